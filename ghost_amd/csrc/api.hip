@@ -1,0 +1,1068 @@
+// api.hip -- C ABI (include/ghx.h) over the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/ghx.h"
+#include "fft_passes.h"
+#include "pseudo.h"
+
+using ghx::FieldSet;
+using ghx::FFTCtx;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return set_err(e_ == hipErrorOutOfMemory ? GHX_ERR_NOMEM : GHX_ERR_HIP, "%s: %s (%s:%d)", #expr, \
+                     hipGetErrorString(e_), __FILE__, __LINE__);                              \
+  } while (0)
+
+#define ARGCHK(cond, msg)                               \
+  do {                                                  \
+    if (!(cond)) return set_err(GHX_ERR_ARG, "%s", msg); \
+  } while (0)
+
+extern "C" const char *ghx_last_error(void) { return g_err; }
+
+extern "C" int ghx_device_ok(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0;
+}
+
+// ---------------------------------------------------------------------------
+// memory
+// ---------------------------------------------------------------------------
+extern "C" int ghx_malloc(void **ptr, size_t bytes) {
+  ARGCHK(ptr, "null pointer");
+  HIPCHK(hipMalloc(ptr, bytes ? bytes : 16));
+  return GHX_OK;
+}
+extern "C" int ghx_free(void *ptr) {
+  if (ptr) HIPCHK(hipFree(ptr));
+  return GHX_OK;
+}
+extern "C" int ghx_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GHX_OK;
+}
+extern "C" int ghx_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GHX_OK;
+}
+extern "C" int ghx_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return GHX_OK;
+}
+extern "C" int ghx_memset_zero(void *ptr, size_t bytes) {
+  HIPCHK(ghx::op_zero(0, ptr, bytes));
+  HIPCHK(hipDeviceSynchronize());
+  return GHX_OK;
+}
+extern "C" int ghx_sync(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return GHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+struct ghx_plan {
+  int nx, ny, nz, nxh, dir;
+  FFTCtx ctx;
+  cplx *tw = nullptr;       // all twiddle tables
+  cplx *q = nullptr;        // Q scratch, q_fields fields
+  int q_fields = 0;
+  bool timing = false;
+  double t_fft = 0.0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool ev_pending = false;
+};
+
+static void fill_tw(std::vector<cplx> &h, size_t off, int n) {
+  const long double twopi = 6.283185307179586476925286766559L;
+  for (int k = 0; k < n; ++k) {
+    long double a = twopi * (long double)k / (long double)n;
+    h[off + k].x = (double)cosl(a);
+    h[off + k].y = -(double)sinl(a);
+  }
+}
+
+static size_t nc_of(int nx, int ny, int nz) { return (size_t)(nx / 2 + 1) * ny * nz; }
+
+static GridK make_gridk(int nx, int ny, int nz, int arbsize, double Lx, double Ly, double Lz) {
+  GridK g;
+  g.nx = nx; g.ny = ny; g.nz = nz; g.nxh = nx / 2 + 1;
+  g.anis = (arbsize || nx != ny || ny != nz) ? 1 : 0;
+  g.dkx = arbsize ? 1.0 / Lx : 1.0;
+  g.dky = arbsize ? 1.0 / Ly : 1.0;
+  g.dkz = arbsize ? 1.0 / Lz : 1.0;
+  g.rmp = g.rmq = g.rms = 1.0;
+  if (g.anis) {
+    g.rmp = 1.0 / ((double)nx * (double)nx);
+    g.rmq = 1.0 / ((double)ny * (double)ny);
+    g.rms = 1.0 / ((double)nz * (double)nz);
+  }
+  g.kmax = 1.0 / 9.0;
+  if (!arbsize && !g.anis) g.kmax = g.kmax * ((double)nx * (double)nx);
+  double t = (double)nx * (double)ny * (double)nz;
+  g.inv_n6 = 1.0 / (t * t);
+  g.tiny = 0.0;
+  return g;
+}
+
+static int plan_init(ghx_plan *p, int nx, int ny, int nz, int dir) {
+  if (nx < 2 || nx % 2 || ny < 2 || nz < 2)
+    return set_err(GHX_ERR_SIZE, "grid %dx%dx%d: nx must be even and all sizes >= 2", nx, ny, nz);
+  if (!ghx::fft_size_supported(nx / 2) || !ghx::fft_size_supported(ny) || !ghx::fft_size_supported(nz))
+    return set_err(GHX_ERR_SIZE, "grid %dx%dx%d has no compiled FFT schedule (supported line lengths: "
+                   "2,3,4,5,6,8,10,12,16,24,32,48,64,96,128,192,256,384,512,1024; x uses nx/2)", nx, ny, nz);
+  if (!ghx_device_ok()) return set_err(GHX_ERR_NODEVICE, "no GPU visible to HIP");
+  p->nx = nx; p->ny = ny; p->nz = nz; p->nxh = nx / 2 + 1; p->dir = dir;
+  std::vector<cplx> h((size_t)nx / 2 + nx + ny + nz);
+  fill_tw(h, 0, nx / 2);
+  fill_tw(h, nx / 2, nx);
+  fill_tw(h, nx / 2 + nx, ny);
+  fill_tw(h, nx / 2 + nx + ny, nz);
+  HIPCHK(hipMalloc(&p->tw, h.size() * sizeof(cplx)));
+  HIPCHK(hipMemcpy(p->tw, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice));
+  p->ctx.gk = make_gridk(nx, ny, nz, 0, 1, 1, 1);
+  p->ctx.tw_xh = p->tw;
+  p->ctx.tw_x = p->tw + nx / 2;
+  p->ctx.tw_y = p->tw + nx / 2 + nx;
+  p->ctx.tw_z = p->tw + nx / 2 + nx + ny;
+  p->ctx.stream = 0;
+  return GHX_OK;
+}
+
+static int plan_scratch(ghx_plan *p, int nf) {
+  if (p->q_fields >= nf) return GHX_OK;
+  if (p->q) HIPCHK(hipFree(p->q));
+  p->q = nullptr;
+  p->q_fields = 0;
+  HIPCHK(hipMalloc(&p->q, (size_t)nf * nc_of(p->nx, p->ny, p->nz) * sizeof(cplx)));
+  p->q_fields = nf;
+  return GHX_OK;
+}
+
+static void plan_release(ghx_plan *p) {
+  if (!p) return;
+  if (p->q) (void)hipFree(p->q);
+  if (p->tw) (void)hipFree(p->tw);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+}
+
+static void timer_collect(ghx_plan *p) {
+  if (p->ev_pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p->ev1) == hipSuccess && hipEventElapsedTime(&ms, p->ev0, p->ev1) == hipSuccess)
+      p->t_fft += ms * 1e-3;
+    p->ev_pending = false;
+  }
+}
+static void timer_begin(ghx_plan *p) {
+  if (!p->timing) return;
+  timer_collect(p);
+  (void)hipEventRecord(p->ev0, p->ctx.stream);
+}
+static void timer_end(ghx_plan *p) {
+  if (!p->timing) return;
+  (void)hipEventRecord(p->ev1, p->ctx.stream);
+  p->ev_pending = true;
+}
+
+extern "C" int ghx_plan_create(ghx_plan **plan, int nx, int ny, int nz, int fftdir, int flags) {
+  (void)flags;
+  ARGCHK(plan, "null plan pointer");
+  ARGCHK(fftdir == GHX_FFTW_REAL_TO_COMPLEX || fftdir == GHX_FFTW_COMPLEX_TO_REAL, "fftdir must be -1 or +1");
+  ghx_plan *p = new (std::nothrow) ghx_plan();
+  if (!p) return set_err(GHX_ERR_NOMEM, "host allocation failed");
+  int rc = plan_init(p, nx, ny, nz, fftdir);
+  if (rc) { plan_release(p); delete p; return rc; }
+  *plan = p;
+  return GHX_OK;
+}
+
+extern "C" int ghx_plan_destroy(ghx_plan *plan) {
+  if (!plan) return GHX_OK;
+  plan_release(plan);
+  delete plan;
+  return GHX_OK;
+}
+
+extern "C" int ghx_plan_set_stream(ghx_plan *plan, void *stream) {
+  ARGCHK(plan, "null plan");
+  plan->ctx.stream = (hipStream_t)stream;
+  return GHX_OK;
+}
+
+static int do_r2c(ghx_plan *p, int nf, const double *const *in, void *const *out) {
+  ARGCHK(p && in && out && nf >= 1 && nf <= ghx::kMaxFields, "bad r2c arguments");
+  int rc = plan_scratch(p, nf);
+  if (rc) return rc;
+  const size_t nc = nc_of(p->nx, p->ny, p->nz);
+  FieldSet q, o;
+  for (int f = 0; f < nf; ++f) {
+    ARGCHK(in[f] && out[f], "null field pointer");
+    q.p[f] = p->q + f * nc;
+    o.p[f] = (cplx *)out[f];
+  }
+  timer_begin(p);
+  HIPCHK(ghx::xfwd(p->ctx, nf, in, q));
+  HIPCHK(ghx::ypass(p->ctx, -1, nf, q));
+  HIPCHK(ghx::zfwd(p->ctx, nf, q, o));
+  timer_end(p);
+  return GHX_OK;
+}
+
+static int do_c2r(ghx_plan *p, int nf, const void *const *in, double *const *out) {
+  ARGCHK(p && in && out && nf >= 1 && nf <= ghx::kMaxFields, "bad c2r arguments");
+  int rc = plan_scratch(p, nf);
+  if (rc) return rc;
+  const size_t nc = nc_of(p->nx, p->ny, p->nz);
+  FieldSet q, i;
+  for (int f = 0; f < nf; ++f) {
+    ARGCHK(in[f] && out[f], "null field pointer");
+    q.p[f] = p->q + f * nc;
+    i.p[f] = (cplx *)in[f];
+  }
+  timer_begin(p);
+  HIPCHK(ghx::zinv(p->ctx, 0, nf, i, q));
+  HIPCHK(ghx::ypass(p->ctx, +1, nf, q));
+  HIPCHK(ghx::xinv(p->ctx, nf, q, out));
+  timer_end(p);
+  return GHX_OK;
+}
+
+extern "C" int ghx_r2c(ghx_plan *plan, const double *in, void *out) { return do_r2c(plan, 1, &in, &out); }
+extern "C" int ghx_c2r(ghx_plan *plan, const void *in, double *out) { return do_c2r(plan, 1, &in, &out); }
+extern "C" int ghx_r2c_many(ghx_plan *plan, int nf, const double *const *in, void *const *out) {
+  return do_r2c(plan, nf, in, out);
+}
+extern "C" int ghx_c2r_many(ghx_plan *plan, int nf, const void *const *in, double *const *out) {
+  return do_c2r(plan, nf, in, out);
+}
+
+extern "C" int ghx_r2c_host(ghx_plan *p, const double *in, void *out) {
+  ARGCHK(p && in && out, "bad arguments");
+  const size_t nr = (size_t)p->nx * p->ny * p->nz, nc = nc_of(p->nx, p->ny, p->nz);
+  double *din = nullptr;
+  void *dout = nullptr;
+  HIPCHK(hipMalloc(&din, nr * sizeof(double)));
+  hipError_t e = hipMalloc(&dout, nc * sizeof(cplx));
+  if (e != hipSuccess) { (void)hipFree(din); HIPCHK(e); }
+  int rc = GHX_OK;
+  if ((e = hipMemcpy(din, in, nr * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) rc = set_err(GHX_ERR_HIP, "%s", hipGetErrorString(e));
+  if (!rc) rc = ghx_r2c(p, din, dout);
+  if (!rc && (e = hipMemcpyAsync(out, dout, nc * sizeof(cplx), hipMemcpyDeviceToHost, p->ctx.stream)) != hipSuccess)
+    rc = set_err(GHX_ERR_HIP, "%s", hipGetErrorString(e));
+  if (!rc && (e = hipStreamSynchronize(p->ctx.stream)) != hipSuccess) rc = set_err(GHX_ERR_HIP, "%s", hipGetErrorString(e));
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return rc;
+}
+
+extern "C" int ghx_c2r_host(ghx_plan *p, const void *in, double *out) {
+  ARGCHK(p && in && out, "bad arguments");
+  const size_t nr = (size_t)p->nx * p->ny * p->nz, nc = nc_of(p->nx, p->ny, p->nz);
+  void *din = nullptr;
+  double *dout = nullptr;
+  HIPCHK(hipMalloc(&din, nc * sizeof(cplx)));
+  hipError_t e = hipMalloc(&dout, nr * sizeof(double));
+  if (e != hipSuccess) { (void)hipFree(din); HIPCHK(e); }
+  int rc = GHX_OK;
+  if ((e = hipMemcpy(din, in, nc * sizeof(cplx), hipMemcpyHostToDevice)) != hipSuccess) rc = set_err(GHX_ERR_HIP, "%s", hipGetErrorString(e));
+  if (!rc) rc = ghx_c2r(p, din, dout);
+  if (!rc && (e = hipMemcpyAsync(out, dout, nr * sizeof(double), hipMemcpyDeviceToHost, p->ctx.stream)) != hipSuccess)
+    rc = set_err(GHX_ERR_HIP, "%s", hipGetErrorString(e));
+  if (!rc && (e = hipStreamSynchronize(p->ctx.stream)) != hipSuccess) rc = set_err(GHX_ERR_HIP, "%s", hipGetErrorString(e));
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return rc;
+}
+
+extern "C" int ghx_plan_timers(ghx_plan *p, double *ffttime, double *tratime, double *comtime, double *tottime) {
+  ARGCHK(p, "null plan");
+  timer_collect(p);
+  if (ffttime) *ffttime = p->t_fft;
+  if (tratime) *tratime = 0.0;
+  if (comtime) *comtime = 0.0;
+  if (tottime) *tottime = p->t_fft;
+  return GHX_OK;
+}
+
+extern "C" int ghx_plan_reset_timers(ghx_plan *p) {
+  ARGCHK(p, "null plan");
+  if (!p->ev0) {
+    HIPCHK(hipEventCreate(&p->ev0));
+    HIPCHK(hipEventCreate(&p->ev1));
+  }
+  timer_collect(p);
+  p->timing = true;
+  p->t_fft = 0.0;
+  return GHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// grid
+// ---------------------------------------------------------------------------
+struct ghx_grid {
+  ghx_plan plan;
+  GridK gk;
+  double Dkk = 1.0;
+  int nmax = 0;
+  hipStream_t stream = 0;
+  double *work = nullptr;   // reductions / spectra
+  size_t work_cap = 0;
+  cplx *ctmp = nullptr;     // complex temporaries
+  int ctmp_fields = 0;
+  double *rtmp = nullptr;   // real temporaries
+  int rtmp_fields = 0;
+  size_t nc = 0, nr = 0;
+};
+
+static int grid_ctmp(ghx_grid *g, int nf) {
+  if (g->ctmp_fields >= nf) return GHX_OK;
+  if (g->ctmp) HIPCHK(hipFree(g->ctmp));
+  g->ctmp = nullptr;
+  g->ctmp_fields = 0;
+  HIPCHK(hipMalloc(&g->ctmp, (size_t)nf * g->nc * sizeof(cplx)));
+  g->ctmp_fields = nf;
+  return GHX_OK;
+}
+static int grid_rtmp(ghx_grid *g, int nf) {
+  if (g->rtmp_fields >= nf) return GHX_OK;
+  if (g->rtmp) HIPCHK(hipFree(g->rtmp));
+  g->rtmp = nullptr;
+  g->rtmp_fields = 0;
+  HIPCHK(hipMalloc(&g->rtmp, (size_t)nf * g->nr * sizeof(double)));
+  g->rtmp_fields = nf;
+  return GHX_OK;
+}
+static int grid_work(ghx_grid *g, size_t n) {
+  if (g->work_cap >= n) return GHX_OK;
+  if (g->work) HIPCHK(hipFree(g->work));
+  g->work = nullptr;
+  g->work_cap = 0;
+  HIPCHK(hipMalloc(&g->work, n * sizeof(double)));
+  g->work_cap = n;
+  return GHX_OK;
+}
+
+extern "C" int ghx_grid_create(ghx_grid **grid, int nx, int ny, int nz, double Lx, double Ly, double Lz, int arbsize,
+                               double Dkk) {
+  ARGCHK(grid, "null grid pointer");
+  if (arbsize) ARGCHK(Lx > 0 && Ly > 0 && Lz > 0, "box lengths must be positive");
+  ghx_grid *g = new (std::nothrow) ghx_grid();
+  if (!g) return set_err(GHX_ERR_NOMEM, "host allocation failed");
+  int rc = plan_init(&g->plan, nx, ny, nz, GHX_FFTW_REAL_TO_COMPLEX);
+  if (rc) { plan_release(&g->plan); delete g; return rc; }
+  GridK k = make_gridk(nx, ny, nz, arbsize, Lx, Ly, Lz);
+  // Dkk, nmax, tiny: main3D.fpp:905-909, 1728-1735
+  double dkk = arbsize ? Dkk : 0.0;
+  if (dkk < 1e-5) dkk = std::fmin(k.dkx, std::fmin(k.dky, k.dkz));
+  double mx = std::fmax(nx * k.dkx, std::fmax(ny * k.dky, nz * k.dkz));
+  g->nmax = (int)(mx / dkk);
+  k.tiny = std::fmin(1e-5, 0.1 / ((double)g->nmax * (double)g->nmax));
+  g->Dkk = dkk;
+  g->gk = k;
+  g->plan.ctx.gk = k;
+  g->nc = nc_of(nx, ny, nz);
+  g->nr = (size_t)nx * ny * nz;
+  *grid = g;
+  return GHX_OK;
+}
+
+extern "C" int ghx_grid_destroy(ghx_grid *g) {
+  if (!g) return GHX_OK;
+  plan_release(&g->plan);
+  if (g->work) (void)hipFree(g->work);
+  if (g->ctmp) (void)hipFree(g->ctmp);
+  if (g->rtmp) (void)hipFree(g->rtmp);
+  delete g;
+  return GHX_OK;
+}
+
+extern "C" int ghx_grid_set_stream(ghx_grid *g, void *stream) {
+  ARGCHK(g, "null grid");
+  g->stream = (hipStream_t)stream;
+  g->plan.ctx.stream = (hipStream_t)stream;
+  return GHX_OK;
+}
+
+extern "C" int ghx_grid_info(const ghx_grid *g, double *kmax, double *tiny, int *nmax, double *Dkk, int *anis) {
+  ARGCHK(g, "null grid");
+  if (kmax) *kmax = g->gk.kmax;
+  if (tiny) *tiny = g->gk.tiny;
+  if (nmax) *nmax = g->nmax;
+  if (Dkk) *Dkk = g->Dkk;
+  if (anis) *anis = g->gk.anis;
+  return GHX_OK;
+}
+
+extern "C" int ghx_grid_r2c(ghx_grid *g, const double *in, void *out) {
+  ARGCHK(g, "null grid");
+  return ghx_r2c(&g->plan, in, out);
+}
+extern "C" int ghx_grid_c2r(ghx_grid *g, const void *in, double *out) {
+  ARGCHK(g, "null grid");
+  return ghx_c2r(&g->plan, in, out);
+}
+
+// ---------------------------------------------------------------------------
+// operators
+// ---------------------------------------------------------------------------
+#define C_(p) ((const cplx *)(p))
+#define M_(p) ((cplx *)(p))
+
+extern "C" int ghx_derivk3(ghx_grid *g, const void *a, void *b, int dir) {
+  ARGCHK(g && a && b && dir >= 1 && dir <= 3, "bad derivk3 arguments");
+  HIPCHK(ghx::op_derivk3(g->gk, g->stream, C_(a), M_(b), dir));
+  return GHX_OK;
+}
+extern "C" int ghx_laplak3(ghx_grid *g, const void *a, void *b) {
+  ARGCHK(g && a && b, "bad laplak3 arguments");
+  HIPCHK(ghx::op_laplak3(g->gk, g->stream, C_(a), M_(b)));
+  return GHX_OK;
+}
+extern "C" int ghx_rotor3(ghx_grid *g, const void *a, const void *b, void *c, int dir) {
+  ARGCHK(g && a && b && c && dir >= 1 && dir <= 3, "bad rotor3 arguments");
+  HIPCHK(ghx::op_rotor3(g->gk, g->stream, C_(a), C_(b), M_(c), dir));
+  return GHX_OK;
+}
+extern "C" int ghx_nonlhd3(ghx_grid *g, const void *a, const void *b, const void *c, void *out, int dir) {
+  ARGCHK(g && a && b && c && out && dir >= 1 && dir <= 3, "bad nonlhd3 arguments");
+  HIPCHK(ghx::op_nonlhd3(g->gk, g->stream, C_(a), C_(b), C_(c), M_(out), dir));
+  return GHX_OK;
+}
+extern "C" int ghx_nonlin3(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                           const void *f, void *out, int dir) {
+  ARGCHK(g && a && b && c && d && e && f && out && dir >= 1 && dir <= 3, "bad nonlin3 arguments");
+  HIPCHK(ghx::op_nonlin3(g->gk, g->stream, C_(a), C_(b), C_(c), C_(d), C_(e), C_(f), M_(out), dir));
+  return GHX_OK;
+}
+extern "C" int ghx_gauge3(ghx_grid *g, const void *a, const void *b, const void *c, void *out, int dir) {
+  ARGCHK(g && a && b && c && out && dir >= 1 && dir <= 3, "bad gauge3 arguments");
+  HIPCHK(ghx::op_gauge3(g->gk, g->stream, C_(a), C_(b), C_(c), M_(out), dir));
+  return GHX_OK;
+}
+
+// (P x Q) products through the fused passes.  mode 1: P = curl(a,b,c), Q = (a,b,c)
+// (prodre3); mode 0: P = (a,b,c), Q = (d,e,f) (vector3).
+static int fused_cross(ghx_grid *g, int mode, const void *a, const void *b, const void *c, const void *d,
+                       const void *e, const void *f, void *x, void *y, void *z) {
+  int rc = plan_scratch(&g->plan, 9);
+  if (rc) return rc;
+  const size_t nc = g->nc;
+  const FFTCtx &ctx = g->plan.ctx;
+  FieldSet in, q6, q3, out;
+  for (int k = 0; k < 6; ++k) q6.p[k] = g->plan.q + k * nc;
+  for (int k = 0; k < 3; ++k) q3.p[k] = g->plan.q + (6 + k) * nc;
+  out.p[0] = M_(x); out.p[1] = M_(y); out.p[2] = M_(z);
+  FieldSet pq;  // P fields first, then Q
+  if (mode == 1) {
+    in.p[0] = M_(a); in.p[1] = M_(b); in.p[2] = M_(c);
+    HIPCHK(ghx::zinv(ctx, 1, 3, in, q6));  // q6 = (u, curl u)
+    for (int k = 0; k < 3; ++k) { pq.p[k] = q6.p[3 + k]; pq.p[3 + k] = q6.p[k]; }
+  } else {
+    in.p[0] = M_(a); in.p[1] = M_(b); in.p[2] = M_(c); in.p[3] = M_(d); in.p[4] = M_(e); in.p[5] = M_(f);
+    HIPCHK(ghx::zinv(ctx, 0, 6, in, q6));
+    for (int k = 0; k < 6; ++k) pq.p[k] = q6.p[k];
+  }
+  HIPCHK(ghx::ypass(ctx, +1, 6, q6));
+  HIPCHK(ghx::xcross(ctx, 0, pq, q3));
+  HIPCHK(ghx::ypass(ctx, -1, 3, q3));
+  HIPCHK(ghx::zfwd(ctx, 3, q3, out));
+  return GHX_OK;
+}
+
+extern "C" int ghx_prodre3(ghx_grid *g, const void *a, const void *b, const void *c, void *d, void *e, void *f) {
+  ARGCHK(g && a && b && c && d && e && f, "bad prodre3 arguments");
+  return fused_cross(g, 1, a, b, c, nullptr, nullptr, nullptr, d, e, f);
+}
+extern "C" int ghx_vector3(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                           const void *f, void *x, void *y, void *z) {
+  ARGCHK(g && a && b && c && d && e && f && x && y && z, "bad vector3 arguments");
+  return fused_cross(g, 0, a, b, c, d, e, f, x, y, z);
+}
+
+static int reduce(ghx_grid *g, int kind, const void *a, const void *b, const void *c, const void *d, const void *e,
+                  const void *f, double *out) {
+  int rc = grid_work(g, ghx::reduce_work_doubles());
+  if (rc) return rc;
+  HIPCHK(ghx::op_reduce(g->gk, g->stream, kind, C_(a), C_(b), C_(c), C_(d), C_(e), C_(f), g->work, out));
+  return GHX_OK;
+}
+
+static int curl3(ghx_grid *g, const void *a, const void *b, const void *c, cplx *o) {
+  HIPCHK(ghx::op_rotor3(g->gk, g->stream, C_(b), C_(c), o, 1));
+  HIPCHK(ghx::op_rotor3(g->gk, g->stream, C_(a), C_(c), o + g->nc, 2));
+  HIPCHK(ghx::op_rotor3(g->gk, g->stream, C_(a), C_(b), o + 2 * g->nc, 3));
+  return GHX_OK;
+}
+
+extern "C" int ghx_energy(ghx_grid *g, const void *a, const void *b, const void *c, int kin, double *out) {
+  ARGCHK(g && a && b && c && out && kin >= 0 && kin <= 2, "bad energy arguments");
+  if (kin == 1) return reduce(g, 0, a, b, c, nullptr, nullptr, nullptr, out);
+  int rc = grid_ctmp(g, 3);
+  if (rc) return rc;
+  cplx *t = g->ctmp;
+  if (kin == 0) {
+    if ((rc = curl3(g, a, b, c, t))) return rc;
+  } else {
+    HIPCHK(ghx::op_laplak3(g->gk, g->stream, C_(a), t));
+    HIPCHK(ghx::op_laplak3(g->gk, g->stream, C_(b), t + g->nc));
+    HIPCHK(ghx::op_laplak3(g->gk, g->stream, C_(c), t + 2 * g->nc));
+  }
+  return reduce(g, 0, t, t + g->nc, t + 2 * g->nc, nullptr, nullptr, nullptr, out);
+}
+
+extern "C" int ghx_helicity(ghx_grid *g, const void *a, const void *b, const void *c, double *out) {
+  ARGCHK(g && a && b && c && out, "bad helicity arguments");
+  int rc = grid_ctmp(g, 3);
+  if (rc) return rc;
+  if ((rc = curl3(g, a, b, c, g->ctmp))) return rc;
+  return reduce(g, 1, a, b, c, g->ctmp, g->ctmp + g->nc, g->ctmp + 2 * g->nc, out);
+}
+
+extern "C" int ghx_cross(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                         const void *f, int kin, double *out) {
+  ARGCHK(g && a && b && c && d && e && f && out && (kin == 0 || kin == 1), "bad cross arguments");
+  return reduce(g, kin == 1 ? 1 : 2, a, b, c, d, e, f, out);
+}
+
+extern "C" int ghx_divergence(ghx_grid *g, const void *a, const void *b, const void *c, double *out) {
+  ARGCHK(g && a && b && c && out, "bad divergence arguments");
+  return reduce(g, 3, a, b, c, nullptr, nullptr, nullptr, out);
+}
+
+extern "C" int ghx_maxabs(ghx_grid *g, const void *a, const void *b, const void *c, int kin, double *out) {
+  ARGCHK(g && a && b && c && out && kin >= 0 && kin <= 2, "bad maxabs arguments");
+  int rc = grid_ctmp(g, 3);
+  if (!rc) rc = grid_rtmp(g, 3);
+  if (!rc) rc = grid_work(g, ghx::reduce_work_doubles());
+  if (rc) return rc;
+  cplx *t = g->ctmp;
+  const void *src[3];
+  if (kin == 0) {
+    if ((rc = curl3(g, a, b, c, t))) return rc;
+    src[0] = t; src[1] = t + g->nc; src[2] = t + 2 * g->nc;
+  } else if (kin == 1) {
+    HIPCHK(ghx::op_laplak3(g->gk, g->stream, C_(a), t));
+    HIPCHK(ghx::op_laplak3(g->gk, g->stream, C_(b), t + g->nc));
+    HIPCHK(ghx::op_laplak3(g->gk, g->stream, C_(c), t + 2 * g->nc));
+    src[0] = t; src[1] = t + g->nc; src[2] = t + 2 * g->nc;
+  } else {
+    src[0] = a; src[1] = b; src[2] = c;
+  }
+  double *r[3] = {g->rtmp, g->rtmp + g->nr, g->rtmp + 2 * g->nr};
+  if ((rc = do_c2r(&g->plan, 3, src, r))) return rc;
+  double m = 0;
+  HIPCHK(ghx::op_maxnorm(g->gk, g->stream, r[0], r[1], r[2], g->work, &m));
+  *out = m / ((double)g->gk.nx * (double)g->gk.ny * (double)g->gk.nz);
+  return GHX_OK;
+}
+
+extern "C" int ghx_spectrumc(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
+                             double *Hk) {
+  ARGCHK(g && a && b && c && kin >= 0 && kin <= 2 && (hel == 0 || hel == 1), "bad spectrumc arguments");
+  const int nb = g->nmax / 2 + 1;
+  int rc = grid_work(g, ghx::spectrum_work_doubles(g->gk, nb));
+  if (rc) return rc;
+  cplx *t = nullptr;
+  if (kin == 0 || hel == 1) {
+    if ((rc = grid_ctmp(g, 3))) return rc;
+    t = g->ctmp;
+    if ((rc = curl3(g, a, b, c, t))) return rc;
+  }
+  if (kin <= 1 && Ek) {
+    if (kin == 1)
+      HIPCHK(ghx::op_spectrum(g->gk, g->stream, 0, g->Dkk, nb, C_(a), C_(b), C_(c), nullptr, nullptr, nullptr, g->work, Ek));
+    else
+      HIPCHK(ghx::op_spectrum(g->gk, g->stream, 0, g->Dkk, nb, t, t + g->nc, t + 2 * g->nc, nullptr, nullptr, nullptr,
+                              g->work, Ek));
+  }
+  if (hel == 1 && Hk)
+    HIPCHK(ghx::op_spectrum(g->gk, g->stream, 1, g->Dkk, nb, C_(a), C_(b), C_(c), t, t + g->nc, t + 2 * g->nc,
+                            g->work, Hk));
+  return GHX_OK;
+}
+
+extern "C" int ghx_phaseshift(ghx_grid *g, void *a, double cre, double cim) {
+  ARGCHK(g && a, "bad phaseshift arguments");
+  HIPCHK(ghx::op_phaseshift(g->gk, g->stream, M_(a), cre, cim));
+  return GHX_OK;
+}
+
+extern "C" int ghx_normalize(ghx_grid *g, void *fx, void *fy, void *fz, double f0, int kin) {
+  ARGCHK(g && fx && fy && fz, "bad normalize arguments");
+  double e = 0;
+  int rc = ghx_energy(g, fx, fy, fz, kin, &e);
+  if (rc) return rc;
+  HIPCHK(ghx::op_normalize_apply(g->gk, g->stream, M_(fx), M_(fy), M_(fz), f0, e));
+  return GHX_OK;
+}
+
+extern "C" int ghx_sharp(ghx_grid *g, const void *a, void *b, double delta) {
+  ARGCHK(g && a && b && delta > 0, "bad sharp arguments");
+  const double pi = 3.14159265358979323846;
+  HIPCHK(ghx::op_sharp(g->gk, g->stream, C_(a), M_(b), (pi / delta) * (pi / delta)));
+  return GHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// initial conditions / forcing
+// ---------------------------------------------------------------------------
+extern "C" double ghx_randu(int32_t *idum) {
+  // Park-Miller minimal standard, Schrage factorisation, XOR mask; the scale
+  // am = 1./im is a single-precision constant in GHOST.
+  const int32_t iq = 127773, ir = 2836, mask = 123459876, ia = 16807, im = 2147483647;
+  const double am = (double)(1.0f / (float)im);
+  int32_t s = *idum ^ mask;
+  const int32_t k = s / iq;
+  s = ia * (s - k * iq) - ir * k;
+  if (s < 0) s += im;
+  double r = am * (double)s;
+  r = (r - 0.5) * 2;
+  *idum = s ^ mask;
+  return r;
+}
+
+extern "C" int ghx_initialv_tg(ghx_grid *g, double kdn, double kup, double u0, void *vx, void *vy, void *vz) {
+  ARGCHK(g && vx && vy && vz, "bad initialv_tg arguments");
+  int rc = grid_rtmp(g, 2);
+  if (rc) return rc;
+  double *r1 = g->rtmp, *r2 = g->rtmp + g->nr;
+  HIPCHK(ghx::op_tg_real(g->gk, g->stream, r1, r2, (int)kdn, (int)kup));
+  HIPCHK(ghx::op_zero(g->stream, vz, g->nc * sizeof(cplx)));
+  const double *in[2] = {r1, r2};
+  void *out[2] = {vx, vy};
+  if ((rc = do_r2c(&g->plan, 2, in, out))) return rc;
+  return ghx_normalize(g, vx, vy, vz, u0, 1);
+}
+
+// Host-side random-phase fill of three fields in the loop order of
+// examples/initialfv.f90_random:16-135 (kx=0 plane with its Hermitian mirrors,
+// then kx>0 planes), one kx plane at a time, uploaded to c1..c3.
+namespace {
+struct HostGrid {
+  int nx, ny, nz;
+  GridK k;
+  double kxv(int i) const { return ghxw(i, nx) * k.dkx; }
+  double kyv(int j) const { return ghxw(j, ny) * k.dky; }
+  double kzv(int q) const { return ghxw(q, nz) * k.dkz; }
+  static double ghxw(int q, int n) { return (double)(q < n / 2 ? q : q - n); }
+  double kn2(int i, int j, int q) const {
+    double a = ghxw(i, nx), b = ghxw(j, ny), c = ghxw(q, nz);
+    return k.rmp * (a * a) + k.rmq * (b * b) + k.rms * (c * c);
+  }
+  double kk2(int i, int j, int q) const {
+    if (!k.anis) return kn2(i, j, q);
+    double a = kxv(i), b = kyv(j), c = kzv(q);
+    return a * a + b * b + c * c;
+  }
+};
+
+template <typename Band, typename Amp>
+int random_fill(ghx_grid *g, Band inband, Amp amp, int32_t *seed, cplx *c1, cplx *c2, cplx *c3) {
+  const double pi = 3.14159265358979323846;
+  HostGrid hg{g->gk.nx, g->gk.ny, g->gk.nz, g->gk};
+  const int ny = hg.ny, nz = hg.nz;
+  const size_t plane = (size_t)ny * nz;
+  std::vector<cplx> h(3 * plane);
+  cplx *C[3] = {h.data(), h.data() + plane, h.data() + 2 * plane};
+  cplx *D[3] = {c1, c2, c3};
+  auto draw = [&](double fac, double dump) {
+    double phase = 2 * pi * ghx_randu(seed);
+    cplx r;
+    r.x = fac * std::cos(phase) * dump;
+    r.y = fac * std::sin(phase) * dump;
+    return r;
+  };
+  auto conjc = [](cplx a) { a.y = -a.y; return a; };
+  const cplx zero = {0.0, 0.0};
+  // kx = 0 plane, m(k,j) = j*nz + k
+  for (int f = 0; f < 3; ++f) C[f][0] = zero;
+  auto pair = [&](size_t m, size_t mm, int j, int q) {
+    if (inband(0, j, q)) {
+      double dump = amp(0, j, q);
+      for (int f = 0; f < 3; ++f) {
+        C[f][m] = draw(1.0, dump);
+        C[f][mm] = conjc(C[f][m]);
+      }
+    } else {
+      for (int f = 0; f < 3; ++f) { C[f][m] = zero; C[f][mm] = zero; }
+    }
+  };
+  for (int j = 2; j <= ny / 2 + 1; ++j) pair((size_t)(j - 1) * nz, (size_t)(ny - j + 1) * nz, j - 1, 0);
+  for (int q = 2; q <= nz / 2 + 1; ++q) pair((size_t)(q - 1), (size_t)(nz - q + 1), 0, q - 1);
+  for (int j = 2; j <= ny; ++j)
+    for (int q = 2; q <= nz / 2 + 1; ++q)
+      pair((size_t)(j - 1) * nz + (q - 1), (size_t)(ny - j + 1) * nz + (nz - q + 1), j - 1, q - 1);
+  for (int f = 0; f < 3; ++f) HIPCHK(hipMemcpy(D[f], C[f], plane * sizeof(cplx), hipMemcpyHostToDevice));
+  for (int i = 1; i < g->gk.nxh; ++i) {
+    for (int j = 0; j < ny; ++j)
+      for (int q = 0; q < nz; ++q) {
+        size_t m = (size_t)j * nz + q;
+        if (inband(i, j, q)) {
+          double dump = amp(i, j, q);
+          for (int f = 0; f < 3; ++f) C[f][m] = draw(2.0, dump);
+        } else {
+          for (int f = 0; f < 3; ++f) C[f][m] = zero;
+        }
+      }
+    for (int f = 0; f < 3; ++f)
+      HIPCHK(hipMemcpy(D[f] + (size_t)i * plane, C[f], plane * sizeof(cplx), hipMemcpyHostToDevice));
+  }
+  return GHX_OK;
+}
+
+}  // namespace
+
+static int finish_random(ghx_grid *g, double f0, void *fx, void *fy, void *fz) {
+  cplx *C = g->ctmp;
+  // rotor3(C2,C3,fx,1); rotor3(C1,C3,fy,2); rotor3(C1,C2,fz,3); normalize
+  HIPCHK(ghx::op_rotor3(g->gk, g->stream, C + g->nc, C + 2 * g->nc, M_(fx), 1));
+  HIPCHK(ghx::op_rotor3(g->gk, g->stream, C, C + 2 * g->nc, M_(fy), 2));
+  HIPCHK(ghx::op_rotor3(g->gk, g->stream, C, C + g->nc, M_(fz), 3));
+  return ghx_normalize(g, fx, fy, fz, f0, 1);
+}
+
+extern "C" int ghx_initialfv_random(ghx_grid *g, double kdn, double kup, double f0, int32_t *seed, void *fx, void *fy,
+                                    void *fz) {
+  ARGCHK(g && seed && fx && fy && fz, "bad initialfv_random arguments");
+  int rc = grid_ctmp(g, 3);
+  if (rc) return rc;
+  HostGrid hg{g->gk.nx, g->gk.ny, g->gk.nz, g->gk};
+  auto band = [&](int i, int j, int q) {
+    double k2 = hg.kk2(i, j, q);
+    return k2 <= kup * kup && k2 >= kdn * kdn;
+  };
+  auto amp = [&](int i, int j, int q) { return 1. / std::sqrt(hg.kk2(i, j, q)); };
+  if ((rc = random_fill(g, band, amp, seed, g->ctmp, g->ctmp + g->nc, g->ctmp + 2 * g->nc))) return rc;
+  return finish_random(g, f0, fx, fy, fz);
+}
+
+extern "C" int ghx_initialv_k4exp(ghx_grid *g, double k0, double u0, int32_t *seed, void *vx, void *vy, void *vz) {
+  ARGCHK(g && seed && vx && vy && vz && k0 > 0, "bad initialv_k4exp arguments");
+  int rc = grid_ctmp(g, 3);
+  if (rc) return rc;
+  HostGrid hg{g->gk.nx, g->gk.ny, g->gk.nz, g->gk};
+  auto band = [&](int i, int j, int q) {
+    double k2 = hg.kn2(i, j, q);
+    return k2 <= g->gk.kmax && k2 >= g->gk.tiny;
+  };
+  auto amp = [&](int i, int j, int q) { return std::exp(-hg.kk2(i, j, q) / (k0 * k0)); };
+  if ((rc = random_fill(g, band, amp, seed, g->ctmp, g->ctmp + g->nc, g->ctmp + 2 * g->nc))) return rc;
+  return finish_random(g, u0, vx, vy, vz);
+}
+
+// ---------------------------------------------------------------------------
+// RK substeps
+// ---------------------------------------------------------------------------
+
+// Fused HD substep: q holds 9 Q-layout fields of scratch.
+static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const cplx *const *v, const cplx *const *c,
+                      const cplx *const *f, const uint8_t *fline, cplx *const *out) {
+  const size_t nc = g->nc;
+  const FFTCtx &ctx = g->plan.ctx;
+  FieldSet in, q6, q3, pq;
+  for (int k = 0; k < 3; ++k) in.p[k] = const_cast<cplx *>(v[k]);
+  for (int k = 0; k < 6; ++k) q6.p[k] = q + k * nc;
+  for (int k = 0; k < 3; ++k) q3.p[k] = q + (6 + k) * nc;
+  for (int k = 0; k < 3; ++k) { pq.p[k] = q6.p[3 + k]; pq.p[3 + k] = q6.p[k]; }
+  HIPCHK(ghx::zinv(ctx, 1, 3, in, q6));
+  HIPCHK(ghx::ypass(ctx, +1, 6, q6));
+  HIPCHK(ghx::xcross(ctx, 0, pq, q3));
+  HIPCHK(ghx::ypass(ctx, -1, 3, q3));
+  ghx::RKParams rk;
+  rk.dt = dt; rk.nu = nu; rk.rmp = 1. / (double)o;
+  for (int k = 0; k < 3; ++k) {
+    rk.v[k] = v[k]; rk.c[k] = c[k]; rk.out[k] = out[k];
+    rk.f[k] = f ? f[k] : nullptr;
+  }
+  rk.fline = f ? fline : nullptr;
+  HIPCHK(ghx::zfwd_rk_hd(ctx, q3, rk));
+  return GHX_OK;
+}
+
+// lines (kx,ky) on which any of the fields is non-zero
+__global__ void k_line_flags(const cplx *a, const cplx *b, const cplx *c, int nz, size_t nlines, uint8_t *flag) {
+  for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < nlines; l += (size_t)gridDim.x * blockDim.x) {
+    uint8_t nzf = 0;
+    const size_t base = l * nz;
+    for (int k = 0; k < nz; ++k) {
+      cplx x = a[base + k], y = b[base + k], z = c[base + k];
+      if (x.x != 0.0 || x.y != 0.0 || y.x != 0.0 || y.y != 0.0 || z.x != 0.0 || z.y != 0.0) nzf = 1;
+    }
+    flag[l] = nzf;
+  }
+}
+
+static int line_flags(ghx_grid *g, const cplx *a, const cplx *b, const cplx *c, uint8_t *flag) {
+  size_t nl = (size_t)g->gk.nxh * g->gk.ny;
+  unsigned nb = (unsigned)((nl + 255) / 256);
+  hipLaunchKernelGGL(k_line_flags, dim3(nb), dim3(256), 0, g->stream, a, b, c, g->gk.nz, nl, flag);
+  HIPCHK(hipGetLastError());
+  return GHX_OK;
+}
+
+extern "C" int ghx_hd_rkstep2(ghx_grid *g, int o, double dt, double nu, const void *const *v_in, const void *const *C,
+                              const void *const *f, void *const *v_out) {
+  ARGCHK(g && o >= 1 && v_in && C && v_out, "bad hd_rkstep2 arguments");
+  int rc = plan_scratch(&g->plan, 9);
+  if (rc) return rc;
+  const cplx *v[3], *c[3], *ff[3];
+  cplx *out[3];
+  for (int k = 0; k < 3; ++k) {
+    v[k] = C_(v_in[k]); c[k] = C_(C[k]); out[k] = M_(v_out[k]);
+    ff[k] = f ? C_(f[k]) : nullptr;
+  }
+  uint8_t *fl = nullptr;
+  if (f) {
+    // dense forcing: every line is read
+    size_t nl = (size_t)g->gk.nxh * g->gk.ny;
+    if ((rc = grid_work(g, (nl + 7) / 8 + 1))) return rc;
+    fl = (uint8_t *)g->work;
+    if ((rc = line_flags(g, ff[0], ff[1], ff[2], fl))) return rc;
+  }
+  return hd_substep(g, g->plan.q, o, dt, nu, v, c, f ? ff : nullptr, fl, out);
+}
+
+struct ghx_hd {
+  ghx_grid *g;
+  int ord;
+  double dt, nu;
+  cplx *mem = nullptr;  // v(3), v1(3), f(3), q(9)
+  cplx *v[3], *v1[3], *f[3], *q;
+  uint8_t *fline = nullptr;
+  bool forced = false;
+};
+
+extern "C" int ghx_hd_create(ghx_hd **s, ghx_grid *g, int ord, double dt, double nu) {
+  ARGCHK(s && g && ord >= 1, "bad hd_create arguments");
+  ghx_hd *h = new (std::nothrow) ghx_hd();
+  if (!h) return set_err(GHX_ERR_NOMEM, "host allocation failed");
+  h->g = g; h->ord = ord; h->dt = dt; h->nu = nu;
+  const size_t nc = g->nc;
+  hipError_t e = hipMalloc(&h->mem, 18 * nc * sizeof(cplx));
+  if (e != hipSuccess) { delete h; HIPCHK(e); }
+  for (int k = 0; k < 3; ++k) {
+    h->v[k] = h->mem + k * nc;
+    h->v1[k] = h->mem + (3 + k) * nc;
+    h->f[k] = h->mem + (6 + k) * nc;
+  }
+  h->q = h->mem + 9 * nc;
+  e = hipMalloc(&h->fline, (size_t)g->gk.nxh * g->gk.ny);
+  if (e != hipSuccess) { (void)hipFree(h->mem); delete h; HIPCHK(e); }
+  e = ghx::op_zero(g->stream, h->mem, 9 * nc * sizeof(cplx));
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) { (void)hipFree(h->mem); (void)hipFree(h->fline); delete h; HIPCHK(e); }
+  *s = h;
+  return GHX_OK;
+}
+
+extern "C" int ghx_hd_destroy(ghx_hd *s) {
+  if (!s) return GHX_OK;
+  (void)hipFree(s->mem);
+  (void)hipFree(s->fline);
+  delete s;
+  return GHX_OK;
+}
+
+extern "C" int ghx_hd_fields(ghx_hd *s, void **v, void **f) {
+  ARGCHK(s, "null solver");
+  for (int k = 0; k < 3; ++k) {
+    if (v) v[k] = s->v[k];
+    if (f) f[k] = s->f[k];
+  }
+  return GHX_OK;
+}
+
+extern "C" int ghx_hd_set_forcing(ghx_hd *s, int enabled) {
+  ARGCHK(s, "null solver");
+  s->forced = enabled != 0;
+  if (s->forced) {
+    int rc = line_flags(s->g, s->f[0], s->f[1], s->f[2], s->fline);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->g->stream));
+  }
+  return GHX_OK;
+}
+
+// main3D.fpp:3202-3231.  The state lives in v; C (step start) is v itself
+// for the first substep, so the intermediate substeps write v1 and the last
+// one writes back into v element-wise over C.
+static int hd_step_once(ghx_hd *s) {
+  const cplx *fc[3] = {s->f[0], s->f[1], s->f[2]};
+  const cplx *const *f = s->forced ? fc : nullptr;
+  const cplx *v0[3] = {s->v[0], s->v[1], s->v[2]};
+  const cplx *v1[3] = {s->v1[0], s->v1[1], s->v1[2]};
+  for (int o = s->ord; o >= 1; --o) {
+    const bool first = (o == s->ord), last = (o == 1);
+    const cplx *const *vin = first ? v0 : v1;
+    cplx *const *out = last ? s->v : s->v1;
+    int rc = hd_substep(s->g, s->q, o, s->dt, s->nu, vin, v0, f, s->fline, out);
+    if (rc) return rc;
+  }
+  return GHX_OK;
+}
+
+extern "C" int ghx_hd_step(ghx_hd *s, int nsteps) {
+  ARGCHK(s && nsteps >= 0, "bad hd_step arguments");
+  for (int n = 0; n < nsteps; ++n) {
+    int rc = hd_step_once(s);
+    if (rc) return rc;
+  }
+  return GHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MHD
+// ---------------------------------------------------------------------------
+struct ghx_mhd {
+  ghx_grid *g;
+  int ord;
+  double dt, nu, mu;
+  cplx *mem = nullptr;  // s(6) s1(6) f(3) m(3) q(21)
+  cplx *s[6], *s1[6], *f[3], *m[3], *q;
+  uint8_t *fline = nullptr;
+  bool forced = false;
+};
+
+extern "C" int ghx_mhd_create(ghx_mhd **out, ghx_grid *g, int ord, double dt, double nu, double mu) {
+  ARGCHK(out && g && ord >= 1, "bad mhd_create arguments");
+  ghx_mhd *h = new (std::nothrow) ghx_mhd();
+  if (!h) return set_err(GHX_ERR_NOMEM, "host allocation failed");
+  h->g = g; h->ord = ord; h->dt = dt; h->nu = nu; h->mu = mu;
+  const size_t nc = g->nc;
+  hipError_t e = hipMalloc(&h->mem, 39 * nc * sizeof(cplx));
+  if (e != hipSuccess) { delete h; HIPCHK(e); }
+  for (int k = 0; k < 6; ++k) { h->s[k] = h->mem + k * nc; h->s1[k] = h->mem + (6 + k) * nc; }
+  for (int k = 0; k < 3; ++k) { h->f[k] = h->mem + (12 + k) * nc; h->m[k] = h->mem + (15 + k) * nc; }
+  h->q = h->mem + 18 * nc;
+  e = hipMalloc(&h->fline, (size_t)g->gk.nxh * g->gk.ny);
+  if (e != hipSuccess) { (void)hipFree(h->mem); delete h; HIPCHK(e); }
+  e = ghx::op_zero(g->stream, h->mem, 18 * nc * sizeof(cplx));
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) { (void)hipFree(h->mem); (void)hipFree(h->fline); delete h; HIPCHK(e); }
+  *out = h;
+  return GHX_OK;
+}
+
+extern "C" int ghx_mhd_destroy(ghx_mhd *s) {
+  if (!s) return GHX_OK;
+  (void)hipFree(s->mem);
+  (void)hipFree(s->fline);
+  delete s;
+  return GHX_OK;
+}
+
+extern "C" int ghx_mhd_fields(ghx_mhd *s, void **v, void **a, void **f, void **m) {
+  ARGCHK(s, "null solver");
+  for (int k = 0; k < 3; ++k) {
+    if (v) v[k] = s->s[k];
+    if (a) a[k] = s->s[3 + k];
+    if (f) f[k] = s->f[k];
+    if (m) m[k] = s->m[k];
+  }
+  return GHX_OK;
+}
+
+__global__ void k_or_flags(uint8_t *a, const uint8_t *b, size_t n) {
+  for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (size_t)gridDim.x * blockDim.x)
+    a[l] = a[l] | b[l];
+}
+
+extern "C" int ghx_mhd_set_forcing(ghx_mhd *s, int enabled) {
+  ARGCHK(s, "null solver");
+  s->forced = enabled != 0;
+  if (s->forced) {
+    ghx_grid *g = s->g;
+    size_t nl = (size_t)g->gk.nxh * g->gk.ny;
+    int rc = grid_work(g, nl / 8 + 2);
+    if (rc) return rc;
+    uint8_t *tmp = (uint8_t *)g->work;
+    if ((rc = line_flags(g, s->f[0], s->f[1], s->f[2], s->fline))) return rc;
+    if ((rc = line_flags(g, s->m[0], s->m[1], s->m[2], tmp))) return rc;
+    hipLaunchKernelGGL(k_or_flags, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, g->stream, s->fline, tmp, nl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g->stream));
+  }
+  return GHX_OK;
+}
+
+// include/mhd_rkstep2.f90:4-65 fused: one inverse chain for (v, A) producing
+// (u, w, b, j); three cross products (w x u, j x b, u x b); one forward chain
+// ending in nonlin3 / gauge3 and the update.
+static int mhd_substep(ghx_mhd *s, int o, const cplx *const *vin, cplx *const *out) {
+  ghx_grid *g = s->g;
+  const size_t nc = g->nc;
+  const FFTCtx &ctx = g->plan.ctx;
+  FieldSet in, q12, q9, pq;
+  for (int k = 0; k < 6; ++k) in.p[k] = const_cast<cplx *>(vin[k]);
+  for (int k = 0; k < 12; ++k) q12.p[k] = s->q + k * nc;
+  for (int k = 0; k < 9; ++k) q9.p[k] = s->q + (12 + k) * nc;
+  HIPCHK(ghx::zinv(ctx, 2, 6, in, q12));  // u(0..2) w(3..5) b(6..8) j(9..11)
+  HIPCHK(ghx::ypass(ctx, +1, 12, q12));
+  FieldSet o3;
+  // w x u
+  for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[3 + k]; pq.p[3 + k] = q12.p[k]; o3.p[k] = q9.p[k]; }
+  HIPCHK(ghx::xcross(ctx, 0, pq, o3));
+  // j x b
+  for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[9 + k]; pq.p[3 + k] = q12.p[6 + k]; o3.p[k] = q9.p[3 + k]; }
+  HIPCHK(ghx::xcross(ctx, 0, pq, o3));
+  // u x b
+  for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[k]; pq.p[3 + k] = q12.p[6 + k]; o3.p[k] = q9.p[6 + k]; }
+  HIPCHK(ghx::xcross(ctx, 0, pq, o3));
+  HIPCHK(ghx::ypass(ctx, -1, 9, q9));
+  ghx::MHDRKParams rk;
+  rk.dt = s->dt; rk.nu = s->nu; rk.mu = s->mu; rk.rmp = 1. / (double)o;
+  for (int k = 0; k < 6; ++k) {
+    rk.v[k] = vin[k];
+    rk.c[k] = s->s[k];
+    rk.out[k] = out[k];
+  }
+  for (int k = 0; k < 3; ++k) {
+    rk.f[k] = s->forced ? s->f[k] : nullptr;
+    rk.f[3 + k] = s->forced ? s->m[k] : nullptr;
+  }
+  rk.fline = s->forced ? s->fline : nullptr;
+  HIPCHK(ghx::zfwd_rk_mhd(ctx, q9, rk));
+  return GHX_OK;
+}
+
+extern "C" int ghx_mhd_step(ghx_mhd *s, int nsteps) {
+  ARGCHK(s && nsteps >= 0, "bad mhd_step arguments");
+  const cplx *v0[6], *v1[6];
+  for (int k = 0; k < 6; ++k) { v0[k] = s->s[k]; v1[k] = s->s1[k]; }
+  for (int n = 0; n < nsteps; ++n) {
+    for (int o = s->ord; o >= 1; --o) {
+      const bool first = (o == s->ord), last = (o == 1);
+      int rc = mhd_substep(s, o, first ? v0 : v1, last ? s->s : s->s1);
+      if (rc) return rc;
+    }
+  }
+  return GHX_OK;
+}

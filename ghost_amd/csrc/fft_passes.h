@@ -1,0 +1,66 @@
+// fft_passes.h -- host-side interface of the FFT pass kernels.
+//
+// Layouts (complex double unless noted):
+//   R  real    r[z][y][x]          GHOST R(nx,ny,nz)
+//   S  spectral s[kx][ky][kz]      GHOST C(nz,ny,nx/2+1), kz fastest
+//   Q  intermediate q[z][kx][ky]   ky fastest; the y pass runs in place on it
+// r2c = X (R->Q, transposed store) . Y (Q in place) . Z (Q->S, transposed load)
+// c2r = Z^-1 (S->Q, transposed store) . Y^-1 (Q in place) . X^-1 (Q->R)
+#pragma once
+#include "ghx_common.h"
+
+namespace ghx {
+
+struct FFTCtx {
+  GridK gk;
+  const cplx *tw_xh;  // nx/2 table
+  const cplx *tw_x;   // nx table (real/half-complex packing)
+  const cplx *tw_y;   // ny table
+  const cplx *tw_z;   // nz table
+  hipStream_t stream;
+};
+
+constexpr int kMaxFields = 12;
+struct FieldSet {
+  cplx *p[kMaxFields];
+};
+
+// Fused-epilogue parameters for the Z forward pass of an RK substep.
+struct RKParams {
+  double dt, nu, rmp;
+  const cplx *v[3];    // state the nonlinear term was computed from (laplacian)
+  const cplx *c[3];    // state at the start of the step
+  cplx *out[3];        // updated state (may alias c or v element-wise)
+  const cplx *f[3];    // forcing (may be null)
+  const uint8_t *fline;  // per (kx,ky) line: 1 if the forcing is non-zero there
+};
+
+struct MHDRKParams {
+  double dt, nu, mu, rmp;
+  const cplx *v[6], *c[6];
+  cplx *out[6];
+  const cplx *f[6];
+  const uint8_t *fline;
+};
+
+bool fft_size_supported(int n);
+
+// Z^-1 pass, S -> Q.  mode 0: nf fields in, nf out.  mode 1 (HD): 3 in
+// (v) -> 6 out (v, curl v).  mode 2 (MHD): 6 in (v, A) -> 12 out
+// (v, curl v, b = curl A, j = curl b).
+hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out);
+// Y pass in place on Q fields (dir -1 forward, +1 inverse).
+hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q);
+// X^-1: Q -> real rows.
+hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out);
+// X forward: real rows -> Q.
+hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet &out);
+// Fused X^-1 . cross product . X: 6 Q fields (u, w) -> 3 Q fields of FFT[(w x u)/N^6].
+// mode 1 (MHD): 12 fields (u, w, b, j) -> 9 (w x u, j x b, u x b).
+hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out);
+// Z forward, Q -> S (mode 0), or fused with projection + RK update (HD/MHD).
+hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out);
+hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk);
+hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &rk);
+
+}  // namespace ghx

@@ -1,0 +1,772 @@
+// fft_passes.hip -- FFT pass kernels for GHOST's fftp layouts on gfx950.
+//
+// Every pass streams whole lines through registers (LineFFT) and uses LDS only
+// for the Stockham exchanges and for the tile transposes that connect the
+// x-contiguous real layout, the ky-contiguous intermediate Q and the
+// kz-contiguous spectral layout S.  The transposes are fused into the first
+// (load) or last (store) step of the x and z passes; global accesses are
+// either whole contiguous lines or L-point chunks along ky (L*16 bytes).
+#include "fft_engine.h"
+#include "fft_passes.h"
+
+// The launchers are split into parts (GHX_PART=1..7) so the build compiles
+// them in parallel; each part instantiates only the kernels it launches.
+#ifndef GHX_PART
+#error "compile with -DGHX_PART=1..7"
+#endif
+
+namespace ghx {
+
+// Sizes with a compiled schedule (complex line lengths).
+#define GHX_SIZES(X) \
+  X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
+
+#if GHX_PART == 1
+bool fft_size_supported(int n) {
+  switch (n) {
+#define GHX_CASE(n_) case n_:
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+    return true;
+  default:
+    return false;
+  }
+}
+
+#endif
+
+static size_t g_max_lds = 0;
+static size_t max_lds() {
+  if (!g_max_lds) {
+    int dev = 0, v = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      v = 65536;
+    g_max_lds = (size_t)v;
+  }
+  return g_max_lds;
+}
+
+// lines per block for a transposed pass: chunk L along ky, bounded by LDS and 1024 threads
+static int pick_L(int T, int rowslots, int ny, int want) {
+  int L = want;
+  while (L > 1 && ((size_t)L * rowslots * sizeof(cplx) > max_lds() || L * T > 512)) L >>= 1;
+  if (L > ny) {
+    L = 1;
+    while (L * 2 <= ny) L *= 2;
+  }
+  return L < 1 ? 1 : L;
+}
+
+template <typename K> static void allow_lds(K kern, size_t bytes) {
+  if (bytes > 49152) hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// ---------------------------------------------------------------------------
+// Tile helpers (block = L lines x T lanes; line lt = threadIdx.x / T)
+// ---------------------------------------------------------------------------
+
+// Store this block's L lines (each NP points, kept in LDS rows) to Q with the
+// line index (ky) fastest: dst[(p*nxh + i)*ny + j0 + l].
+GHX_DEV void tile_store_q(cplx *lds, int stride, int L, int NP, cplx *dst, size_t z_or_p_stride,
+                          size_t base, int j0, int ny) {
+  for (int idx = threadIdx.x; idx < L * NP; idx += blockDim.x) {
+    int l = idx % L, p = idx / L;
+    if (j0 + l < ny) dst[base + (size_t)p * z_or_p_stride + j0 + l] = lds[l * stride + lds_pad(p)];
+  }
+}
+GHX_DEV void tile_load_q(cplx *lds, int stride, int L, int NP, const cplx *src, size_t pstride,
+                         size_t base, int j0, int ny) {
+  for (int idx = threadIdx.x; idx < L * NP; idx += blockDim.x) {
+    int l = idx % L, p = idx / L;
+    cplx v = c_make(0.0, 0.0);
+    if (j0 + l < ny) v = src[base + (size_t)p * pstride + j0 + l];
+    lds[l * stride + lds_pad(p)] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Y pass: contiguous lines q[z][kx][:] in place.
+// ---------------------------------------------------------------------------
+template <int N, int DIR>
+__global__ __launch_bounds__(256) void k_ypass(FieldSet q, long nlines, const cplx *__restrict__ tw) {
+  using F = LineFFT<N, DIR>;
+  constexpr int T = F::T, E = F::E;
+  extern __shared__ cplx lds[];
+  const int lpb = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const long line = (long)blockIdx.x * lpb + lt;
+  const bool ok = line < nlines;
+  cplx *row = lds + lt * lds_line_stride(N);
+  cplx *base = q.p[blockIdx.y] + (ok ? line : 0) * N;
+  cplx v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = ok ? base[t + T * m] : c_make(0.0, 0.0);
+  F::run(v, row, t, tw);
+  if (ok) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) base[t + T * m] = v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Z^-1: S -> Q.  Block = (kx index i, ky chunk j0..j0+L-1).
+// ---------------------------------------------------------------------------
+template <int N, int E>
+GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *tw,
+                       cplx *dst, const GridK &g, int i, int j0) {
+  using F = LineFFT<N, +1>;
+  constexpr int T = F::T;
+  F::run(v, row, t, tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
+  __syncthreads();
+  tile_store_q(lds, stride, L, N, dst, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny);
+  __syncthreads();
+}
+
+template <int N, int MODE>
+__global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ tw) {
+  using F = LineFFT<N, +1>;
+  constexpr int T = F::T, E = F::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  const bool ok = j < g.ny;
+  const int stride = lds_line_stride(N);
+  cplx *row = lds + lt * stride;
+  const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
+  if constexpr (MODE == 0) {
+    for (int f = 0; f < nf; ++f) {
+      cplx v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = ok ? in.p[f][sline + t + T * m] : c_make(0.0, 0.0);
+      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0);
+    }
+  } else {
+    // MODE 1: v -> (v, curl v).  MODE 2: (v, A) -> (v, curl v, b = curl A, curl b).
+    const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
+    cplx u[3][E];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int m = 0; m < E; ++m) u[c][m] = ok ? in.p[c][sline + t + T * m] : c_make(0.0, 0.0);
+    // rotor3 (pseudospec3D_hd.f90:124-200) on (a0,a1,a2) -> component c
+    auto curl = [&](const cplx (&a)[3][E], int c, cplx (&o)[E]) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double kz = wavenum(t + T * m, g.nz, g.dkz);
+        if (c == 0) o[m] = c_sub(c_ik(ky, a[2][m]), c_ik(kz, a[1][m]));
+        else if (c == 1) o[m] = c_sub(c_ik(kz, a[0][m]), c_ik(kx, a[2][m]));
+        else o[m] = c_sub(c_ik(kx, a[1][m]), c_ik(ky, a[0][m]));
+      }
+    };
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cplx v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = u[c][m];
+      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[c], g, i, j0);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cplx v[E];
+      curl(u, c, v);
+      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[3 + c], g, i, j0);
+    }
+    if constexpr (MODE == 2) {
+      cplx a[3][E], b[3][E];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int m = 0; m < E; ++m) a[c][m] = ok ? in.p[3 + c][sline + t + T * m] : c_make(0.0, 0.0);
+      curl(a, 0, b[0]);
+      curl(a, 1, b[1]);
+      curl(a, 2, b[2]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        cplx v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = b[c][m];
+        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[6 + c], g, i, j0);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        cplx v[E];
+        curl(b, c, v);
+        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[9 + c], g, i, j0);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Z forward: Q -> S.  Block = (kx index i, ky chunk).  Transposed load.
+// ---------------------------------------------------------------------------
+template <int N, int E>
+GHX_DEV void zfwd_load(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *tw,
+                       const cplx *src, const GridK &g, int i, int j0) {
+  using F = LineFFT<N, -1>;
+  constexpr int T = F::T;
+  tile_load_q(lds, stride, L, N, src, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = row[lds_pad(t + T * m)];
+  __syncthreads();
+  F::run(v, row, t, tw);
+}
+
+template <int N>
+__global__ __launch_bounds__(512) void k_zfwd(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ tw) {
+  using F = LineFFT<N, -1>;
+  constexpr int T = F::T, E = F::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  const bool ok = j < g.ny;
+  const int stride = lds_line_stride(N);
+  cplx *row = lds + lt * stride;
+  const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
+  for (int f = 0; f < nf; ++f) {
+    cplx v[E];
+    zfwd_load<N, E>(v, lds, row, stride, L, t, tw, in.p[f], g, i, j0);
+    if (ok) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) out.p[f][sline + t + T * m] = v[m];
+    }
+    __syncthreads();
+  }
+}
+
+// Projection of pseudospec3D_hd.f90:399-509 (nonlhd3) for one mode.
+GHX_DEV void project_hd(const cplx &a, const cplx &b, const cplx &c, double kx, double ky, double kz,
+                        double kk2, int i, int j, int k, cplx &gx, cplx &gy, cplx &gz) {
+  cplx dot = c_add(c_add(c_scale(a, kx), c_scale(b, ky)), c_scale(c, kz));
+  gx = i == 0 ? c_make(-a.x, -a.y) : c_add(c_make(-a.x, -a.y), c_make(kx * dot.x / kk2, kx * dot.y / kk2));
+  gy = j == 0 ? c_make(-b.x, -b.y) : c_add(c_make(-b.x, -b.y), c_make(ky * dot.x / kk2, ky * dot.y / kk2));
+  gz = k == 0 ? c_make(-c.x, -c.y) : c_add(c_make(-c.x, -c.y), c_make(kz * dot.x / kk2, kz * dot.y / kk2));
+}
+
+// RK update of include/hd_rkstep2.f90:12-32 for one component:
+// v = C + dt*(nu*(-kk2 v) + N + f)*rmp
+GHX_DEV cplx rk_update(cplx cc, cplx vv, cplx nl, cplx ff, double kk2, double nu, double dt, double rmp) {
+  double lx = -kk2 * vv.x, ly = -kk2 * vv.y;
+  double sx = nu * lx + nl.x + ff.x, sy = nu * ly + nl.y + ff.y;
+  return c_make(cc.x + dt * sx * rmp, cc.y + dt * sy * rmp);
+}
+
+template <int N>
+__global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const cplx *__restrict__ tw, RKParams rk) {
+  using F = LineFFT<N, -1>;
+  constexpr int T = F::T, E = F::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  const bool ok = j < g.ny;
+  const int stride = lds_line_stride(N);
+  cplx *row = lds + lt * stride;
+  cplx P[3][E];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0);
+    __syncthreads();
+  }
+  if (!ok) return;
+  const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
+  const size_t sline = ((size_t)i * g.ny + j) * N;
+  const bool forced = rk.fline && rk.fline[(size_t)i * g.ny + j];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    const size_t s = sline + k;
+    const double kz = wavenum(k, g.nz, g.dkz);
+    const double kn2 = kn2_of(g, i, j, k);
+    cplx o0 = c_make(0.0, 0.0), o1 = o0, o2 = o0;
+    if (kn2 <= g.kmax && kn2 >= g.tiny) {
+      const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
+      cplx n0, n1, n2;
+      project_hd(P[0][m], P[1][m], P[2][m], kx, ky, kz, kk2, i, j, k, n0, n1, n2);
+      const cplx z = c_make(0.0, 0.0);
+      o0 = rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+      o1 = rk_update(rk.c[1][s], rk.v[1][s], n1, forced ? rk.f[1][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+      o2 = rk_update(rk.c[2][s], rk.v[2][s], n2, forced ? rk.f[2][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+    }
+    rk.out[0][s] = o0;
+    rk.out[1][s] = o1;
+    rk.out[2][s] = o2;
+  }
+}
+
+// MHD: nonlin3 (pseudospec3D_mhd.f90:108-227) for the velocity from
+// (w x u, j x b) and gauge3 (:230-336) for the potential from u x b.
+template <int N>
+__global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const cplx *__restrict__ tw, MHDRKParams rk) {
+  using F = LineFFT<N, -1>;
+  constexpr int T = F::T, E = F::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  const bool ok = j < g.ny;
+  const int stride = lds_line_stride(N);
+  cplx *row = lds + lt * stride;
+  const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
+  const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
+  const bool forced = ok && rk.fline && rk.fline[(size_t)i * g.ny + j];
+  const cplx z0 = c_make(0.0, 0.0);
+  // velocity: fields 0..2 = w x u, 3..5 = j x b
+  {
+    cplx P[6][E];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0);
+      __syncthreads();
+    }
+    if (ok) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + T * m;
+        const size_t s = sline + k;
+        const double kz = wavenum(k, g.nz, g.dkz);
+        const double kn2 = kn2_of(g, i, j, k);
+        cplx o[3] = {z0, z0, z0};
+        if (kn2 <= g.kmax && kn2 >= g.tiny) {
+          const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
+          cplx d0 = c_sub(P[0][m], P[3][m]), d1 = c_sub(P[1][m], P[4][m]), d2 = c_sub(P[2][m], P[5][m]);
+          cplx dot = c_add(c_add(c_scale(d0, kx), c_scale(d1, ky)), c_scale(d2, kz));
+          const double kd[3] = {kx, ky, kz};
+          const int pl[3] = {i, j, k};
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            cplx base = c_add(c_make(-P[c][m].x, -P[c][m].y), P[3 + c][m]);
+            cplx nl = pl[c] == 0 ? base : c_add(base, c_make(kd[c] * dot.x / kk2, kd[c] * dot.y / kk2));
+            o[c] = rk_update(rk.c[c][s], rk.v[c][s], nl, forced && rk.f[c] ? rk.f[c][s] : z0, kk2, rk.nu,
+                             rk.dt, rk.rmp);
+          }
+        }
+        rk.out[0][s] = o[0];
+        rk.out[1][s] = o[1];
+        rk.out[2][s] = o[2];
+      }
+    }
+  }
+  // vector potential: fields 6..8 = u x b
+  {
+    cplx P[3][E];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[6 + f], g, i, j0);
+      __syncthreads();
+    }
+    if (ok) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + T * m;
+        const size_t s = sline + k;
+        const double kz = wavenum(k, g.nz, g.dkz);
+        const double kn2 = kn2_of(g, i, j, k);
+        cplx o[3] = {z0, z0, z0};
+        if (kn2 <= g.kmax && kn2 >= g.tiny) {
+          const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
+          cplx dot = c_add(c_add(c_scale(P[0][m], kx), c_scale(P[1][m], ky)), c_scale(P[2][m], kz));
+          const double kd[3] = {kx, ky, kz};
+          const int pl[3] = {i, j, k};
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            cplx nl = pl[c] == 0 ? P[c][m] : c_sub(P[c][m], c_make(kd[c] * dot.x / kk2, kd[c] * dot.y / kk2));
+            o[c] = rk_update(rk.c[3 + c][s], rk.v[3 + c][s], nl,
+                             forced && rk.f[3 + c] ? rk.f[3 + c][s] : z0, kk2, rk.mu, rk.dt, rk.rmp);
+          }
+        }
+        rk.out[3][s] = o[0];
+        rk.out[4][s] = o[1];
+        rk.out[5][s] = o[2];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// X passes.  Block = (z, ky chunk j0..j0+L-1); a real row of nx points is
+// held as N = nx/2 packed complex points.
+// ---------------------------------------------------------------------------
+
+// Q rows -> packed half-length spectrum in registers, then inverse FFT:
+// v = real row (x[2n], x[2n+1]) * nx, unnormalised c2r.
+template <int N, int E>
+GHX_DEV void xinv_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *twh,
+                      const cplx *twx, const cplx *src, const GridK &g, int z, int j0) {
+  using F = LineFFT<N, +1>;
+  constexpr int T = F::T;
+  tile_load_q(lds, stride, L, N + 1, src, g.ny, (size_t)z * g.nxh * g.ny, j0, g.ny);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    cplx A = row[lds_pad(k)];
+    cplx Bs = row[lds_pad(N - k)];
+    if (k == 0) { A.y = 0.0; Bs.y = 0.0; }
+    cplx B = c_conj(Bs);
+    cplx Xe = c_add(A, B);
+    cplx Xo = c_mul(c_sub(A, B), c_conj(twx[k]));
+    v[m] = c_add(Xe, c_make(-Xo.y, Xo.x));
+  }
+  __syncthreads();
+  F::run(v, row, t, twh);
+}
+
+// forward FFT of a packed real row, unpack to nx/2+1 points, transposed store to Q.
+template <int N, int E>
+GHX_DEV void xfwd_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *twh,
+                      const cplx *twx, cplx *dst, const GridK &g, int z, int j0) {
+  using F = LineFFT<N, -1>;
+  constexpr int T = F::T;
+  F::run(v, row, t, twh);
+#pragma unroll
+  for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
+  __syncthreads();
+  cplx X[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    cplx A = v[m];
+    cplx B = c_conj(row[lds_pad((N - k) % N)]);
+    cplx Xe = c_scale(c_add(A, B), 0.5);
+    cplx d = c_sub(A, B);
+    cplx Xo = c_make(0.5 * d.y, -0.5 * d.x);  // -0.5 i (A - B)
+    X[m] = c_add(Xe, c_mul(twx[k], Xo));
+  }
+  cplx nyq = row[lds_pad(0)];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = X[m];
+  if (t == 0) row[lds_pad(N)] = c_make(nyq.x - nyq.y, 0.0);
+  __syncthreads();
+  tile_store_q(lds, stride, L, N + 1, dst, g.ny, (size_t)z * g.nxh * g.ny, j0, g.ny);
+  __syncthreads();
+}
+
+struct RealSet {
+  double *p[kMaxFields];
+};
+
+template <int N>
+__global__ __launch_bounds__(512) void k_xinv(FieldSet in, RealSet out, int nf, GridK g, const cplx *__restrict__ twh,
+                       const cplx *__restrict__ twx) {
+  constexpr int T = FFTCfg<N>::T, E = FFTCfg<N>::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const int z = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  const bool ok = j < g.ny;
+  const int stride = lds_line_stride(N + 1);
+  cplx *row = lds + lt * stride;
+  for (int f = 0; f < nf; ++f) {
+    cplx v[E];
+    xinv_row<N, E>(v, lds, row, stride, L, t, twh, twx, in.p[f], g, z, j0);
+    if (ok) {
+      cplx *dst = reinterpret_cast<cplx *>(out.p[f] + ((size_t)z * g.ny + j) * g.nx);
+#pragma unroll
+      for (int m = 0; m < E; ++m) dst[t + T * m] = v[m];
+    }
+    __syncthreads();
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(512) void k_xfwd(RealSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twh,
+                       const cplx *__restrict__ twx) {
+  constexpr int T = FFTCfg<N>::T, E = FFTCfg<N>::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const int z = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  const bool ok = j < g.ny;
+  const int stride = lds_line_stride(N + 1);
+  cplx *row = lds + lt * stride;
+  for (int f = 0; f < nf; ++f) {
+    cplx v[E];
+    const cplx *src = reinterpret_cast<const cplx *>(in.p[f] + ((size_t)z * g.ny + (ok ? j : 0)) * g.nx);
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = ok ? src[t + T * m] : c_make(0.0, 0.0);
+    xfwd_row<N, E>(v, lds, row, stride, L, t, twh, twx, out.p[f], g, z, j0);
+  }
+}
+
+// Fused X^-1 . (P x Q)/N^6 . X with GHOST's operand order
+// (pseudospec3D_hd.f90:377-389): out_x = (P_y Q_z - Q_y P_z) tmp,
+// out_y = (P_z Q_x - Q_z P_x) tmp, out_z = (P_x Q_y - Q_x P_y) tmp.
+// Fields: in.p[0..2] = P, in.p[3..5] = Q; out.p[0..2].
+template <int N>
+__global__ __launch_bounds__(512) void k_xcross(FieldSet in, FieldSet out, GridK g, const cplx *__restrict__ twh,
+                         const cplx *__restrict__ twx) {
+  constexpr int T = FFTCfg<N>::T, E = FFTCfg<N>::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const int z = blockIdx.y, j0 = blockIdx.x * L;
+  const int stride = lds_line_stride(N + 1);
+  cplx *row = lds + lt * stride;
+  const double tmp = g.inv_n6;
+  cplx qx[E], qy[E], qz[E], ay[E], az[E];
+  xinv_row<N, E>(qx, lds, row, stride, L, t, twh, twx, in.p[3], g, z, j0);
+  xinv_row<N, E>(qy, lds, row, stride, L, t, twh, twx, in.p[4], g, z, j0);
+  xinv_row<N, E>(qz, lds, row, stride, L, t, twh, twx, in.p[5], g, z, j0);
+  {
+    cplx px[E];
+    xinv_row<N, E>(px, lds, row, stride, L, t, twh, twx, in.p[0], g, z, j0);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      ay[m] = c_make(qz[m].x * px[m].x, qz[m].y * px[m].y);  // Q_z P_x
+      az[m] = c_make(px[m].x * qy[m].x, px[m].y * qy[m].y);  // P_x Q_y
+    }
+  }
+  {
+    cplx py[E];
+    xinv_row<N, E>(py, lds, row, stride, L, t, twh, twx, in.p[1], g, z, j0);
+    cplx oz[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      oz[m] = c_make((az[m].x - qx[m].x * py[m].x) * tmp, (az[m].y - qx[m].y * py[m].y) * tmp);
+      az[m] = c_make(py[m].x * qz[m].x, py[m].y * qz[m].y);  // now P_y Q_z
+    }
+    xfwd_row<N, E>(oz, lds, row, stride, L, t, twh, twx, out.p[2], g, z, j0);
+  }
+  {
+    cplx pz[E];
+    xinv_row<N, E>(pz, lds, row, stride, L, t, twh, twx, in.p[2], g, z, j0);
+    cplx ox[E], oy[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      ox[m] = c_make((az[m].x - qy[m].x * pz[m].x) * tmp, (az[m].y - qy[m].y * pz[m].y) * tmp);
+      oy[m] = c_make((pz[m].x * qx[m].x - ay[m].x) * tmp, (pz[m].y * qx[m].y - ay[m].y) * tmp);
+    }
+    xfwd_row<N, E>(ox, lds, row, stride, L, t, twh, twx, out.p[0], g, z, j0);
+    xfwd_row<N, E>(oy, lds, row, stride, L, t, twh, twx, out.p[1], g, z, j0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static int ydesired_lpb(int T) {
+  int lpb = 256 / T;
+  return lpb < 1 ? 1 : lpb;
+}
+
+#if GHX_PART == 1
+hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q) {
+  const GridK &g = c.gk;
+  const long nlines = (long)g.nz * g.nxh;
+  switch (g.ny) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int lpb = ydesired_lpb(T);                                                              \
+    size_t lds = (size_t)lpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    dim3 grid((unsigned)((nlines + lpb - 1) / lpb), nf);                                    \
+    if (dir < 0) {                                                                          \
+      allow_lds(k_ypass<n_, -1>, lds);                                                      \
+      hipLaunchKernelGGL((k_ypass<n_, -1>), grid, dim3(lpb * T), lds, c.stream, q, nlines, c.tw_y); \
+    } else {                                                                                \
+      allow_lds(k_ypass<n_, 1>, lds);                                                       \
+      hipLaunchKernelGGL((k_ypass<n_, 1>), grid, dim3(lpb * T), lds, c.stream, q, nlines, c.tw_y); \
+    }                                                                                       \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#endif
+
+#if GHX_PART == 2
+hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out) {
+  const GridK &g = c.gk;
+  switch (g.nz) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, 8);                                        \
+    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
+    if (mode == 0) {                                                                        \
+      allow_lds(k_zinv<n_, 0>, lds);                                                        \
+      hipLaunchKernelGGL((k_zinv<n_, 0>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
+    } else if (mode == 1) {                                                                 \
+      allow_lds(k_zinv<n_, 1>, lds);                                                        \
+      hipLaunchKernelGGL((k_zinv<n_, 1>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
+    } else {                                                                                \
+      allow_lds(k_zinv<n_, 2>, lds);                                                        \
+      hipLaunchKernelGGL((k_zinv<n_, 2>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
+    }                                                                                       \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#endif
+
+#if GHX_PART == 3
+hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out) {
+  const GridK &g = c.gk;
+  switch (g.nz) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, 8);                                        \
+    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
+    allow_lds(k_zfwd<n_>, lds);                                                             \
+    hipLaunchKernelGGL((k_zfwd<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#endif
+
+#if GHX_PART == 4
+hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk) {
+  const GridK &g = c.gk;
+  switch (g.nz) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, 8);                                        \
+    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
+    allow_lds(k_zfwd_rk_hd<n_>, lds);                                                       \
+    hipLaunchKernelGGL((k_zfwd_rk_hd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#endif
+
+#if GHX_PART == 5
+hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &rk) {
+  const GridK &g = c.gk;
+  switch (g.nz) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, 8);                                        \
+    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
+    allow_lds(k_zfwd_rk_mhd<n_>, lds);                                                      \
+    hipLaunchKernelGGL((k_zfwd_rk_mhd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#endif
+
+#if GHX_PART == 6
+hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out) {
+  const GridK &g = c.gk;
+  if (g.nx % 2) return hipErrorInvalidValue;
+  RealSet rs;
+  for (int f = 0; f < nf; ++f) rs.p[f] = out[f];
+  switch (g.nx / 2) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, 16);                                   \
+    size_t lds = (size_t)L * lds_line_stride(n_ + 1) * sizeof(cplx);                        \
+    dim3 grid((g.ny + L - 1) / L, g.nz);                                                    \
+    allow_lds(k_xinv<n_>, lds);                                                             \
+    hipLaunchKernelGGL((k_xinv<n_>), grid, dim3(L * T), lds, c.stream, in, rs, nf, g, c.tw_xh, c.tw_x); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet &out) {
+  const GridK &g = c.gk;
+  if (g.nx % 2) return hipErrorInvalidValue;
+  RealSet rs;
+  for (int f = 0; f < nf; ++f) rs.p[f] = const_cast<double *>(in[f]);
+  switch (g.nx / 2) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, 16);                                   \
+    size_t lds = (size_t)L * lds_line_stride(n_ + 1) * sizeof(cplx);                        \
+    dim3 grid((g.ny + L - 1) / L, g.nz);                                                    \
+    allow_lds(k_xfwd<n_>, lds);                                                             \
+    hipLaunchKernelGGL((k_xfwd<n_>), grid, dim3(L * T), lds, c.stream, rs, out, nf, g, c.tw_xh, c.tw_x); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#endif
+
+#if GHX_PART == 7
+hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out) {
+  (void)mode;
+  const GridK &g = c.gk;
+  if (g.nx % 2) return hipErrorInvalidValue;
+  switch (g.nx / 2) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, 16);                                   \
+    size_t lds = (size_t)L * lds_line_stride(n_ + 1) * sizeof(cplx);                        \
+    dim3 grid((g.ny + L - 1) / L, g.nz);                                                    \
+    allow_lds(k_xcross<n_>, lds);                                                           \
+    hipLaunchKernelGGL((k_xcross<n_>), grid, dim3(L * T), lds, c.stream, in, out, g, c.tw_xh, c.tw_x); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#endif
+
+}  // namespace ghx

@@ -1,0 +1,47 @@
+// pseudo.h -- pointwise spectral operators and reductions (host interface).
+#pragma once
+#include "ghx_common.h"
+
+namespace ghx {
+
+// Elementwise operators on spectral fields s[kx][ky][kz] (GHOST C(nz,ny,nx/2+1)).
+hipError_t op_derivk3(const GridK &g, hipStream_t st, const cplx *a, cplx *b, int dir);
+hipError_t op_laplak3(const GridK &g, hipStream_t st, const cplx *a, cplx *b);
+hipError_t op_rotor3(const GridK &g, hipStream_t st, const cplx *a, const cplx *b, cplx *c, int dir);
+hipError_t op_nonlhd3(const GridK &g, hipStream_t st, const cplx *a, const cplx *b, const cplx *c, cplx *o, int dir);
+hipError_t op_nonlin3(const GridK &g, hipStream_t st, const cplx *a, const cplx *b, const cplx *c,
+                      const cplx *d, const cplx *e, const cplx *f, cplx *o, int dir);
+hipError_t op_gauge3(const GridK &g, hipStream_t st, const cplx *a, const cplx *b, const cplx *c, cplx *o, int dir);
+hipError_t op_phaseshift(const GridK &g, hipStream_t st, cplx *a, double cre, double cim);
+hipError_t op_scale(const GridK &g, hipStream_t st, cplx *a, const cplx *src, double s);  // a = src*s
+// a = a*f0/sqrt(e) (module_dns.f90 normalize) for three fields
+hipError_t op_normalize_apply(const GridK &g, hipStream_t st, cplx *x, cplx *y, cplx *z, double f0, double e);
+hipError_t op_sharp(const GridK &g, hipStream_t st, const cplx *a, cplx *b, double kc2);
+hipError_t op_zero(hipStream_t st, void *p, size_t bytes);
+// real-space cross product r7=(r2 r6-r5 r3)tmp ... in place (prodre3's loop)
+hipError_t op_cross_real(const GridK &g, hipStream_t st, double *r1, double *r2, double *r3,
+                         const double *r4, const double *r5, const double *r6, double *r7);
+// RK update of include/hd_rkstep2.f90 for one component over all modes:
+// v = C + dt*(nu*lap + N + f)*rmp where kept, else 0; lap given (already -kk2 v).
+hipError_t op_rk_update(const GridK &g, hipStream_t st, cplx *v, const cplx *c, const cplx *lap,
+                        const cplx *nl, const cplx *f, double nu, double dt, double rmp);
+// TG velocity in real space (examples/initialv.f90_tg:37-56)
+hipError_t op_tg_real(const GridK &g, hipStream_t st, double *r1, double *r2, int kdn, int kup);
+
+// Reductions.  kind: 0 = |a|^2+|b|^2+|c|^2, 1 = Re(a d* + b e* + c f*),
+// 2 = that * kk2, 3 = |i kx a + i ky b + i kz c|^2 (divergence).
+// Each mode is weighted 1 on the kx=0 plane, 2 elsewhere, times 1/N^6.
+// `work` must hold reduce_work_doubles() doubles.  Result to *out (host).
+size_t reduce_work_doubles();
+hipError_t op_reduce(const GridK &g, hipStream_t st, int kind, const cplx *a, const cplx *b, const cplx *c,
+                     const cplx *d, const cplx *e, const cplx *f, double *work, double *out);
+// spectrumc binning (pseudospec3D_hd.f90:1190-1415); kind 0: energy of (a,b,c),
+// kind 1: helicity Re(a d* + b e* + c f*).  nb bins; out[nb] host.
+size_t spectrum_work_doubles(const GridK &g, int nb);
+hipError_t op_spectrum(const GridK &g, hipStream_t st, int kind, double dkk, int nb, const cplx *a, const cplx *b,
+                       const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *work, double *out);
+// max over real points of sqrt(r1^2+r2^2+r3^2) (maxabs)
+hipError_t op_maxnorm(const GridK &g, hipStream_t st, const double *r1, const double *r2, const double *r3,
+                      double *work, double *out);
+
+}  // namespace ghx

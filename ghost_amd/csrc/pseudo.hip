@@ -1,0 +1,439 @@
+// pseudo.hip -- pointwise spectral operators, RK update, diagnostics and
+// reductions on device-resident GHOST fields (gfx950, FP64).
+//
+// Spectral index m = (i*ny + j)*nz + k for kx index i, ky index j, kz index k
+// (GHOST's C(nz,ny,nx/2+1) in column-major order).
+#include "pseudo.h"
+
+namespace ghx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Idx {
+  int i, j, k;
+};
+GHX_DEV Idx decompose(size_t m, const GridK &g) {
+  Idx r;
+  r.k = (int)(m % (size_t)g.nz);
+  size_t line = m / (size_t)g.nz;
+  r.j = (int)(line % (size_t)g.ny);
+  r.i = (int)(line / (size_t)g.ny);
+  return r;
+}
+
+__host__ __device__ inline size_t ncplx(const GridK &g) { return (size_t)g.nxh * g.ny * g.nz; }
+__host__ __device__ inline size_t nreal(const GridK &g) { return (size_t)g.nx * g.ny * g.nz; }
+inline unsigned nblocks(size_t n) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  if (b > 65536u * 4u) b = 65536u * 4u;
+  return (unsigned)(b ? b : 1);
+}
+
+#define GRID_STRIDE(m, n) for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < (n); m += (size_t)gridDim.x * blockDim.x)
+
+// derivk3 (pseudospec3D_hd.f90:22-88): b = i k_dir a
+__global__ void k_derivk3(GridK g, const cplx *a, cplx *b, int dir) {
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    double kk = dir == 1 ? wavenum(q.i, g.nx, g.dkx) : dir == 2 ? wavenum(q.j, g.ny, g.dky) : wavenum(q.k, g.nz, g.dkz);
+    b[m] = c_ik(kk, a[m]);
+  }
+}
+
+GHX_DEV double kk2_at(const GridK &g, const Idx &q) {
+  if (!g.anis) return kn2_of(g, q.i, q.j, q.k);
+  return kk2_of(g, wavenum(q.i, g.nx, g.dkx), wavenum(q.j, g.ny, g.dky), wavenum(q.k, g.nz, g.dkz));
+}
+
+// laplak3 (:91-121): b = -kk2 a
+__global__ void k_laplak3(GridK g, const cplx *a, cplx *b) {
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    double kk2 = kk2_at(g, q);
+    cplx v = a[m];
+    b[m] = c_make(-kk2 * v.x, -kk2 * v.y);
+  }
+}
+
+// rotor3 (:124-200)
+__global__ void k_rotor3(GridK g, const cplx *a, const cplx *b, cplx *c, int dir) {
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky), kz = wavenum(q.k, g.nz, g.dkz);
+    cplx av = a[m], bv = b[m];
+    if (dir == 1) c[m] = c_sub(c_ik(ky, bv), c_ik(kz, av));
+    else if (dir == 2) c[m] = c_sub(c_ik(kz, av), c_ik(kx, bv));
+    else c[m] = c_sub(c_ik(kx, bv), c_ik(ky, av));
+  }
+}
+
+// nonlhd3 (:399-509), nonlin3 (mhd :108-227), gauge3 (mhd :230-336)
+// mode 0: g = -a_dir + k_dir (k.a)/k^2;  mode 1: g = -a_dir + d_dir + k_dir (k.(a-d))/k^2;
+// mode 2: g = a_dir - k_dir (k.a)/k^2.  The k_dir = 0 plane keeps only the first terms.
+__global__ void k_project(GridK g, int mode, const cplx *a, const cplx *b, const cplx *c, const cplx *d,
+                          const cplx *e, const cplx *f, cplx *o, int dir) {
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky), kz = wavenum(q.k, g.nz, g.dkz);
+    cplx A = a[m], B = b[m], Cc = c[m];
+    cplx s = dir == 1 ? A : dir == 2 ? B : Cc;
+    int plane = dir == 1 ? q.i : dir == 2 ? q.j : q.k;
+    double kd = dir == 1 ? kx : dir == 2 ? ky : kz;
+    cplx r;
+    if (mode == 1) {
+      cplx D = d[m], Ee = e[m], F = f[m];
+      cplx sd = dir == 1 ? D : dir == 2 ? Ee : F;
+      r = c_add(c_make(-s.x, -s.y), sd);
+      if (plane != 0) {
+        double kk2 = kk2_at(g, q);
+        cplx dot = c_add(c_add(c_scale(c_sub(A, D), kx), c_scale(c_sub(B, Ee), ky)), c_scale(c_sub(Cc, F), kz));
+        r = c_add(r, c_make(kd * dot.x / kk2, kd * dot.y / kk2));
+      }
+    } else {
+      r = mode == 0 ? c_make(-s.x, -s.y) : s;
+      if (plane != 0) {
+        double kk2 = kk2_at(g, q);
+        cplx dot = c_add(c_add(c_scale(A, kx), c_scale(B, ky)), c_scale(Cc, kz));
+        cplx t = c_make(kd * dot.x / kk2, kd * dot.y / kk2);
+        r = mode == 0 ? c_add(r, t) : c_sub(r, t);
+      }
+    }
+    o[m] = r;
+  }
+}
+
+// phaseshift (:2084-2150).  On the kx=0 plane GHOST's loops multiply some
+// entries by cdump and some by its conjugate; the kz and ky Nyquist entries
+// get both, in the loop order reproduced here.
+__global__ void k_phaseshift(GridK g, cplx *a, double cre, double cim) {
+  const cplx cd = c_make(cre, cim), jd = c_make(cre, -cim);
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    cplx v = a[m];
+    if (q.i > 0) {
+      v = c_mul(v, cd);
+    } else {
+      const int ny = g.ny, nz = g.nz, j = q.j, k = q.k;
+      if (k == 0) {
+        if (j >= 1 && j <= ny / 2) v = c_mul(v, cd);
+        if (j >= ny / 2 && j <= ny - 1 && j >= 1) v = c_mul(v, jd);
+      } else if (j == 0) {
+        if (k <= nz / 2) v = c_mul(v, cd);
+        if (k >= nz / 2) v = c_mul(v, jd);
+      } else {
+        // loop over j (outer, 1..ny-1) then k (inner, 1..nz/2): entry (k,j) with
+        // k <= nz/2 gets cd at iteration (j,k); entry (k,j) with k >= nz/2 gets
+        // jd at iteration (ny-j, nz-k).
+        const bool gets_cd = k <= nz / 2;
+        const bool gets_jd = k >= nz / 2;
+        if (gets_cd && gets_jd) {
+          if (j <= ny - j) { v = c_mul(v, cd); v = c_mul(v, jd); }
+          else { v = c_mul(v, jd); v = c_mul(v, cd); }
+        } else if (gets_cd) {
+          v = c_mul(v, cd);
+        } else {
+          v = c_mul(v, jd);
+        }
+      }
+    }
+    a[m] = v;
+  }
+}
+
+__global__ void k_scale(size_t n, cplx *a, const cplx *src, double s) {
+  GRID_STRIDE(m, n) {
+    cplx v = src[m];
+    a[m] = c_make(v.x * s, v.y * s);
+  }
+}
+
+// module_dns.f90:33-38: f = f*f0/sqrt(tmp)
+__global__ void k_normalize(size_t n, cplx *x, cplx *y, cplx *z, double f0, double sq) {
+  GRID_STRIDE(m, n) {
+    cplx a = x[m], b = y[m], c = z[m];
+    x[m] = c_make(a.x * f0 / sq, a.y * f0 / sq);
+    y[m] = c_make(b.x * f0 / sq, b.y * f0 / sq);
+    z[m] = c_make(c.x * f0 / sq, c.y * f0 / sq);
+  }
+}
+
+// sharp (pseudospec3D_filt.f90:200-241): keep kn2 <= kc2
+__global__ void k_sharp(GridK g, const cplx *a, cplx *b, double kc2) {
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    b[m] = kn2_of(g, q.i, q.j, q.k) <= kc2 ? a[m] : c_make(0.0, 0.0);
+  }
+}
+
+__global__ void k_zero(size_t n, double *p) {
+  GRID_STRIDE(m, n) p[m] = 0.0;
+}
+
+__global__ void k_cross_real(size_t n, double tmp, double *r1, double *r2, double *r3, const double *r4,
+                             const double *r5, const double *r6, double *r7) {
+  GRID_STRIDE(m, n) {
+    double a = (r2[m] * r6[m] - r5[m] * r3[m]) * tmp;
+    double b = (r3[m] * r4[m] - r6[m] * r1[m]) * tmp;
+    double c = (r1[m] * r5[m] - r4[m] * r2[m]) * tmp;
+    r7[m] = a;
+    r3[m] = b;
+    r1[m] = c;
+  }
+}
+
+__global__ void k_rk_update(GridK g, cplx *v, const cplx *c, const cplx *lap, const cplx *nl, const cplx *f,
+                            double nu, double dt, double rmp) {
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    double kn2 = kn2_of(g, q.i, q.j, q.k);
+    cplx out = c_make(0.0, 0.0);
+    if (kn2 <= g.kmax && kn2 >= g.tiny) {
+      cplx L = lap[m], N = nl[m], C = c[m];
+      cplx F = f ? f[m] : c_make(0.0, 0.0);
+      double sx = nu * L.x + N.x + F.x, sy = nu * L.y + N.y + F.y;
+      out = c_make(C.x + dt * sx * rmp, C.y + dt * sy * rmp);
+    }
+    v[m] = out;
+  }
+}
+
+// examples/initialv.f90_tg:37-56 (real space, x fastest)
+__global__ void k_tg_real(GridK g, double *r1, double *r2, int kdn, int kup) {
+  const double pi = 3.14159265358979323846;
+  GRID_STRIDE(m, nreal(g)) {
+    int i = (int)(m % (size_t)g.nx) + 1;
+    size_t rest = m / (size_t)g.nx;
+    int j = (int)(rest % (size_t)g.ny) + 1;
+    int k = (int)(rest / (size_t)g.ny) + 1;
+    double a = 0.0, b = 0.0;
+    for (int ki = kdn; ki <= kup; ++ki) {
+      double ax = 2 * pi * ki * ((double)i - 1) / (double)g.nx;
+      double ay = 2 * pi * ki * ((double)j - 1) / (double)g.ny;
+      double az = 2 * pi * ki * ((double)k - 1) / (double)g.nz;
+      a = a + sin(ax) * cos(ay) * cos(az);
+      b = b - cos(ax) * sin(ay) * cos(az);
+    }
+    r1[m] = a;
+    r2[m] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions: per-block partial sums in a fixed order, then one block sums
+// the partials in index order, so results are reproducible run to run.
+// ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 2048;
+
+GHX_DEV double block_sum(double v) {
+  __shared__ double sh[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x / 64); ++q) r += sh[q];
+  __syncthreads();
+  return r;
+}
+
+GHX_DEV double cabs2(cplx a) { return a.x * a.x + a.y * a.y; }
+GHX_DEV double cre_dot(cplx a, cplx b) { return a.x * b.x + a.y * b.y; }  // Re(a conj(b))
+
+__global__ void k_reduce(GridK g, int kind, const cplx *a, const cplx *b, const cplx *c, const cplx *d,
+                         const cplx *e, const cplx *f, double *part) {
+  double acc = 0.0;
+  const double tmp = g.inv_n6;
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    const double w = q.i == 0 ? 1.0 : 2.0;
+    double s;
+    if (kind == 0) {
+      s = cabs2(a[m]) + cabs2(b[m]) + cabs2(c[m]);
+    } else if (kind == 1 || kind == 2) {
+      s = cre_dot(a[m], d[m]) + cre_dot(b[m], e[m]) + cre_dot(c[m], f[m]);
+      if (kind == 2) s *= kk2_at(g, q);
+    } else {
+      double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky), kz = wavenum(q.k, g.nz, g.dkz);
+      cplx dv = c_add(c_add(c_ik(kx, a[m]), c_ik(ky, b[m])), c_ik(kz, c[m]));
+      s = cabs2(dv);
+    }
+    acc += w * s * tmp;
+  }
+  double r = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ void k_sum_partials(const double *part, int n, double *out) {
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) acc += part[q];
+  double r = block_sum(acc);
+  if (threadIdx.x == 0) *out = r;
+}
+
+// spectrumc: shell kmn = int(sqrt(kk2)/Dkk + .501) (single-precision literal),
+// bins 1..nb.
+__global__ void k_spectrum(GridK g, int kind, double dkk, int nb, const cplx *a, const cplx *b, const cplx *c,
+                           const cplx *d, const cplx *e, const cplx *f, double *part) {
+  extern __shared__ double bins[];
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) bins[q] = 0.0;
+  __syncthreads();
+  const double tmp = g.inv_n6;
+  const double off = (double)0.501f;
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    double kk2 = kk2_at(g, q);
+    int kmn = (int)(sqrt(kk2) / dkk + off);
+    if (kmn > 0 && kmn <= nb) {
+      const double w = q.i == 0 ? 1.0 : 2.0;
+      double s;
+      if (kind == 0) s = cabs2(a[m]) + cabs2(b[m]) + cabs2(c[m]);
+      else s = cre_dot(a[m], d[m]) + cre_dot(b[m], e[m]) + cre_dot(c[m], f[m]);
+      atomicAdd(&bins[kmn - 1], w * s * tmp);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) part[(size_t)blockIdx.x * nb + q] = bins[q];
+}
+
+__global__ void k_spectrum_sum(const double *part, int nblk, int nb, double *out) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) acc += part[(size_t)b * nb + q];
+    out[q] = acc;
+  }
+}
+
+__global__ void k_maxnorm(size_t n, const double *r1, const double *r2, const double *r3, double *part) {
+  double v = 0.0;
+  GRID_STRIDE(m, n) {
+    double s = sqrt(r1[m] * r1[m] + r2[m] * r2[m] + r3[m] * r3[m]);
+    v = fmax(v, s);
+  }
+  __shared__ double sh[kBlock];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_max_partials(const double *part, int n, double *out) {
+  __shared__ double sh[kBlock];
+  double v = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) v = fmax(v, part[q]);
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+}  // namespace
+
+#define LAUNCH(kern, n, st, ...)                                                     \
+  do {                                                                               \
+    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, st, __VA_ARGS__);    \
+    return hipGetLastError();                                                        \
+  } while (0)
+
+hipError_t op_derivk3(const GridK &g, hipStream_t st, const cplx *a, cplx *b, int dir) {
+  LAUNCH(k_derivk3, ncplx(g), st, g, a, b, dir);
+}
+hipError_t op_laplak3(const GridK &g, hipStream_t st, const cplx *a, cplx *b) { LAUNCH(k_laplak3, ncplx(g), st, g, a, b); }
+hipError_t op_rotor3(const GridK &g, hipStream_t st, const cplx *a, const cplx *b, cplx *c, int dir) {
+  LAUNCH(k_rotor3, ncplx(g), st, g, a, b, c, dir);
+}
+hipError_t op_nonlhd3(const GridK &g, hipStream_t st, const cplx *a, const cplx *b, const cplx *c, cplx *o, int dir) {
+  LAUNCH(k_project, ncplx(g), st, g, 0, a, b, c, (const cplx *)nullptr, (const cplx *)nullptr, (const cplx *)nullptr, o, dir);
+}
+hipError_t op_nonlin3(const GridK &g, hipStream_t st, const cplx *a, const cplx *b, const cplx *c, const cplx *d,
+                      const cplx *e, const cplx *f, cplx *o, int dir) {
+  LAUNCH(k_project, ncplx(g), st, g, 1, a, b, c, d, e, f, o, dir);
+}
+hipError_t op_gauge3(const GridK &g, hipStream_t st, const cplx *a, const cplx *b, const cplx *c, cplx *o, int dir) {
+  LAUNCH(k_project, ncplx(g), st, g, 2, a, b, c, (const cplx *)nullptr, (const cplx *)nullptr, (const cplx *)nullptr, o, dir);
+}
+hipError_t op_phaseshift(const GridK &g, hipStream_t st, cplx *a, double cre, double cim) {
+  LAUNCH(k_phaseshift, ncplx(g), st, g, a, cre, cim);
+}
+hipError_t op_scale(const GridK &g, hipStream_t st, cplx *a, const cplx *src, double s) {
+  LAUNCH(k_scale, ncplx(g), st, ncplx(g), a, src, s);
+}
+hipError_t op_normalize_apply(const GridK &g, hipStream_t st, cplx *x, cplx *y, cplx *z, double f0, double e) {
+  LAUNCH(k_normalize, ncplx(g), st, ncplx(g), x, y, z, f0, sqrt(e));
+}
+hipError_t op_sharp(const GridK &g, hipStream_t st, const cplx *a, cplx *b, double kc2) {
+  LAUNCH(k_sharp, ncplx(g), st, g, a, b, kc2);
+}
+hipError_t op_zero(hipStream_t st, void *p, size_t bytes) {
+  size_t n = bytes / sizeof(double);
+  LAUNCH(k_zero, n, st, n, (double *)p);
+}
+hipError_t op_cross_real(const GridK &g, hipStream_t st, double *r1, double *r2, double *r3, const double *r4,
+                         const double *r5, const double *r6, double *r7) {
+  LAUNCH(k_cross_real, nreal(g), st, nreal(g), g.inv_n6, r1, r2, r3, r4, r5, r6, r7);
+}
+hipError_t op_rk_update(const GridK &g, hipStream_t st, cplx *v, const cplx *c, const cplx *lap, const cplx *nl,
+                        const cplx *f, double nu, double dt, double rmp) {
+  LAUNCH(k_rk_update, ncplx(g), st, g, v, c, lap, nl, f, nu, dt, rmp);
+}
+hipError_t op_tg_real(const GridK &g, hipStream_t st, double *r1, double *r2, int kdn, int kup) {
+  LAUNCH(k_tg_real, nreal(g), st, g, r1, r2, kdn, kup);
+}
+
+size_t reduce_work_doubles() { return kRedBlocks + 8; }
+
+hipError_t op_reduce(const GridK &g, hipStream_t st, int kind, const cplx *a, const cplx *b, const cplx *c,
+                     const cplx *d, const cplx *e, const cplx *f, double *work, double *out) {
+  unsigned nb = nblocks(ncplx(g));
+  if (nb > (unsigned)kRedBlocks) nb = kRedBlocks;
+  hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(kBlock), 0, st, g, kind, a, b, c, d, e, f, work);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, work, (int)nb, work + kRedBlocks);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipMemcpyAsync(out, work + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, st);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(st);
+}
+
+static unsigned spectrum_blocks(const GridK &g) {
+  unsigned nb = nblocks(ncplx(g));
+  return nb > 1024u ? 1024u : nb;
+}
+size_t spectrum_work_doubles(const GridK &g, int nb) { return (size_t)spectrum_blocks(g) * nb + nb; }
+
+hipError_t op_spectrum(const GridK &g, hipStream_t st, int kind, double dkk, int nb, const cplx *a, const cplx *b,
+                       const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *work, double *out) {
+  unsigned nblk = spectrum_blocks(g);
+  hipLaunchKernelGGL(k_spectrum, dim3(nblk), dim3(kBlock), nb * sizeof(double), st, g, kind, dkk, nb, a, b, c, d,
+                     e, f, work);
+  double *res = work + (size_t)nblk * nb;
+  hipLaunchKernelGGL(k_spectrum_sum, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, st, work, (int)nblk, nb, res);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipMemcpyAsync(out, res, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(st);
+}
+
+hipError_t op_maxnorm(const GridK &g, hipStream_t st, const double *r1, const double *r2, const double *r3,
+                      double *work, double *out) {
+  unsigned nb = nblocks(nreal(g));
+  if (nb > (unsigned)kRedBlocks) nb = kRedBlocks;
+  hipLaunchKernelGGL(k_maxnorm, dim3(nb), dim3(kBlock), 0, st, nreal(g), r1, r2, r3, work);
+  hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(kBlock), 0, st, work, (int)nb, work + kRedBlocks);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipMemcpyAsync(out, work + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, st);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(st);
+}
+
+}  // namespace ghx

@@ -1,0 +1,197 @@
+"""Python mirror of GHOST's pseudospectral subroutines on device-resident fields.
+
+The routines keep the Fortran names, argument order and meaning of
+3D/src/pseudo/pseudospec3D_hd.f90, pseudospec3D_mhd.f90, pseudospec3D_filt.f90
+and module_dns.f90; the module state GHOST keeps in ``grid``, ``kes``,
+``ali``, ``boxsize`` and ``fft`` (pseudospec3D_mod.fpp:13-120) lives in
+:class:`Grid`.  Spectral fields are complex128 GPU tensors of shape
+(nx//2+1, ny, nz); real fields are float64 of shape (nz, ny, nx).
+"""
+import ctypes as C
+
+import torch
+
+from . import _lib
+from .fftp import _dptr
+
+
+class Grid:
+    """Grid, wavenumbers and dealias bounds (main3D.fpp:737-749, 881-909, 1727-1800)."""
+
+    def __init__(self, nx, ny=None, nz=None, L=(1.0, 1.0, 1.0), arbsize=False, Dkk=0.0):
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.nxh = self.nx // 2 + 1
+        L_ = _lib.load()
+        h = C.c_void_p()
+        _lib.check(L_.ghx_grid_create(C.byref(h), self.nx, self.ny, self.nz, float(L[0]), float(L[1]),
+                                      float(L[2]), int(bool(arbsize)), float(Dkk)))
+        self._h = h
+        _lib.check(L_.ghx_grid_set_stream(h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        kmax, tiny, nmax, dkk, anis = C.c_double(), C.c_double(), C.c_int(), C.c_double(), C.c_int()
+        _lib.check(L_.ghx_grid_info(h, C.byref(kmax), C.byref(tiny), C.byref(nmax), C.byref(dkk), C.byref(anis)))
+        self.kmax, self.tiny, self.nmax, self.Dkk, self.anis = kmax.value, tiny.value, nmax.value, dkk.value, anis.value
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _lib.load().ghx_grid_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # ---- shapes and allocation -------------------------------------------
+    @property
+    def cshape(self):
+        return (self.nxh, self.ny, self.nz)
+
+    @property
+    def rshape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def nbins(self):
+        return self.nmax // 2 + 1
+
+    def czeros(self):
+        return torch.zeros(self.cshape, dtype=torch.complex128, device=self.device)
+
+    def rzeros(self):
+        return torch.zeros(self.rshape, dtype=torch.float64, device=self.device)
+
+    def _c(self, t):
+        return C.c_void_p(_dptr(t, torch.complex128, self.cshape))
+
+    def _r(self, t):
+        return C.c_void_p(_dptr(t, torch.float64, self.rshape))
+
+    # ---- fftp -------------------------------------------------------------
+    def r2c(self, r, out=None):
+        out = self.czeros() if out is None else out
+        _lib.check(_lib.load().ghx_grid_r2c(self._h, self._r(r), self._c(out)))
+        return out
+
+    def c2r(self, c, out=None):
+        out = self.rzeros() if out is None else out
+        _lib.check(_lib.load().ghx_grid_c2r(self._h, self._c(c), self._r(out)))
+        return out
+
+    # ---- pseudospec3D_hd.f90 ----------------------------------------------
+    def derivk3(self, a, b, dir):
+        _lib.check(_lib.load().ghx_derivk3(self._h, self._c(a), self._c(b), dir))
+        return b
+
+    def laplak3(self, a, b):
+        _lib.check(_lib.load().ghx_laplak3(self._h, self._c(a), self._c(b)))
+        return b
+
+    def rotor3(self, a, b, c, dir):
+        _lib.check(_lib.load().ghx_rotor3(self._h, self._c(a), self._c(b), self._c(c), dir))
+        return c
+
+    def prodre3(self, a, b, c, d, e, f):
+        _lib.check(_lib.load().ghx_prodre3(self._h, *map(self._c, (a, b, c, d, e, f))))
+        return d, e, f
+
+    def nonlhd3(self, a, b, c, g, dir):
+        _lib.check(_lib.load().ghx_nonlhd3(self._h, *map(self._c, (a, b, c, g)), dir))
+        return g
+
+    def energy(self, a, b, c, kin):
+        d = C.c_double()
+        _lib.check(_lib.load().ghx_energy(self._h, *map(self._c, (a, b, c)), kin, C.byref(d)))
+        return d.value
+
+    def helicity(self, a, b, c):
+        d = C.c_double()
+        _lib.check(_lib.load().ghx_helicity(self._h, *map(self._c, (a, b, c)), C.byref(d)))
+        return d.value
+
+    def cross(self, a, b, c, d, e, f, kin):
+        g = C.c_double()
+        _lib.check(_lib.load().ghx_cross(self._h, *map(self._c, (a, b, c, d, e, f)), kin, C.byref(g)))
+        return g.value
+
+    def maxabs(self, a, b, c, kin):
+        d = C.c_double()
+        _lib.check(_lib.load().ghx_maxabs(self._h, *map(self._c, (a, b, c)), kin, C.byref(d)))
+        return d.value
+
+    def divergence(self, a, b, c):
+        d = C.c_double()
+        _lib.check(_lib.load().ghx_divergence(self._h, *map(self._c, (a, b, c)), C.byref(d)))
+        return d.value
+
+    def spectrumc(self, a, b, c, kin=1, hel=0):
+        """Returns (Ektot, Hktot) as lists of nmax/2+1 doubles (spectrumc :1190)."""
+        Ek = (C.c_double * self.nbins)()
+        Hk = (C.c_double * self.nbins)()
+        _lib.check(_lib.load().ghx_spectrumc(self._h, *map(self._c, (a, b, c)), kin, hel,
+                                             C.cast(Ek, C.POINTER(C.c_double)), C.cast(Hk, C.POINTER(C.c_double))))
+        return list(Ek), list(Hk)
+
+    def phaseshift(self, a, cdump):
+        _lib.check(_lib.load().ghx_phaseshift(self._h, self._c(a), float(cdump.real), float(cdump.imag)))
+        return a
+
+    def normalize(self, fx, fy, fz, f0, kin=1):
+        _lib.check(_lib.load().ghx_normalize(self._h, *map(self._c, (fx, fy, fz)), float(f0), kin))
+
+    def sharp(self, a, b, delta):
+        _lib.check(_lib.load().ghx_sharp(self._h, self._c(a), self._c(b), float(delta)))
+        return b
+
+    # ---- pseudospec3D_mhd.f90 ---------------------------------------------
+    def vector3(self, a, b, c, d, e, f, x, y, z):
+        _lib.check(_lib.load().ghx_vector3(self._h, *map(self._c, (a, b, c, d, e, f, x, y, z))))
+        return x, y, z
+
+    def nonlin3(self, a, b, c, d, e, f, g, dir):
+        _lib.check(_lib.load().ghx_nonlin3(self._h, *map(self._c, (a, b, c, d, e, f, g)), dir))
+        return g
+
+    def gauge3(self, a, b, c, g, dir):
+        _lib.check(_lib.load().ghx_gauge3(self._h, *map(self._c, (a, b, c, g)), dir))
+        return g
+
+    # ---- IC / forcing -----------------------------------------------------
+    def initialv_tg(self, kdn=1.0, kup=1.0, u0=0.5):
+        v = [self.czeros() for _ in range(3)]
+        _lib.check(_lib.load().ghx_initialv_tg(self._h, kdn, kup, u0, *map(self._c, v)))
+        return v
+
+    def initialfv_random(self, kdn, kup, f0, seed, out=None):
+        f = [self.czeros() for _ in range(3)] if out is None else out
+        s = C.c_int32(seed)
+        _lib.check(_lib.load().ghx_initialfv_random(self._h, kdn, kup, f0, C.byref(s), *map(self._c, f)))
+        return f, s.value
+
+    def initialv_k4exp(self, k0, u0, seed, out=None):
+        v = [self.czeros() for _ in range(3)] if out is None else out
+        s = C.c_int32(seed)
+        _lib.check(_lib.load().ghx_initialv_k4exp(self._h, k0, u0, C.byref(s), *map(self._c, v)))
+        return v, s.value
+
+    # ---- RK ----------------------------------------------------------------
+    def hd_rkstep2(self, o, dt, nu, v_in, C_, f, v_out):
+        """include/hd_rkstep2.f90 for one value of o, fused on the GPU."""
+        L_ = _lib.load()
+        pv, _a = _lib.ptr_array([t.data_ptr() for t in map(lambda t: t, v_in)])
+        pc, _b = _lib.ptr_array([t.data_ptr() for t in C_])
+        po, _d = _lib.ptr_array([t.data_ptr() for t in v_out])
+        for t in list(v_in) + list(C_) + list(v_out) + (list(f) if f is not None else []):
+            _dptr(t, torch.complex128, self.cshape)
+        if f is not None:
+            pf, _c = _lib.ptr_array([t.data_ptr() for t in f])
+        else:
+            pf = None
+        _lib.check(L_.ghx_hd_rkstep2(self._h, int(o), float(dt), float(nu), pv, pc, pf, po))
+
+
+def randu(seed):
+    """randu (pseudospec3D_mod.fpp:130-148); returns (value, new seed)."""
+    s = C.c_int32(seed)
+    v = _lib.load().ghx_randu(C.byref(s))
+    return v, s.value

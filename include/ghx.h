@@ -1,0 +1,187 @@
+/*
+ * ghx.h -- C ABI of the MI355X (gfx950) implementation of GHOST's
+ * pseudospectral hot path.
+ *
+ * Reference paths below are relative to the GHOST repository root.
+ *
+ * Conventions (GHOST's, 3D/src/fftp-3/fftp3D.fpp:204-431):
+ *   real field      double r[nz][ny][nx]              == Fortran REAL(GP) R(nx,ny,nz)
+ *   spectral field  complex double c[nx/2+1][ny][nz]  == Fortran COMPLEX(GP) C(nz,ny,nx/2+1)
+ *                   (16-byte interleaved re,im; kz is the contiguous index)
+ * Transforms are unnormalised: forward sign -1, inverse +1, r2c->c2r round
+ * trip multiplies by nx*ny*nz.  Nonlinear products carry 1/(nx ny nz)^2.
+ *
+ * Unless a name ends in _host, every field pointer is a DEVICE pointer
+ * (from ghx_malloc, hipMalloc or a torch tensor), and calls are
+ * asynchronous on the handle's stream; functions returning host scalars
+ * synchronise that stream.
+ *
+ * Every function returns 0 on success or a negative GHX_ERR_* code;
+ * ghx_last_error() describes the most recent failure on the calling thread.
+ * Handles are not re-entrant: one host thread per handle at a time.
+ */
+#ifndef GHX_H
+#define GHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GHX_OK 0
+#define GHX_ERR_ARG (-1)      /* bad argument (null handle, bad dir, ...) */
+#define GHX_ERR_SIZE (-2)     /* grid size without a compiled FFT schedule */
+#define GHX_ERR_HIP (-3)      /* HIP runtime error */
+#define GHX_ERR_NOMEM (-4)    /* device allocation failed */
+#define GHX_ERR_NODEVICE (-5) /* no GPU visible */
+
+/* Direction constants of fftp-3/fftp_mod.fpp:27-28 */
+#define GHX_FFTW_REAL_TO_COMPLEX (-1)
+#define GHX_FFTW_COMPLEX_TO_REAL (1)
+/* Planner flags of fftp-3/fftp_mod.fpp:24-26 (accepted, ignored on the GPU) */
+#define GHX_FFTW_ESTIMATE 64
+#define GHX_FFTW_MEASURE 0
+
+const char *ghx_last_error(void);
+/* 1 if the library found a usable GPU, else 0 */
+int ghx_device_ok(void);
+
+/* ---------------------------------------------------------------------- */
+/* Device memory and streams                                              */
+/* ---------------------------------------------------------------------- */
+int ghx_malloc(void **ptr, size_t bytes);
+int ghx_free(void *ptr);
+int ghx_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int ghx_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int ghx_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int ghx_memset_zero(void *ptr, size_t bytes);
+/* hipDeviceSynchronize */
+int ghx_sync(void);
+
+/* ---------------------------------------------------------------------- */
+/* fftp: GHOST's parallel FFT module on one rank                          */
+/* ---------------------------------------------------------------------- */
+typedef struct ghx_plan ghx_plan;
+
+/* fftp3d_create_plan(plan,n,fftdir,flags)  fftp-3/fftp3D.fpp:56-124 */
+int ghx_plan_create(ghx_plan **plan, int nx, int ny, int nz, int fftdir, int flags);
+/* fftp3d_destroy_plan(plan)  fftp-3/fftp3D.fpp:126-155 */
+int ghx_plan_destroy(ghx_plan *plan);
+/* stream for this plan's launches (hipStream_t cast to void*, NULL = default) */
+int ghx_plan_set_stream(ghx_plan *plan, void *stream);
+/* fftp3d_real_to_complex(plan,in,out,comm)  fftp-3/fftp3D.fpp:204-315
+ * device in[nz][ny][nx] -> device out[nx/2+1][ny][nz] */
+int ghx_r2c(ghx_plan *plan, const double *in, void *out);
+/* fftp3d_complex_to_real(plan,in,out,comm)  fftp-3/fftp3D.fpp:318-431
+ * device in -> device out.  Unlike GHOST, the input is preserved. */
+int ghx_c2r(ghx_plan *plan, const void *in, double *out);
+/* Batched forms: nf fields (nf <= 12) in one launch sequence. */
+int ghx_r2c_many(ghx_plan *plan, int nf, const double *const *in, void *const *out);
+int ghx_c2r_many(ghx_plan *plan, int nf, const void *const *in, double *const *out);
+/* Host-array forms of fftp3d_real_to_complex / fftp3d_complex_to_real, for a
+ * caller whose fields live on the host (copies H->D, transforms, D->H). */
+int ghx_r2c_host(ghx_plan *plan, const double *in, void *out);
+int ghx_c2r_host(ghx_plan *plan, const void *in, double *out);
+/* fftp module timers ffttime/tratime/comtime/tottime (fftp-3/fftp_mod.fpp:29-33),
+ * accumulated seconds of GPU time of this plan's transforms.  On one GPU the
+ * transposes are fused into the FFT passes and there is no communication, so
+ * tra and com stay 0. */
+int ghx_plan_timers(ghx_plan *plan, double *ffttime, double *tratime, double *comtime, double *tottime);
+int ghx_plan_reset_timers(ghx_plan *plan);
+
+/* ---------------------------------------------------------------------- */
+/* Grid: GHOST's grid/kes/ali/boxsize/fft module state                    */
+/* main3D.fpp:737-749 (anis), 881-909 (box), 1727-1800 (kx,ky,kz,kn2,kk2,  */
+/* kmax, tiny, nmax); pseudospec3D_mod.fpp:13-120                          */
+/* ---------------------------------------------------------------------- */
+typedef struct ghx_grid ghx_grid;
+
+/* arbsize=0: 2*pi-periodic box (L ignored); arbsize=1: box of Lx,Ly,Lz
+ * (units of 2*pi) as with GHOST's ARBSIZE build; Dkk<1e-5 selects the default
+ * shell width min(1/Lx,1/Ly,1/Lz). */
+int ghx_grid_create(ghx_grid **grid, int nx, int ny, int nz, double Lx, double Ly, double Lz, int arbsize,
+                    double Dkk);
+int ghx_grid_destroy(ghx_grid *grid);
+int ghx_grid_set_stream(ghx_grid *grid, void *stream);
+/* kmax, tiny (dealias bounds on kn2), nmax (spectra have nmax/2+1 bins), Dkk, anis */
+int ghx_grid_info(const ghx_grid *grid, double *kmax, double *tiny, int *nmax, double *Dkk, int *anis);
+/* fftp plans of the grid (planrc / plancr of pseudospec3D_mod.fpp:45-50) */
+int ghx_grid_r2c(ghx_grid *grid, const double *in, void *out);
+int ghx_grid_c2r(ghx_grid *grid, const void *in, double *out);
+
+/* ---------------------------------------------------------------------- */
+/* pseudospec operators (pseudospec3D_hd.f90 / _mhd.f90 / _filt.f90)      */
+/* Same argument order and meaning as the Fortran routines.               */
+/* ---------------------------------------------------------------------- */
+int ghx_derivk3(ghx_grid *g, const void *a, void *b, int dir);                                /* hd :22   */
+int ghx_laplak3(ghx_grid *g, const void *a, void *b);                                         /* hd :91   */
+int ghx_rotor3(ghx_grid *g, const void *a, const void *b, void *c, int dir);                  /* hd :124  */
+int ghx_prodre3(ghx_grid *g, const void *a, const void *b, const void *c, void *d, void *e, void *f); /* hd :316 */
+int ghx_nonlhd3(ghx_grid *g, const void *a, const void *b, const void *c, void *out, int dir); /* hd :399 */
+int ghx_vector3(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                const void *f, void *x, void *y, void *z);                                    /* mhd :22  */
+int ghx_nonlin3(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                const void *f, void *out, int dir);                                           /* mhd :108 */
+int ghx_gauge3(ghx_grid *g, const void *a, const void *b, const void *c, void *out, int dir);  /* mhd :230 */
+int ghx_energy(ghx_grid *g, const void *a, const void *b, const void *c, int kin, double *out); /* hd :512 */
+int ghx_helicity(ghx_grid *g, const void *a, const void *b, const void *c, double *out);       /* hd :665  */
+int ghx_cross(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+              const void *f, int kin, double *out);                                           /* hd :787  */
+int ghx_maxabs(ghx_grid *g, const void *a, const void *b, const void *c, int kin, double *out); /* hd :910 */
+/* divergence term of hdcheck (hd :1036-1071): <|div v|^2> */
+int ghx_divergence(ghx_grid *g, const void *a, const void *b, const void *c, double *out);
+/* spectrumc (hd :1190-1415): Ek, Hk host arrays of nmax/2+1 doubles (either may be NULL) */
+int ghx_spectrumc(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
+                  double *Hk);
+int ghx_phaseshift(ghx_grid *g, void *a, double cdump_re, double cdump_im);                   /* hd :2084 */
+int ghx_normalize(ghx_grid *g, void *fx, void *fy, void *fz, double f0, int kin);             /* module_dns.f90:13 */
+int ghx_sharp(ghx_grid *g, const void *a, void *b, double delta);                             /* filt :200 */
+
+/* ---------------------------------------------------------------------- */
+/* Initial conditions / forcing (examples/initialv.f90_tg,                 */
+/* examples/initialfv.f90_random; randu pseudospec3D_mod.fpp:130-148)      */
+/* ---------------------------------------------------------------------- */
+double ghx_randu(int32_t *seed);
+int ghx_initialv_tg(ghx_grid *g, double kdn, double kup, double u0, void *vx, void *vy, void *vz);
+int ghx_initialfv_random(ghx_grid *g, double kdn, double kup, double f0, int32_t *seed, void *fx, void *fy,
+                         void *fz);
+/* random-phase solenoidal field with shell spectrum ~ k^4 exp(-2 (k/k0)^2),
+ * drawn in the fill order of initialfv.f90_random, normalised to <u^2> = u0^2 */
+int ghx_initialv_k4exp(ghx_grid *g, double k0, double u0, int32_t *seed, void *vx, void *vy, void *vz);
+
+/* ---------------------------------------------------------------------- */
+/* Runge-Kutta substeps with fields resident in HBM                        */
+/* ---------------------------------------------------------------------- */
+/* One pass of include/hd_rkstep2.f90 for order index o (main3D.fpp:3229):
+ *   v = C + dt*(nu*lap v + P[-(curl v) x v] + f)/o   where tiny <= kn2 <= kmax, else 0,
+ * computed by the fused path (curl in the first inverse pass, cross product
+ * between the x passes, projection and update in the last forward pass).
+ * v_in/C/f: 3 fields each (f may be NULL); v_out may equal v_in or C. */
+int ghx_hd_rkstep2(ghx_grid *g, int o, double dt, double nu, const void *const *v_in, const void *const *C,
+                   const void *const *f, void *const *v_out);
+
+typedef struct ghx_hd ghx_hd;
+/* HD solver state: v (3), forcing f (3), ping-pong and pass scratch, all on device. */
+int ghx_hd_create(ghx_hd **s, ghx_grid *g, int ord, double dt, double nu);
+int ghx_hd_destroy(ghx_hd *s);
+/* device pointers of the resident fields: v[3], f[3] */
+int ghx_hd_fields(ghx_hd *s, void **v, void **f);
+/* call after writing f: records which (kx,ky) lines carry forcing, or 0 for none */
+int ghx_hd_set_forcing(ghx_hd *s, int enabled);
+/* nsteps full RK steps (main3D.fpp:3202-3231 without output/diagnostics) */
+int ghx_hd_step(ghx_hd *s, int nsteps);
+
+typedef struct ghx_mhd ghx_mhd;
+/* MHD solver state: v (3), a (3, vector potential), f (3), m (3). */
+int ghx_mhd_create(ghx_mhd **s, ghx_grid *g, int ord, double dt, double nu, double mu);
+int ghx_mhd_destroy(ghx_mhd *s);
+int ghx_mhd_fields(ghx_mhd *s, void **v, void **a, void **f, void **m);
+int ghx_mhd_set_forcing(ghx_mhd *s, int enabled);
+int ghx_mhd_step(ghx_mhd *s, int nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GHX_H */

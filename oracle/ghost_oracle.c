@@ -1,0 +1,1014 @@
+/*
+ * ghost_oracle.c -- CPU restatement of GHOST's pseudospectral hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the timed CPU
+ * baseline.  Never linked into, or called by, the product library.
+ *
+ * Every routine cites the GHOST source it restates (paths relative to the
+ * reference repository root, 3D/src/...).  Arithmetic is FP64 throughout and
+ * the file is compiled with -ffp-contract=off so that each expression rounds
+ * the way it is written, in GHOST's operand order.
+ *
+ * The FFT is this file's own mixed-radix Stockham transform (radix 8/4/3/2
+ * plus a generic odd-prime DFT), so the oracle needs no FFT library; its
+ * conventions are FFTW's (sign -1 forward, +1 inverse, unnormalised; the
+ * 1D c2r ignores the imaginary part of the DC and Nyquist inputs).
+ */
+#include "ghost_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+
+static int g_threads = 0;
+void og_set_threads(int n) {
+  g_threads = n;
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#endif
+}
+int og_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* 1D mixed-radix Stockham FFT                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fft1 {
+  int n, nf;
+  int fac[64];
+  ocplx *tw; /* tw[k] = exp(-2 pi i k / n), k = 0..n-1 */
+} fft1;
+
+static fft1 *fft1_new(int n) {
+  fft1 *p = (fft1 *)calloc(1, sizeof(fft1));
+  p->n = n;
+  int m = n;
+  while (m % 8 == 0) { p->fac[p->nf++] = 8; m /= 8; }
+  while (m % 4 == 0) { p->fac[p->nf++] = 4; m /= 4; }
+  while (m % 2 == 0) { p->fac[p->nf++] = 2; m /= 2; }
+  for (int q = 3; m > 1; q += 2)
+    while (m % q == 0) { p->fac[p->nf++] = q; m /= q; }
+  p->tw = (ocplx *)malloc(sizeof(ocplx) * (size_t)n);
+  const long double twopi = 6.283185307179586476925286766559L;
+  for (int k = 0; k < n; ++k) {
+    long double a = twopi * (long double)k / (long double)n;
+    p->tw[k] = (double)cosl(a) - I * (double)sinl(a);
+  }
+  return p;
+}
+
+static void fft1_free(fft1 *p) {
+  if (!p) return;
+  free(p->tw);
+  free(p);
+}
+
+static inline ocplx twid(const fft1 *p, int e, int sign) {
+  ocplx w = p->tw[e];
+  return sign < 0 ? w : conj(w);
+}
+
+/* multiply by sign*i */
+static inline ocplx mul_si(ocplx z, int sign) {
+  return sign < 0 ? cimag(z) - I * creal(z) : -cimag(z) + I * creal(z);
+}
+
+static void dft_small(ocplx *v, int R, const fft1 *p, int sign) {
+  const double s3 = 0.86602540378443864676; /* sqrt(3)/2 */
+  const double r2 = 0.70710678118654752440; /* 1/sqrt(2) */
+  switch (R) {
+  case 2: {
+    ocplx a = v[0] + v[1], b = v[0] - v[1];
+    v[0] = a; v[1] = b;
+    return;
+  }
+  case 3: {
+    ocplx t = v[1] + v[2], d = v[1] - v[2];
+    ocplx m = v[0] - 0.5 * t;
+    ocplx e = mul_si(s3 * d, sign);
+    v[0] = v[0] + t;
+    v[1] = m + e;
+    v[2] = m - e;
+    return;
+  }
+  case 4: {
+    ocplx a0 = v[0] + v[2], a1 = v[0] - v[2];
+    ocplx a2 = v[1] + v[3], a3 = mul_si(v[1] - v[3], sign);
+    v[0] = a0 + a2; v[2] = a0 - a2;
+    v[1] = a1 + a3; v[3] = a1 - a3;
+    return;
+  }
+  case 8: {
+    ocplx e[4] = {v[0], v[2], v[4], v[6]};
+    ocplx o[4] = {v[1], v[3], v[5], v[7]};
+    dft_small(e, 4, p, sign);
+    dft_small(o, 4, p, sign);
+    /* W8^q for q=1,2,3 with W8 = exp(sign*2 pi i/8) */
+    ocplx w1 = sign < 0 ? r2 - I * r2 : r2 + I * r2;
+    ocplx o1 = o[1] * w1;
+    ocplx o2 = mul_si(o[2], sign);
+    ocplx o3 = mul_si(o[3] * w1, sign);
+    v[0] = e[0] + o[0]; v[4] = e[0] - o[0];
+    v[1] = e[1] + o1;   v[5] = e[1] - o1;
+    v[2] = e[2] + o2;   v[6] = e[2] - o2;
+    v[3] = e[3] + o3;   v[7] = e[3] - o3;
+    return;
+  }
+  default: {
+    ocplx out[64];
+    int step = p->n / R;
+    for (int q = 0; q < R; ++q) {
+      ocplx acc = 0;
+      for (int r = 0; r < R; ++r) acc += v[r] * twid(p, ((q * r) % R) * step, sign);
+      out[q] = acc;
+    }
+    memcpy(v, out, sizeof(ocplx) * (size_t)R);
+  }
+  }
+}
+
+/* In place on x, using work w (length n). */
+static void fft1_exec(const fft1 *p, ocplx *x, ocplx *w, int sign) {
+  const int n = p->n;
+  if (n == 1) return;
+  ocplx *src = x, *dst = w;
+  int Ns = 1;
+  for (int f = 0; f < p->nf; ++f) {
+    const int R = p->fac[f];
+    const int m = n / R;
+    const int tws = n / (Ns * R);
+    for (int j = 0; j < m; ++j) {
+      const int k = j % Ns;
+      ocplx v[64];
+      for (int r = 0; r < R; ++r) {
+        ocplx t = src[j + r * m];
+        if (k) t *= twid(p, (int)(((long)k * r * tws) % n), sign);
+        v[r] = t;
+      }
+      dft_small(v, R, p, sign);
+      const int base = (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+    }
+    ocplx *t = src; src = dst; dst = t;
+    Ns *= R;
+  }
+  if (src != x) memcpy(x, src, sizeof(ocplx) * (size_t)n);
+}
+
+void og_fft1d(int n, ocplx *x, int sign) {
+  fft1 *p = fft1_new(n);
+  ocplx *w = (ocplx *)malloc(sizeof(ocplx) * (size_t)n);
+  fft1_exec(p, x, w, sign);
+  free(w);
+  fft1_free(p);
+}
+
+/* Real row (length nx) -> half spectrum (nx/2+1), sign -1.  Even nx uses the
+ * packed half-length transform, odd nx the full complex one. */
+static void rfft_row(const fft1 *ph, const fft1 *pf, int nx, const double *x, ocplx *X,
+                     ocplx *wa, ocplx *wb) {
+  if (nx % 2 == 0) {
+    const int M = nx / 2;
+    for (int n = 0; n < M; ++n) wa[n] = x[2 * n] + I * x[2 * n + 1];
+    fft1_exec(ph, wa, wb, -1);
+    for (int k = 0; k <= M; ++k) {
+      ocplx A = wa[k % M];
+      ocplx B = conj(wa[(M - k) % M]);
+      ocplx Xe = 0.5 * (A + B);
+      ocplx Xo = -0.5 * I * (A - B);
+      X[k] = Xe + pf->tw[k] * Xo;
+    }
+  } else {
+    for (int n = 0; n < nx; ++n) wa[n] = x[n];
+    fft1_exec(pf, wa, wb, -1);
+    for (int k = 0; k <= nx / 2; ++k) X[k] = wa[k];
+  }
+}
+
+/* Half spectrum (nx/2+1) -> real row, sign +1, unnormalised. */
+static void irfft_row(const fft1 *ph, const fft1 *pf, int nx, const ocplx *Xin, double *x,
+                      ocplx *wa, ocplx *wb) {
+  if (nx % 2 == 0) {
+    const int M = nx / 2;
+    for (int k = 0; k < M; ++k) {
+      ocplx A = (k == 0) ? creal(Xin[0]) : Xin[k];
+      ocplx Bs = (k == 0) ? creal(Xin[M]) : Xin[M - k];
+      ocplx B = conj(Bs);
+      ocplx Xe = A + B;
+      ocplx Xo = (A - B) * conj(pf->tw[k]);
+      wa[k] = Xe + I * Xo;
+    }
+    fft1_exec(ph, wa, wb, +1);
+    for (int n = 0; n < M; ++n) {
+      x[2 * n] = creal(wa[n]);
+      x[2 * n + 1] = cimag(wa[n]);
+    }
+  } else {
+    wa[0] = creal(Xin[0]);
+    for (int k = 1; k <= nx / 2; ++k) {
+      wa[k] = Xin[k];
+      wa[nx - k] = conj(Xin[k]);
+    }
+    fft1_exec(pf, wa, wb, +1);
+    for (int n = 0; n < nx; ++n) x[n] = creal(wa[n]);
+  }
+}
+
+typedef struct fftset {
+  fft1 *px, *pxh, *py, *pz;
+} fftset;
+
+/* ------------------------------------------------------------------------ */
+/* Grid / wavenumbers: main3D.fpp:737-749 (anis), 881-909 (box), 1727-1800  */
+/* ------------------------------------------------------------------------ */
+
+#define IDX(g, i, j, k) (((size_t)(i) * (g)->ny + (size_t)(j)) * (g)->nz + (size_t)(k))
+#define RIDX(g, i, j, k) (((size_t)(k) * (g)->ny + (size_t)(j)) * (g)->nx + (size_t)(i))
+
+og_grid *og_grid_new(int nx, int ny, int nz, double Lx, double Ly, double Lz, int arbsize,
+                     double Dkk) {
+  og_grid *g = (og_grid *)calloc(1, sizeof(og_grid));
+  g->nx = nx; g->ny = ny; g->nz = nz; g->nxh = nx / 2 + 1;
+  g->Lx = arbsize ? Lx : 1.0;
+  g->Ly = arbsize ? Ly : 1.0;
+  g->Lz = arbsize ? Lz : 1.0;
+  g->Dkx = arbsize ? 1.0 / g->Lx : 1.0;
+  g->Dky = arbsize ? 1.0 / g->Ly : 1.0;
+  g->Dkz = arbsize ? 1.0 / g->Lz : 1.0;
+  g->Dkk = arbsize ? Dkk : 0.0;
+  if (g->Dkk < 1e-5) g->Dkk = fmin(g->Dkx, fmin(g->Dky, g->Dkz));
+  g->anis = (arbsize || nx != ny || ny != nz) ? 1 : 0;
+
+  g->kmax = 1.0 / 9.0;
+  double mx = fmax(nx * g->Dkx, fmax(ny * g->Dky, nz * g->Dkz));
+  g->nmax = (int)(mx / g->Dkk);
+  if (!arbsize && g->anis == 0) g->kmax = g->kmax * ((double)nx * (double)nx);
+  g->tiny = fmin(1e-5, 0.1 / ((double)g->nmax * (double)g->nmax));
+
+  g->kx = (double *)malloc(sizeof(double) * nx);
+  g->ky = (double *)malloc(sizeof(double) * ny);
+  g->kz = (double *)malloc(sizeof(double) * nz);
+  for (int i = 1; i <= nx / 2; ++i) { g->kx[i - 1] = i - 1; g->kx[i + nx / 2 - 1] = i - nx / 2 - 1; }
+  if (nx == 1) g->kx[0] = 0;
+  for (int j = 1; j <= ny / 2; ++j) { g->ky[j - 1] = j - 1; g->ky[j + ny / 2 - 1] = j - ny / 2 - 1; }
+  if (ny == 1) g->ky[0] = 0;
+  for (int k = 1; k <= nz / 2; ++k) { g->kz[k - 1] = k - 1; g->kz[k + nz / 2 - 1] = k - nz / 2 - 1; }
+  double rmp = 1, rmq = 1, rms = 1;
+  if (g->anis) {
+    rmp = 1.0 / ((double)nx * (double)nx);
+    rmq = 1.0 / ((double)ny * (double)ny);
+    rms = 1.0 / ((double)nz * (double)nz);
+  }
+  size_t nc = (size_t)g->nxh * ny * nz;
+  g->kn2 = (double *)malloc(sizeof(double) * nc);
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k)
+        g->kn2[IDX(g, i, j, k)] = rmp * (g->kx[i] * g->kx[i]) + rmq * (g->ky[j] * g->ky[j]) +
+                                  rms * (g->kz[k] * g->kz[k]);
+  if (arbsize) {
+    for (int i = 0; i < nx; ++i) g->kx[i] *= g->Dkx;
+    for (int j = 0; j < ny; ++j) g->ky[j] *= g->Dky;
+    for (int k = 0; k < nz; ++k) g->kz[k] *= g->Dkz;
+  }
+  if (g->anis) {
+    g->kk2 = (double *)malloc(sizeof(double) * nc);
+    for (int i = 0; i < g->nxh; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k)
+          g->kk2[IDX(g, i, j, k)] = g->kx[i] * g->kx[i] + g->ky[j] * g->ky[j] + g->kz[k] * g->kz[k];
+  } else {
+    g->kk2 = g->kn2;
+  }
+  fftset *fs = (fftset *)calloc(1, sizeof(fftset));
+  fs->px = fft1_new(nx);
+  fs->pxh = fft1_new(nx % 2 == 0 ? nx / 2 : 1);
+  fs->py = fft1_new(ny);
+  fs->pz = fft1_new(nz);
+  g->fft = fs;
+  return g;
+}
+
+void og_grid_free(og_grid *g) {
+  if (!g) return;
+  fftset *fs = (fftset *)g->fft;
+  fft1_free(fs->px); fft1_free(fs->pxh); fft1_free(fs->py); fft1_free(fs->pz);
+  free(fs);
+  if (g->kk2 != g->kn2) free(g->kk2);
+  free(g->kn2);
+  free(g->kx); free(g->ky); free(g->kz);
+  free(g);
+}
+
+/* ------------------------------------------------------------------------ */
+/* fftp3d_real_to_complex (fftp-3/fftp3D.fpp:204-315): 2D r2c per z-plane,   */
+/* transpose to out(nz,ny,nx/2+1), then c2c along z.                         */
+/* ------------------------------------------------------------------------ */
+
+void og_r2c(const og_grid *g, const double *in, ocplx *out) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz, nxh = g->nxh;
+  const fftset *fs = (const fftset *)g->fft;
+  int nmaxl = nx > ny ? nx : ny;
+  if (nz > nmaxl) nmaxl = nz;
+  ocplx *carr = (ocplx *)malloc(sizeof(ocplx) * (size_t)nz * ny * nxh); /* [z][y][kx] */
+#pragma omp parallel
+  {
+    ocplx *wa = (ocplx *)malloc(sizeof(ocplx) * (size_t)nmaxl);
+    ocplx *wb = (ocplx *)malloc(sizeof(ocplx) * (size_t)nmaxl);
+#pragma omp for schedule(static)
+    for (int z = 0; z < nz; ++z) {
+      for (int y = 0; y < ny; ++y)
+        rfft_row(fs->pxh, fs->px, nx, in + ((size_t)z * ny + y) * nx,
+                 carr + ((size_t)z * ny + y) * nxh, wa, wb);
+      ocplx *line = (ocplx *)malloc(sizeof(ocplx) * (size_t)ny);
+      for (int kx = 0; kx < nxh; ++kx) {
+        for (int y = 0; y < ny; ++y) line[y] = carr[((size_t)z * ny + y) * nxh + kx];
+        fft1_exec(fs->py, line, wa, -1);
+        for (int y = 0; y < ny; ++y) carr[((size_t)z * ny + y) * nxh + kx] = line[y];
+      }
+      free(line);
+    }
+#pragma omp for schedule(static)
+    for (int kx = 0; kx < nxh; ++kx)
+      for (int y = 0; y < ny; ++y) {
+        ocplx *o = out + ((size_t)kx * ny + y) * nz;
+        for (int z = 0; z < nz; ++z) o[z] = carr[((size_t)z * ny + y) * nxh + kx];
+        fft1_exec(fs->pz, o, wa, -1);
+      }
+    free(wa);
+    free(wb);
+  }
+  free(carr);
+}
+
+/* fftp3d_complex_to_real (fftp-3/fftp3D.fpp:318-431): c2c along z, transpose,
+ * 2D c2r per z-plane.  The input is copied first, so it is preserved. */
+void og_c2r(const og_grid *g, const ocplx *in, double *out) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz, nxh = g->nxh;
+  const fftset *fs = (const fftset *)g->fft;
+  int nmaxl = nx > ny ? nx : ny;
+  if (nz > nmaxl) nmaxl = nz;
+  ocplx *tmp = (ocplx *)malloc(sizeof(ocplx) * (size_t)nxh * ny * nz);
+  ocplx *carr = (ocplx *)malloc(sizeof(ocplx) * (size_t)nz * ny * nxh);
+#pragma omp parallel
+  {
+    ocplx *wa = (ocplx *)malloc(sizeof(ocplx) * (size_t)nmaxl);
+    ocplx *wb = (ocplx *)malloc(sizeof(ocplx) * (size_t)nmaxl);
+#pragma omp for schedule(static)
+    for (int kx = 0; kx < nxh; ++kx)
+      for (int y = 0; y < ny; ++y) {
+        ocplx *t = tmp + ((size_t)kx * ny + y) * nz;
+        memcpy(t, in + ((size_t)kx * ny + y) * nz, sizeof(ocplx) * (size_t)nz);
+        fft1_exec(fs->pz, t, wa, +1);
+      }
+#pragma omp for schedule(static)
+    for (int z = 0; z < nz; ++z) {
+      for (int y = 0; y < ny; ++y)
+        for (int kx = 0; kx < nxh; ++kx)
+          carr[((size_t)z * ny + y) * nxh + kx] = tmp[((size_t)kx * ny + y) * nz + z];
+      ocplx *line = (ocplx *)malloc(sizeof(ocplx) * (size_t)ny);
+      for (int kx = 0; kx < nxh; ++kx) {
+        for (int y = 0; y < ny; ++y) line[y] = carr[((size_t)z * ny + y) * nxh + kx];
+        fft1_exec(fs->py, line, wa, +1);
+        for (int y = 0; y < ny; ++y) carr[((size_t)z * ny + y) * nxh + kx] = line[y];
+      }
+      free(line);
+      for (int y = 0; y < ny; ++y)
+        irfft_row(fs->pxh, fs->px, nx, carr + ((size_t)z * ny + y) * nxh,
+                  out + ((size_t)z * ny + y) * nx, wa, wb);
+    }
+    free(wa);
+    free(wb);
+  }
+  free(carr);
+  free(tmp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* pseudospec3D_hd.f90 pointwise operators                                  */
+/* ------------------------------------------------------------------------ */
+
+static inline size_t ncplx(const og_grid *g) { return (size_t)g->nxh * g->ny * g->nz; }
+static inline size_t nreal(const og_grid *g) { return (size_t)g->nx * g->ny * g->nz; }
+
+/* i*k*a evaluated as GHOST's im*k*a: (-k*Im a, k*Re a) */
+static inline ocplx ika(double k, ocplx a) { return -k * cimag(a) + I * (k * creal(a)); }
+
+/* derivk3 (:22-88) */
+void og_derivk3(const og_grid *g, const ocplx *a, ocplx *b, int dir) {
+  OMP_FOR
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        double kk = dir == 1 ? g->kx[i] : (dir == 2 ? g->ky[j] : g->kz[k]);
+        b[IDX(g, i, j, k)] = ika(kk, a[IDX(g, i, j, k)]);
+      }
+}
+
+/* laplak3 (:91-121) */
+void og_laplak3(const og_grid *g, const ocplx *a, ocplx *b) {
+  const size_t n = ncplx(g);
+  OMP_FOR
+  for (size_t m = 0; m < n; ++m) b[m] = -g->kk2[m] * a[m];
+}
+
+/* rotor3 (:124-200): dir 1: d_y b - d_z a; dir 2: d_z a - d_x b; dir 3: d_x b - d_y a */
+void og_rotor3(const og_grid *g, const ocplx *a, const ocplx *b, ocplx *c, int dir) {
+  OMP_FOR
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        size_t m = IDX(g, i, j, k);
+        ocplx c1, c2;
+        if (dir == 1) { c1 = ika(g->kz[k], a[m]); c2 = ika(g->ky[j], b[m]); c[m] = c2 - c1; }
+        else if (dir == 2) { c1 = ika(g->kz[k], a[m]); c2 = ika(g->kx[i], b[m]); c[m] = c1 - c2; }
+        else { c1 = ika(g->ky[j], a[m]); c2 = ika(g->kx[i], b[m]); c[m] = c2 - c1; }
+      }
+}
+
+static inline double nsix(const og_grid *g) {
+  double t = (double)g->nx * (double)g->ny * (double)g->nz;
+  return 1.0 / (t * t);
+}
+
+/* real-space cross product of prodre3/vector3 (:377-389, mhd :80-92):
+ * r7=(r2 r6-r5 r3)tmp, r3=(r3 r4-r6 r1)tmp, r1=(r1 r5-r4 r2)tmp */
+static void cross_real(const og_grid *g, double *r1, double *r2, double *r3, const double *r4,
+                       const double *r5, const double *r6, double *r7) {
+  const double tmp = nsix(g);
+  const size_t n = nreal(g);
+  OMP_FOR
+  for (size_t m = 0; m < n; ++m) {
+    double a = (r2[m] * r6[m] - r5[m] * r3[m]) * tmp;
+    double b = (r3[m] * r4[m] - r6[m] * r1[m]) * tmp;
+    double c = (r1[m] * r5[m] - r4[m] * r2[m]) * tmp;
+    r7[m] = a; r3[m] = b; r1[m] = c;
+  }
+}
+
+/* prodre3 (:316-396): FFT[curl(A) x A] / N^6 */
+void og_prodre3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, ocplx *d,
+                ocplx *e, ocplx *f) {
+  const size_t nr = nreal(g), nc = ncplx(g);
+  double *r = (double *)malloc(sizeof(double) * nr * 7);
+  ocplx *t = (ocplx *)malloc(sizeof(ocplx) * nc);
+  og_rotor3(g, b, c, t, 1); og_c2r(g, t, r + 0 * nr);
+  og_rotor3(g, a, c, t, 2); og_c2r(g, t, r + 1 * nr);
+  og_rotor3(g, a, b, t, 3); og_c2r(g, t, r + 2 * nr);
+  og_c2r(g, a, r + 3 * nr);
+  og_c2r(g, b, r + 4 * nr);
+  og_c2r(g, c, r + 5 * nr);
+  cross_real(g, r, r + nr, r + 2 * nr, r + 3 * nr, r + 4 * nr, r + 5 * nr, r + 6 * nr);
+  og_r2c(g, r + 6 * nr, d);
+  og_r2c(g, r + 2 * nr, e);
+  og_r2c(g, r + 0 * nr, f);
+  free(t);
+  free(r);
+}
+
+/* nonlhd3 (:399-509): g = -a_dir + k_dir (k.a)/k^2, with g = -a_dir on the
+ * k_dir = 0 plane (index 1 in GHOST) */
+void og_nonlhd3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, ocplx *gout,
+                int dir) {
+  OMP_FOR
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        size_t m = IDX(g, i, j, k);
+        ocplx s = dir == 1 ? a[m] : (dir == 2 ? b[m] : c[m]);
+        int plane = dir == 1 ? i : (dir == 2 ? j : k);
+        if (plane == 0) {
+          gout[m] = -s;
+        } else {
+          double kd = dir == 1 ? g->kx[i] : (dir == 2 ? g->ky[j] : g->kz[k]);
+          ocplx dot = g->kx[i] * a[m] + g->ky[j] * b[m] + g->kz[k] * c[m];
+          gout[m] = -s + kd * dot / g->kk2[m];
+        }
+      }
+}
+
+/* nonlin3 (mhd :108-227): g = -a_dir + d_dir + k_dir (k.(A-D))/k^2 */
+void og_nonlin3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                const ocplx *e, const ocplx *f, ocplx *gout, int dir) {
+  OMP_FOR
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        size_t m = IDX(g, i, j, k);
+        ocplx s = dir == 1 ? a[m] : (dir == 2 ? b[m] : c[m]);
+        ocplx sd = dir == 1 ? d[m] : (dir == 2 ? e[m] : f[m]);
+        int plane = dir == 1 ? i : (dir == 2 ? j : k);
+        if (plane == 0) {
+          gout[m] = -s + sd;
+        } else {
+          double kd = dir == 1 ? g->kx[i] : (dir == 2 ? g->ky[j] : g->kz[k]);
+          ocplx dot = g->kx[i] * (a[m] - d[m]) + g->ky[j] * (b[m] - e[m]) + g->kz[k] * (c[m] - f[m]);
+          gout[m] = -s + sd + kd * dot / g->kk2[m];
+        }
+      }
+}
+
+/* gauge3 (mhd :230-336): g = a_dir - k_dir (k.a)/k^2 */
+void og_gauge3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, ocplx *gout,
+               int dir) {
+  OMP_FOR
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        size_t m = IDX(g, i, j, k);
+        ocplx s = dir == 1 ? a[m] : (dir == 2 ? b[m] : c[m]);
+        int plane = dir == 1 ? i : (dir == 2 ? j : k);
+        if (plane == 0) {
+          gout[m] = s;
+        } else {
+          double kd = dir == 1 ? g->kx[i] : (dir == 2 ? g->ky[j] : g->kz[k]);
+          ocplx dot = g->kx[i] * a[m] + g->ky[j] * b[m] + g->kz[k] * c[m];
+          gout[m] = s - kd * dot / g->kk2[m];
+        }
+      }
+}
+
+/* vector3 (mhd :22-105): FFT[A x B] / N^6 */
+void og_vector3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                const ocplx *e, const ocplx *f, ocplx *x, ocplx *y, ocplx *z) {
+  const size_t nr = nreal(g);
+  double *r = (double *)malloc(sizeof(double) * nr * 7);
+  og_c2r(g, a, r + 0 * nr);
+  og_c2r(g, b, r + 1 * nr);
+  og_c2r(g, c, r + 2 * nr);
+  og_c2r(g, d, r + 3 * nr);
+  og_c2r(g, e, r + 4 * nr);
+  og_c2r(g, f, r + 5 * nr);
+  cross_real(g, r, r + nr, r + 2 * nr, r + 3 * nr, r + 4 * nr, r + 5 * nr, r + 6 * nr);
+  og_r2c(g, r + 6 * nr, x);
+  og_r2c(g, r + 2 * nr, y);
+  og_r2c(g, r + 0 * nr, z);
+  free(r);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Global quantities                                                        */
+/* ------------------------------------------------------------------------ */
+
+static inline double abs2(ocplx z) { return creal(z) * creal(z) + cimag(z) * cimag(z); }
+
+/* Hermitian-weighted sum over modes: weight 1 on the kx=0 plane, 2 elsewhere */
+#define HSUM(g, expr, out)                                                            \
+  do {                                                                                \
+    double acc_ = 0.0;                                                                \
+    _Pragma("omp parallel for reduction(+:acc_) schedule(static)")                    \
+    for (int i = 0; i < (g)->nxh; ++i) {                                              \
+      double w_ = i == 0 ? 1.0 : 2.0;                                                 \
+      for (int j = 0; j < (g)->ny; ++j)                                               \
+        for (int k = 0; k < (g)->nz; ++k) {                                           \
+          size_t m = IDX(g, i, j, k);                                                 \
+          (void)m;                                                                    \
+          acc_ += w_ * (expr);                                                        \
+        }                                                                             \
+    }                                                                                 \
+    out = acc_;                                                                       \
+  } while (0)
+
+/* energy (:512-662): kin 1 <v^2>, 0 <|curl v|^2>, 2 <|lap v|^2> */
+double og_energy(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, int kin) {
+  const double tmp = nsix(g);
+  double d;
+  if (kin == 1) {
+    HSUM(g, (abs2(a[m]) + abs2(b[m]) + abs2(c[m])) * tmp, d);
+    return d;
+  }
+  size_t nc = ncplx(g);
+  ocplx *c1 = (ocplx *)malloc(sizeof(ocplx) * nc * 3);
+  ocplx *c2 = c1 + nc, *c3 = c1 + 2 * nc;
+  if (kin == 0) {
+    og_rotor3(g, b, c, c1, 1); og_rotor3(g, a, c, c2, 2); og_rotor3(g, a, b, c3, 3);
+  } else {
+    og_laplak3(g, a, c1); og_laplak3(g, b, c2); og_laplak3(g, c, c3);
+  }
+  HSUM(g, (abs2(c1[m]) + abs2(c2[m]) + abs2(c3[m])) * tmp, d);
+  free(c1);
+  return d;
+}
+
+/* helicity (:665-784): <v . curl v> */
+double og_helicity(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c) {
+  const double tmp = nsix(g);
+  size_t nc = ncplx(g);
+  ocplx *c1 = (ocplx *)malloc(sizeof(ocplx) * nc);
+  double d = 0, t;
+  og_rotor3(g, b, c, c1, 1);
+  HSUM(g, creal(a[m] * conj(c1[m])) * tmp, t); d += t;
+  og_rotor3(g, a, c, c1, 2);
+  HSUM(g, creal(b[m] * conj(c1[m])) * tmp, t); d += t;
+  og_rotor3(g, a, b, c1, 3);
+  HSUM(g, creal(c[m] * conj(c1[m])) * tmp, t); d += t;
+  free(c1);
+  return d;
+}
+
+/* cross (:787-907): kin 1 <A.B>, kin 0 sum k^2 A.B */
+double og_cross(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                const ocplx *e, const ocplx *f, int kin) {
+  const double tmp = nsix(g);
+  double r;
+  if (kin == 1) {
+    HSUM(g, creal(a[m] * conj(d[m]) + b[m] * conj(e[m]) + c[m] * conj(f[m])) * tmp, r);
+  } else {
+    HSUM(g, creal(a[m] * conj(d[m]) + b[m] * conj(e[m]) + c[m] * conj(f[m])) * g->kk2[m] * tmp, r);
+  }
+  return r;
+}
+
+/* hdcheck divergence (:1036-1071): <(div v)^2> */
+double og_divergence(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c) {
+  const double tmq = nsix(g);
+  double r;
+  HSUM(g, abs2(ika(g->kx[i], a[m]) + ika(g->ky[j], b[m]) + ika(g->kz[k], c[m])) * tmq, r);
+  return r;
+}
+
+/* maxabs (:910-981) */
+double og_maxabs(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, int kin) {
+  size_t nc = ncplx(g), nr = nreal(g);
+  ocplx *cc = (ocplx *)malloc(sizeof(ocplx) * nc * 3);
+  double *r = (double *)malloc(sizeof(double) * nr * 3);
+  if (kin == 0) {
+    og_rotor3(g, b, c, cc, 1); og_rotor3(g, a, c, cc + nc, 2); og_rotor3(g, a, b, cc + 2 * nc, 3);
+  } else if (kin == 1) {
+    og_laplak3(g, a, cc); og_laplak3(g, b, cc + nc); og_laplak3(g, c, cc + 2 * nc);
+  } else {
+    memcpy(cc, a, sizeof(ocplx) * nc);
+    memcpy(cc + nc, b, sizeof(ocplx) * nc);
+    memcpy(cc + 2 * nc, c, sizeof(ocplx) * nc);
+  }
+  og_c2r(g, cc, r); og_c2r(g, cc + nc, r + nr); og_c2r(g, cc + 2 * nc, r + 2 * nr);
+  double dl = 0.0;
+  for (size_t m = 0; m < nr; ++m) {
+    double v = sqrt(r[m] * r[m] + r[nr + m] * r[nr + m] + r[2 * nr + m] * r[2 * nr + m]);
+    if (v > dl) dl = v;
+  }
+  free(cc); free(r);
+  return dl / ((double)g->nx * (double)g->ny * (double)g->nz);
+}
+
+/* spectrumc (:1190-1415).  The shell offset .501 is a default-real literal in
+ * GHOST, i.e. the single-precision value promoted to double. */
+void og_spectrumc(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, int kin,
+                  int hel, double *Ek, double *Hk) {
+  const double tmp = nsix(g);
+  const double off = (double)0.501f;
+  const int nb = g->nmax / 2 + 1;
+  size_t nc = ncplx(g);
+  ocplx *cu = NULL;
+  if (kin == 0 || hel == 1) {
+    cu = (ocplx *)malloc(sizeof(ocplx) * nc * 3);
+    og_rotor3(g, b, c, cu, 1); og_rotor3(g, a, c, cu + nc, 2); og_rotor3(g, a, b, cu + 2 * nc, 3);
+  }
+  if (kin <= 1 && Ek) {
+    for (int q = 0; q < nb; ++q) Ek[q] = 0.0;
+    const ocplx *x = kin == 1 ? a : cu, *y = kin == 1 ? b : cu + nc, *z = kin == 1 ? c : cu + 2 * nc;
+    for (int i = 0; i < g->nxh; ++i)
+      for (int j = 0; j < g->ny; ++j)
+        for (int k = 0; k < g->nz; ++k) {
+          size_t m = IDX(g, i, j, k);
+          int kmn = (int)(sqrt(g->kk2[m]) / g->Dkk + off);
+          if (kmn > 0 && kmn <= nb) {
+            double w = i == 0 ? 1.0 : 2.0;
+            Ek[kmn - 1] += w * (abs2(x[m]) + abs2(y[m]) + abs2(z[m])) * tmp;
+          }
+        }
+  }
+  if (hel == 1 && Hk) {
+    for (int q = 0; q < nb; ++q) Hk[q] = 0.0;
+    for (int i = 0; i < g->nxh; ++i)
+      for (int j = 0; j < g->ny; ++j)
+        for (int k = 0; k < g->nz; ++k) {
+          size_t m = IDX(g, i, j, k);
+          int kmn = (int)(sqrt(g->kk2[m]) / g->Dkk + off);
+          if (kmn > 0 && kmn <= nb) {
+            double w = i == 0 ? 1.0 : 2.0;
+            Hk[kmn - 1] += w * (creal(a[m] * conj(cu[m])) + creal(b[m] * conj(cu[nc + m])) +
+                                creal(c[m] * conj(cu[2 * nc + m]))) * tmp;
+          }
+        }
+  }
+  free(cu);
+}
+
+/* phaseshift (:2084-2150).  The Nyquist entries of the kx=0 plane are hit
+ * twice (by cdump and by its conjugate), as in GHOST's loops. */
+void og_phaseshift(const og_grid *g, ocplx *a, double cre, double cim) {
+  const ocplx cd = cre + I * cim, jd = conj(cd);
+  const int ny = g->ny, nz = g->nz;
+#define A0(k, j) a[IDX(g, 0, (j), (k))]
+  for (int j = 2; j <= ny / 2 + 1; ++j) {
+    A0(0, j - 1) *= cd;
+    A0(0, ny - j + 1) *= jd;
+  }
+  for (int k = 2; k <= nz / 2 + 1; ++k) {
+    A0(k - 1, 0) *= cd;
+    A0(nz - k + 1, 0) *= jd;
+  }
+  for (int j = 2; j <= ny; ++j)
+    for (int k = 2; k <= nz / 2 + 1; ++k) {
+      A0(k - 1, j - 1) *= cd;
+      A0(nz - k + 1, ny - j + 1) *= jd;
+    }
+#undef A0
+  OMP_FOR
+  for (int i = 1; i < g->nxh; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) a[IDX(g, i, j, k)] *= cd;
+}
+
+/* normalize (module_dns.f90:13-41): f = f*f0/sqrt(energy(f,kin)) */
+void og_normalize(const og_grid *g, ocplx *fx, ocplx *fy, ocplx *fz, double f0, int kin) {
+  double tmp = og_energy(g, fx, fy, fz, kin);
+  double s = sqrt(tmp);
+  const size_t n = ncplx(g);
+  OMP_FOR
+  for (size_t m = 0; m < n; ++m) {
+    fx[m] = fx[m] * f0 / s;
+    fy[m] = fy[m] * f0 / s;
+    fz[m] = fz[m] * f0 / s;
+  }
+}
+
+/* randu (pseudospec3D_mod.fpp:130-148): Park-Miller minimal standard with
+ * Schrage factorisation and an XOR mask; am = 1./im is a default-real
+ * (single precision) constant. */
+double og_randu(int32_t *idum) {
+  const int32_t iq = 127773, ir = 2836, mask = 123459876, ia = 16807, im = 2147483647;
+  const double am = (double)(1.0f / (float)im);
+  int32_t s = *idum ^ mask;
+  int32_t k = s / iq;
+  s = ia * (s - k * iq) - ir * k;
+  if (s < 0) s += im;
+  double r = am * (double)s;
+  r = (r - 0.5) * 2;
+  *idum = s ^ mask;
+  return r;
+}
+
+/* sharp (pseudospec3D_filt.f90:200-241) */
+void og_sharp(const og_grid *g, const ocplx *a, ocplx *b, double delta) {
+  const double pi = 3.14159265358979323846;
+  const double tmp = (pi / delta) * (pi / delta);
+  const size_t n = ncplx(g);
+  OMP_FOR
+  for (size_t m = 0; m < n; ++m) b[m] = g->kn2[m] <= tmp ? a[m] : 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Initial conditions and forcing                                           */
+/* ------------------------------------------------------------------------ */
+
+/* examples/initialv.f90_tg:22-61 */
+void og_initialv_tg(const og_grid *g, double kdn, double kup, double u0, ocplx *vx, ocplx *vy,
+                    ocplx *vz) {
+  const double pi = 3.14159265358979323846;
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  size_t nr = nreal(g);
+  double *R1 = (double *)malloc(sizeof(double) * nr);
+  double *R2 = (double *)malloc(sizeof(double) * nr);
+  OMP_FOR
+  for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        double r1 = 0.0, r2 = 0.0;
+        for (int ki = (int)kdn; ki <= (int)kup; ++ki) {
+          double ax = 2 * pi * ki * ((double)i - 1) / (double)nx;
+          double ay = 2 * pi * ki * ((double)j - 1) / (double)ny;
+          double az = 2 * pi * ki * ((double)k - 1) / (double)nz;
+          r1 = r1 + sin(ax) * cos(ay) * cos(az);
+          r2 = r2 - cos(ax) * sin(ay) * cos(az);
+        }
+        R1[RIDX(g, i - 1, j - 1, k - 1)] = r1;
+        R2[RIDX(g, i - 1, j - 1, k - 1)] = r2;
+      }
+  memset(vz, 0, sizeof(ocplx) * ncplx(g));
+  og_r2c(g, R1, vx);
+  og_r2c(g, R2, vy);
+  free(R1);
+  free(R2);
+  og_normalize(g, vx, vy, vz, u0, 1);
+}
+
+/* Random-phase fill of three complex fields in the loop and Hermitian-mirror
+ * order of examples/initialfv.f90_random:16-135.  amp(kk2) gives the modulus
+ * for kx=0 modes (kx>0 modes get 2*amp), inband(m) selects the modes drawn. */
+typedef double (*amp_fn)(double kk2, const void *ctx);
+typedef int (*band_fn)(const og_grid *g, size_t m, const void *ctx);
+
+static void random_fill(const og_grid *g, amp_fn amp, band_fn inband, const void *ctx,
+                        int32_t *seed, ocplx *C1, ocplx *C2, ocplx *C3) {
+  const double pi = 3.14159265358979323846;
+  const int ny = g->ny, nz = g->nz;
+#define DRAW(dst, m, fac, dump)                                                   \
+  do {                                                                            \
+    double phase = 2 * pi * og_randu(seed);                                       \
+    (dst)[m] = (fac) * (cos(phase) + I * sin(phase)) * (dump);                    \
+  } while (0)
+#define M0(k, j) IDX(g, 0, (j), (k))
+  C1[M0(0, 0)] = 0; C2[M0(0, 0)] = 0; C3[M0(0, 0)] = 0;
+  for (int j = 2; j <= ny / 2 + 1; ++j) {
+    size_t m = M0(0, j - 1), mm = M0(0, ny - j + 1);
+    if (inband(g, m, ctx)) {
+      double dump = amp(g->kk2[m], ctx);
+      DRAW(C1, m, 1.0, dump); C1[mm] = conj(C1[m]);
+      DRAW(C2, m, 1.0, dump); C2[mm] = conj(C2[m]);
+      DRAW(C3, m, 1.0, dump); C3[mm] = conj(C3[m]);
+    } else {
+      C1[m] = 0; C1[mm] = 0; C2[m] = 0; C2[mm] = 0; C3[m] = 0; C3[mm] = 0;
+    }
+  }
+  for (int k = 2; k <= nz / 2 + 1; ++k) {
+    size_t m = M0(k - 1, 0), mm = M0(nz - k + 1, 0);
+    if (inband(g, m, ctx)) {
+      double dump = amp(g->kk2[m], ctx);
+      DRAW(C1, m, 1.0, dump); C1[mm] = conj(C1[m]);
+      DRAW(C2, m, 1.0, dump); C2[mm] = conj(C2[m]);
+      DRAW(C3, m, 1.0, dump); C3[mm] = conj(C3[m]);
+    } else {
+      C1[m] = 0; C1[mm] = 0; C2[m] = 0; C2[mm] = 0; C3[m] = 0; C3[mm] = 0;
+    }
+  }
+  for (int j = 2; j <= ny; ++j)
+    for (int k = 2; k <= nz / 2 + 1; ++k) {
+      size_t m = M0(k - 1, j - 1), mm = M0(nz - k + 1, ny - j + 1);
+      if (inband(g, m, ctx)) {
+        double dump = amp(g->kk2[m], ctx);
+        DRAW(C1, m, 1.0, dump); C1[mm] = conj(C1[m]);
+        DRAW(C2, m, 1.0, dump); C2[mm] = conj(C2[m]);
+        DRAW(C3, m, 1.0, dump); C3[mm] = conj(C3[m]);
+      } else {
+        C1[m] = 0; C1[mm] = 0; C2[m] = 0; C2[mm] = 0; C3[m] = 0; C3[mm] = 0;
+      }
+    }
+  for (int i = 1; i < g->nxh; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        size_t m = IDX(g, i, j, k);
+        if (inband(g, m, ctx)) {
+          double dump = amp(g->kk2[m], ctx);
+          DRAW(C1, m, 2, dump);
+          DRAW(C2, m, 2, dump);
+          DRAW(C3, m, 2, dump);
+        } else {
+          C1[m] = 0; C2[m] = 0; C3[m] = 0;
+        }
+      }
+#undef M0
+#undef DRAW
+}
+
+typedef struct { double kdn, kup, k0; } bandctx;
+static double amp_invk(double kk2, const void *ctx) { (void)ctx; return 1. / sqrt(kk2); }
+static int band_shell(const og_grid *g, size_t m, const void *ctx) {
+  const bandctx *b = (const bandctx *)ctx;
+  return g->kk2[m] <= b->kup * b->kup && g->kk2[m] >= b->kdn * b->kdn;
+}
+static double amp_gauss(double kk2, const void *ctx) {
+  const bandctx *b = (const bandctx *)ctx;
+  return exp(-kk2 / (b->k0 * b->k0));
+}
+static int band_dealiased(const og_grid *g, size_t m, const void *ctx) {
+  (void)ctx;
+  return g->kn2[m] <= g->kmax && g->kn2[m] >= g->tiny;
+}
+
+static void curl_normalize(const og_grid *g, ocplx *C, double f0, ocplx *fx, ocplx *fy,
+                           ocplx *fz) {
+  size_t nc = ncplx(g);
+  og_rotor3(g, C + nc, C + 2 * nc, fx, 1);
+  og_rotor3(g, C, C + 2 * nc, fy, 2);
+  og_rotor3(g, C, C + nc, fz, 3);
+  og_normalize(g, fx, fy, fz, f0, 1);
+}
+
+/* examples/initialfv.f90_random:16-139 */
+void og_initialfv_random(const og_grid *g, double kdn, double kup, double f0, int32_t *seed,
+                         ocplx *fx, ocplx *fy, ocplx *fz) {
+  size_t nc = ncplx(g);
+  ocplx *C = (ocplx *)calloc(nc * 3, sizeof(ocplx));
+  bandctx b = {kdn, kup, 0};
+  random_fill(g, amp_invk, band_shell, &b, seed, C, C + nc, C + 2 * nc);
+  curl_normalize(g, C, f0, fx, fy, fz);
+  free(C);
+}
+
+/* Random-phase solenoidal velocity with shell spectrum E(k) ~ k^4 exp(-2 (k/k0)^2):
+ * a vector potential of modulus exp(-(k/k0)^2) on every dealiased mode, filled
+ * in the order of examples/initialfv.f90_random, then curl and normalize. */
+void og_initialv_k4exp(const og_grid *g, double k0, double u0, int32_t *seed, ocplx *vx,
+                       ocplx *vy, ocplx *vz) {
+  size_t nc = ncplx(g);
+  ocplx *C = (ocplx *)calloc(nc * 3, sizeof(ocplx));
+  bandctx b = {0, 0, k0};
+  random_fill(g, amp_gauss, band_dealiased, &b, seed, C, C + nc, C + 2 * nc);
+  curl_normalize(g, C, u0, vx, vy, vz);
+  free(C);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Runge-Kutta steps                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* include/hd_rkstep2.f90:3-32 for one value of o */
+void og_hd_substep(const og_grid *g, int o, double dt, double nu, ocplx *vx, ocplx *vy, ocplx *vz,
+                   const ocplx *c1, const ocplx *c2, const ocplx *c3, const ocplx *fx,
+                   const ocplx *fy, const ocplx *fz) {
+  size_t nc = ncplx(g);
+  ocplx *W = (ocplx *)malloc(sizeof(ocplx) * nc * 5);
+  ocplx *C4 = W, *C5 = W + nc, *C6 = W + 2 * nc, *C7 = W + 3 * nc, *C8 = W + 4 * nc;
+  og_prodre3(g, vx, vy, vz, C4, C5, C6);
+  og_nonlhd3(g, C4, C5, C6, C7, 1);
+  og_nonlhd3(g, C4, C5, C6, C8, 2);
+  og_nonlhd3(g, C4, C5, C6, C4, 3);
+  og_laplak3(g, vx, vx);
+  og_laplak3(g, vy, vy);
+  og_laplak3(g, vz, vz);
+  const double rmp = 1. / (double)o;
+  OMP_FOR
+  for (size_t m = 0; m < nc; ++m) {
+    if (g->kn2[m] <= g->kmax && g->kn2[m] >= g->tiny) {
+      ocplx f1 = fx ? fx[m] : 0, f2 = fy ? fy[m] : 0, f3 = fz ? fz[m] : 0;
+      vx[m] = c1[m] + dt * (nu * vx[m] + C7[m] + f1) * rmp;
+      vy[m] = c2[m] + dt * (nu * vy[m] + C8[m] + f2) * rmp;
+      vz[m] = c3[m] + dt * (nu * vz[m] + C4[m] + f3) * rmp;
+    } else {
+      vx[m] = 0; vy[m] = 0; vz[m] = 0;
+    }
+  }
+  free(W);
+}
+
+/* main3D.fpp:3202-3231 with include/hd_rkstep1.f90 */
+void og_hd_step(const og_grid *g, int ord, double dt, double nu, ocplx *vx, ocplx *vy, ocplx *vz,
+                const ocplx *fx, const ocplx *fy, const ocplx *fz) {
+  size_t nc = ncplx(g);
+  ocplx *C = (ocplx *)malloc(sizeof(ocplx) * nc * 3);
+  memcpy(C, vx, sizeof(ocplx) * nc);
+  memcpy(C + nc, vy, sizeof(ocplx) * nc);
+  memcpy(C + 2 * nc, vz, sizeof(ocplx) * nc);
+  for (int o = ord; o >= 1; --o)
+    og_hd_substep(g, o, dt, nu, vx, vy, vz, C, C + nc, C + 2 * nc, fx, fy, fz);
+  free(C);
+}
+
+/* main3D.fpp:3202-3231 with include/mhd_rkstep1.f90 and mhd_rkstep2.f90:4-65 */
+void og_mhd_step(const og_grid *g, int ord, double dt, double nu, double mu, ocplx *vx, ocplx *vy,
+                 ocplx *vz, ocplx *ax, ocplx *ay, ocplx *az, const ocplx *fx, const ocplx *fy,
+                 const ocplx *fz, const ocplx *mx, const ocplx *my, const ocplx *mz) {
+  size_t nc = ncplx(g);
+  ocplx *W = (ocplx *)malloc(sizeof(ocplx) * nc * 17);
+  ocplx *C[18];
+  for (int q = 1; q <= 17; ++q) C[q] = W + (size_t)(q - 1) * nc;
+  memcpy(C[1], vx, sizeof(ocplx) * nc); memcpy(C[2], vy, sizeof(ocplx) * nc);
+  memcpy(C[3], vz, sizeof(ocplx) * nc); memcpy(C[4], ax, sizeof(ocplx) * nc);
+  memcpy(C[5], ay, sizeof(ocplx) * nc); memcpy(C[6], az, sizeof(ocplx) * nc);
+  for (int o = ord; o >= 1; --o) {
+    og_rotor3(g, ay, az, C[7], 1);
+    og_rotor3(g, ax, az, C[8], 2);
+    og_rotor3(g, ax, ay, C[9], 3);
+    og_prodre3(g, vx, vy, vz, C[10], C[11], C[12]);
+    og_prodre3(g, C[7], C[8], C[9], C[13], C[14], C[15]);
+    og_nonlin3(g, C[10], C[11], C[12], C[13], C[14], C[15], C[16], 1);
+    og_nonlin3(g, C[10], C[11], C[12], C[13], C[14], C[15], C[17], 2);
+    og_nonlin3(g, C[10], C[11], C[12], C[13], C[14], C[15], C[10], 3);
+    og_vector3(g, vx, vy, vz, C[7], C[8], C[9], C[11], C[12], C[13]);
+    og_gauge3(g, C[11], C[12], C[13], C[7], 1);
+    og_gauge3(g, C[11], C[12], C[13], C[8], 2);
+    og_gauge3(g, C[11], C[12], C[13], C[9], 3);
+    og_laplak3(g, vx, vx); og_laplak3(g, vy, vy); og_laplak3(g, vz, vz);
+    og_laplak3(g, ax, ax); og_laplak3(g, ay, ay); og_laplak3(g, az, az);
+    const double rmp = 1. / (double)o;
+    OMP_FOR
+    for (size_t m = 0; m < nc; ++m) {
+      if (g->kn2[m] <= g->kmax && g->kn2[m] >= g->tiny) {
+        ocplx f1 = fx ? fx[m] : 0, f2 = fy ? fy[m] : 0, f3 = fz ? fz[m] : 0;
+        ocplx m1 = mx ? mx[m] : 0, m2 = my ? my[m] : 0, m3 = mz ? mz[m] : 0;
+        vx[m] = C[1][m] + dt * (nu * vx[m] + C[16][m] + f1) * rmp;
+        vy[m] = C[2][m] + dt * (nu * vy[m] + C[17][m] + f2) * rmp;
+        vz[m] = C[3][m] + dt * (nu * vz[m] + C[10][m] + f3) * rmp;
+        ax[m] = C[4][m] + dt * (mu * ax[m] + C[7][m] + m1) * rmp;
+        ay[m] = C[5][m] + dt * (mu * ay[m] + C[8][m] + m2) * rmp;
+        az[m] = C[6][m] + dt * (mu * az[m] + C[9][m] + m3) * rmp;
+      } else {
+        vx[m] = 0; vy[m] = 0; vz[m] = 0; ax[m] = 0; ay[m] = 0; az[m] = 0;
+      }
+    }
+  }
+  free(W);
+}

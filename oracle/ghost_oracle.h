@@ -1,0 +1,112 @@
+/*
+ * ghost_oracle.h -- CPU restatement of GHOST's pseudospectral hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * implementation is compared against, and the timed CPU baseline.  Only the
+ * test-suite, the smoke check and the CPU-baseline leg of the benchmark may
+ * load it; the product library never links or calls it.
+ *
+ * Layouts follow GHOST exactly (reference 3D/src/fftp-3/fftp3D.fpp:204-431):
+ *   real     r[nz][ny][nx]          (Fortran R(nx,ny,nz), x fastest)
+ *   spectral c[nx/2+1][ny][nz]      (Fortran C(nz,ny,nx/2+1), kz fastest)
+ * Transforms are unnormalised, forward sign -1, inverse sign +1.
+ */
+#ifndef GHOST_ORACLE_H
+#define GHOST_ORACLE_H
+
+#include <complex.h>
+#include <stdint.h>
+
+typedef double _Complex ocplx;
+
+typedef struct og_grid {
+  int nx, ny, nz;          /* grid points */
+  int nxh;                 /* nx/2+1 */
+  int anis;                /* 0: cubic, 1: anisotropic/arbitrary box */
+  double Lx, Ly, Lz;       /* box lengths in units of 2*pi */
+  double Dkx, Dky, Dkz, Dkk;
+  double kmax, tiny;       /* dealias bounds on kn2 */
+  int nmax;                /* spectrum uses bins 1..nmax/2+1 */
+  double *kx, *ky, *kz;    /* wavenumbers (with box scaling), sizes nx,ny,nz */
+  double *kn2;             /* [nxh][ny][nz] dimensionless, for dealiasing */
+  double *kk2;             /* [nxh][ny][nz] physical k^2 (== kn2 when cubic) */
+  void *fft;               /* private FFT plans */
+} og_grid;
+
+/* grid / wavenumbers: main3D.fpp:737-749, 881-909, 1727-1800 */
+og_grid *og_grid_new(int nx, int ny, int nz, double Lx, double Ly, double Lz,
+                     int arbsize, double Dkk);
+void og_grid_free(og_grid *g);
+void og_set_threads(int n);
+int og_get_threads(void);
+
+/* fftp3d_real_to_complex / fftp3d_complex_to_real (fftp3D.fpp:204-431).
+ * og_c2r preserves its input (GHOST's destroys it). */
+void og_r2c(const og_grid *g, const double *in, ocplx *out);
+void og_c2r(const og_grid *g, const ocplx *in, double *out);
+/* 1D complex transform of length n (sign -1 forward, +1 inverse), unnormalised */
+void og_fft1d(int n, ocplx *x, int sign);
+
+/* pseudospec3D_hd.f90 */
+void og_derivk3(const og_grid *g, const ocplx *a, ocplx *b, int dir);          /* :22  */
+void og_laplak3(const og_grid *g, const ocplx *a, ocplx *b);                   /* :91  */
+void og_rotor3(const og_grid *g, const ocplx *a, const ocplx *b, ocplx *c, int dir); /* :124 */
+void og_prodre3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c,
+                ocplx *d, ocplx *e, ocplx *f);                                 /* :316 */
+void og_nonlhd3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c,
+                ocplx *gout, int dir);                                         /* :399 */
+double og_energy(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, int kin); /* :512 */
+double og_helicity(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c);        /* :665 */
+double og_cross(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c,
+                const ocplx *d, const ocplx *e, const ocplx *f, int kin);      /* :787 */
+double og_maxabs(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, int kin); /* :910 */
+/* spectrumc (:1190): Ek,Hk have nmax/2+1 entries; kin in {0,1,2}, hel in {0,1} */
+void og_spectrumc(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c,
+                  int kin, int hel, double *Ek, double *Hk);
+/* divergence term of hdcheck (:1036-1071) */
+double og_divergence(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c);
+void og_phaseshift(const og_grid *g, ocplx *a, double cre, double cim);       /* :2084 */
+
+/* module_dns.f90:13-41 */
+void og_normalize(const og_grid *g, ocplx *fx, ocplx *fy, ocplx *fz, double f0, int kin);
+
+/* pseudospec3D_mod.fpp:130-148 */
+double og_randu(int32_t *idum);
+
+/* IC / forcing generators (examples/initialv.f90_tg, examples/initialfv.f90_random,
+ * and the k^4 exp(-2(k/k0)^2) random-phase field in the same fill order) */
+void og_initialv_tg(const og_grid *g, double kdn, double kup, double u0,
+                    ocplx *vx, ocplx *vy, ocplx *vz);
+void og_initialfv_random(const og_grid *g, double kdn, double kup, double f0,
+                         int32_t *seed, ocplx *fx, ocplx *fy, ocplx *fz);
+void og_initialv_k4exp(const og_grid *g, double k0, double u0, int32_t *seed,
+                       ocplx *vx, ocplx *vy, ocplx *vz);
+
+/* HD step (main3D.fpp:3202-3231, include/hd_rkstep1.f90, hd_rkstep2.f90).
+ * fx..fz may be NULL (no forcing). ord = RK order. */
+void og_hd_step(const og_grid *g, int ord, double dt, double nu,
+                ocplx *vx, ocplx *vy, ocplx *vz,
+                const ocplx *fx, const ocplx *fy, const ocplx *fz);
+/* one substep o of that loop, given C = state at step start */
+void og_hd_substep(const og_grid *g, int o, double dt, double nu,
+                   ocplx *vx, ocplx *vy, ocplx *vz,
+                   const ocplx *c1, const ocplx *c2, const ocplx *c3,
+                   const ocplx *fx, const ocplx *fy, const ocplx *fz);
+
+/* MHD (pseudospec3D_mhd.f90, include/mhd_rkstep2.f90) */
+void og_vector3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c,
+                const ocplx *d, const ocplx *e, const ocplx *f,
+                ocplx *x, ocplx *y, ocplx *z);                                  /* :22  */
+void og_nonlin3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c,
+                const ocplx *d, const ocplx *e, const ocplx *f, ocplx *gout, int dir); /* :108 */
+void og_gauge3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c,
+               ocplx *gout, int dir);                                          /* :230 */
+void og_mhd_step(const og_grid *g, int ord, double dt, double nu, double mu,
+                 ocplx *vx, ocplx *vy, ocplx *vz, ocplx *ax, ocplx *ay, ocplx *az,
+                 const ocplx *fx, const ocplx *fy, const ocplx *fz,
+                 const ocplx *mx, const ocplx *my, const ocplx *mz);
+
+/* pseudospec3D_filt.f90:200-241 */
+void og_sharp(const og_grid *g, const ocplx *a, ocplx *b, double delta);
+
+#endif

@@ -1,0 +1,275 @@
+"""ctypes binding of the CPU oracle (``libghost_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by the test-suite, the smoke check and the
+CPU-baseline leg of the benchmark.  The product package never imports it.
+
+Array conventions (GHOST's, reference 3D/src/fftp-3/fftp3D.fpp:204-431):
+  real fields     numpy float64    shape (nz, ny, nx)        == Fortran R(nx,ny,nz)
+  spectral fields numpy complex128 shape (nx//2+1, ny, nz)   == Fortran C(nz,ny,nx/2+1)
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libghost_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class _Grid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nxh", C.c_int), ("anis", C.c_int),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+        ("Dkx", C.c_double), ("Dky", C.c_double), ("Dkz", C.c_double), ("Dkk", C.c_double),
+        ("kmax", C.c_double), ("tiny", C.c_double), ("nmax", C.c_int),
+        ("kx", C.POINTER(C.c_double)), ("ky", C.POINTER(C.c_double)), ("kz", C.POINTER(C.c_double)),
+        ("kn2", C.POINTER(C.c_double)), ("kk2", C.POINTER(C.c_double)), ("fft", C.c_void_p),
+    ]
+
+
+_P = C.c_void_p
+_G = C.POINTER(_Grid)
+
+
+def _declare(L):
+    L.og_grid_new.restype = _G
+    L.og_grid_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                              C.c_int, C.c_double]
+    L.og_grid_free.argtypes = [_G]
+    L.og_set_threads.argtypes = [C.c_int]
+    L.og_get_threads.restype = C.c_int
+    L.og_r2c.argtypes = [_G, _P, _P]
+    L.og_c2r.argtypes = [_G, _P, _P]
+    L.og_fft1d.argtypes = [C.c_int, _P, C.c_int]
+    L.og_derivk3.argtypes = [_G, _P, _P, C.c_int]
+    L.og_laplak3.argtypes = [_G, _P, _P]
+    L.og_rotor3.argtypes = [_G, _P, _P, _P, C.c_int]
+    L.og_prodre3.argtypes = [_G, _P, _P, _P, _P, _P, _P]
+    L.og_nonlhd3.argtypes = [_G, _P, _P, _P, _P, C.c_int]
+    L.og_nonlin3.argtypes = [_G, _P, _P, _P, _P, _P, _P, _P, C.c_int]
+    L.og_gauge3.argtypes = [_G, _P, _P, _P, _P, C.c_int]
+    L.og_vector3.argtypes = [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    for name in ("og_energy", "og_maxabs"):
+        getattr(L, name).restype = C.c_double
+        getattr(L, name).argtypes = [_G, _P, _P, _P, C.c_int]
+    L.og_helicity.restype = C.c_double
+    L.og_helicity.argtypes = [_G, _P, _P, _P]
+    L.og_divergence.restype = C.c_double
+    L.og_divergence.argtypes = [_G, _P, _P, _P]
+    L.og_cross.restype = C.c_double
+    L.og_cross.argtypes = [_G, _P, _P, _P, _P, _P, _P, C.c_int]
+    L.og_spectrumc.argtypes = [_G, _P, _P, _P, C.c_int, C.c_int, _P, _P]
+    L.og_phaseshift.argtypes = [_G, _P, C.c_double, C.c_double]
+    L.og_normalize.argtypes = [_G, _P, _P, _P, C.c_double, C.c_int]
+    L.og_randu.restype = C.c_double
+    L.og_randu.argtypes = [C.POINTER(C.c_int32)]
+    L.og_initialv_tg.argtypes = [_G, C.c_double, C.c_double, C.c_double, _P, _P, _P]
+    L.og_initialfv_random.argtypes = [_G, C.c_double, C.c_double, C.c_double,
+                                      C.POINTER(C.c_int32), _P, _P, _P]
+    L.og_initialv_k4exp.argtypes = [_G, C.c_double, C.c_double, C.POINTER(C.c_int32), _P, _P, _P]
+    L.og_hd_step.argtypes = [_G, C.c_int, C.c_double, C.c_double, _P, _P, _P, _P, _P, _P]
+    L.og_hd_substep.argtypes = [_G, C.c_int, C.c_double, C.c_double, _P, _P, _P, _P, _P, _P,
+                                _P, _P, _P]
+    L.og_mhd_step.argtypes = [_G, C.c_int, C.c_double, C.c_double, C.c_double] + [_P] * 12
+    L.og_sharp.argtypes = [_G, _P, _P, C.c_double]
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def set_threads(n):
+    lib().og_set_threads(int(n))
+
+
+def get_threads():
+    return lib().og_get_threads()
+
+
+def randu(seed):
+    """Draw one value; returns (value, new_seed)."""
+    s = C.c_int32(seed)
+    v = lib().og_randu(C.byref(s))
+    return v, s.value
+
+
+def fft1d(x, sign):
+    x = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().og_fft1d(x.size, _p(x), int(sign))
+    return x
+
+
+class Grid:
+    """GHOST grid: wavenumbers, dealias bounds and FFT plans (main3D.fpp:1727-1800)."""
+
+    def __init__(self, nx, ny=None, nz=None, L=(1.0, 1.0, 1.0), arbsize=False, Dkk=0.0):
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        self._g = lib().og_grid_new(nx, ny, nz, L[0], L[1], L[2], int(bool(arbsize)), Dkk)
+        g = self._g.contents
+        self.nx, self.ny, self.nz, self.nxh = g.nx, g.ny, g.nz, g.nxh
+        self.kmax, self.tiny, self.nmax, self.Dkk = g.kmax, g.tiny, g.nmax, g.Dkk
+        self.anis = g.anis
+        self.kx = np.ctypeslib.as_array(g.kx, shape=(nx,)).copy()
+        self.ky = np.ctypeslib.as_array(g.ky, shape=(ny,)).copy()
+        self.kz = np.ctypeslib.as_array(g.kz, shape=(nz,)).copy()
+        self.kn2 = np.ctypeslib.as_array(g.kn2, shape=(self.nxh, ny, nz)).copy()
+        self.kk2 = np.ctypeslib.as_array(g.kk2, shape=(self.nxh, ny, nz)).copy()
+
+    def __del__(self):
+        try:
+            lib().og_grid_free(self._g)
+        except Exception:
+            pass
+
+    # shapes / allocation
+    @property
+    def cshape(self):
+        return (self.nxh, self.ny, self.nz)
+
+    @property
+    def rshape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def czeros(self, n=None):
+        return np.zeros(self.cshape if n is None else (n,) + self.cshape, np.complex128)
+
+    def rzeros(self):
+        return np.zeros(self.rshape, np.float64)
+
+    @property
+    def nbins(self):
+        return self.nmax // 2 + 1
+
+    # FFT
+    def r2c(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = self.czeros()
+        lib().og_r2c(self._g, _p(r), _p(out))
+        return out
+
+    def c2r(self, c):
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        out = self.rzeros()
+        lib().og_c2r(self._g, _p(c), _p(out))
+        return out
+
+    # operators
+    def derivk3(self, a, d):
+        b = self.czeros(); lib().og_derivk3(self._g, _p(a), _p(b), d); return b
+
+    def laplak3(self, a):
+        b = self.czeros(); lib().og_laplak3(self._g, _p(a), _p(b)); return b
+
+    def rotor3(self, a, b, d):
+        c = self.czeros(); lib().og_rotor3(self._g, _p(a), _p(b), _p(c), d); return c
+
+    def curl(self, vx, vy, vz):
+        return self.rotor3(vy, vz, 1), self.rotor3(vx, vz, 2), self.rotor3(vx, vy, 3)
+
+    def prodre3(self, a, b, c):
+        d, e, f = self.czeros(), self.czeros(), self.czeros()
+        lib().og_prodre3(self._g, _p(a), _p(b), _p(c), _p(d), _p(e), _p(f))
+        return d, e, f
+
+    def nonlhd3(self, a, b, c, d):
+        g = self.czeros(); lib().og_nonlhd3(self._g, _p(a), _p(b), _p(c), _p(g), d); return g
+
+    def nonlin3(self, a, b, c, d, e, f, dr):
+        g = self.czeros()
+        lib().og_nonlin3(self._g, _p(a), _p(b), _p(c), _p(d), _p(e), _p(f), _p(g), dr)
+        return g
+
+    def gauge3(self, a, b, c, d):
+        g = self.czeros(); lib().og_gauge3(self._g, _p(a), _p(b), _p(c), _p(g), d); return g
+
+    def vector3(self, a, b, c, d, e, f):
+        x, y, z = self.czeros(), self.czeros(), self.czeros()
+        lib().og_vector3(self._g, *map(_p, (a, b, c, d, e, f, x, y, z)))
+        return x, y, z
+
+    def energy(self, a, b, c, kin=1):
+        return lib().og_energy(self._g, _p(a), _p(b), _p(c), kin)
+
+    def helicity(self, a, b, c):
+        return lib().og_helicity(self._g, _p(a), _p(b), _p(c))
+
+    def cross(self, a, b, c, d, e, f, kin=1):
+        return lib().og_cross(self._g, *map(_p, (a, b, c, d, e, f)), kin)
+
+    def divergence(self, a, b, c):
+        return lib().og_divergence(self._g, _p(a), _p(b), _p(c))
+
+    def maxabs(self, a, b, c, kin):
+        return lib().og_maxabs(self._g, _p(a), _p(b), _p(c), kin)
+
+    def spectrumc(self, a, b, c, kin=1, hel=0):
+        Ek = np.zeros(self.nbins)
+        Hk = np.zeros(self.nbins)
+        lib().og_spectrumc(self._g, _p(a), _p(b), _p(c), kin, hel, _p(Ek), _p(Hk))
+        return Ek, Hk
+
+    def phaseshift(self, a, phase_c):
+        a = a.copy()
+        lib().og_phaseshift(self._g, _p(a), float(np.real(phase_c)), float(np.imag(phase_c)))
+        return a
+
+    def normalize(self, fx, fy, fz, f0, kin=1):
+        fx, fy, fz = fx.copy(), fy.copy(), fz.copy()
+        lib().og_normalize(self._g, _p(fx), _p(fy), _p(fz), f0, kin)
+        return fx, fy, fz
+
+    def sharp(self, a, delta):
+        b = self.czeros(); lib().og_sharp(self._g, _p(a), _p(b), delta); return b
+
+    # initial conditions / forcing
+    def initialv_tg(self, kdn=1.0, kup=1.0, u0=0.5):
+        v = [self.czeros() for _ in range(3)]
+        lib().og_initialv_tg(self._g, kdn, kup, u0, *map(_p, v))
+        return v
+
+    def initialfv_random(self, kdn, kup, f0, seed):
+        s = C.c_int32(seed)
+        f = [self.czeros() for _ in range(3)]
+        lib().og_initialfv_random(self._g, kdn, kup, f0, C.byref(s), *map(_p, f))
+        return f, s.value
+
+    def initialv_k4exp(self, k0, u0, seed):
+        s = C.c_int32(seed)
+        v = [self.czeros() for _ in range(3)]
+        lib().og_initialv_k4exp(self._g, k0, u0, C.byref(s), *map(_p, v))
+        return v, s.value
+
+    # time stepping
+    def hd_step(self, v, dt, nu, f=None, ord=2):
+        """One RK step in place on the list v=[vx,vy,vz]."""
+        fp = [None] * 3 if f is None else list(map(_p, f))
+        lib().og_hd_step(self._g, ord, dt, nu, *map(_p, v), *fp)
+
+    def hd_substep(self, v, c, o, dt, nu, f=None):
+        fp = [None] * 3 if f is None else list(map(_p, f))
+        lib().og_hd_substep(self._g, o, dt, nu, *map(_p, v), *map(_p, c), *fp)
+
+    def mhd_step(self, v, a, dt, nu, mu, f=None, m=None, ord=2):
+        fp = [None] * 3 if f is None else list(map(_p, f))
+        mp = [None] * 3 if m is None else list(map(_p, m))
+        lib().og_mhd_step(self._g, ord, dt, nu, mu, *map(_p, v), *map(_p, a), *fp, *mp)
