@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""HD3D FP64 RK2 steps/s on one MI355X (gfx950), GHOST's pseudospectral hot path.
+
+Workload (the 512^3 forced-turbulence configuration): 512^3 cubic 2*pi box,
+FP64, random-phase solenoidal initial velocity with E(k) ~ k^4 exp(-2 (k/3)^2)
+normalised to total energy 0.5, constant-amplitude random-phase forcing in the
+shell 2 <= |k| <= 3 whose phase is shifted every FSTEP steps (GHOST rand=1),
+nu = 2e-4, RK2, dt = 2.5e-4.  A step is one full RK2 step of the fused HIP
+path with all fields resident in HBM.
+
+Prints one JSON line on stdout.  Everything else goes to stderr.
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "HD3D 512³ FP64 steps/s + FFT HBM GB/s; learned-LES speedup at ≤5% spectrum err"
+HBM_PEAK_GBS = 8000.0
+NU, DT, K0, U0, F0 = 2e-4, 2.5e-4, 3.0, 1.0, 0.4
+KDN, KUP = 2.0, 3.0
+FSTEP = 20
+IC_SEED, F_SEED = 1000, 2000
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"traffic_hd{n}.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        return data.get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(n, threads):
+    """Time one RK substep of the same workload with the CPU oracle (test infrastructure)."""
+    from oracle import oracle as O
+    O.set_threads(threads)
+    og = O.Grid(n)
+    v, _ = og.initialv_k4exp(K0, U0, IC_SEED)
+    f, _ = og.initialfv_random(KDN, KUP, F0, F_SEED)
+    c = [x.copy() for x in v]
+    t0 = time.perf_counter()
+    og.hd_substep(v, c, 2, DT, NU, f=f)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / (2.0 * dt), "unit": "steps/s", "cores": int(threads), "kind": "port",
+            "sample": f"one RK2 substep (o=2: 9 FFTs + kernels) of the {n}^3 workload timed with the C/OpenMP "
+                      f"oracle on {threads} host threads ({dt:.2f} s), steps/s = 1/(2 x substep)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        log("bench.py runs on exactly one GPU: this build has no multi-GPU path (one MI355X per run); "
+            "use --gpus 1")
+        sys.exit(2)
+
+    import torch
+
+    import ghost_amd
+    from ghost_amd import _lib
+    from ghost_amd.pseudospec import Grid, randu
+    from ghost_amd.solvers import HDSolver
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+    ghost_amd.load()
+    torch.cuda.set_device(0)
+    n = args.n
+
+    g = Grid(n)
+    s = HDSolver(g, dt=DT, nu=NU, ord=2)
+    v, _ = g.initialv_k4exp(K0, U0, IC_SEED)
+    f, fseed = g.initialfv_random(KDN, KUP, F0, F_SEED)
+    s.set_velocity(v)
+    s.set_forcing(f)
+    e0 = g.energy(*v, 1)
+    del v, f
+    torch.cuda.empty_cache()
+    log(f"grid {n}^3, <u^2> = {e0:.6f} (E = {0.5 * e0:.6f})")
+
+    state = {"timef": FSTEP, "seed": fseed}
+
+    def one_step():
+        # main3D.fpp:2544-2556: phase shift of the forcing when timef == fstep
+        if state["timef"] == FSTEP:
+            state["timef"] = 0
+            r, state["seed"] = randu(state["seed"])
+            phase = 2 * math.pi * r
+            s.phaseshift_forcing(complex(math.cos(phase), math.sin(phase)))
+        s.step(1)
+        state["timef"] += 1
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+
+    _lib.prof_reset()
+    _lib.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.prof_enable(False)
+    prof = _lib.prof_read()
+    elapsed = t1 - t0
+    steps_per_s = args.steps / elapsed
+    v = s.velocity()
+    e1 = g.energy(*v, 1)
+    assert math.isfinite(e1), "energy is not finite"
+    log(f"{args.steps} steps in {elapsed:.4f} s -> {steps_per_s:.3f} steps/s; <u^2> = {e1:.6f}")
+
+    passes = {}
+    for name, (cnt, ms, by) in prof.items():
+        avg_s = ms / cnt * 1e-3
+        passes[name] = {"launches": cnt, "avg_ms": round(ms / cnt, 4),
+                        "algorithmic_GB_per_launch": round(by / cnt / 1e9, 4),
+                        "achieved_GBps": round(by / cnt / avg_s / 1e9, 1),
+                        "share_of_step": round(ms * 1e-3 / elapsed, 4)}
+        log(f"  {name:12s} x{cnt:4d}  avg {ms / cnt:8.3f} ms  {by / cnt / 1e9:7.3f} GB  "
+            f"{by / cnt / avg_s / 1e9:8.1f} GB/s")
+    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    cnt, ms, by = prof[dom]
+    achieved = by / cnt / (ms / cnt * 1e-3) / 1e9
+    traffic = read_traffic(dom, n)
+
+    # algorithmic bytes of a step (18 transforms read once + written once, and the
+    # 30 spectral field reads/writes of the two updates)
+    s_r, s_c = n ** 3 * 8.0, (n // 2 + 1) * n * n * 16.0
+    b_step = 18 * (s_r + s_c) + 30 * s_c
+
+    out = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random-phase k^4 exp IC and shell-2..3 forcing generated on the host)",
+        "config": {"workload": f"HD3D {n}^3 FP64 forced turbulence, RK2, dt={DT}, nu={NU}, k0={K0}, "
+                               f"E0=0.5, forcing f0={F0} in 2<=|k|<=3 with phase shift every {FSTEP} steps; "
+                               f"no Lagrangian tracers",
+                   "grid": [n, n, n], "ord": 2, "dt": DT, "nu": NU, "tracers": 0},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "fft_passes": passes,
+        "step_algorithmic_GB": round(b_step / 1e9, 3),
+        "step_effective_GBps": round(b_step * steps_per_s / 1e9, 1),
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: oracle substep at {n}^3 on {threads} threads ...")
+        del s, v
+        torch.cuda.empty_cache()
+        out["cpu_baseline"] = cpu_baseline(n, threads)
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.5f} steps/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -104,6 +104,9 @@ def _declare(L):
         "ghx_mhd_fields": (_I, [_P, _PP, _PP, _PP, _PP]),
         "ghx_mhd_set_forcing": (_I, [_P, _I]),
         "ghx_mhd_step": (_I, [_P, _I]),
+        "ghx_prof_enable": (_I, [_I]),
+        "ghx_prof_reset": (_I, []),
+        "ghx_prof_read": (_I, [_I, C.POINTER(C.c_char_p), C.POINTER(_I), _DP, _DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -121,3 +124,26 @@ def check(rc):
 def ptr_array(ptrs):
     arr = (C.c_void_p * len(ptrs))(*[C.c_void_p(p) for p in ptrs])
     return C.cast(arr, _PP), arr
+
+
+def prof_enable(on=True):
+    check(load().ghx_prof_enable(1 if on else 0))
+
+
+def prof_reset():
+    check(load().ghx_prof_reset())
+
+
+def prof_read():
+    """{pass name: (launches, total ms, total algorithmic bytes)} from the pass profiler."""
+    L = load()
+    out = {}
+    slot = 0
+    while True:
+        name, cnt, ms, by = C.c_char_p(), C.c_int(), C.c_double(), C.c_double()
+        if L.ghx_prof_read(slot, C.byref(name), C.byref(cnt), C.byref(ms), C.byref(by)) != 0:
+            break
+        if cnt.value:
+            out[name.value.decode()] = (cnt.value, ms.value, by.value)
+        slot += 1
+    return out

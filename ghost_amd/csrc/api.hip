@@ -85,6 +85,97 @@ extern "C" int ghx_sync(void) {
 }
 
 // ---------------------------------------------------------------------------
+// pass profiler: HIP events around each pass launch on its own stream
+// ---------------------------------------------------------------------------
+namespace {
+enum ProfSlot {
+  P_ZINV, P_ZINV_CURL, P_ZINV_MHD, P_YINV, P_YFWD, P_XINV, P_XFWD, P_XCROSS, P_ZFWD, P_ZFWD_RK_HD, P_ZFWD_RK_MHD,
+  P_NSLOTS
+};
+const char *kProfNames[P_NSLOTS] = {"zinv", "zinv_curl", "zinv_mhd", "ypass_inv", "ypass_fwd", "xinv",
+                                    "xfwd", "xcross", "zfwd", "zfwd_rk_hd", "zfwd_rk_mhd"};
+struct ProfRec {
+  int slot;
+  double bytes;
+  hipEvent_t a, b;
+};
+struct Prof {
+  bool on = false;
+  std::vector<ProfRec> pending;
+  std::vector<hipEvent_t> pool;
+  int count[P_NSLOTS] = {};
+  double ms[P_NSLOTS] = {};
+  double bytes[P_NSLOTS] = {};
+} g_prof;
+
+hipEvent_t prof_event() {
+  if (!g_prof.pool.empty()) {
+    hipEvent_t e = g_prof.pool.back();
+    g_prof.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void prof_collect() {
+  for (ProfRec &r : g_prof.pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      g_prof.count[r.slot] += 1;
+      g_prof.ms[r.slot] += ms;
+      g_prof.bytes[r.slot] += r.bytes;
+    }
+    g_prof.pool.push_back(r.a);
+    g_prof.pool.push_back(r.b);
+  }
+  g_prof.pending.clear();
+}
+}  // namespace
+
+// Wraps one pass launch: expr must evaluate to hipError_t.  bytes is the
+// pass's algorithmic traffic (each field it touches read or written once).
+#define PROF(slot_, nbytes_, stream_, expr_)                              \
+  do {                                                                    \
+    if (g_prof.on) {                                                      \
+      ProfRec r_;                                                         \
+      r_.slot = (slot_);                                                  \
+      r_.bytes = (double)(nbytes_);                                       \
+      r_.a = prof_event();                                                \
+      r_.b = prof_event();                                                \
+      (void)hipEventRecord(r_.a, (stream_));                              \
+      HIPCHK(expr_);                                                      \
+      (void)hipEventRecord(r_.b, (stream_));                              \
+      g_prof.pending.push_back(r_);                                       \
+      if (g_prof.pending.size() > 4096) prof_collect();                   \
+    } else {                                                              \
+      HIPCHK(expr_);                                                      \
+    }                                                                     \
+  } while (0)
+
+extern "C" int ghx_prof_enable(int on) {
+  prof_collect();
+  g_prof.on = on != 0;
+  return GHX_OK;
+}
+extern "C" int ghx_prof_reset(void) {
+  prof_collect();
+  for (int k = 0; k < P_NSLOTS; ++k) { g_prof.count[k] = 0; g_prof.ms[k] = 0; g_prof.bytes[k] = 0; }
+  return GHX_OK;
+}
+extern "C" int ghx_prof_read(int slot, const char **name, int *count, double *total_ms, double *total_bytes) {
+  ARGCHK(slot >= 0, "bad slot");
+  if (slot >= P_NSLOTS) return set_err(GHX_ERR_ARG, "slot out of range (%d slots)", (int)P_NSLOTS);
+  prof_collect();
+  if (name) *name = kProfNames[slot];
+  if (count) *count = g_prof.count[slot];
+  if (total_ms) *total_ms = g_prof.ms[slot];
+  if (total_bytes) *total_bytes = g_prof.bytes[slot];
+  return GHX_OK;
+}
+
+// ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
 struct ghx_plan {
@@ -228,10 +319,11 @@ static int do_r2c(ghx_plan *p, int nf, const double *const *in, void *const *out
     q.p[f] = p->q + f * nc;
     o.p[f] = (cplx *)out[f];
   }
+  const double sc = (double)nc * sizeof(cplx), sr = (double)p->nx * p->ny * p->nz * sizeof(double);
   timer_begin(p);
-  HIPCHK(ghx::xfwd(p->ctx, nf, in, q));
-  HIPCHK(ghx::ypass(p->ctx, -1, nf, q));
-  HIPCHK(ghx::zfwd(p->ctx, nf, q, o));
+  PROF(P_XFWD, nf * (sr + sc), p->ctx.stream, ghx::xfwd(p->ctx, nf, in, q));
+  PROF(P_YFWD, nf * 2 * sc, p->ctx.stream, ghx::ypass(p->ctx, -1, nf, q));
+  PROF(P_ZFWD, nf * 2 * sc, p->ctx.stream, ghx::zfwd(p->ctx, nf, q, o));
   timer_end(p);
   return GHX_OK;
 }
@@ -247,10 +339,11 @@ static int do_c2r(ghx_plan *p, int nf, const void *const *in, double *const *out
     q.p[f] = p->q + f * nc;
     i.p[f] = (cplx *)in[f];
   }
+  const double sc = (double)nc * sizeof(cplx), sr = (double)p->nx * p->ny * p->nz * sizeof(double);
   timer_begin(p);
-  HIPCHK(ghx::zinv(p->ctx, 0, nf, i, q));
-  HIPCHK(ghx::ypass(p->ctx, +1, nf, q));
-  HIPCHK(ghx::xinv(p->ctx, nf, q, out));
+  PROF(P_ZINV, nf * 2 * sc, p->ctx.stream, ghx::zinv(p->ctx, 0, nf, i, q));
+  PROF(P_YINV, nf * 2 * sc, p->ctx.stream, ghx::ypass(p->ctx, +1, nf, q));
+  PROF(P_XINV, nf * (sr + sc), p->ctx.stream, ghx::xinv(p->ctx, nf, q, out));
   timer_end(p);
   return GHX_OK;
 }
@@ -480,20 +573,21 @@ static int fused_cross(ghx_grid *g, int mode, const void *a, const void *b, cons
   for (int k = 0; k < 6; ++k) q6.p[k] = g->plan.q + k * nc;
   for (int k = 0; k < 3; ++k) q3.p[k] = g->plan.q + (6 + k) * nc;
   out.p[0] = M_(x); out.p[1] = M_(y); out.p[2] = M_(z);
+  const double sc = (double)nc * sizeof(cplx);
   FieldSet pq;  // P fields first, then Q
   if (mode == 1) {
     in.p[0] = M_(a); in.p[1] = M_(b); in.p[2] = M_(c);
-    HIPCHK(ghx::zinv(ctx, 1, 3, in, q6));  // q6 = (u, curl u)
+    PROF(P_ZINV_CURL, 9 * sc, ctx.stream, ghx::zinv(ctx, 1, 3, in, q6));  // q6 = (u, curl u)
     for (int k = 0; k < 3; ++k) { pq.p[k] = q6.p[3 + k]; pq.p[3 + k] = q6.p[k]; }
   } else {
     in.p[0] = M_(a); in.p[1] = M_(b); in.p[2] = M_(c); in.p[3] = M_(d); in.p[4] = M_(e); in.p[5] = M_(f);
-    HIPCHK(ghx::zinv(ctx, 0, 6, in, q6));
+    PROF(P_ZINV, 12 * sc, ctx.stream, ghx::zinv(ctx, 0, 6, in, q6));
     for (int k = 0; k < 6; ++k) pq.p[k] = q6.p[k];
   }
-  HIPCHK(ghx::ypass(ctx, +1, 6, q6));
-  HIPCHK(ghx::xcross(ctx, 0, pq, q3));
-  HIPCHK(ghx::ypass(ctx, -1, 3, q3));
-  HIPCHK(ghx::zfwd(ctx, 3, q3, out));
+  PROF(P_YINV, 12 * sc, ctx.stream, ghx::ypass(ctx, +1, 6, q6));
+  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, q3));
+  PROF(P_YFWD, 6 * sc, ctx.stream, ghx::ypass(ctx, -1, 3, q3));
+  PROF(P_ZFWD, 6 * sc, ctx.stream, ghx::zfwd(ctx, 3, q3, out));
   return GHX_OK;
 }
 
@@ -792,10 +886,11 @@ static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const c
   for (int k = 0; k < 6; ++k) q6.p[k] = q + k * nc;
   for (int k = 0; k < 3; ++k) q3.p[k] = q + (6 + k) * nc;
   for (int k = 0; k < 3; ++k) { pq.p[k] = q6.p[3 + k]; pq.p[3 + k] = q6.p[k]; }
-  HIPCHK(ghx::zinv(ctx, 1, 3, in, q6));
-  HIPCHK(ghx::ypass(ctx, +1, 6, q6));
-  HIPCHK(ghx::xcross(ctx, 0, pq, q3));
-  HIPCHK(ghx::ypass(ctx, -1, 3, q3));
+  const double sc = (double)nc * sizeof(cplx);
+  PROF(P_ZINV_CURL, 9 * sc, ctx.stream, ghx::zinv(ctx, 1, 3, in, q6));
+  PROF(P_YINV, 12 * sc, ctx.stream, ghx::ypass(ctx, +1, 6, q6));
+  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, q3));
+  PROF(P_YFWD, 6 * sc, ctx.stream, ghx::ypass(ctx, -1, 3, q3));
   ghx::RKParams rk;
   rk.dt = dt; rk.nu = nu; rk.rmp = 1. / (double)o;
   for (int k = 0; k < 3; ++k) {
@@ -803,7 +898,9 @@ static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const c
     rk.f[k] = f ? f[k] : nullptr;
   }
   rk.fline = f ? fline : nullptr;
-  HIPCHK(ghx::zfwd_rk_hd(ctx, q3, rk));
+  // reads the three products, v and C (once if they alias), writes v_out
+  const int nread = 3 + 3 + ((const void *)c[0] == (const void *)v[0] ? 0 : 3);
+  PROF(P_ZFWD_RK_HD, (nread + 3) * sc, ctx.stream, ghx::zfwd_rk_hd(ctx, q3, rk));
   return GHX_OK;
 }
 
@@ -1024,19 +1121,20 @@ static int mhd_substep(ghx_mhd *s, int o, const cplx *const *vin, cplx *const *o
   for (int k = 0; k < 6; ++k) in.p[k] = const_cast<cplx *>(vin[k]);
   for (int k = 0; k < 12; ++k) q12.p[k] = s->q + k * nc;
   for (int k = 0; k < 9; ++k) q9.p[k] = s->q + (12 + k) * nc;
-  HIPCHK(ghx::zinv(ctx, 2, 6, in, q12));  // u(0..2) w(3..5) b(6..8) j(9..11)
-  HIPCHK(ghx::ypass(ctx, +1, 12, q12));
+  const double sc = (double)nc * sizeof(cplx);
+  PROF(P_ZINV_MHD, 18 * sc, ctx.stream, ghx::zinv(ctx, 2, 6, in, q12));  // u(0..2) w(3..5) b(6..8) j(9..11)
+  PROF(P_YINV, 24 * sc, ctx.stream, ghx::ypass(ctx, +1, 12, q12));
   FieldSet o3;
   // w x u
   for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[3 + k]; pq.p[3 + k] = q12.p[k]; o3.p[k] = q9.p[k]; }
-  HIPCHK(ghx::xcross(ctx, 0, pq, o3));
+  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3));
   // j x b
   for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[9 + k]; pq.p[3 + k] = q12.p[6 + k]; o3.p[k] = q9.p[3 + k]; }
-  HIPCHK(ghx::xcross(ctx, 0, pq, o3));
+  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3));
   // u x b
   for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[k]; pq.p[3 + k] = q12.p[6 + k]; o3.p[k] = q9.p[6 + k]; }
-  HIPCHK(ghx::xcross(ctx, 0, pq, o3));
-  HIPCHK(ghx::ypass(ctx, -1, 9, q9));
+  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3));
+  PROF(P_YFWD, 18 * sc, ctx.stream, ghx::ypass(ctx, -1, 9, q9));
   ghx::MHDRKParams rk;
   rk.dt = s->dt; rk.nu = s->nu; rk.mu = s->mu; rk.rmp = 1. / (double)o;
   for (int k = 0; k < 6; ++k) {
@@ -1049,7 +1147,8 @@ static int mhd_substep(ghx_mhd *s, int o, const cplx *const *vin, cplx *const *o
     rk.f[3 + k] = s->forced ? s->m[k] : nullptr;
   }
   rk.fline = s->forced ? s->fline : nullptr;
-  HIPCHK(ghx::zfwd_rk_mhd(ctx, q9, rk));
+  const int nread = 9 + 6 + (vin[0] == s->s[0] ? 0 : 6);
+  PROF(P_ZFWD_RK_MHD, (nread + 6) * sc, ctx.stream, ghx::zfwd_rk_mhd(ctx, q9, rk));
   return GHX_OK;
 }
 

@@ -181,6 +181,16 @@ int ghx_mhd_fields(ghx_mhd *s, void **v, void **a, void **f, void **m);
 int ghx_mhd_set_forcing(ghx_mhd *s, int enabled);
 int ghx_mhd_step(ghx_mhd *s, int nsteps);
 
+/* ---------------------------------------------------------------------- */
+/* Pass profiler (replaces the GTStart/GTStop handles around the fftp      */
+/* phases, utils/gtimer_mod.f90:37-185 and fftp-3/fftp3D.fpp:244-312):      */
+/* HIP events recorded around every pass launch on its stream.  Slots name  */
+/* the pass kernels; total_bytes sums each launch's algorithmic traffic.    */
+/* ---------------------------------------------------------------------- */
+int ghx_prof_enable(int on);
+int ghx_prof_reset(void);
+int ghx_prof_read(int slot, const char **name, int *count, double *total_ms, double *total_bytes);
+
 #ifdef __cplusplus
 }
 #endif
