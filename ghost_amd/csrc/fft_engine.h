@@ -57,6 +57,27 @@ GHX_FFTCFG(384, 24, 4, 3, 2, 8, 8)
 GHX_FFTCFG(512, 8, 3, 8, 8, 8)
 GHX_FFTCFG(768, 24, 4, 3, 4, 8, 8)
 GHX_FFTCFG(1024, 8, 4, 2, 8, 8, 8)
+
+// Low-register schedules (4 points per lane) for kernels that keep several
+// fields live at once (the fused x pass holds six); other sizes fall back to
+// the default schedule.
+template <int N> struct FFTCfgLow : FFTCfg<N> {};
+#define GHX_FFTCFGLOW(N_, E_, NS_, ...)                                 \
+  template <> struct FFTCfgLow<N_> {                                    \
+    static constexpr int E = E_;                                        \
+    static constexpr int NS = NS_;                                      \
+    static constexpr int T = N_ / E_;                                   \
+    GHX_DEV static constexpr int radix(int s) {                         \
+      constexpr int r[] = {__VA_ARGS__};                                \
+      return r[s];                                                      \
+    }                                                                   \
+  };
+GHX_FFTCFGLOW(32, 4, 3, 2, 4, 4)
+GHX_FFTCFGLOW(64, 4, 3, 4, 4, 4)
+GHX_FFTCFGLOW(128, 4, 4, 2, 4, 4, 4)
+GHX_FFTCFGLOW(256, 4, 4, 4, 4, 4, 4)
+GHX_FFTCFGLOW(512, 4, 5, 2, 4, 4, 4, 4)
+#undef GHX_FFTCFGLOW
 #undef GHX_FFTCFG
 
 // ---------------------------------------------------------------------------
@@ -127,10 +148,26 @@ template <int DIR> struct SmallDFT<8, DIR> {
 };
 
 // ---------------------------------------------------------------------------
+// Synchronisation of one line's LDS exchange.  A line of T <= 64 lanes lives
+// inside one wavefront (T is a power of two and blocks are whole lines), so
+// ordering the wave's own LDS traffic is enough and the other waves of the
+// block run on; longer lines need the block barrier.
+// ---------------------------------------------------------------------------
+template <int T> GHX_DEV void line_sync() {
+  if constexpr (T <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // The line transform.
 // ---------------------------------------------------------------------------
-template <int N, int DIR> struct LineFFT {
-  using C = FFTCfg<N>;
+template <int N, int DIR, typename Cfg = FFTCfg<N>> struct LineFFT {
+  using C = Cfg;
   static constexpr int E = C::E;
   static constexpr int T = C::T;
 
@@ -168,18 +205,19 @@ template <int N, int DIR> struct LineFFT {
         }
       }
       if constexpr (!LAST) {
-        __syncthreads();
+        line_sync<T>();
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = buf[lds_pad(t + T * m)];
-        __syncthreads();
+        line_sync<T>();
         stages<S + 1, Ns * R>(v, buf, t, tw);
       }
     }
   }
 
   // v holds x[t + T m] on entry and X[t + T m] on exit.  buf is this line's
-  // LDS buffer (lds_line_stride(N) slots).  Every thread of the block must
-  // call this (it contains block barriers).
+  // LDS buffer (lds_line_stride(N) slots), used by this line only.  Every
+  // thread of the block must call this (lines longer than a wavefront use
+  // block barriers).
   GHX_DEV static void run(cplx (&v)[E], cplx *buf, int t, const cplx *tw) {
     stages<0, 1>(v, buf, t, tw);
   }

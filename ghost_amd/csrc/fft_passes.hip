@@ -58,8 +58,34 @@ static int pick_L(int T, int rowslots, int ny, int want) {
   return L < 1 ? 1 : L;
 }
 
+// Tile widths (lines per block along ky) for the z and x passes; overridable
+// with GHX_LZ / GHX_LX for tuning sweeps.
+static int tile_want(const char *env, int dflt) {
+  const char *e = getenv(env);
+  int v = e ? atoi(e) : 0;
+  return v > 0 ? v : dflt;
+}
+
 template <typename K> static void allow_lds(K kern, size_t bytes) {
   if (bytes > 49152) hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware tile order.  Workgroups are dealt round-robin to the 8 XCDs by
+// linear id, and each XCD has its own L2.  Tiles narrower than a 128-byte line
+// share lines with their ky neighbours, so consecutive logical tiles are
+// mapped to linear ids 8 apart: they land on the same XCD and hit its L2.
+// ---------------------------------------------------------------------------
+GHX_DEV void xcd_tile(int remap, int &bx, int &by) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  if (!remap) return;
+  const int nb = gridDim.x * gridDim.y;
+  if (nb % 8) return;
+  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  const int t = (b % 8) * (nb / 8) + b / 8;
+  bx = t % gridDim.x;
+  by = t / gridDim.x;
 }
 
 // ---------------------------------------------------------------------------
@@ -297,6 +323,8 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const 
     rk.out[0][s] = o0;
     rk.out[1][s] = o1;
     rk.out[2][s] = o2;
+    // keep the loads of one mode from being hoisted over the others (register pressure)
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -396,10 +424,10 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
 
 // Q rows -> packed half-length spectrum in registers, then inverse FFT:
 // v = real row (x[2n], x[2n+1]) * nx, unnormalised c2r.
-template <int N, int E>
+template <int N, int E, typename Cfg = FFTCfgLow<N>>
 GHX_DEV void xinv_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *twh,
                       const cplx *twx, const cplx *src, const GridK &g, int z, int j0) {
-  using F = LineFFT<N, +1>;
+  using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T;
   tile_load_q(lds, stride, L, N + 1, src, g.ny, (size_t)z * g.nxh * g.ny, j0, g.ny);
   __syncthreads();
@@ -419,10 +447,10 @@ GHX_DEV void xinv_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int
 }
 
 // forward FFT of a packed real row, unpack to nx/2+1 points, transposed store to Q.
-template <int N, int E>
+template <int N, int E, typename Cfg = FFTCfgLow<N>>
 GHX_DEV void xfwd_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *twh,
                       const cplx *twx, cplx *dst, const GridK &g, int z, int j0) {
-  using F = LineFFT<N, -1>;
+  using F = LineFFT<N, -1, Cfg>;
   constexpr int T = F::T;
   F::run(v, row, t, twh);
 #pragma unroll
@@ -456,7 +484,7 @@ struct RealSet {
 template <int N>
 __global__ __launch_bounds__(512) void k_xinv(FieldSet in, RealSet out, int nf, GridK g, const cplx *__restrict__ twh,
                        const cplx *__restrict__ twx) {
-  constexpr int T = FFTCfg<N>::T, E = FFTCfg<N>::E;
+  constexpr int T = FFTCfgLow<N>::T, E = FFTCfgLow<N>::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
@@ -479,7 +507,7 @@ __global__ __launch_bounds__(512) void k_xinv(FieldSet in, RealSet out, int nf, 
 template <int N>
 __global__ __launch_bounds__(512) void k_xfwd(RealSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twh,
                        const cplx *__restrict__ twx) {
-  constexpr int T = FFTCfg<N>::T, E = FFTCfg<N>::E;
+  constexpr int T = FFTCfgLow<N>::T, E = FFTCfgLow<N>::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
@@ -500,24 +528,26 @@ __global__ __launch_bounds__(512) void k_xfwd(RealSet in, FieldSet out, int nf, 
 // (pseudospec3D_hd.f90:377-389): out_x = (P_y Q_z - Q_y P_z) tmp,
 // out_y = (P_z Q_x - Q_z P_x) tmp, out_z = (P_x Q_y - Q_x P_y) tmp.
 // Fields: in.p[0..2] = P, in.p[3..5] = Q; out.p[0..2].
-template <int N>
-__global__ __launch_bounds__(512) void k_xcross(FieldSet in, FieldSet out, GridK g, const cplx *__restrict__ twh,
-                         const cplx *__restrict__ twx) {
-  constexpr int T = FFTCfg<N>::T, E = FFTCfg<N>::E;
+template <int N, typename Cfg, int W>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W))) void k_xcross(FieldSet in, FieldSet out, GridK g, const cplx *__restrict__ twh,
+                         const cplx *__restrict__ twx, int remap) {
+  constexpr int T = Cfg::T, E = Cfg::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  const int z = blockIdx.y, j0 = blockIdx.x * L;
+  int bx, by;
+  xcd_tile(remap, bx, by);
+  const int z = by, j0 = bx * L;
   const int stride = lds_line_stride(N + 1);
   cplx *row = lds + lt * stride;
   const double tmp = g.inv_n6;
   cplx qx[E], qy[E], qz[E], ay[E], az[E];
-  xinv_row<N, E>(qx, lds, row, stride, L, t, twh, twx, in.p[3], g, z, j0);
-  xinv_row<N, E>(qy, lds, row, stride, L, t, twh, twx, in.p[4], g, z, j0);
-  xinv_row<N, E>(qz, lds, row, stride, L, t, twh, twx, in.p[5], g, z, j0);
+  xinv_row<N, E, Cfg>(qx, lds, row, stride, L, t, twh, twx, in.p[3], g, z, j0);
+  xinv_row<N, E, Cfg>(qy, lds, row, stride, L, t, twh, twx, in.p[4], g, z, j0);
+  xinv_row<N, E, Cfg>(qz, lds, row, stride, L, t, twh, twx, in.p[5], g, z, j0);
   {
     cplx px[E];
-    xinv_row<N, E>(px, lds, row, stride, L, t, twh, twx, in.p[0], g, z, j0);
+    xinv_row<N, E, Cfg>(px, lds, row, stride, L, t, twh, twx, in.p[0], g, z, j0);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       ay[m] = c_make(qz[m].x * px[m].x, qz[m].y * px[m].y);  // Q_z P_x
@@ -526,26 +556,26 @@ __global__ __launch_bounds__(512) void k_xcross(FieldSet in, FieldSet out, GridK
   }
   {
     cplx py[E];
-    xinv_row<N, E>(py, lds, row, stride, L, t, twh, twx, in.p[1], g, z, j0);
+    xinv_row<N, E, Cfg>(py, lds, row, stride, L, t, twh, twx, in.p[1], g, z, j0);
     cplx oz[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       oz[m] = c_make((az[m].x - qx[m].x * py[m].x) * tmp, (az[m].y - qx[m].y * py[m].y) * tmp);
       az[m] = c_make(py[m].x * qz[m].x, py[m].y * qz[m].y);  // now P_y Q_z
     }
-    xfwd_row<N, E>(oz, lds, row, stride, L, t, twh, twx, out.p[2], g, z, j0);
+    xfwd_row<N, E, Cfg>(oz, lds, row, stride, L, t, twh, twx, out.p[2], g, z, j0);
   }
   {
     cplx pz[E];
-    xinv_row<N, E>(pz, lds, row, stride, L, t, twh, twx, in.p[2], g, z, j0);
+    xinv_row<N, E, Cfg>(pz, lds, row, stride, L, t, twh, twx, in.p[2], g, z, j0);
     cplx ox[E], oy[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       ox[m] = c_make((az[m].x - qy[m].x * pz[m].x) * tmp, (az[m].y - qy[m].y * pz[m].y) * tmp);
       oy[m] = c_make((pz[m].x * qx[m].x - ay[m].x) * tmp, (pz[m].y * qx[m].y - ay[m].y) * tmp);
     }
-    xfwd_row<N, E>(ox, lds, row, stride, L, t, twh, twx, out.p[0], g, z, j0);
-    xfwd_row<N, E>(oy, lds, row, stride, L, t, twh, twx, out.p[1], g, z, j0);
+    xfwd_row<N, E, Cfg>(ox, lds, row, stride, L, t, twh, twx, out.p[0], g, z, j0);
+    xfwd_row<N, E, Cfg>(oy, lds, row, stride, L, t, twh, twx, out.p[1], g, z, j0);
   }
 }
 
@@ -595,7 +625,7 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, 8);                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 8));                                        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     if (mode == 0) {                                                                        \
@@ -627,7 +657,7 @@ hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, 8);                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 8));                                        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd<n_>, lds);                                                             \
@@ -651,7 +681,7 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, 8);                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 8));                                        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd_rk_hd<n_>, lds);                                                       \
@@ -675,7 +705,7 @@ hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &r
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, 8);                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 8));                                        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd_rk_mhd<n_>, lds);                                                      \
@@ -701,8 +731,8 @@ hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out)
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
-    constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, 16);                                   \
+    constexpr int T = FFTCfgLow<n_>::T;                                                           \
+    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, tile_want("GHX_LX", 16));                                   \
     size_t lds = (size_t)L * lds_line_stride(n_ + 1) * sizeof(cplx);                        \
     dim3 grid((g.ny + L - 1) / L, g.nz);                                                    \
     allow_lds(k_xinv<n_>, lds);                                                             \
@@ -725,8 +755,8 @@ hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
-    constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, 16);                                   \
+    constexpr int T = FFTCfgLow<n_>::T;                                                           \
+    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, tile_want("GHX_LX", 16));                                   \
     size_t lds = (size_t)L * lds_line_stride(n_ + 1) * sizeof(cplx);                        \
     dim3 grid((g.ny + L - 1) / L, g.nz);                                                    \
     allow_lds(k_xfwd<n_>, lds);                                                             \
@@ -744,27 +774,45 @@ hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet
 #endif
 
 #if GHX_PART == 7
+template <typename Cfg, int W, int N>
+static hipError_t launch_xcross(const FFTCtx &c, const FieldSet &in, const FieldSet &out, int want) {
+  const GridK &g = c.gk;
+  constexpr int T = Cfg::T;
+  int L = pick_L(T, lds_line_stride(N + 1), g.ny, want);
+  size_t lds = (size_t)L * lds_line_stride(N + 1) * sizeof(cplx);
+  dim3 grid((g.ny + L - 1) / L, g.nz);
+  allow_lds(k_xcross<N, Cfg, W>, lds);
+  hipLaunchKernelGGL((k_xcross<N, Cfg, W>), grid, dim3(L * T), lds, c.stream, in, out, g, c.tw_xh, c.tw_x,
+                     tile_want("GHX_XCD", 1) == 1 ? 1 : 0);
+  return hipGetLastError();
+}
+
+static int xcross_variant() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("GHX_XCROSS_VARIANT");
+    v = e ? atoi(e) : 0;
+  }
+  return v;
+}
+
 hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out) {
   (void)mode;
   const GridK &g = c.gk;
   if (g.nx % 2) return hipErrorInvalidValue;
+  const int var = xcross_variant();
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
-  case n_: {                                                                                \
-    constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, 16);                                   \
-    size_t lds = (size_t)L * lds_line_stride(n_ + 1) * sizeof(cplx);                        \
-    dim3 grid((g.ny + L - 1) / L, g.nz);                                                    \
-    allow_lds(k_xcross<n_>, lds);                                                           \
-    hipLaunchKernelGGL((k_xcross<n_>), grid, dim3(L * T), lds, c.stream, in, out, g, c.tw_xh, c.tw_x); \
-    break;                                                                                  \
-  }
+  case n_:                                                                                  \
+    switch (var) {                                                                          \
+    case 1: return launch_xcross<FFTCfgLow<n_>, 1, n_>(c, in, out, 16);                     \
+    default: return launch_xcross<FFTCfgLow<n_>, 3, n_>(c, in, out, tile_want("GHX_LX", 4)); \
+    }
     GHX_SIZES(GHX_CASE)
 #undef GHX_CASE
   default:
     return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 #endif
