@@ -99,6 +99,8 @@ def _declare(L):
         "ghx_hd_fields": (_I, [_P, _PP, _PP]),
         "ghx_hd_set_forcing": (_I, [_P, _I]),
         "ghx_hd_step": (_I, [_P, _I]),
+        "ghx_hd_touch": (_I, [_P]),
+        "ghx_mhd_touch": (_I, [_P]),
         "ghx_mhd_create": (_I, [_PP, _P, _I, _D, _D, _D]),
         "ghx_mhd_destroy": (_I, [_P]),
         "ghx_mhd_fields": (_I, [_P, _PP, _PP, _PP, _PP]),

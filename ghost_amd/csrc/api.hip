@@ -433,6 +433,8 @@ struct ghx_grid {
   double *rtmp = nullptr;   // real temporaries
   int rtmp_fields = 0;
   size_t nc = 0, nr = 0;
+  double ps[3] = {1, 1, 1};  // pruned shares (rows, lines, modes), computed on first use
+  bool ps_valid = false;
 };
 
 static int grid_ctmp(ghx_grid *g, int nf) {
@@ -876,9 +878,45 @@ extern "C" int ghx_initialv_k4exp(ghx_grid *g, double k0, double u0, int32_t *se
 // RK substeps
 // ---------------------------------------------------------------------------
 
+// Shares of the spectral volume touched by the pruned passes (dealiased
+// fields): fx = kx rows with kept modes, fl = (kx,ky) lines with kept modes,
+// fm = kept modes (tiny <= kn2 <= kmax).
+struct PruneShares {
+  double fx, fl, fm;
+};
+static PruneShares prune_shares_compute(const GridK &g) {
+  auto kn2 = [&](int i, int j, int k) {
+    double a = (double)(i < g.nx / 2 ? i : i - g.nx), b = (double)(j < g.ny / 2 ? j : j - g.ny),
+           c = (double)(k < g.nz / 2 ? k : k - g.nz);
+    return g.rmp * (a * a) + g.rmq * (b * b) + g.rms * (c * c);
+  };
+  double rows = 0, lines = 0, modes = 0;
+  for (int i = 0; i < g.nxh; ++i) {
+    if (kn2(i, 0, 0) <= g.kmax) rows += 1;
+    for (int j = 0; j < g.ny; ++j) {
+      if (kn2(i, j, 0) > g.kmax) continue;
+      lines += 1;
+      for (int k = 0; k < g.nz; ++k) {
+        double q = kn2(i, j, k);
+        if (q <= g.kmax && q >= g.tiny) modes += 1;
+      }
+    }
+  }
+  const double nl = (double)g.nxh * g.ny;
+  return PruneShares{rows / g.nxh, lines / nl, modes / (nl * g.nz)};
+}
+static PruneShares prune_shares(ghx_grid *g) {
+  if (!g->ps_valid) {
+    PruneShares p = prune_shares_compute(g->gk);
+    g->ps[0] = p.fx; g->ps[1] = p.fl; g->ps[2] = p.fm;
+    g->ps_valid = true;
+  }
+  return PruneShares{g->ps[0], g->ps[1], g->ps[2]};
+}
+
 // Fused HD substep: q holds 9 Q-layout fields of scratch.
 static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const cplx *const *v, const cplx *const *c,
-                      const cplx *const *f, const uint8_t *fline, cplx *const *out) {
+                      const cplx *const *f, const uint8_t *fline, cplx *const *out, int prune = 0) {
   const size_t nc = g->nc;
   const FFTCtx &ctx = g->plan.ctx;
   FieldSet in, q6, q3, pq;
@@ -887,10 +925,13 @@ static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const c
   for (int k = 0; k < 3; ++k) q3.p[k] = q + (6 + k) * nc;
   for (int k = 0; k < 3; ++k) { pq.p[k] = q6.p[3 + k]; pq.p[3 + k] = q6.p[k]; }
   const double sc = (double)nc * sizeof(cplx);
-  PROF(P_ZINV_CURL, 9 * sc, ctx.stream, ghx::zinv(ctx, 1, 3, in, q6));
-  PROF(P_YINV, 12 * sc, ctx.stream, ghx::ypass(ctx, +1, 6, q6));
-  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, q3));
-  PROF(P_YFWD, 6 * sc, ctx.stream, ghx::ypass(ctx, -1, 3, q3));
+  // Algorithmic traffic of each pass.  Pruned: fx = share of kx rows carrying
+  // kept modes, fl = share of (kx,ky) lines, fm = share of kept modes.
+  const PruneShares ps = (prune && g_prof.on) ? prune_shares(g) : PruneShares{1.0, 1.0, 1.0};
+  PROF(P_ZINV_CURL, (3 * ps.fm + 6 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 1, 3, in, q6, prune));
+  PROF(P_YINV, (6 * ps.fl + 6 * ps.fx) * sc, ctx.stream, ghx::ypass(ctx, +1, 6, q6, prune));
+  PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, q3, prune));
+  PROF(P_YFWD, (3 * ps.fx + 3 * ps.fl) * sc, ctx.stream, ghx::ypass(ctx, -1, 3, q3, prune));
   ghx::RKParams rk;
   rk.dt = dt; rk.nu = nu; rk.rmp = 1. / (double)o;
   for (int k = 0; k < 3; ++k) {
@@ -898,9 +939,12 @@ static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const c
     rk.f[k] = f ? f[k] : nullptr;
   }
   rk.fline = f ? fline : nullptr;
-  // reads the three products, v and C (once if they alias), writes v_out
-  const int nread = 3 + 3 + ((const void *)c[0] == (const void *)v[0] ? 0 : 3);
-  PROF(P_ZFWD_RK_HD, (nread + 3) * sc, ctx.stream, ghx::zfwd_rk_hd(ctx, q3, rk));
+  // reads the three products on the transformed lines, v and C (once if they
+  // alias) and the forcing on the kept modes, writes v_out on the kept modes
+  // (on every mode without pruning)
+  const int nvc = (const void *)c[0] == (const void *)v[0] ? 3 : 6;
+  PROF(P_ZFWD_RK_HD, (3 * ps.fl + nvc * ps.fm + 3 * (prune ? ps.fm : 1.0)) * sc, ctx.stream,
+       ghx::zfwd_rk_hd(ctx, q3, rk, prune));
   return GHX_OK;
 }
 
@@ -955,13 +999,20 @@ struct ghx_hd {
   cplx *v[3], *v1[3], *f[3], *q;
   uint8_t *fline = nullptr;
   bool forced = false;
+  bool clean = false;  // v known to be zero outside the kept modes
 };
+
+// Count of non-zero modes of (a,b,c) outside the dealias set.
+static int count_unkept(ghx_grid *g, const cplx *a, const cplx *b, const cplx *c, double *n) {
+  return reduce(g, 4, a, b, c, nullptr, nullptr, nullptr, n);
+}
 
 extern "C" int ghx_hd_create(ghx_hd **s, ghx_grid *g, int ord, double dt, double nu) {
   ARGCHK(s && g && ord >= 1, "bad hd_create arguments");
   ghx_hd *h = new (std::nothrow) ghx_hd();
   if (!h) return set_err(GHX_ERR_NOMEM, "host allocation failed");
   h->g = g; h->ord = ord; h->dt = dt; h->nu = nu;
+  (void)prune_shares(g);  // host-side table of pruned shares, computed once per grid
   const size_t nc = g->nc;
   hipError_t e = hipMalloc(&h->mem, 18 * nc * sizeof(cplx));
   if (e != hipSuccess) { delete h; HIPCHK(e); }
@@ -990,6 +1041,7 @@ extern "C" int ghx_hd_destroy(ghx_hd *s) {
 
 extern "C" int ghx_hd_fields(ghx_hd *s, void **v, void **f) {
   ARGCHK(s, "null solver");
+  s->clean = false;  // the caller may write through these pointers
   for (int k = 0; k < 3; ++k) {
     if (v) v[k] = s->v[k];
     if (f) f[k] = s->f[k];
@@ -1011,7 +1063,20 @@ extern "C" int ghx_hd_set_forcing(ghx_hd *s, int enabled) {
 // main3D.fpp:3202-3231.  The state lives in v; C (step start) is v itself
 // for the first substep, so the intermediate substeps write v1 and the last
 // one writes back into v element-wise over C.
+extern "C" int ghx_hd_touch(ghx_hd *s) {
+  ARGCHK(s, "null solver");
+  s->clean = false;
+  return GHX_OK;
+}
+
 static int hd_step_once(ghx_hd *s) {
+  if (!s->clean) {
+    double n = 0;
+    int rc = count_unkept(s->g, s->v[0], s->v[1], s->v[2], &n);
+    if (rc) return rc;
+    s->clean = (n == 0.0);
+  }
+  const int prune = s->clean && !getenv("GHX_NO_PRUNE") ? 1 : 0;
   const cplx *fc[3] = {s->f[0], s->f[1], s->f[2]};
   const cplx *const *f = s->forced ? fc : nullptr;
   const cplx *v0[3] = {s->v[0], s->v[1], s->v[2]};
@@ -1020,9 +1085,10 @@ static int hd_step_once(ghx_hd *s) {
     const bool first = (o == s->ord), last = (o == 1);
     const cplx *const *vin = first ? v0 : v1;
     cplx *const *out = last ? s->v : s->v1;
-    int rc = hd_substep(s->g, s->q, o, s->dt, s->nu, vin, v0, f, s->fline, out);
+    int rc = hd_substep(s->g, s->q, o, s->dt, s->nu, vin, v0, f, s->fline, out, prune);
     if (rc) return rc;
   }
+  s->clean = true;  // every substep zeroes the modes outside the dealias set
   return GHX_OK;
 }
 
@@ -1046,6 +1112,7 @@ struct ghx_mhd {
   cplx *s[6], *s1[6], *f[3], *m[3], *q;
   uint8_t *fline = nullptr;
   bool forced = false;
+  bool clean = false;
 };
 
 extern "C" int ghx_mhd_create(ghx_mhd **out, ghx_grid *g, int ord, double dt, double nu, double mu) {
@@ -1053,6 +1120,7 @@ extern "C" int ghx_mhd_create(ghx_mhd **out, ghx_grid *g, int ord, double dt, do
   ghx_mhd *h = new (std::nothrow) ghx_mhd();
   if (!h) return set_err(GHX_ERR_NOMEM, "host allocation failed");
   h->g = g; h->ord = ord; h->dt = dt; h->nu = nu; h->mu = mu;
+  (void)prune_shares(g);
   const size_t nc = g->nc;
   hipError_t e = hipMalloc(&h->mem, 39 * nc * sizeof(cplx));
   if (e != hipSuccess) { delete h; HIPCHK(e); }
@@ -1078,6 +1146,7 @@ extern "C" int ghx_mhd_destroy(ghx_mhd *s) {
 
 extern "C" int ghx_mhd_fields(ghx_mhd *s, void **v, void **a, void **f, void **m) {
   ARGCHK(s, "null solver");
+  s->clean = false;
   for (int k = 0; k < 3; ++k) {
     if (v) v[k] = s->s[k];
     if (a) a[k] = s->s[3 + k];
@@ -1113,7 +1182,13 @@ extern "C" int ghx_mhd_set_forcing(ghx_mhd *s, int enabled) {
 // include/mhd_rkstep2.f90:4-65 fused: one inverse chain for (v, A) producing
 // (u, w, b, j); three cross products (w x u, j x b, u x b); one forward chain
 // ending in nonlin3 / gauge3 and the update.
-static int mhd_substep(ghx_mhd *s, int o, const cplx *const *vin, cplx *const *out) {
+extern "C" int ghx_mhd_touch(ghx_mhd *s) {
+  ARGCHK(s, "null solver");
+  s->clean = false;
+  return GHX_OK;
+}
+
+static int mhd_substep(ghx_mhd *s, int o, const cplx *const *vin, cplx *const *out, int prune) {
   ghx_grid *g = s->g;
   const size_t nc = g->nc;
   const FFTCtx &ctx = g->plan.ctx;
@@ -1122,19 +1197,20 @@ static int mhd_substep(ghx_mhd *s, int o, const cplx *const *vin, cplx *const *o
   for (int k = 0; k < 12; ++k) q12.p[k] = s->q + k * nc;
   for (int k = 0; k < 9; ++k) q9.p[k] = s->q + (12 + k) * nc;
   const double sc = (double)nc * sizeof(cplx);
-  PROF(P_ZINV_MHD, 18 * sc, ctx.stream, ghx::zinv(ctx, 2, 6, in, q12));  // u(0..2) w(3..5) b(6..8) j(9..11)
-  PROF(P_YINV, 24 * sc, ctx.stream, ghx::ypass(ctx, +1, 12, q12));
+  const PruneShares ps = (prune && g_prof.on) ? prune_shares(g) : PruneShares{1.0, 1.0, 1.0};
+  PROF(P_ZINV_MHD, (6 * ps.fm + 12 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 2, 6, in, q12, prune));  // u w b j
+  PROF(P_YINV, (12 * ps.fl + 12 * ps.fx) * sc, ctx.stream, ghx::ypass(ctx, +1, 12, q12, prune));
   FieldSet o3;
   // w x u
   for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[3 + k]; pq.p[3 + k] = q12.p[k]; o3.p[k] = q9.p[k]; }
-  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3));
+  PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3, prune));
   // j x b
   for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[9 + k]; pq.p[3 + k] = q12.p[6 + k]; o3.p[k] = q9.p[3 + k]; }
-  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3));
+  PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3, prune));
   // u x b
   for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[k]; pq.p[3 + k] = q12.p[6 + k]; o3.p[k] = q9.p[6 + k]; }
-  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3));
-  PROF(P_YFWD, 18 * sc, ctx.stream, ghx::ypass(ctx, -1, 9, q9));
+  PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3, prune));
+  PROF(P_YFWD, (9 * ps.fx + 9 * ps.fl) * sc, ctx.stream, ghx::ypass(ctx, -1, 9, q9, prune));
   ghx::MHDRKParams rk;
   rk.dt = s->dt; rk.nu = s->nu; rk.mu = s->mu; rk.rmp = 1. / (double)o;
   for (int k = 0; k < 6; ++k) {
@@ -1147,8 +1223,9 @@ static int mhd_substep(ghx_mhd *s, int o, const cplx *const *vin, cplx *const *o
     rk.f[3 + k] = s->forced ? s->m[k] : nullptr;
   }
   rk.fline = s->forced ? s->fline : nullptr;
-  const int nread = 9 + 6 + (vin[0] == s->s[0] ? 0 : 6);
-  PROF(P_ZFWD_RK_MHD, (nread + 6) * sc, ctx.stream, ghx::zfwd_rk_mhd(ctx, q9, rk));
+  const int nvc = vin[0] == s->s[0] ? 6 : 12;
+  PROF(P_ZFWD_RK_MHD, (9 * ps.fl + nvc * ps.fm + 6 * (prune ? ps.fm : 1.0)) * sc, ctx.stream,
+       ghx::zfwd_rk_mhd(ctx, q9, rk, prune));
   return GHX_OK;
 }
 
@@ -1157,11 +1234,20 @@ extern "C" int ghx_mhd_step(ghx_mhd *s, int nsteps) {
   const cplx *v0[6], *v1[6];
   for (int k = 0; k < 6; ++k) { v0[k] = s->s[k]; v1[k] = s->s1[k]; }
   for (int n = 0; n < nsteps; ++n) {
+    if (!s->clean) {
+      double a = 0, b = 0;
+      int rc = count_unkept(s->g, s->s[0], s->s[1], s->s[2], &a);
+      if (!rc) rc = count_unkept(s->g, s->s[3], s->s[4], s->s[5], &b);
+      if (rc) return rc;
+      s->clean = (a == 0.0 && b == 0.0);
+    }
+    const int prune = s->clean && !getenv("GHX_NO_PRUNE") ? 1 : 0;
     for (int o = s->ord; o >= 1; --o) {
       const bool first = (o == s->ord), last = (o == 1);
-      int rc = mhd_substep(s, o, first ? v0 : v1, last ? s->s : s->s1);
+      int rc = mhd_substep(s, o, first ? v0 : v1, last ? s->s : s->s1, prune);
       if (rc) return rc;
     }
+    s->clean = true;
   }
   return GHX_OK;
 }

@@ -48,19 +48,24 @@ bool fft_size_supported(int n);
 // Z^-1 pass, S -> Q.  mode 0: nf fields in, nf out.  mode 1 (HD): 3 in
 // (v) -> 6 out (v, curl v).  mode 2 (MHD): 6 in (v, A) -> 12 out
 // (v, curl v, b = curl A, j = curl b).
-hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out);
+// prune=1: the input is dealiased (zero outside tiny <= kn2 <= kmax): only
+// lines that carry kept modes are read and written (every pass below takes
+// the same flag; see the kernels for exactly what each one skips).
+hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out, int prune = 0);
 // Y pass in place on Q fields (dir -1 forward, +1 inverse).
-hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q);
+hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q, int prune = 0);
 // X^-1: Q -> real rows.
 hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out);
 // X forward: real rows -> Q.
 hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet &out);
 // Fused X^-1 . cross product . X: 6 Q fields (u, w) -> 3 Q fields of FFT[(w x u)/N^6].
 // mode 1 (MHD): 12 fields (u, w, b, j) -> 9 (w x u, j x b, u x b).
-hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out);
+hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out, int prune = 0);
 // Z forward, Q -> S (mode 0), or fused with projection + RK update (HD/MHD).
 hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out);
-hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk);
-hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &rk);
+hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, int prune = 0);
+hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &rk, int prune = 0);
+// number of kx indices (from 0) whose kn2(kx,0,0) <= kmax
+int active_kx(const GridK &g);
 
 }  // namespace ghx

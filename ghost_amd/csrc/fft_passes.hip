@@ -94,44 +94,72 @@ GHX_DEV void xcd_tile(int remap, int &bx, int &by) {
 
 // Store this block's L lines (each NP points, kept in LDS rows) to Q with the
 // line index (ky) fastest: dst[(p*nxh + i)*ny + j0 + l].
+// Only lines whose bit is set in lmask and points p < pmax are stored; the
+// loader zero-fills the rest without touching memory (dealias pruning).
 GHX_DEV void tile_store_q(cplx *lds, int stride, int L, int NP, cplx *dst, size_t z_or_p_stride,
-                          size_t base, int j0, int ny) {
+                          size_t base, int j0, int ny, unsigned lmask = ~0u, int pmax = 1 << 30) {
   for (int idx = threadIdx.x; idx < L * NP; idx += blockDim.x) {
     int l = idx % L, p = idx / L;
-    if (j0 + l < ny) dst[base + (size_t)p * z_or_p_stride + j0 + l] = lds[l * stride + lds_pad(p)];
+    if (j0 + l < ny && ((lmask >> l) & 1u) && p < pmax)
+      dst[base + (size_t)p * z_or_p_stride + j0 + l] = lds[l * stride + lds_pad(p)];
   }
 }
 GHX_DEV void tile_load_q(cplx *lds, int stride, int L, int NP, const cplx *src, size_t pstride,
-                         size_t base, int j0, int ny) {
+                         size_t base, int j0, int ny, unsigned lmask = ~0u, int pmax = 1 << 30) {
   for (int idx = threadIdx.x; idx < L * NP; idx += blockDim.x) {
     int l = idx % L, p = idx / L;
     cplx v = c_make(0.0, 0.0);
-    if (j0 + l < ny) v = src[base + (size_t)p * pstride + j0 + l];
+    if (j0 + l < ny && ((lmask >> l) & 1u) && p < pmax) v = src[base + (size_t)p * pstride + j0 + l];
     lds[l * stride + lds_pad(p)] = v;
   }
+}
+
+// A (kx, ky) line carries dealiased modes iff kn2(kx, ky, 0) <= kmax.
+GHX_DEV bool line_active(const GridK &g, int i, int j) { return kn2_of(g, i, j, 0) <= g.kmax; }
+GHX_DEV unsigned tile_lines_active(const GridK &g, int i, int j0, int L) {
+  unsigned m = 0;
+  for (int l = 0; l < L; ++l)
+    if (j0 + l < g.ny && line_active(g, i, j0 + l)) m |= 1u << l;
+  return m;
 }
 
 // ---------------------------------------------------------------------------
 // Y pass: contiguous lines q[z][kx][:] in place.
 // ---------------------------------------------------------------------------
+// prune (dealiased fields, line (z,kx) over ky): 0 none; 1 inverse: lines with
+// no dealiased mode are skipped and only the ky inside the dealias disc are
+// read; 2 forward: such lines are skipped and only those ky are written.
 template <int N, int DIR>
-__global__ __launch_bounds__(256) void k_ypass(FieldSet q, long nlines, const cplx *__restrict__ tw) {
+__global__ __launch_bounds__(256) void k_ypass(FieldSet q, long nlines, const cplx *__restrict__ tw, GridK g,
+                                              int prune) {
   using F = LineFFT<N, DIR>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int lpb = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   const long line = (long)blockIdx.x * lpb + lt;
-  const bool ok = line < nlines;
+  bool ok = line < nlines;
+  const int kx = (int)(line % g.nxh);
+  if (prune && ok && !line_active(g, kx, 0)) ok = false;
+  if constexpr (T == 64) {
+    if (!ok) return;  // one line per wavefront and only wave-level syncs
+  }
   cplx *row = lds + lt * lds_line_stride(N);
   cplx *base = q.p[blockIdx.y] + (ok ? line : 0) * N;
   cplx v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = ok ? base[t + T * m] : c_make(0.0, 0.0);
+  for (int m = 0; m < E; ++m) {
+    const int j = t + T * m;
+    const bool need = ok && (prune != 1 || kn2_of(g, kx, j, 0) <= g.kmax);
+    v[m] = need ? base[j] : c_make(0.0, 0.0);
+  }
   F::run(v, row, t, tw);
   if (ok) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) base[t + T * m] = v[m];
+    for (int m = 0; m < E; ++m) {
+      const int j = t + T * m;
+      if (prune != 2 || kn2_of(g, kx, j, 0) <= g.kmax) base[j] = v[m];
+    }
   }
 }
 
@@ -140,35 +168,41 @@ __global__ __launch_bounds__(256) void k_ypass(FieldSet q, long nlines, const cp
 // ---------------------------------------------------------------------------
 template <int N, int E>
 GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *tw,
-                       cplx *dst, const GridK &g, int i, int j0) {
+                       cplx *dst, const GridK &g, int i, int j0, unsigned lmask) {
   using F = LineFFT<N, +1>;
   constexpr int T = F::T;
   F::run(v, row, t, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
   __syncthreads();
-  tile_store_q(lds, stride, L, N, dst, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny);
+  tile_store_q(lds, stride, L, N, dst, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny, lmask);
   __syncthreads();
 }
 
 template <int N, int MODE>
-__global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ tw) {
+__global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ tw,
+                                             int prune) {
   using F = LineFFT<N, +1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
-  const bool ok = j < g.ny;
+  // prune: the input is dealiased, so only lines inside the dealias disc
+  // carry data; only their kept modes are read and only they are written.
+  const unsigned lmask = prune ? tile_lines_active(g, i, j0, L) : ~0u;
+  if (lmask == 0u) return;  // uniform over the block
+  const bool ok = j < g.ny && ((lmask >> lt) & 1u);
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
   const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
+  auto need = [&](int k) { return ok && (!prune || kn2_of(g, i, j, k) <= g.kmax); };
   if constexpr (MODE == 0) {
     for (int f = 0; f < nf; ++f) {
       cplx v[E];
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = ok ? in.p[f][sline + t + T * m] : c_make(0.0, 0.0);
-      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0);
+      for (int m = 0; m < E; ++m) v[m] = need(t + T * m) ? in.p[f][sline + t + T * m] : c_make(0.0, 0.0);
+      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0, lmask);
     }
   } else {
     // MODE 1: v -> (v, curl v).  MODE 2: (v, A) -> (v, curl v, b = curl A, curl b).
@@ -177,7 +211,7 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int m = 0; m < E; ++m) u[c][m] = ok ? in.p[c][sline + t + T * m] : c_make(0.0, 0.0);
+      for (int m = 0; m < E; ++m) u[c][m] = need(t + T * m) ? in.p[c][sline + t + T * m] : c_make(0.0, 0.0);
     // rotor3 (pseudospec3D_hd.f90:124-200) on (a0,a1,a2) -> component c
     auto curl = [&](const cplx (&a)[3][E], int c, cplx (&o)[E]) {
 #pragma unroll
@@ -193,20 +227,20 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
       cplx v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = u[c][m];
-      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[c], g, i, j0);
+      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[c], g, i, j0, lmask);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       cplx v[E];
       curl(u, c, v);
-      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[3 + c], g, i, j0);
+      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[3 + c], g, i, j0, lmask);
     }
     if constexpr (MODE == 2) {
       cplx a[3][E], b[3][E];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int m = 0; m < E; ++m) a[c][m] = ok ? in.p[3 + c][sline + t + T * m] : c_make(0.0, 0.0);
+        for (int m = 0; m < E; ++m) a[c][m] = need(t + T * m) ? in.p[3 + c][sline + t + T * m] : c_make(0.0, 0.0);
       curl(a, 0, b[0]);
       curl(a, 1, b[1]);
       curl(a, 2, b[2]);
@@ -215,13 +249,13 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
         cplx v[E];
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = b[c][m];
-        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[6 + c], g, i, j0);
+        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[6 + c], g, i, j0, lmask);
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         cplx v[E];
         curl(b, c, v);
-        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[9 + c], g, i, j0);
+        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[9 + c], g, i, j0, lmask);
       }
     }
   }
@@ -232,10 +266,10 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
 // ---------------------------------------------------------------------------
 template <int N, int E>
 GHX_DEV void zfwd_load(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *tw,
-                       const cplx *src, const GridK &g, int i, int j0) {
+                       const cplx *src, const GridK &g, int i, int j0, unsigned lmask = ~0u) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T;
-  tile_load_q(lds, stride, L, N, src, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny);
+  tile_load_q(lds, stride, L, N, src, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny, lmask);
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = row[lds_pad(t + T * m)];
@@ -284,20 +318,26 @@ GHX_DEV cplx rk_update(cplx cc, cplx vv, cplx nl, cplx ff, double kk2, double nu
 }
 
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const cplx *__restrict__ tw, RKParams rk) {
+__global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const cplx *__restrict__ tw, RKParams rk,
+                                                   int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
-  const bool ok = j < g.ny;
+  // prune: all state buffers are dealiased (zero outside the kept modes), so
+  // only lines inside the dealias disc are transformed and only kept modes
+  // are read and written.
+  const unsigned lmask = prune ? tile_lines_active(g, i, j0, L) : ~0u;
+  if (lmask == 0u) return;  // uniform over the block
+  const bool ok = j < g.ny && ((lmask >> lt) & 1u);
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
   cplx P[3][E];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0);
+    zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
     __syncthreads();
   }
   if (!ok) return;
@@ -319,10 +359,14 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const 
       o0 = rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
       o1 = rk_update(rk.c[1][s], rk.v[1][s], n1, forced ? rk.f[1][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
       o2 = rk_update(rk.c[2][s], rk.v[2][s], n2, forced ? rk.f[2][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+      rk.out[0][s] = o0;
+      rk.out[1][s] = o1;
+      rk.out[2][s] = o2;
+    } else if (!prune) {
+      rk.out[0][s] = o0;
+      rk.out[1][s] = o1;
+      rk.out[2][s] = o2;
     }
-    rk.out[0][s] = o0;
-    rk.out[1][s] = o1;
-    rk.out[2][s] = o2;
     // keep the loads of one mode from being hoisted over the others (register pressure)
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -331,14 +375,17 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const 
 // MHD: nonlin3 (pseudospec3D_mhd.f90:108-227) for the velocity from
 // (w x u, j x b) and gauge3 (:230-336) for the potential from u x b.
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const cplx *__restrict__ tw, MHDRKParams rk) {
+__global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const cplx *__restrict__ tw, MHDRKParams rk,
+                                                    int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
-  const bool ok = j < g.ny;
+  const unsigned lmask = prune ? tile_lines_active(g, i, j0, L) : ~0u;
+  if (lmask == 0u) return;  // uniform over the block
+  const bool ok = j < g.ny && ((lmask >> lt) & 1u);
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
   const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
@@ -350,7 +397,7 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
     cplx P[6][E];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-      zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0);
+      zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
       __syncthreads();
     }
     if (ok) {
@@ -375,9 +422,11 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
                              rk.dt, rk.rmp);
           }
         }
-        rk.out[0][s] = o[0];
-        rk.out[1][s] = o[1];
-        rk.out[2][s] = o[2];
+        if (!prune || (kn2 <= g.kmax && kn2 >= g.tiny)) {
+          rk.out[0][s] = o[0];
+          rk.out[1][s] = o[1];
+          rk.out[2][s] = o[2];
+        }
       }
     }
   }
@@ -386,7 +435,7 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
     cplx P[3][E];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[6 + f], g, i, j0);
+      zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[6 + f], g, i, j0, lmask);
       __syncthreads();
     }
     if (ok) {
@@ -409,9 +458,11 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
                              forced && rk.f[3 + c] ? rk.f[3 + c][s] : z0, kk2, rk.mu, rk.dt, rk.rmp);
           }
         }
-        rk.out[3][s] = o[0];
-        rk.out[4][s] = o[1];
-        rk.out[5][s] = o[2];
+        if (!prune || (kn2 <= g.kmax && kn2 >= g.tiny)) {
+          rk.out[3][s] = o[0];
+          rk.out[4][s] = o[1];
+          rk.out[5][s] = o[2];
+        }
       }
     }
   }
@@ -426,10 +477,10 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
 // v = real row (x[2n], x[2n+1]) * nx, unnormalised c2r.
 template <int N, int E, typename Cfg = FFTCfgLow<N>>
 GHX_DEV void xinv_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *twh,
-                      const cplx *twx, const cplx *src, const GridK &g, int z, int j0) {
+                      const cplx *twx, const cplx *src, const GridK &g, int z, int j0, int pmax = 1 << 30) {
   using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T;
-  tile_load_q(lds, stride, L, N + 1, src, g.ny, (size_t)z * g.nxh * g.ny, j0, g.ny);
+  tile_load_q(lds, stride, L, N + 1, src, g.ny, (size_t)z * g.nxh * g.ny, j0, g.ny, ~0u, pmax);
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -449,7 +500,7 @@ GHX_DEV void xinv_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int
 // forward FFT of a packed real row, unpack to nx/2+1 points, transposed store to Q.
 template <int N, int E, typename Cfg = FFTCfgLow<N>>
 GHX_DEV void xfwd_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *twh,
-                      const cplx *twx, cplx *dst, const GridK &g, int z, int j0) {
+                      const cplx *twx, cplx *dst, const GridK &g, int z, int j0, int pmax = 1 << 30) {
   using F = LineFFT<N, -1, Cfg>;
   constexpr int T = F::T;
   F::run(v, row, t, twh);
@@ -473,7 +524,7 @@ GHX_DEV void xfwd_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int
   for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = X[m];
   if (t == 0) row[lds_pad(N)] = c_make(nyq.x - nyq.y, 0.0);
   __syncthreads();
-  tile_store_q(lds, stride, L, N + 1, dst, g.ny, (size_t)z * g.nxh * g.ny, j0, g.ny);
+  tile_store_q(lds, stride, L, N + 1, dst, g.ny, (size_t)z * g.nxh * g.ny, j0, g.ny, ~0u, pmax);
   __syncthreads();
 }
 
@@ -530,7 +581,7 @@ __global__ __launch_bounds__(512) void k_xfwd(RealSet in, FieldSet out, int nf, 
 // Fields: in.p[0..2] = P, in.p[3..5] = Q; out.p[0..2].
 template <int N, typename Cfg, int W>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W))) void k_xcross(FieldSet in, FieldSet out, GridK g, const cplx *__restrict__ twh,
-                         const cplx *__restrict__ twx, int remap) {
+                         const cplx *__restrict__ twx, int remap, int pmax) {
   constexpr int T = Cfg::T, E = Cfg::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
@@ -542,12 +593,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W))) void k
   cplx *row = lds + lt * stride;
   const double tmp = g.inv_n6;
   cplx qx[E], qy[E], qz[E], ay[E], az[E];
-  xinv_row<N, E, Cfg>(qx, lds, row, stride, L, t, twh, twx, in.p[3], g, z, j0);
-  xinv_row<N, E, Cfg>(qy, lds, row, stride, L, t, twh, twx, in.p[4], g, z, j0);
-  xinv_row<N, E, Cfg>(qz, lds, row, stride, L, t, twh, twx, in.p[5], g, z, j0);
+  xinv_row<N, E, Cfg>(qx, lds, row, stride, L, t, twh, twx, in.p[3], g, z, j0, pmax);
+  xinv_row<N, E, Cfg>(qy, lds, row, stride, L, t, twh, twx, in.p[4], g, z, j0, pmax);
+  xinv_row<N, E, Cfg>(qz, lds, row, stride, L, t, twh, twx, in.p[5], g, z, j0, pmax);
   {
     cplx px[E];
-    xinv_row<N, E, Cfg>(px, lds, row, stride, L, t, twh, twx, in.p[0], g, z, j0);
+    xinv_row<N, E, Cfg>(px, lds, row, stride, L, t, twh, twx, in.p[0], g, z, j0, pmax);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       ay[m] = c_make(qz[m].x * px[m].x, qz[m].y * px[m].y);  // Q_z P_x
@@ -556,26 +607,26 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W))) void k
   }
   {
     cplx py[E];
-    xinv_row<N, E, Cfg>(py, lds, row, stride, L, t, twh, twx, in.p[1], g, z, j0);
+    xinv_row<N, E, Cfg>(py, lds, row, stride, L, t, twh, twx, in.p[1], g, z, j0, pmax);
     cplx oz[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       oz[m] = c_make((az[m].x - qx[m].x * py[m].x) * tmp, (az[m].y - qx[m].y * py[m].y) * tmp);
       az[m] = c_make(py[m].x * qz[m].x, py[m].y * qz[m].y);  // now P_y Q_z
     }
-    xfwd_row<N, E, Cfg>(oz, lds, row, stride, L, t, twh, twx, out.p[2], g, z, j0);
+    xfwd_row<N, E, Cfg>(oz, lds, row, stride, L, t, twh, twx, out.p[2], g, z, j0, pmax);
   }
   {
     cplx pz[E];
-    xinv_row<N, E, Cfg>(pz, lds, row, stride, L, t, twh, twx, in.p[2], g, z, j0);
+    xinv_row<N, E, Cfg>(pz, lds, row, stride, L, t, twh, twx, in.p[2], g, z, j0, pmax);
     cplx ox[E], oy[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       ox[m] = c_make((az[m].x - qy[m].x * pz[m].x) * tmp, (az[m].y - qy[m].y * pz[m].y) * tmp);
       oy[m] = c_make((pz[m].x * qx[m].x - ay[m].x) * tmp, (pz[m].y * qx[m].y - ay[m].y) * tmp);
     }
-    xfwd_row<N, E, Cfg>(ox, lds, row, stride, L, t, twh, twx, out.p[0], g, z, j0);
-    xfwd_row<N, E, Cfg>(oy, lds, row, stride, L, t, twh, twx, out.p[1], g, z, j0);
+    xfwd_row<N, E, Cfg>(ox, lds, row, stride, L, t, twh, twx, out.p[0], g, z, j0, pmax);
+    xfwd_row<N, E, Cfg>(oy, lds, row, stride, L, t, twh, twx, out.p[1], g, z, j0, pmax);
   }
 }
 
@@ -589,7 +640,16 @@ static int ydesired_lpb(int T) {
 }
 
 #if GHX_PART == 1
-hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q) {
+int active_kx(const GridK &g) {
+  int n = 0;
+  for (int q = 0; q < g.nxh; ++q) {
+    double kx = (double)(q < g.nx / 2 ? q : q - g.nx);
+    if (g.rmp * (kx * kx) <= g.kmax) n = q + 1;
+  }
+  return n;
+}
+
+hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q, int prune) {
   const GridK &g = c.gk;
   const long nlines = (long)g.nz * g.nxh;
   switch (g.ny) {
@@ -601,10 +661,10 @@ hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q) {
     dim3 grid((unsigned)((nlines + lpb - 1) / lpb), nf);                                    \
     if (dir < 0) {                                                                          \
       allow_lds(k_ypass<n_, -1>, lds);                                                      \
-      hipLaunchKernelGGL((k_ypass<n_, -1>), grid, dim3(lpb * T), lds, c.stream, q, nlines, c.tw_y); \
+      hipLaunchKernelGGL((k_ypass<n_, -1>), grid, dim3(lpb * T), lds, c.stream, q, nlines, c.tw_y, g, prune ? 2 : 0); \
     } else {                                                                                \
       allow_lds(k_ypass<n_, 1>, lds);                                                       \
-      hipLaunchKernelGGL((k_ypass<n_, 1>), grid, dim3(lpb * T), lds, c.stream, q, nlines, c.tw_y); \
+      hipLaunchKernelGGL((k_ypass<n_, 1>), grid, dim3(lpb * T), lds, c.stream, q, nlines, c.tw_y, g, prune ? 1 : 0); \
     }                                                                                       \
     break;                                                                                  \
   }
@@ -619,7 +679,7 @@ hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q) {
 #endif
 
 #if GHX_PART == 2
-hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out) {
+hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out, int prune) {
   const GridK &g = c.gk;
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
@@ -630,13 +690,13 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     if (mode == 0) {                                                                        \
       allow_lds(k_zinv<n_, 0>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 0>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
+      hipLaunchKernelGGL((k_zinv<n_, 0>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune); \
     } else if (mode == 1) {                                                                 \
       allow_lds(k_zinv<n_, 1>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 1>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
+      hipLaunchKernelGGL((k_zinv<n_, 1>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune); \
     } else {                                                                                \
       allow_lds(k_zinv<n_, 2>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 2>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
+      hipLaunchKernelGGL((k_zinv<n_, 2>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune); \
     }                                                                                       \
     break;                                                                                  \
   }
@@ -675,7 +735,7 @@ hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #endif
 
 #if GHX_PART == 4
-hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk) {
+hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, int prune) {
   const GridK &g = c.gk;
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
@@ -685,7 +745,7 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk) {
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd_rk_hd<n_>, lds);                                                       \
-    hipLaunchKernelGGL((k_zfwd_rk_hd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk); \
+    hipLaunchKernelGGL((k_zfwd_rk_hd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -699,7 +759,7 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk) {
 #endif
 
 #if GHX_PART == 5
-hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &rk) {
+hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &rk, int prune) {
   const GridK &g = c.gk;
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
@@ -709,7 +769,7 @@ hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &r
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd_rk_mhd<n_>, lds);                                                      \
-    hipLaunchKernelGGL((k_zfwd_rk_mhd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk); \
+    hipLaunchKernelGGL((k_zfwd_rk_mhd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -775,7 +835,7 @@ hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet
 
 #if GHX_PART == 7
 template <typename Cfg, int W, int N>
-static hipError_t launch_xcross(const FFTCtx &c, const FieldSet &in, const FieldSet &out, int want) {
+static hipError_t launch_xcross(const FFTCtx &c, const FieldSet &in, const FieldSet &out, int want, int pmax) {
   const GridK &g = c.gk;
   constexpr int T = Cfg::T;
   int L = pick_L(T, lds_line_stride(N + 1), g.ny, want);
@@ -783,7 +843,7 @@ static hipError_t launch_xcross(const FFTCtx &c, const FieldSet &in, const Field
   dim3 grid((g.ny + L - 1) / L, g.nz);
   allow_lds(k_xcross<N, Cfg, W>, lds);
   hipLaunchKernelGGL((k_xcross<N, Cfg, W>), grid, dim3(L * T), lds, c.stream, in, out, g, c.tw_xh, c.tw_x,
-                     tile_want("GHX_XCD", 1) == 1 ? 1 : 0);
+                     tile_want("GHX_XCD", 1) == 1 ? 1 : 0, pmax);
   return hipGetLastError();
 }
 
@@ -796,17 +856,18 @@ static int xcross_variant() {
   return v;
 }
 
-hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out) {
+hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out, int prune) {
   (void)mode;
   const GridK &g = c.gk;
   if (g.nx % 2) return hipErrorInvalidValue;
   const int var = xcross_variant();
+  const int pmax = prune ? active_kx(g) : g.nxh;
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_:                                                                                  \
     switch (var) {                                                                          \
-    case 1: return launch_xcross<FFTCfgLow<n_>, 1, n_>(c, in, out, 16);                     \
-    default: return launch_xcross<FFTCfgLow<n_>, 3, n_>(c, in, out, tile_want("GHX_LX", 4)); \
+    case 1: return launch_xcross<FFTCfgLow<n_>, 1, n_>(c, in, out, 16, pmax);               \
+    default: return launch_xcross<FFTCfgLow<n_>, 3, n_>(c, in, out, tile_want("GHX_LX", 4), pmax); \
     }
     GHX_SIZES(GHX_CASE)
 #undef GHX_CASE

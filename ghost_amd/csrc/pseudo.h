@@ -29,7 +29,8 @@ hipError_t op_rk_update(const GridK &g, hipStream_t st, cplx *v, const cplx *c, 
 hipError_t op_tg_real(const GridK &g, hipStream_t st, double *r1, double *r2, int kdn, int kup);
 
 // Reductions.  kind: 0 = |a|^2+|b|^2+|c|^2, 1 = Re(a d* + b e* + c f*),
-// 2 = that * kk2, 3 = |i kx a + i ky b + i kz c|^2 (divergence).
+// 2 = that * kk2, 3 = |i kx a + i ky b + i kz c|^2 (divergence),
+// 4 = number of modes outside tiny <= kn2 <= kmax where (a,b,c) is non-zero.
 // Each mode is weighted 1 on the kx=0 plane, 2 elsewhere, times 1/N^6.
 // `work` must hold reduce_work_doubles() doubles.  Result to *out (host).
 size_t reduce_work_doubles();

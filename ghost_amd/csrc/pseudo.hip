@@ -255,10 +255,17 @@ __global__ void k_reduce(GridK g, int kind, const cplx *a, const cplx *b, const 
     } else if (kind == 1 || kind == 2) {
       s = cre_dot(a[m], d[m]) + cre_dot(b[m], e[m]) + cre_dot(c[m], f[m]);
       if (kind == 2) s *= kk2_at(g, q);
-    } else {
+    } else if (kind == 3) {
       double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky), kz = wavenum(q.k, g.nz, g.dkz);
       cplx dv = c_add(c_add(c_ik(kx, a[m]), c_ik(ky, b[m])), c_ik(kz, c[m]));
       s = cabs2(dv);
+    } else {
+      // kind 4: count of modes outside tiny <= kn2 <= kmax holding a non-zero value
+      const double kn2 = kn2_of(g, q.i, q.j, q.k);
+      const cplx A = a[m], B = b[m], Cc = c[m];
+      const bool nz = A.x != 0.0 || A.y != 0.0 || B.x != 0.0 || B.y != 0.0 || Cc.x != 0.0 || Cc.y != 0.0;
+      acc += (!(kn2 <= g.kmax && kn2 >= g.tiny) && nz) ? 1.0 : 0.0;
+      continue;
     }
     acc += w * s * tmp;
   }

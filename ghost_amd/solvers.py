@@ -46,6 +46,7 @@ class HDSolver(_Resident):
         torch.cuda.synchronize()
         for k in range(3):
             self._copy_in(self._v[k], v[k])
+        _lib.check(_lib.load().ghx_hd_touch(self._h))
 
     def set_forcing(self, f):
         torch.cuda.synchronize()
@@ -97,6 +98,7 @@ class MHDSolver(_Resident):
         for k in range(3):
             self._copy_in(self._v[k], v[k])
             self._copy_in(self._a[k], a[k])
+        _lib.check(_lib.load().ghx_mhd_touch(self._h))
 
     def set_forcing(self, f=None, m=None):
         torch.cuda.synchronize()

@@ -168,6 +168,11 @@ int ghx_hd_create(ghx_hd **s, ghx_grid *g, int ord, double dt, double nu);
 int ghx_hd_destroy(ghx_hd *s);
 /* device pointers of the resident fields: v[3], f[3] */
 int ghx_hd_fields(ghx_hd *s, void **v, void **f);
+/* call after writing v through those pointers.  The next step checks whether
+ * v is zero outside the dealias set (tiny <= kn2 <= kmax); while it is, the
+ * passes skip every line that carries no kept mode (exact zeros in, discarded
+ * values out), otherwise the first step runs the full transforms. */
+int ghx_hd_touch(ghx_hd *s);
 /* call after writing f: records which (kx,ky) lines carry forcing, or 0 for none */
 int ghx_hd_set_forcing(ghx_hd *s, int enabled);
 /* nsteps full RK steps (main3D.fpp:3202-3231 without output/diagnostics) */
@@ -178,6 +183,8 @@ typedef struct ghx_mhd ghx_mhd;
 int ghx_mhd_create(ghx_mhd **s, ghx_grid *g, int ord, double dt, double nu, double mu);
 int ghx_mhd_destroy(ghx_mhd *s);
 int ghx_mhd_fields(ghx_mhd *s, void **v, void **a, void **f, void **m);
+/* as ghx_hd_touch, for v and a */
+int ghx_mhd_touch(ghx_mhd *s);
 int ghx_mhd_set_forcing(ghx_mhd *s, int enabled);
 int ghx_mhd_step(ghx_mhd *s, int nsteps);
 
