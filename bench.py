@@ -34,14 +34,19 @@ def log(*a):
 
 
 def read_traffic(kernel, n):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"traffic_hd{n}.json")
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if any.
+
+    The summary carries one line per pass: `traffic <pass> <bytes per launch>`."""
+    path = os.path.join(ROOT, "profiles", f"hd{n}_rk2_hbm_traffic.md")
     try:
         with open(path) as fh:
-            data = json.load(fh)
-        return data.get(kernel)
-    except (OSError, ValueError):
-        return None
+            for line in fh:
+                f = line.split()
+                if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
+                    return float(f[2])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(n, threads):
@@ -68,6 +73,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (the reference kernel for the rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE scale factors) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -88,6 +96,15 @@ def main():
     ghost_amd.load()
     torch.cuda.set_device(0)
     n = args.n
+
+    if args.calibrate_copy:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 1 GiB device to device")
+        return
 
     g = Grid(n)
     s = HDSolver(g, dt=DT, nu=NU, ord=2)

@@ -246,12 +246,13 @@ static int plan_init(ghx_plan *p, int nx, int ny, int nz, int dir) {
   return GHX_OK;
 }
 
+// nf scratch slots, each large enough for a field in the Q or P layout
 static int plan_scratch(ghx_plan *p, int nf) {
   if (p->q_fields >= nf) return GHX_OK;
   if (p->q) HIPCHK(hipFree(p->q));
   p->q = nullptr;
   p->q_fields = 0;
-  HIPCHK(hipMalloc(&p->q, (size_t)nf * nc_of(p->nx, p->ny, p->nz) * sizeof(cplx)));
+  HIPCHK(hipMalloc(&p->q, (size_t)nf * ghx::slot_points(p->ctx.gk) * sizeof(cplx)));
   p->q_fields = nf;
   return GHX_OK;
 }
@@ -310,19 +311,20 @@ extern "C" int ghx_plan_set_stream(ghx_plan *plan, void *stream) {
 
 static int do_r2c(ghx_plan *p, int nf, const double *const *in, void *const *out) {
   ARGCHK(p && in && out && nf >= 1 && nf <= ghx::kMaxFields, "bad r2c arguments");
-  int rc = plan_scratch(p, nf);
+  int rc = plan_scratch(p, 2 * nf);
   if (rc) return rc;
-  const size_t nc = nc_of(p->nx, p->ny, p->nz);
-  FieldSet q, o;
+  const size_t nc = nc_of(p->nx, p->ny, p->nz), slot = ghx::slot_points(p->ctx.gk);
+  FieldSet pp, q, o;
   for (int f = 0; f < nf; ++f) {
     ARGCHK(in[f] && out[f], "null field pointer");
-    q.p[f] = p->q + f * nc;
+    pp.p[f] = p->q + f * slot;
+    q.p[f] = p->q + (nf + f) * slot;
     o.p[f] = (cplx *)out[f];
   }
   const double sc = (double)nc * sizeof(cplx), sr = (double)p->nx * p->ny * p->nz * sizeof(double);
   timer_begin(p);
-  PROF(P_XFWD, nf * (sr + sc), p->ctx.stream, ghx::xfwd(p->ctx, nf, in, q));
-  PROF(P_YFWD, nf * 2 * sc, p->ctx.stream, ghx::ypass(p->ctx, -1, nf, q));
+  PROF(P_XFWD, nf * (sr + sc), p->ctx.stream, ghx::xfwd(p->ctx, nf, in, pp));
+  PROF(P_YFWD, nf * 2 * sc, p->ctx.stream, ghx::yfwd(p->ctx, nf, pp, q));
   PROF(P_ZFWD, nf * 2 * sc, p->ctx.stream, ghx::zfwd(p->ctx, nf, q, o));
   timer_end(p);
   return GHX_OK;
@@ -330,20 +332,21 @@ static int do_r2c(ghx_plan *p, int nf, const double *const *in, void *const *out
 
 static int do_c2r(ghx_plan *p, int nf, const void *const *in, double *const *out) {
   ARGCHK(p && in && out && nf >= 1 && nf <= ghx::kMaxFields, "bad c2r arguments");
-  int rc = plan_scratch(p, nf);
+  int rc = plan_scratch(p, 2 * nf);
   if (rc) return rc;
-  const size_t nc = nc_of(p->nx, p->ny, p->nz);
-  FieldSet q, i;
+  const size_t nc = nc_of(p->nx, p->ny, p->nz), slot = ghx::slot_points(p->ctx.gk);
+  FieldSet q, pp, i;
   for (int f = 0; f < nf; ++f) {
     ARGCHK(in[f] && out[f], "null field pointer");
-    q.p[f] = p->q + f * nc;
+    q.p[f] = p->q + f * slot;
+    pp.p[f] = p->q + (nf + f) * slot;
     i.p[f] = (cplx *)in[f];
   }
   const double sc = (double)nc * sizeof(cplx), sr = (double)p->nx * p->ny * p->nz * sizeof(double);
   timer_begin(p);
   PROF(P_ZINV, nf * 2 * sc, p->ctx.stream, ghx::zinv(p->ctx, 0, nf, i, q));
-  PROF(P_YINV, nf * 2 * sc, p->ctx.stream, ghx::ypass(p->ctx, +1, nf, q));
-  PROF(P_XINV, nf * (sr + sc), p->ctx.stream, ghx::xinv(p->ctx, nf, q, out));
+  PROF(P_YINV, nf * 2 * sc, p->ctx.stream, ghx::yinv(p->ctx, nf, q, pp));
+  PROF(P_XINV, nf * (sr + sc), p->ctx.stream, ghx::xinv(p->ctx, nf, pp, out));
   timer_end(p);
   return GHX_OK;
 }
@@ -565,31 +568,43 @@ extern "C" int ghx_gauge3(ghx_grid *g, const void *a, const void *b, const void 
 
 // (P x Q) products through the fused passes.  mode 1: P = curl(a,b,c), Q = (a,b,c)
 // (prodre3); mode 0: P = (a,b,c), Q = (d,e,f) (vector3).
+// Scratch slots of the fused chain: Q6 (slots 0-5) -> P6 (6-11) -> P3 (reuses
+// 0-2) -> Q3 (reuses 6-8).
+struct Chain {
+  FieldSet q6, p6, p3, q3;
+};
+static Chain chain_slots(cplx *base, size_t slot) {
+  Chain c;
+  for (int k = 0; k < 6; ++k) { c.q6.p[k] = base + k * slot; c.p6.p[k] = base + (6 + k) * slot; }
+  for (int k = 0; k < 3; ++k) { c.p3.p[k] = base + k * slot; c.q3.p[k] = base + (6 + k) * slot; }
+  return c;
+}
+
 static int fused_cross(ghx_grid *g, int mode, const void *a, const void *b, const void *c, const void *d,
                        const void *e, const void *f, void *x, void *y, void *z) {
-  int rc = plan_scratch(&g->plan, 9);
+  int rc = plan_scratch(&g->plan, 12);
   if (rc) return rc;
   const size_t nc = g->nc;
   const FFTCtx &ctx = g->plan.ctx;
-  FieldSet in, q6, q3, out;
-  for (int k = 0; k < 6; ++k) q6.p[k] = g->plan.q + k * nc;
-  for (int k = 0; k < 3; ++k) q3.p[k] = g->plan.q + (6 + k) * nc;
+  Chain ch = chain_slots(g->plan.q, ghx::slot_points(g->gk));
+  FieldSet in, out;
+  FieldSet &q6 = ch.q6;
   out.p[0] = M_(x); out.p[1] = M_(y); out.p[2] = M_(z);
   const double sc = (double)nc * sizeof(cplx);
   FieldSet pq;  // P fields first, then Q
   if (mode == 1) {
     in.p[0] = M_(a); in.p[1] = M_(b); in.p[2] = M_(c);
     PROF(P_ZINV_CURL, 9 * sc, ctx.stream, ghx::zinv(ctx, 1, 3, in, q6));  // q6 = (u, curl u)
-    for (int k = 0; k < 3; ++k) { pq.p[k] = q6.p[3 + k]; pq.p[3 + k] = q6.p[k]; }
+    for (int k = 0; k < 3; ++k) { pq.p[k] = ch.p6.p[3 + k]; pq.p[3 + k] = ch.p6.p[k]; }
   } else {
     in.p[0] = M_(a); in.p[1] = M_(b); in.p[2] = M_(c); in.p[3] = M_(d); in.p[4] = M_(e); in.p[5] = M_(f);
     PROF(P_ZINV, 12 * sc, ctx.stream, ghx::zinv(ctx, 0, 6, in, q6));
-    for (int k = 0; k < 6; ++k) pq.p[k] = q6.p[k];
+    for (int k = 0; k < 6; ++k) pq.p[k] = ch.p6.p[k];
   }
-  PROF(P_YINV, 12 * sc, ctx.stream, ghx::ypass(ctx, +1, 6, q6));
-  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, q3));
-  PROF(P_YFWD, 6 * sc, ctx.stream, ghx::ypass(ctx, -1, 3, q3));
-  PROF(P_ZFWD, 6 * sc, ctx.stream, ghx::zfwd(ctx, 3, q3, out));
+  PROF(P_YINV, 12 * sc, ctx.stream, ghx::yinv(ctx, 6, q6, ch.p6));
+  PROF(P_XCROSS, 9 * sc, ctx.stream, ghx::xcross(ctx, 0, pq, ch.p3));
+  PROF(P_YFWD, 6 * sc, ctx.stream, ghx::yfwd(ctx, 3, ch.p3, ch.q3));
+  PROF(P_ZFWD, 6 * sc, ctx.stream, ghx::zfwd(ctx, 3, ch.q3, out));
   return GHX_OK;
 }
 
@@ -919,19 +934,18 @@ static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const c
                       const cplx *const *f, const uint8_t *fline, cplx *const *out, int prune = 0) {
   const size_t nc = g->nc;
   const FFTCtx &ctx = g->plan.ctx;
-  FieldSet in, q6, q3, pq;
+  Chain ch = chain_slots(q, ghx::slot_points(g->gk));
+  FieldSet in, pq;
   for (int k = 0; k < 3; ++k) in.p[k] = const_cast<cplx *>(v[k]);
-  for (int k = 0; k < 6; ++k) q6.p[k] = q + k * nc;
-  for (int k = 0; k < 3; ++k) q3.p[k] = q + (6 + k) * nc;
-  for (int k = 0; k < 3; ++k) { pq.p[k] = q6.p[3 + k]; pq.p[3 + k] = q6.p[k]; }
+  for (int k = 0; k < 3; ++k) { pq.p[k] = ch.p6.p[3 + k]; pq.p[3 + k] = ch.p6.p[k]; }
   const double sc = (double)nc * sizeof(cplx);
   // Algorithmic traffic of each pass.  Pruned: fx = share of kx rows carrying
   // kept modes, fl = share of (kx,ky) lines, fm = share of kept modes.
   const PruneShares ps = (prune && g_prof.on) ? prune_shares(g) : PruneShares{1.0, 1.0, 1.0};
-  PROF(P_ZINV_CURL, (3 * ps.fm + 6 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 1, 3, in, q6, prune));
-  PROF(P_YINV, (6 * ps.fl + 6 * ps.fx) * sc, ctx.stream, ghx::ypass(ctx, +1, 6, q6, prune));
-  PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, q3, prune));
-  PROF(P_YFWD, (3 * ps.fx + 3 * ps.fl) * sc, ctx.stream, ghx::ypass(ctx, -1, 3, q3, prune));
+  PROF(P_ZINV_CURL, (3 * ps.fm + 6 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 1, 3, in, ch.q6, prune));
+  PROF(P_YINV, (6 * ps.fl + 6 * ps.fx) * sc, ctx.stream, ghx::yinv(ctx, 6, ch.q6, ch.p6, prune));
+  PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, ch.p3, prune));
+  PROF(P_YFWD, (3 * ps.fx + 3 * ps.fl) * sc, ctx.stream, ghx::yfwd(ctx, 3, ch.p3, ch.q3, prune));
   ghx::RKParams rk;
   rk.dt = dt; rk.nu = nu; rk.rmp = 1. / (double)o;
   for (int k = 0; k < 3; ++k) {
@@ -944,7 +958,7 @@ static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const c
   // (on every mode without pruning)
   const int nvc = (const void *)c[0] == (const void *)v[0] ? 3 : 6;
   PROF(P_ZFWD_RK_HD, (3 * ps.fl + nvc * ps.fm + 3 * (prune ? ps.fm : 1.0)) * sc, ctx.stream,
-       ghx::zfwd_rk_hd(ctx, q3, rk, prune));
+       ghx::zfwd_rk_hd(ctx, ch.q3, rk, prune));
   return GHX_OK;
 }
 
@@ -972,7 +986,7 @@ static int line_flags(ghx_grid *g, const cplx *a, const cplx *b, const cplx *c, 
 extern "C" int ghx_hd_rkstep2(ghx_grid *g, int o, double dt, double nu, const void *const *v_in, const void *const *C,
                               const void *const *f, void *const *v_out) {
   ARGCHK(g && o >= 1 && v_in && C && v_out, "bad hd_rkstep2 arguments");
-  int rc = plan_scratch(&g->plan, 9);
+  int rc = plan_scratch(&g->plan, 12);
   if (rc) return rc;
   const cplx *v[3], *c[3], *ff[3];
   cplx *out[3];
@@ -995,7 +1009,7 @@ struct ghx_hd {
   ghx_grid *g;
   int ord;
   double dt, nu;
-  cplx *mem = nullptr;  // v(3), v1(3), f(3), q(9)
+  cplx *mem = nullptr;  // v(3), v1(3), f(3), 12 scratch slots
   cplx *v[3], *v1[3], *f[3], *q;
   uint8_t *fline = nullptr;
   bool forced = false;
@@ -1014,14 +1028,14 @@ extern "C" int ghx_hd_create(ghx_hd **s, ghx_grid *g, int ord, double dt, double
   h->g = g; h->ord = ord; h->dt = dt; h->nu = nu;
   (void)prune_shares(g);  // host-side table of pruned shares, computed once per grid
   const size_t nc = g->nc;
-  hipError_t e = hipMalloc(&h->mem, 18 * nc * sizeof(cplx));
+  hipError_t e = hipMalloc(&h->mem, (9 * nc + 12 * ghx::slot_points(g->gk)) * sizeof(cplx));
   if (e != hipSuccess) { delete h; HIPCHK(e); }
   for (int k = 0; k < 3; ++k) {
     h->v[k] = h->mem + k * nc;
     h->v1[k] = h->mem + (3 + k) * nc;
     h->f[k] = h->mem + (6 + k) * nc;
   }
-  h->q = h->mem + 9 * nc;
+  h->q = h->mem + 9 * nc;  // 12 scratch slots (Q/P layouts)
   e = hipMalloc(&h->fline, (size_t)g->gk.nxh * g->gk.ny);
   if (e != hipSuccess) { (void)hipFree(h->mem); delete h; HIPCHK(e); }
   e = ghx::op_zero(g->stream, h->mem, 9 * nc * sizeof(cplx));
@@ -1108,7 +1122,7 @@ struct ghx_mhd {
   ghx_grid *g;
   int ord;
   double dt, nu, mu;
-  cplx *mem = nullptr;  // s(6) s1(6) f(3) m(3) q(21)
+  cplx *mem = nullptr;  // s(6) s1(6) f(3) m(3), 24 scratch slots
   cplx *s[6], *s1[6], *f[3], *m[3], *q;
   uint8_t *fline = nullptr;
   bool forced = false;
@@ -1122,11 +1136,11 @@ extern "C" int ghx_mhd_create(ghx_mhd **out, ghx_grid *g, int ord, double dt, do
   h->g = g; h->ord = ord; h->dt = dt; h->nu = nu; h->mu = mu;
   (void)prune_shares(g);
   const size_t nc = g->nc;
-  hipError_t e = hipMalloc(&h->mem, 39 * nc * sizeof(cplx));
+  hipError_t e = hipMalloc(&h->mem, (18 * nc + 24 * ghx::slot_points(g->gk)) * sizeof(cplx));
   if (e != hipSuccess) { delete h; HIPCHK(e); }
   for (int k = 0; k < 6; ++k) { h->s[k] = h->mem + k * nc; h->s1[k] = h->mem + (6 + k) * nc; }
   for (int k = 0; k < 3; ++k) { h->f[k] = h->mem + (12 + k) * nc; h->m[k] = h->mem + (15 + k) * nc; }
-  h->q = h->mem + 18 * nc;
+  h->q = h->mem + 18 * nc;  // 24 scratch slots
   e = hipMalloc(&h->fline, (size_t)g->gk.nxh * g->gk.ny);
   if (e != hipSuccess) { (void)hipFree(h->mem); delete h; HIPCHK(e); }
   e = ghx::op_zero(g->stream, h->mem, 18 * nc * sizeof(cplx));
@@ -1190,27 +1204,34 @@ extern "C" int ghx_mhd_touch(ghx_mhd *s) {
 
 static int mhd_substep(ghx_mhd *s, int o, const cplx *const *vin, cplx *const *out, int prune) {
   ghx_grid *g = s->g;
-  const size_t nc = g->nc;
+  const size_t nc = g->nc, slot = ghx::slot_points(g->gk);
   const FFTCtx &ctx = g->plan.ctx;
-  FieldSet in, q12, q9, pq;
+  // Q12 (slots 0-11) -> P12 (12-23) -> P9 (reuses 0-8) -> Q9 (reuses 12-20)
+  FieldSet in, q12, p12, p9, q9, pq;
   for (int k = 0; k < 6; ++k) in.p[k] = const_cast<cplx *>(vin[k]);
-  for (int k = 0; k < 12; ++k) q12.p[k] = s->q + k * nc;
-  for (int k = 0; k < 9; ++k) q9.p[k] = s->q + (12 + k) * nc;
+  for (int k = 0; k < 12; ++k) { q12.p[k] = s->q + k * slot; p12.p[k] = s->q + (12 + k) * slot; }
+  for (int k = 0; k < 9; ++k) { p9.p[k] = s->q + k * slot; q9.p[k] = s->q + (12 + k) * slot; }
   const double sc = (double)nc * sizeof(cplx);
   const PruneShares ps = (prune && g_prof.on) ? prune_shares(g) : PruneShares{1.0, 1.0, 1.0};
   PROF(P_ZINV_MHD, (6 * ps.fm + 12 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 2, 6, in, q12, prune));  // u w b j
-  PROF(P_YINV, (12 * ps.fl + 12 * ps.fx) * sc, ctx.stream, ghx::ypass(ctx, +1, 12, q12, prune));
+  // the Y^-1 pass takes at most kMaxFields fields per launch
+  PROF(P_YINV, (6 * ps.fl + 6 * ps.fx) * sc, ctx.stream, ghx::yinv(ctx, 6, q12, p12, prune));
+  {
+    FieldSet qa, pa;
+    for (int k = 0; k < 6; ++k) { qa.p[k] = q12.p[6 + k]; pa.p[k] = p12.p[6 + k]; }
+    PROF(P_YINV, (6 * ps.fl + 6 * ps.fx) * sc, ctx.stream, ghx::yinv(ctx, 6, qa, pa, prune));
+  }
   FieldSet o3;
   // w x u
-  for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[3 + k]; pq.p[3 + k] = q12.p[k]; o3.p[k] = q9.p[k]; }
+  for (int k = 0; k < 3; ++k) { pq.p[k] = p12.p[3 + k]; pq.p[3 + k] = p12.p[k]; o3.p[k] = p9.p[k]; }
   PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3, prune));
   // j x b
-  for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[9 + k]; pq.p[3 + k] = q12.p[6 + k]; o3.p[k] = q9.p[3 + k]; }
+  for (int k = 0; k < 3; ++k) { pq.p[k] = p12.p[9 + k]; pq.p[3 + k] = p12.p[6 + k]; o3.p[k] = p9.p[3 + k]; }
   PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3, prune));
   // u x b
-  for (int k = 0; k < 3; ++k) { pq.p[k] = q12.p[k]; pq.p[3 + k] = q12.p[6 + k]; o3.p[k] = q9.p[6 + k]; }
+  for (int k = 0; k < 3; ++k) { pq.p[k] = p12.p[k]; pq.p[3 + k] = p12.p[6 + k]; o3.p[k] = p9.p[6 + k]; }
   PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3, prune));
-  PROF(P_YFWD, (9 * ps.fx + 9 * ps.fl) * sc, ctx.stream, ghx::ypass(ctx, -1, 9, q9, prune));
+  PROF(P_YFWD, (9 * ps.fx + 9 * ps.fl) * sc, ctx.stream, ghx::yfwd(ctx, 9, p9, q9, prune));
   ghx::MHDRKParams rk;
   rk.dt = s->dt; rk.nu = s->nu; rk.mu = s->mu; rk.rmp = 1. / (double)o;
   for (int k = 0; k < 6; ++k) {

@@ -148,6 +148,20 @@ template <int DIR> struct SmallDFT<8, DIR> {
 };
 
 // ---------------------------------------------------------------------------
+// Twiddle powers w^1..w^(R-1) from one table load by a product tree of depth
+// <= 3 (rounding error of a few ulp), instead of R-1 table loads.
+// ---------------------------------------------------------------------------
+template <int R> GHX_DEV void tw_powers(cplx w1, cplx *w) {
+  w[1] = w1;
+  if constexpr (R > 2) w[2] = c_mul(w1, w1);
+  if constexpr (R > 3) w[3] = c_mul(w[2], w1);
+  if constexpr (R > 4) w[4] = c_mul(w[2], w[2]);
+  if constexpr (R > 5) w[5] = c_mul(w[4], w1);
+  if constexpr (R > 6) w[6] = c_mul(w[3], w[3]);
+  if constexpr (R > 7) w[7] = c_mul(w[4], w[3]);
+}
+
+// ---------------------------------------------------------------------------
 // Synchronisation of one line's LDS exchange.  A line of T <= 64 lanes lives
 // inside one wavefront (T is a power of two and blocks are whole lines), so
 // ordering the wave's own LDS traffic is enough and the other waves of the
@@ -191,8 +205,10 @@ template <int N, int DIR, typename Cfg = FFTCfg<N>> struct LineFFT {
         if constexpr (Ns > 1) {
           const int k = j % Ns;
           constexpr int tws = N / (Ns * R);
+          cplx w[R];
+          tw_powers<R>(c_tw<DIR>(tw, k * tws), w);
 #pragma unroll
-          for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], c_tw<DIR>(tw, k * r * tws));
+          for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], w[r]);
         }
         SmallDFT<R, DIR>::run(u);
         if constexpr (LAST) {

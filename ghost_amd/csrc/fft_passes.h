@@ -3,9 +3,10 @@
 // Layouts (complex double unless noted):
 //   R  real    r[z][y][x]          GHOST R(nx,ny,nz)
 //   S  spectral s[kx][ky][kz]      GHOST C(nz,ny,nx/2+1), kz fastest
-//   Q  intermediate q[z][kx][ky]   ky fastest; the y pass runs in place on it
-// r2c = X (R->Q, transposed store) . Y (Q in place) . Z (Q->S, transposed load)
-// c2r = Z^-1 (S->Q, transposed store) . Y^-1 (Q in place) . X^-1 (Q->R)
+//   Q  intermediate q[z][kx][ky]   ky fastest
+//   P  intermediate p[z][ky][kx]   kx fastest, rows of p_row() >= nx/2+1 points
+// r2c = X (R->P, contiguous rows) . Y (P->Q, transposed load) . Z (Q->S, transposed load)
+// c2r = Z^-1 (S->Q, transposed store) . Y^-1 (Q->P, transposed store) . X^-1 (P->R)
 #pragma once
 #include "ghx_common.h"
 
@@ -45,6 +46,11 @@ struct MHDRKParams {
 
 bool fft_size_supported(int n);
 
+// row length of the P layout: nx/2+1 rounded up to 8 points (128-byte rows)
+inline int p_row(const GridK &g) { return (g.nxh + 7) & ~7; }
+// complex points of one scratch field slot, large enough for Q or P
+inline size_t slot_points(const GridK &g) { return (size_t)g.nz * g.ny * p_row(g); }
+
 // Z^-1 pass, S -> Q.  mode 0: nf fields in, nf out.  mode 1 (HD): 3 in
 // (v) -> 6 out (v, curl v).  mode 2 (MHD): 6 in (v, A) -> 12 out
 // (v, curl v, b = curl A, j = curl b).
@@ -52,14 +58,15 @@ bool fft_size_supported(int n);
 // lines that carry kept modes are read and written (every pass below takes
 // the same flag; see the kernels for exactly what each one skips).
 hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out, int prune = 0);
-// Y pass in place on Q fields (dir -1 forward, +1 inverse).
-hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q, int prune = 0);
-// X^-1: Q -> real rows.
+// Y^-1: Q -> P.  Y forward: P -> Q.
+hipError_t yinv(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune = 0);
+hipError_t yfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune = 0);
+// X^-1: P -> real rows.
 hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out);
-// X forward: real rows -> Q.
+// X forward: real rows -> P.
 hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet &out);
-// Fused X^-1 . cross product . X: 6 Q fields (u, w) -> 3 Q fields of FFT[(w x u)/N^6].
-// mode 1 (MHD): 12 fields (u, w, b, j) -> 9 (w x u, j x b, u x b).
+// Fused X^-1 . cross product . X on P rows: in.p[0..2] = A, in.p[3..5] = B
+// -> out.p[0..2] = FFT[(A x B)/N^6] in GHOST's operand order.
 hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out, int prune = 0);
 // Z forward, Q -> S (mode 0), or fused with projection + RK update (HD/MHD).
 hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out);

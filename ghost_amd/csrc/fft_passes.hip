@@ -48,6 +48,11 @@ static size_t max_lds() {
 }
 
 // lines per block for a transposed pass: chunk L along ky, bounded by LDS and 1024 threads
+static int rows_per_block(int T) {
+  int r = 256 / T;
+  return r < 1 ? 1 : r;
+}
+
 static int pick_L(int T, int rowslots, int ny, int want) {
   int L = want;
   while (L > 1 && ((size_t)L * rowslots * sizeof(cplx) > max_lds() || L * T > 512)) L >>= 1;
@@ -72,20 +77,18 @@ template <typename K> static void allow_lds(K kern, size_t bytes) {
 
 // ---------------------------------------------------------------------------
 // XCD-aware tile order.  Workgroups are dealt round-robin to the 8 XCDs by
-// linear id, and each XCD has its own L2.  Tiles narrower than a 128-byte line
-// share lines with their ky neighbours, so consecutive logical tiles are
-// mapped to linear ids 8 apart: they land on the same XCD and hit its L2.
+// linear id and each XCD has its own L2.  A tile narrower than a 128-byte
+// line shares its lines with the neighbouring tile along the chunked axis, so
+// within each group of 16 linear ids the two ids that land on the same XCD
+// (b and b+8) are given neighbouring tiles.  Tiles stay spread over the XCDs
+// in their natural order, which keeps pruned passes load-balanced.
 // ---------------------------------------------------------------------------
-GHX_DEV void xcd_tile(int remap, int &bx, int &by) {
-  bx = blockIdx.x;
-  by = blockIdx.y;
-  if (!remap) return;
+GHX_DEV void xcd_tile(int &bx, int &by) {
   const int nb = gridDim.x * gridDim.y;
-  if (nb % 8) return;
-  const int b = blockIdx.y * gridDim.x + blockIdx.x;
-  const int t = (b % 8) * (nb / 8) + b / 8;
-  bx = t % gridDim.x;
-  by = t / gridDim.x;
+  int b = blockIdx.y * gridDim.x + blockIdx.x;
+  if (b < (nb & ~15)) b = (b & ~15) | ((b & 7) << 1) | ((b >> 3) & 1);
+  bx = b % gridDim.x;
+  by = b / gridDim.x;
 }
 
 // ---------------------------------------------------------------------------
@@ -124,46 +127,6 @@ GHX_DEV unsigned tile_lines_active(const GridK &g, int i, int j0, int L) {
 }
 
 // ---------------------------------------------------------------------------
-// Y pass: contiguous lines q[z][kx][:] in place.
-// ---------------------------------------------------------------------------
-// prune (dealiased fields, line (z,kx) over ky): 0 none; 1 inverse: lines with
-// no dealiased mode are skipped and only the ky inside the dealias disc are
-// read; 2 forward: such lines are skipped and only those ky are written.
-template <int N, int DIR>
-__global__ __launch_bounds__(256) void k_ypass(FieldSet q, long nlines, const cplx *__restrict__ tw, GridK g,
-                                              int prune) {
-  using F = LineFFT<N, DIR>;
-  constexpr int T = F::T, E = F::E;
-  extern __shared__ cplx lds[];
-  const int lpb = blockDim.x / T;
-  const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  const long line = (long)blockIdx.x * lpb + lt;
-  bool ok = line < nlines;
-  const int kx = (int)(line % g.nxh);
-  if (prune && ok && !line_active(g, kx, 0)) ok = false;
-  if constexpr (T == 64) {
-    if (!ok) return;  // one line per wavefront and only wave-level syncs
-  }
-  cplx *row = lds + lt * lds_line_stride(N);
-  cplx *base = q.p[blockIdx.y] + (ok ? line : 0) * N;
-  cplx v[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int j = t + T * m;
-    const bool need = ok && (prune != 1 || kn2_of(g, kx, j, 0) <= g.kmax);
-    v[m] = need ? base[j] : c_make(0.0, 0.0);
-  }
-  F::run(v, row, t, tw);
-  if (ok) {
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int j = t + T * m;
-      if (prune != 2 || kn2_of(g, kx, j, 0) <= g.kmax) base[j] = v[m];
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Z^-1: S -> Q.  Block = (kx index i, ky chunk j0..j0+L-1).
 // ---------------------------------------------------------------------------
 template <int N, int E>
@@ -187,7 +150,9 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i = by, j0 = bx * L, j = j0 + lt;
   // prune: the input is dealiased, so only lines inside the dealias disc
   // carry data; only their kept modes are read and only they are written.
   const unsigned lmask = prune ? tile_lines_active(g, i, j0, L) : ~0u;
@@ -284,7 +249,9 @@ __global__ __launch_bounds__(512) void k_zfwd(FieldSet in, FieldSet out, int nf,
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i = by, j0 = bx * L, j = j0 + lt;
   const bool ok = j < g.ny;
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
@@ -325,7 +292,9 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const 
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i = by, j0 = bx * L, j = j0 + lt;
   // prune: all state buffers are dealiased (zero outside the kept modes), so
   // only lines inside the dealias disc are transformed and only kept modes
   // are read and written.
@@ -382,7 +351,9 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  const int i = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i = by, j0 = bx * L, j = j0 + lt;
   const unsigned lmask = prune ? tile_lines_active(g, i, j0, L) : ~0u;
   if (lmask == 0u) return;  // uniform over the block
   const bool ok = j < g.ny && ((lmask >> lt) & 1u);
@@ -469,63 +440,155 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
 }
 
 // ---------------------------------------------------------------------------
-// X passes.  Block = (z, ky chunk j0..j0+L-1); a real row of nx points is
-// held as N = nx/2 packed complex points.
+// Y passes between Q[z][kx][ky] (ky fastest) and P[z][ky][kx] (kx fastest,
+// rows of nxp >= nx/2+1 points).  Block = (z, kx chunk kx0..kx0+L-1); the
+// ky lines are contiguous in Q and the transpose to or from P goes through
+// an LDS tile with L-point (L*16-byte) chunks along kx.  Only kx < kxlim is
+// touched (kxlim < nx/2+1 when the fields are dealiased).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(512) void k_yinv(FieldSet in, FieldSet out, int nf, GridK g, int nxp,
+                                             const cplx *__restrict__ tw, int kxlim, int prune) {
+  using F = LineFFT<N, +1>;
+  constexpr int T = F::T, E = F::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int z = by, kx0 = bx * L, kx = kx0 + lt;
+  if (kx0 >= kxlim) return;  // uniform over the block
+  const bool ok = kx < kxlim;
+  const int stride = lds_line_stride(N);
+  cplx *row = lds + lt * stride;
+  for (int f = 0; f < nf; ++f) {
+    const cplx *src = in.p[f] + ((size_t)z * g.nxh + (ok ? kx : 0)) * N;
+    cplx v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int j = t + T * m;
+      // dealiased input: Q holds data only on the lines (kx,ky) of the dealias disc
+      const bool need = ok && (!prune || kn2_of(g, kx, j, 0) <= g.kmax);
+      v[m] = need ? src[j] : c_make(0.0, 0.0);
+    }
+    F::run(v, row, t, tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
+    __syncthreads();
+    cplx *dst = out.p[f] + (size_t)z * g.ny * nxp;
+    for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+      const int l = idx % L, p = idx / L;
+      if (kx0 + l < kxlim) dst[(size_t)p * nxp + kx0 + l] = lds[l * stride + lds_pad(p)];
+    }
+    __syncthreads();
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(512) void k_yfwd(FieldSet in, FieldSet out, int nf, GridK g, int nxp,
+                                             const cplx *__restrict__ tw, int kxlim, int prune) {
+  using F = LineFFT<N, -1>;
+  constexpr int T = F::T, E = F::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int z = by, kx0 = bx * L, kx = kx0 + lt;
+  if (kx0 >= kxlim) return;  // uniform over the block
+  const bool ok = kx < kxlim;
+  const int stride = lds_line_stride(N);
+  cplx *row = lds + lt * stride;
+  for (int f = 0; f < nf; ++f) {
+    const cplx *src = in.p[f] + (size_t)z * g.ny * nxp;
+    for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+      const int l = idx % L, p = idx / L;
+      lds[l * stride + lds_pad(p)] = kx0 + l < kxlim ? src[(size_t)p * nxp + kx0 + l] : c_make(0.0, 0.0);
+    }
+    __syncthreads();
+    cplx v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = row[lds_pad(t + T * m)];
+    __syncthreads();
+    F::run(v, row, t, tw);
+    if (ok) {
+      cplx *dst = out.p[f] + ((size_t)z * g.nxh + kx) * N;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int j = t + T * m;
+        // dealiased output: only the ky of the dealias disc are used downstream
+        if (!prune || kn2_of(g, kx, j, 0) <= g.kmax) dst[j] = v[m];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// X passes on contiguous rows: P rows (nx/2+1 points, kx fastest) <-> real
+// rows of nx points held as N = nx/2 packed complex points.  A row is one
+// line of T <= 64 lanes (one wavefront for nx = 512), so these kernels use
+// only wave-level synchronisation.  Block = RPB rows.
 // ---------------------------------------------------------------------------
 
-// Q rows -> packed half-length spectrum in registers, then inverse FFT:
-// v = real row (x[2n], x[2n+1]) * nx, unnormalised c2r.
-template <int N, int E, typename Cfg = FFTCfgLow<N>>
-GHX_DEV void xinv_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *twh,
-                      const cplx *twx, const cplx *src, const GridK &g, int z, int j0, int pmax = 1 << 30) {
+// Half-spectrum row -> unnormalised c2r of the row, packed: v[m] holds
+// (x[2k], x[2k+1]) * nx for k = t + T m.  GHOST's c2r ignores the imaginary
+// parts of the kx = 0 and Nyquist entries.
+template <int N, int E, typename Cfg>
+GHX_DEV void xinv_row(cplx (&v)[E], cplx *buf, int t, const cplx *twh, const cplx *twx, const cplx *src,
+                      int kxlim) {
   using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T;
-  tile_load_q(lds, stride, L, N + 1, src, g.ny, (size_t)z * g.nxh * g.ny, j0, g.ny, ~0u, pmax);
-  __syncthreads();
+  // each point is read once; the mirrored points X[N-k] come through the
+  // line's LDS buffer (slot N holds the Nyquist point)
+  cplx a[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + T * m;
-    cplx A = row[lds_pad(k)];
-    cplx Bs = row[lds_pad(N - k)];
+    a[m] = k < kxlim ? src[k] : c_make(0.0, 0.0);
+    buf[lds_pad(k)] = a[m];
+  }
+  if (t == 0) buf[lds_pad(N)] = N < kxlim ? src[N] : c_make(0.0, 0.0);
+  line_sync<T>();
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    cplx A = a[m];
+    cplx Bs = buf[lds_pad(N - k)];
     if (k == 0) { A.y = 0.0; Bs.y = 0.0; }
     cplx B = c_conj(Bs);
     cplx Xe = c_add(A, B);
     cplx Xo = c_mul(c_sub(A, B), c_conj(twx[k]));
     v[m] = c_add(Xe, c_make(-Xo.y, Xo.x));
   }
-  __syncthreads();
-  F::run(v, row, t, twh);
+  line_sync<T>();
+  F::run(v, buf, t, twh);
 }
 
-// forward FFT of a packed real row, unpack to nx/2+1 points, transposed store to Q.
-template <int N, int E, typename Cfg = FFTCfgLow<N>>
-GHX_DEV void xfwd_row(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *twh,
-                      const cplx *twx, cplx *dst, const GridK &g, int z, int j0, int pmax = 1 << 30) {
+// Packed real row -> r2c half spectrum, stored to a P row (kx < kxlim only).
+template <int N, int E, typename Cfg>
+GHX_DEV void xfwd_row(cplx (&v)[E], cplx *buf, int t, const cplx *twh, const cplx *twx, cplx *dst, int kxlim) {
   using F = LineFFT<N, -1, Cfg>;
   constexpr int T = F::T;
-  F::run(v, row, t, twh);
+  F::run(v, buf, t, twh);
 #pragma unroll
-  for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
-  __syncthreads();
-  cplx X[E];
+  for (int m = 0; m < E; ++m) buf[lds_pad(t + T * m)] = v[m];
+  line_sync<T>();
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + T * m;
     cplx A = v[m];
-    cplx B = c_conj(row[lds_pad((N - k) % N)]);
+    cplx B = c_conj(buf[lds_pad((N - k) % N)]);
     cplx Xe = c_scale(c_add(A, B), 0.5);
     cplx d = c_sub(A, B);
     cplx Xo = c_make(0.5 * d.y, -0.5 * d.x);  // -0.5 i (A - B)
-    X[m] = c_add(Xe, c_mul(twx[k], Xo));
+    if (k < kxlim) dst[k] = c_add(Xe, c_mul(twx[k], Xo));
   }
-  cplx nyq = row[lds_pad(0)];
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = X[m];
-  if (t == 0) row[lds_pad(N)] = c_make(nyq.x - nyq.y, 0.0);
-  __syncthreads();
-  tile_store_q(lds, stride, L, N + 1, dst, g.ny, (size_t)z * g.nxh * g.ny, j0, g.ny, ~0u, pmax);
-  __syncthreads();
+  if (t == 0 && N < kxlim) {
+    cplx z0 = v[0];
+    dst[N] = c_make(z0.x - z0.y, 0.0);
+  }
+  line_sync<T>();
 }
 
 struct RealSet {
@@ -533,111 +596,131 @@ struct RealSet {
 };
 
 template <int N>
-__global__ __launch_bounds__(512) void k_xinv(FieldSet in, RealSet out, int nf, GridK g, const cplx *__restrict__ twh,
-                       const cplx *__restrict__ twx) {
-  constexpr int T = FFTCfgLow<N>::T, E = FFTCfgLow<N>::E;
+__global__ __launch_bounds__(256) void k_xinv(FieldSet in, RealSet out, int nf, GridK g, int nxp,
+                                             const cplx *__restrict__ twh, const cplx *__restrict__ twx) {
+  using Cfg = FFTCfgLow<N>;
+  constexpr int T = Cfg::T, E = Cfg::E;
   extern __shared__ cplx lds[];
-  const int L = blockDim.x / T;
+  const int rpb = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  const int z = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
-  const bool ok = j < g.ny;
-  const int stride = lds_line_stride(N + 1);
-  cplx *row = lds + lt * stride;
+  const long nrows = (long)g.nz * g.ny;
+  const long r = (long)blockIdx.x * rpb + lt;
+  const bool ok = r < nrows;
+  if constexpr (T <= 64) {
+    if (!ok) return;  // only wave-level syncs
+  }
+  cplx *buf = lds + lt * lds_line_stride(N);
   for (int f = 0; f < nf; ++f) {
     cplx v[E];
-    xinv_row<N, E>(v, lds, row, stride, L, t, twh, twx, in.p[f], g, z, j0);
+    xinv_row<N, E, Cfg>(v, buf, t, twh, twx, in.p[f] + (ok ? r : 0) * nxp, ok ? g.nxh : 0);
     if (ok) {
-      cplx *dst = reinterpret_cast<cplx *>(out.p[f] + ((size_t)z * g.ny + j) * g.nx);
+      cplx *dst = reinterpret_cast<cplx *>(out.p[f] + r * g.nx);
 #pragma unroll
       for (int m = 0; m < E; ++m) dst[t + T * m] = v[m];
     }
-    __syncthreads();
+    line_sync<T>();
   }
 }
 
 template <int N>
-__global__ __launch_bounds__(512) void k_xfwd(RealSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twh,
-                       const cplx *__restrict__ twx) {
-  constexpr int T = FFTCfgLow<N>::T, E = FFTCfgLow<N>::E;
+__global__ __launch_bounds__(256) void k_xfwd(RealSet in, FieldSet out, int nf, GridK g, int nxp,
+                                             const cplx *__restrict__ twh, const cplx *__restrict__ twx) {
+  using Cfg = FFTCfgLow<N>;
+  constexpr int T = Cfg::T, E = Cfg::E;
   extern __shared__ cplx lds[];
-  const int L = blockDim.x / T;
+  const int rpb = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  const int z = blockIdx.y, j0 = blockIdx.x * L, j = j0 + lt;
-  const bool ok = j < g.ny;
-  const int stride = lds_line_stride(N + 1);
-  cplx *row = lds + lt * stride;
+  const long nrows = (long)g.nz * g.ny;
+  const long r = (long)blockIdx.x * rpb + lt;
+  const bool ok = r < nrows;
+  if constexpr (T <= 64) {
+    if (!ok) return;
+  }
+  cplx *buf = lds + lt * lds_line_stride(N);
   for (int f = 0; f < nf; ++f) {
+    const cplx *src = reinterpret_cast<const cplx *>(in.p[f] + (ok ? r : 0) * g.nx);
     cplx v[E];
-    const cplx *src = reinterpret_cast<const cplx *>(in.p[f] + ((size_t)z * g.ny + (ok ? j : 0)) * g.nx);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = ok ? src[t + T * m] : c_make(0.0, 0.0);
-    xfwd_row<N, E>(v, lds, row, stride, L, t, twh, twx, out.p[f], g, z, j0);
+    xfwd_row<N, E, Cfg>(v, buf, t, twh, twx, out.p[f] + (ok ? r : 0) * nxp, ok ? g.nxh : 0);
   }
 }
 
-// Fused X^-1 . (P x Q)/N^6 . X with GHOST's operand order
+// Fused X^-1 . (P x Q)/N^6 . X on P rows with GHOST's operand order
 // (pseudospec3D_hd.f90:377-389): out_x = (P_y Q_z - Q_y P_z) tmp,
 // out_y = (P_z Q_x - Q_z P_x) tmp, out_z = (P_x Q_y - Q_x P_y) tmp.
-// Fields: in.p[0..2] = P, in.p[3..5] = Q; out.p[0..2].
-template <int N, typename Cfg, int W>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W))) void k_xcross(FieldSet in, FieldSet out, GridK g, const cplx *__restrict__ twh,
-                         const cplx *__restrict__ twx, int remap, int pmax) {
+// Fields: in.p[0..2] = P, in.p[3..5] = Q; out.p[0..2].  Only kx < kxlim is
+// read and written (dealiased inputs, dealiased use of the outputs).
+template <int N, int W, int SB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_xcross(
+    FieldSet in, FieldSet out, GridK g, int nxp, const cplx *__restrict__ twh, const cplx *__restrict__ twx,
+    int kxlim) {
+  using Cfg = FFTCfgLow<N>;
   constexpr int T = Cfg::T, E = Cfg::E;
   extern __shared__ cplx lds[];
-  const int L = blockDim.x / T;
+  const int rpb = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  int bx, by;
-  xcd_tile(remap, bx, by);
-  const int z = by, j0 = bx * L;
-  const int stride = lds_line_stride(N + 1);
-  cplx *row = lds + lt * stride;
+  const long nrows = (long)g.nz * g.ny;
+  const long r = (long)blockIdx.x * rpb + lt;
+  const bool ok = r < nrows;
+  if constexpr (T <= 64) {
+    if (!ok) return;
+  }
+  const int lim = ok ? kxlim : 0;
+  const size_t off = (size_t)(ok ? r : 0) * nxp;
+  cplx *buf = lds + lt * lds_line_stride(N);
   const double tmp = g.inv_n6;
   cplx qx[E], qy[E], qz[E], ay[E], az[E];
-  xinv_row<N, E, Cfg>(qx, lds, row, stride, L, t, twh, twx, in.p[3], g, z, j0, pmax);
-  xinv_row<N, E, Cfg>(qy, lds, row, stride, L, t, twh, twx, in.p[4], g, z, j0, pmax);
-  xinv_row<N, E, Cfg>(qz, lds, row, stride, L, t, twh, twx, in.p[5], g, z, j0, pmax);
+#define GHX_PHASE() do { if constexpr (SB) __builtin_amdgcn_sched_barrier(0); } while (0)
+  xinv_row<N, E, Cfg>(qx, buf, t, twh, twx, in.p[3] + off, lim);
+  GHX_PHASE();
+  xinv_row<N, E, Cfg>(qy, buf, t, twh, twx, in.p[4] + off, lim);
+  GHX_PHASE();
+  xinv_row<N, E, Cfg>(qz, buf, t, twh, twx, in.p[5] + off, lim);
+  GHX_PHASE();
   {
     cplx px[E];
-    xinv_row<N, E, Cfg>(px, lds, row, stride, L, t, twh, twx, in.p[0], g, z, j0, pmax);
+    xinv_row<N, E, Cfg>(px, buf, t, twh, twx, in.p[0] + off, lim);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       ay[m] = c_make(qz[m].x * px[m].x, qz[m].y * px[m].y);  // Q_z P_x
       az[m] = c_make(px[m].x * qy[m].x, px[m].y * qy[m].y);  // P_x Q_y
     }
   }
+  GHX_PHASE();
   {
     cplx py[E];
-    xinv_row<N, E, Cfg>(py, lds, row, stride, L, t, twh, twx, in.p[1], g, z, j0, pmax);
+    xinv_row<N, E, Cfg>(py, buf, t, twh, twx, in.p[1] + off, lim);
     cplx oz[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       oz[m] = c_make((az[m].x - qx[m].x * py[m].x) * tmp, (az[m].y - qx[m].y * py[m].y) * tmp);
       az[m] = c_make(py[m].x * qz[m].x, py[m].y * qz[m].y);  // now P_y Q_z
     }
-    xfwd_row<N, E, Cfg>(oz, lds, row, stride, L, t, twh, twx, out.p[2], g, z, j0, pmax);
+    xfwd_row<N, E, Cfg>(oz, buf, t, twh, twx, out.p[2] + off, lim);
   }
+  GHX_PHASE();
   {
     cplx pz[E];
-    xinv_row<N, E, Cfg>(pz, lds, row, stride, L, t, twh, twx, in.p[2], g, z, j0, pmax);
+    xinv_row<N, E, Cfg>(pz, buf, t, twh, twx, in.p[2] + off, lim);
     cplx ox[E], oy[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       ox[m] = c_make((az[m].x - qy[m].x * pz[m].x) * tmp, (az[m].y - qy[m].y * pz[m].y) * tmp);
       oy[m] = c_make((pz[m].x * qx[m].x - ay[m].x) * tmp, (pz[m].y * qx[m].y - ay[m].y) * tmp);
     }
-    xfwd_row<N, E, Cfg>(ox, lds, row, stride, L, t, twh, twx, out.p[0], g, z, j0, pmax);
-    xfwd_row<N, E, Cfg>(oy, lds, row, stride, L, t, twh, twx, out.p[1], g, z, j0, pmax);
+    GHX_PHASE();
+    xfwd_row<N, E, Cfg>(ox, buf, t, twh, twx, out.p[0] + off, lim);
+    GHX_PHASE();
+    xfwd_row<N, E, Cfg>(oy, buf, t, twh, twx, out.p[1] + off, lim);
   }
+#undef GHX_PHASE
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 
-static int ydesired_lpb(int T) {
-  int lpb = 256 / T;
-  return lpb < 1 ? 1 : lpb;
-}
 
 #if GHX_PART == 1
 int active_kx(const GridK &g) {
@@ -649,33 +732,6 @@ int active_kx(const GridK &g) {
   return n;
 }
 
-hipError_t ypass(const FFTCtx &c, int dir, int nf, const FieldSet &q, int prune) {
-  const GridK &g = c.gk;
-  const long nlines = (long)g.nz * g.nxh;
-  switch (g.ny) {
-#define GHX_CASE(n_)                                                                        \
-  case n_: {                                                                                \
-    constexpr int T = FFTCfg<n_>::T;                                                        \
-    int lpb = ydesired_lpb(T);                                                              \
-    size_t lds = (size_t)lpb * lds_line_stride(n_) * sizeof(cplx);                          \
-    dim3 grid((unsigned)((nlines + lpb - 1) / lpb), nf);                                    \
-    if (dir < 0) {                                                                          \
-      allow_lds(k_ypass<n_, -1>, lds);                                                      \
-      hipLaunchKernelGGL((k_ypass<n_, -1>), grid, dim3(lpb * T), lds, c.stream, q, nlines, c.tw_y, g, prune ? 2 : 0); \
-    } else {                                                                                \
-      allow_lds(k_ypass<n_, 1>, lds);                                                       \
-      hipLaunchKernelGGL((k_ypass<n_, 1>), grid, dim3(lpb * T), lds, c.stream, q, nlines, c.tw_y, g, prune ? 1 : 0); \
-    }                                                                                       \
-    break;                                                                                  \
-  }
-    GHX_SIZES(GHX_CASE)
-#undef GHX_CASE
-  default:
-    return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
 #endif
 
 #if GHX_PART == 2
@@ -685,7 +741,7 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 8));                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", mode == 0 ? 8 : 4));                       \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     if (mode == 0) {                                                                        \
@@ -741,7 +797,7 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, i
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 8));                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4));                                        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd_rk_hd<n_>, lds);                                                       \
@@ -765,7 +821,7 @@ hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &r
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 8));                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4));                                        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd_rk_mhd<n_>, lds);                                                      \
@@ -782,21 +838,70 @@ hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &r
 
 #endif
 
+#if GHX_PART == 1
+hipError_t yinv(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune) {
+  const GridK &g = c.gk;
+  const int kxlim = prune ? active_kx(g) : g.nxh;
+  const int nxp = p_row(g);
+  switch (g.ny) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.nxh, tile_want("GHX_LY", 8));                  \
+    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    dim3 grid((g.nxh + L - 1) / L, g.nz);                                                   \
+    allow_lds(k_yinv<n_>, lds);                                                             \
+    hipLaunchKernelGGL((k_yinv<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t yfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune) {
+  const GridK &g = c.gk;
+  const int kxlim = prune ? active_kx(g) : g.nxh;
+  const int nxp = p_row(g);
+  switch (g.ny) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.nxh, tile_want("GHX_LY", 4));                  \
+    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    dim3 grid((g.nxh + L - 1) / L, g.nz);                                                   \
+    allow_lds(k_yfwd<n_>, lds);                                                             \
+    hipLaunchKernelGGL((k_yfwd<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+#endif
+
 #if GHX_PART == 6
+
 hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out) {
   const GridK &g = c.gk;
   if (g.nx % 2) return hipErrorInvalidValue;
   RealSet rs;
   for (int f = 0; f < nf; ++f) rs.p[f] = out[f];
+  const long nrows = (long)g.nz * g.ny;
+  const int nxp = p_row(g);
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
-    constexpr int T = FFTCfgLow<n_>::T;                                                           \
-    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, tile_want("GHX_LX", 16));                                   \
-    size_t lds = (size_t)L * lds_line_stride(n_ + 1) * sizeof(cplx);                        \
-    dim3 grid((g.ny + L - 1) / L, g.nz);                                                    \
-    allow_lds(k_xinv<n_>, lds);                                                             \
-    hipLaunchKernelGGL((k_xinv<n_>), grid, dim3(L * T), lds, c.stream, in, rs, nf, g, c.tw_xh, c.tw_x); \
+    constexpr int T = FFTCfgLow<n_>::T;                                                     \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    hipLaunchKernelGGL((k_xinv<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, rs, nf, g, nxp, c.tw_xh, c.tw_x); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -812,15 +917,15 @@ hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet
   if (g.nx % 2) return hipErrorInvalidValue;
   RealSet rs;
   for (int f = 0; f < nf; ++f) rs.p[f] = const_cast<double *>(in[f]);
+  const long nrows = (long)g.nz * g.ny;
+  const int nxp = p_row(g);
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
-    constexpr int T = FFTCfgLow<n_>::T;                                                           \
-    int L = pick_L(T, lds_line_stride(n_ + 1), g.ny, tile_want("GHX_LX", 16));                                   \
-    size_t lds = (size_t)L * lds_line_stride(n_ + 1) * sizeof(cplx);                        \
-    dim3 grid((g.ny + L - 1) / L, g.nz);                                                    \
-    allow_lds(k_xfwd<n_>, lds);                                                             \
-    hipLaunchKernelGGL((k_xfwd<n_>), grid, dim3(L * T), lds, c.stream, rs, out, nf, g, c.tw_xh, c.tw_x); \
+    constexpr int T = FFTCfgLow<n_>::T;                                                     \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    hipLaunchKernelGGL((k_xfwd<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, rs, out, nf, g, nxp, c.tw_xh, c.tw_x); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -830,52 +935,38 @@ hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet
   }
   return hipGetLastError();
 }
-
 #endif
 
 #if GHX_PART == 7
-template <typename Cfg, int W, int N>
-static hipError_t launch_xcross(const FFTCtx &c, const FieldSet &in, const FieldSet &out, int want, int pmax) {
-  const GridK &g = c.gk;
-  constexpr int T = Cfg::T;
-  int L = pick_L(T, lds_line_stride(N + 1), g.ny, want);
-  size_t lds = (size_t)L * lds_line_stride(N + 1) * sizeof(cplx);
-  dim3 grid((g.ny + L - 1) / L, g.nz);
-  allow_lds(k_xcross<N, Cfg, W>, lds);
-  hipLaunchKernelGGL((k_xcross<N, Cfg, W>), grid, dim3(L * T), lds, c.stream, in, out, g, c.tw_xh, c.tw_x,
-                     tile_want("GHX_XCD", 1) == 1 ? 1 : 0, pmax);
-  return hipGetLastError();
-}
-
-static int xcross_variant() {
-  static int v = -1;
-  if (v < 0) {
-    const char *e = getenv("GHX_XCROSS_VARIANT");
-    v = e ? atoi(e) : 0;
-  }
-  return v;
-}
-
 hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out, int prune) {
   (void)mode;
   const GridK &g = c.gk;
   if (g.nx % 2) return hipErrorInvalidValue;
-  const int var = xcross_variant();
-  const int pmax = prune ? active_kx(g) : g.nxh;
+  const int kxlim = prune ? active_kx(g) : g.nxh;
+  const long nrows = (long)g.nz * g.ny;
+  const int nxp = p_row(g);
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
-  case n_:                                                                                  \
-    switch (var) {                                                                          \
-    case 1: return launch_xcross<FFTCfgLow<n_>, 1, n_>(c, in, out, 16, pmax);               \
-    default: return launch_xcross<FFTCfgLow<n_>, 3, n_>(c, in, out, tile_want("GHX_LX", 4), pmax); \
-    }
+  case n_: {                                                                                \
+    constexpr int T = FFTCfgLow<n_>::T;                                                     \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                         \
+    switch (tile_want("GHX_XV", 1)) {                                                       \
+    case 2: hipLaunchKernelGGL((k_xcross<n_, 3, 0>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
+    case 3: hipLaunchKernelGGL((k_xcross<n_, 3, 1>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
+    case 4: hipLaunchKernelGGL((k_xcross<n_, 2, 1>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
+    default: hipLaunchKernelGGL((k_xcross<n_, 2, 0>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
+    }                                                                                       \
+    break;                                                                                  \
+  }
     GHX_SIZES(GHX_CASE)
 #undef GHX_CASE
   default:
     return hipErrorInvalidValue;
   }
+  return hipGetLastError();
 }
-
 #endif
 
 }  // namespace ghx
