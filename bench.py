@@ -5,11 +5,14 @@ Workload (the 512^3 forced-turbulence configuration): 512^3 cubic 2*pi box,
 FP64, random-phase solenoidal initial velocity with E(k) ~ k^4 exp(-2 (k/3)^2)
 normalised to total energy 0.5, constant-amplitude random-phase forcing in the
 shell 2 <= |k| <= 3 whose phase is shifted every FSTEP steps (GHOST rand=1),
-nu = 2e-4, RK2, dt = 2.5e-4.  A step is one full RK2 step of the fused HIP
-path with all fields resident in HBM.
+nu = 2e-4, RK2, dt = 2.5e-4, plus 10^6 Lagrangian tracers at uniform-random
+seeded positions advanced with cubic-spline velocity interpolation (one tracer
+stage after each fluid substep, GHOST's lagpart coupling).  A step is one
+full RK2 step of the fused HIP path, tracers included, with all fields and
+particles resident in HBM.
 
 Prints one JSON line on stdout.  Everything else goes to stderr.
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--tracers 1000000] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -27,6 +30,7 @@ NU, DT, K0, U0, F0 = 2e-4, 2.5e-4, 3.0, 1.0, 0.4
 KDN, KUP = 2.0, 3.0
 FSTEP = 20
 IC_SEED, F_SEED = 1000, 2000
+PART_SEED = 3000
 
 
 def log(*a):
@@ -49,8 +53,16 @@ def read_traffic(kernel, n):
     return None
 
 
-def cpu_baseline(n, threads):
-    """Time one RK substep of the same workload with the CPU oracle (test infrastructure)."""
+def tracer_positions(n, npart):
+    """Uniform-random seeded tracer positions in grid units (the seed file of GPINIT_USERLOC)."""
+    import numpy as np
+    rng = np.random.default_rng(PART_SEED)
+    return [rng.uniform(0.0, float(n), npart) for _ in range(3)]
+
+
+def cpu_baseline(n, threads, npart):
+    """Time one RK substep and one tracer stage of the same workload with the
+    CPU oracle (test infrastructure)."""
     from oracle import oracle as O
     O.set_threads(threads)
     og = O.Grid(n)
@@ -59,10 +71,20 @@ def cpu_baseline(n, threads):
     c = [x.copy() for x in v]
     t0 = time.perf_counter()
     og.hd_substep(v, c, 2, DT, NU, f=f)
-    dt = time.perf_counter() - t0
+    t_sub = time.perf_counter() - t0
+    t_tr = 0.0
+    if npart:
+        pos = tracer_positions(n, npart)
+        x0 = [p.copy() for p in pos]
+        t0 = time.perf_counter()
+        og.tracer_stage(v, pos, x0, DT, 0.5)
+        t_tr = time.perf_counter() - t0
+    dt = t_sub + t_tr
     return {"value": 1.0 / (2.0 * dt), "unit": "steps/s", "cores": int(threads), "kind": "port",
-            "sample": f"one RK2 substep (o=2: 9 FFTs + kernels) of the {n}^3 workload timed with the C/OpenMP "
-                      f"oracle on {threads} host threads ({dt:.2f} s), steps/s = 1/(2 x substep)"}
+            "sample": f"one RK2 substep (o=2: 9 FFTs + kernels, {t_sub:.2f} s) and one tracer stage "
+                      f"({npart} tracers: 3 c2r + spline sweeps + interpolation, {t_tr:.2f} s) of the {n}^3 "
+                      f"workload timed with the C/OpenMP oracle on {threads} host threads, "
+                      f"steps/s = 1/(2 x (substep + stage))"}
 
 
 def main():
@@ -71,6 +93,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--tracers", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -87,6 +110,7 @@ def main():
 
     import ghost_amd
     from ghost_amd import _lib
+    from ghost_amd.lagpart import GPart
     from ghost_amd.pseudospec import Grid, randu
     from ghost_amd.solvers import HDSolver
 
@@ -114,8 +138,14 @@ def main():
     s.set_forcing(f)
     e0 = g.energy(*v, 1)
     del v, f
+    npart = max(0, args.tracers)
+    parts = None
+    if npart:
+        parts = GPart(g, npart, zcompat=True)
+        parts.set_positions(*[torch.from_numpy(p).cuda() for p in tracer_positions(n, npart)])
+        s.set_tracers(parts)
     torch.cuda.empty_cache()
-    log(f"grid {n}^3, <u^2> = {e0:.6f} (E = {0.5 * e0:.6f})")
+    log(f"grid {n}^3, <u^2> = {e0:.6f} (E = {0.5 * e0:.6f}), {npart} tracers")
 
     state = {"timef": FSTEP, "seed": fseed}
 
@@ -148,6 +178,9 @@ def main():
     v = s.velocity()
     e1 = g.energy(*v, 1)
     assert math.isfinite(e1), "energy is not finite"
+    if parts is not None:
+        for c, p in enumerate(parts.positions()):
+            assert bool(((p >= 0) & (p < n)).all()), "tracer outside the box"
     log(f"{args.steps} steps in {elapsed:.4f} s -> {steps_per_s:.3f} steps/s; <u^2> = {e1:.6f}")
 
     passes = {}
@@ -165,9 +198,12 @@ def main():
     traffic = read_traffic(dom, n)
 
     # algorithmic bytes of a step (18 transforms read once + written once, and the
-    # 30 spectral field reads/writes of the two updates)
+    # 30 spectral field reads/writes of the two updates), plus per tracer stage
+    # 3 c2r and per particle 3 x 64 control points, position in/out and x0
     s_r, s_c = n ** 3 * 8.0, (n // 2 + 1) * n * n * 16.0
     b_step = 18 * (s_r + s_c) + 30 * s_c
+    if npart:
+        b_step += 2 * (3 * (s_r + s_c) + npart * (3 * 64 * 8 + 24 + 24 + 24))
 
     out = {
         "metric": METRIC,
@@ -181,11 +217,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random-phase k^4 exp IC and shell-2..3 forcing generated on the host)",
+        "data": "synthetic (seeded random-phase k^4 exp IC, shell-2..3 forcing and uniform tracer positions generated on the host)",
         "config": {"workload": f"HD3D {n}^3 FP64 forced turbulence, RK2, dt={DT}, nu={NU}, k0={K0}, "
                                f"E0=0.5, forcing f0={F0} in 2<=|k|<=3 with phase shift every {FSTEP} steps; "
-                               f"no Lagrangian tracers",
-                   "grid": [n, n, n], "ord": 2, "dt": DT, "nu": NU, "tracers": 0},
+                               f"{npart} Lagrangian tracers, cubic-spline interpolation, GHOST z stencil",
+                   "grid": [n, n, n], "ord": 2, "dt": DT, "nu": NU, "tracers": npart},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "fft_passes": passes,
@@ -195,9 +231,9 @@ def main():
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle substep at {n}^3 on {threads} threads ...")
-        del s, v
+        del s, v, parts
         torch.cuda.empty_cache()
-        out["cpu_baseline"] = cpu_baseline(n, threads)
+        out["cpu_baseline"] = cpu_baseline(n, threads, npart)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.5f} steps/s")
     print(json.dumps(out), flush=True)
 

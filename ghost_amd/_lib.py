@@ -106,6 +106,15 @@ def _declare(L):
         "ghx_mhd_fields": (_I, [_P, _PP, _PP, _PP, _PP]),
         "ghx_mhd_set_forcing": (_I, [_P, _I]),
         "ghx_mhd_step": (_I, [_P, _I]),
+        "ghx_tracers_create": (_I, [_PP, _P, C.c_longlong, _I]),
+        "ghx_tracers_destroy": (_I, [_P]),
+        "ghx_tracers_positions": (_I, [_P, _PP, _PP, _PP]),
+        "ghx_tracers_velocities": (_I, [_P, _PP, _PP, _PP]),
+        "ghx_tracers_set_step": (_I, [_P]),
+        "ghx_tracers_step": (_I, [_P, _PP, _D, _D]),
+        "ghx_tracers_euler_to_lag": (_I, [_P, _P, _P]),
+        "ghx_hd_set_tracers": (_I, [_P, _P]),
+        "ghx_mhd_set_tracers": (_I, [_P, _P]),
         "ghx_prof_enable": (_I, [_I]),
         "ghx_prof_reset": (_I, []),
         "ghx_prof_read": (_I, [_I, C.POINTER(C.c_char_p), C.POINTER(_I), _DP, _DP]),

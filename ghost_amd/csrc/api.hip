@@ -12,6 +12,7 @@
 #include "../../include/ghx.h"
 #include "fft_passes.h"
 #include "pseudo.h"
+#include "tracers.h"
 
 using ghx::FieldSet;
 using ghx::FFTCtx;
@@ -90,10 +91,11 @@ extern "C" int ghx_sync(void) {
 namespace {
 enum ProfSlot {
   P_ZINV, P_ZINV_CURL, P_ZINV_MHD, P_YINV, P_YFWD, P_XINV, P_XFWD, P_XCROSS, P_ZFWD, P_ZFWD_RK_HD, P_ZFWD_RK_MHD,
-  P_NSLOTS
+  P_ZINV_SPLINE, P_TRACER_STAGE, P_NSLOTS
 };
 const char *kProfNames[P_NSLOTS] = {"zinv", "zinv_curl", "zinv_mhd", "ypass_inv", "ypass_fwd", "xinv",
-                                    "xfwd", "xcross", "zfwd", "zfwd_rk_hd", "zfwd_rk_mhd"};
+                                    "xfwd", "xcross", "zfwd", "zfwd_rk_hd", "zfwd_rk_mhd", "zinv_spline",
+                                    "tracer_stage"};
 struct ProfRec {
   int slot;
   double bytes;
@@ -427,6 +429,7 @@ struct ghx_grid {
   ghx_plan plan;
   GridK gk;
   double Dkk = 1.0;
+  double L[3] = {1, 1, 1};  // box lengths in units of 2 pi (1 unless arbsize)
   int nmax = 0;
   hipStream_t stream = 0;
   double *work = nullptr;   // reductions / spectra
@@ -484,6 +487,7 @@ extern "C" int ghx_grid_create(ghx_grid **grid, int nx, int ny, int nz, double L
   g->nmax = (int)(mx / dkk);
   k.tiny = std::fmin(1e-5, 0.1 / ((double)g->nmax * (double)g->nmax));
   g->Dkk = dkk;
+  if (arbsize) { g->L[0] = Lx; g->L[1] = Ly; g->L[2] = Lz; }
   g->gk = k;
   g->plan.ctx.gk = k;
   g->nc = nc_of(nx, ny, nz);
@@ -1014,7 +1018,11 @@ struct ghx_hd {
   uint8_t *fline = nullptr;
   bool forced = false;
   bool clean = false;  // v known to be zero outside the kept modes
+  ghx_tracers *tr = nullptr;  // Lagrangian tracers advanced with the flow
 };
+
+static int tracers_begin_step(ghx_tracers *t);
+static int tracers_stage(ghx_tracers *t, cplx *q, const cplx *const *v, double dt, double xk, int prune);
 
 // Count of non-zero modes of (a,b,c) outside the dealias set.
 static int count_unkept(ghx_grid *g, const cplx *a, const cplx *b, const cplx *c, double *n) {
@@ -1095,12 +1103,23 @@ static int hd_step_once(ghx_hd *s) {
   const cplx *const *f = s->forced ? fc : nullptr;
   const cplx *v0[3] = {s->v[0], s->v[1], s->v[2]};
   const cplx *v1[3] = {s->v1[0], s->v1[1], s->v1[2]};
+  if (s->tr) {
+    int rc = tracers_begin_step(s->tr);
+    if (rc) return rc;
+  }
   for (int o = s->ord; o >= 1; --o) {
     const bool first = (o == s->ord), last = (o == 1);
     const cplx *const *vin = first ? v0 : v1;
     cplx *const *out = last ? s->v : s->v1;
     int rc = hd_substep(s->g, s->q, o, s->dt, s->nu, vin, v0, f, s->fline, out, prune);
     if (rc) return rc;
+    // main3D.fpp:3233-3267: the tracers take a stage on the velocity this
+    // substep produced, which is dealiased whatever the input was
+    if (s->tr) {
+      const cplx *vo[3] = {out[0], out[1], out[2]};
+      rc = tracers_stage(s->tr, s->q, vo, s->dt, 1.0 / (double)o, getenv("GHX_NO_PRUNE") ? 0 : 1);
+      if (rc) return rc;
+    }
   }
   s->clean = true;  // every substep zeroes the modes outside the dealias set
   return GHX_OK;
@@ -1127,6 +1146,7 @@ struct ghx_mhd {
   uint8_t *fline = nullptr;
   bool forced = false;
   bool clean = false;
+  ghx_tracers *tr = nullptr;
 };
 
 extern "C" int ghx_mhd_create(ghx_mhd **out, ghx_grid *g, int ord, double dt, double nu, double mu) {
@@ -1263,12 +1283,206 @@ extern "C" int ghx_mhd_step(ghx_mhd *s, int nsteps) {
       s->clean = (a == 0.0 && b == 0.0);
     }
     const int prune = s->clean && !getenv("GHX_NO_PRUNE") ? 1 : 0;
+    if (s->tr) {
+      int rc = tracers_begin_step(s->tr);
+      if (rc) return rc;
+    }
     for (int o = s->ord; o >= 1; --o) {
       const bool first = (o == s->ord), last = (o == 1);
-      int rc = mhd_substep(s, o, first ? v0 : v1, last ? s->s : s->s1, prune);
+      cplx *const *out = last ? s->s : s->s1;
+      int rc = mhd_substep(s, o, first ? v0 : v1, out, prune);
       if (rc) return rc;
+      if (s->tr) {  // main3D.fpp:3233-3267, velocity after the substep
+        const cplx *vo[3] = {out[0], out[1], out[2]};
+        rc = tracers_stage(s->tr, s->q, vo, s->dt, 1.0 / (double)o, getenv("GHX_NO_PRUNE") ? 0 : 1);
+        if (rc) return rc;
+      }
     }
     s->clean = true;
   }
+  return GHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lagrangian tracers (lagpart/gparts_mod.f90, gpsplineint_mod.fpp), one rank
+// ---------------------------------------------------------------------------
+struct ghx_tracers {
+  ghx_grid *g;
+  long long n;
+  int zcompat;
+  double *mem = nullptr;  // pos(3n) x0(3n) lv(3n)
+  double *pos[3], *x0[3], *lv[3];
+  double *spl = nullptr;  // spline prefilter factors: nx/2+1 + ny + nz
+};
+
+// Spectral form of GPSplineInt_CompSpline3D (gpsplineint_mod.fpp:1158-1579):
+// the periodic spline system (1/6, 2/3, 1/6) along each axis is the circulant
+// whose symbol is b(k) = 2/3 + cos(2 pi k / n) / 3, so the control points of
+// c2r(v / (nx ny nz)) are c2r(v sx[kx] sy[ky] sz[kz]) with s = 1 / b and the
+// 1 / (nx ny nz) folded into sx.
+static int tracer_prefilter(ghx_tracers *t) {
+  const GridK &k = t->g->gk;
+  std::vector<double> h((size_t)k.nxh + k.ny + k.nz);
+  const long double twopi = 6.283185307179586476925286766559L;
+  auto inv_b = [&](int q, int n) {
+    return 1.0L / (2.0L / 3.0L + cosl(twopi * (long double)q / (long double)n) / 3.0L);
+  };
+  const long double rmp = 1.0L / ((long double)k.nx * k.ny * k.nz);
+  for (int i = 0; i < k.nxh; ++i) h[i] = (double)(rmp * inv_b(i, k.nx));
+  for (int j = 0; j < k.ny; ++j) h[k.nxh + j] = (double)inv_b(j, k.ny);
+  for (int q = 0; q < k.nz; ++q) h[k.nxh + k.ny + q] = (double)inv_b(q, k.nz);
+  HIPCHK(hipMalloc(&t->spl, h.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(t->spl, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  return GHX_OK;
+}
+
+extern "C" int ghx_tracers_create(ghx_tracers **out, ghx_grid *g, long long n, int zcompat) {
+  ARGCHK(out && g && n >= 0, "bad tracers_create arguments");
+  if (g->gk.nx < 4 || g->gk.ny < 4 || g->gk.nz < 4)
+    return set_err(GHX_ERR_SIZE, "cubic-spline tracers need at least 4 points along each axis");
+  ghx_tracers *t = new (std::nothrow) ghx_tracers();
+  if (!t) return set_err(GHX_ERR_NOMEM, "host allocation failed");
+  t->g = g; t->n = n; t->zcompat = zcompat ? 1 : 0;
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  hipError_t e = hipMalloc(&t->mem, 9 * nn * sizeof(double));
+  if (e != hipSuccess) { delete t; HIPCHK(e); }
+  for (int c = 0; c < 3; ++c) {
+    t->pos[c] = t->mem + c * nn;
+    t->x0[c] = t->mem + (3 + c) * nn;
+    t->lv[c] = t->mem + (6 + c) * nn;
+  }
+  int rc = tracer_prefilter(t);
+  if (!rc) {
+    e = ghx::op_zero(g->stream, t->mem, 9 * nn * sizeof(double));
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) rc = set_err(GHX_ERR_HIP, "zeroing tracer arrays: %s", hipGetErrorString(e));
+  }
+  if (rc) {
+    (void)hipFree(t->mem);
+    if (t->spl) (void)hipFree(t->spl);
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return GHX_OK;
+}
+
+extern "C" int ghx_tracers_destroy(ghx_tracers *t) {
+  if (!t) return GHX_OK;
+  (void)hipFree(t->mem);
+  (void)hipFree(t->spl);
+  delete t;
+  return GHX_OK;
+}
+
+extern "C" int ghx_tracers_positions(ghx_tracers *t, double **px, double **py, double **pz) {
+  ARGCHK(t, "null tracers");
+  if (px) *px = t->pos[0];
+  if (py) *py = t->pos[1];
+  if (pz) *pz = t->pos[2];
+  return GHX_OK;
+}
+
+extern "C" int ghx_tracers_velocities(ghx_tracers *t, double **ux, double **uy, double **uz) {
+  ARGCHK(t, "null tracers");
+  if (ux) *ux = t->lv[0];
+  if (uy) *uy = t->lv[1];
+  if (uz) *uz = t->lv[2];
+  return GHX_OK;
+}
+
+// GPart_SetStepRKK (gparts_mod.f90:1839-1908): x0 = x.
+static int tracers_begin_step(ghx_tracers *t) {
+  const double *src[3] = {t->pos[0], t->pos[1], t->pos[2]};
+  HIPCHK(ghx::tracer_copy(t->g->stream, t->x0, src, t->n));
+  return GHX_OK;
+}
+
+extern "C" int ghx_tracers_set_step(ghx_tracers *t) {
+  ARGCHK(t, "null tracers");
+  return tracers_begin_step(t);
+}
+
+// One stage on 9 scratch slots of q: Q (0-2) -> P (3-5) -> real control
+// points (6-8), then the interpolation and position update.
+static int tracers_stage(ghx_tracers *t, cplx *q, const cplx *const *v, double dt, double xk, int prune) {
+  ghx_grid *g = t->g;
+  const FFTCtx &ctx = g->plan.ctx;
+  const size_t slot = ghx::slot_points(g->gk);
+  FieldSet in, qq, pp;
+  double *r[3];
+  for (int c = 0; c < 3; ++c) {
+    in.p[c] = const_cast<cplx *>(v[c]);
+    qq.p[c] = q + c * slot;
+    pp.p[c] = q + (3 + c) * slot;
+    r[c] = reinterpret_cast<double *>(q + (6 + c) * slot);
+  }
+  const double sc = (double)g->nc * sizeof(cplx), sr = (double)g->nr * sizeof(double);
+  const PruneShares ps = (prune && g_prof.on) ? prune_shares(g) : PruneShares{1.0, 1.0, 1.0};
+  PROF(P_ZINV_SPLINE, (3 * ps.fm + 3 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 3, 3, in, qq, prune, t->spl));
+  PROF(P_YINV, (3 * ps.fl + 3 * ps.fx) * sc, ctx.stream, ghx::yinv(ctx, 3, qq, pp, prune));
+  PROF(P_XINV, 3 * (ps.fx * sc + sr), ctx.stream, ghx::xinv(ctx, 3, pp, r, prune));
+  ghx::TracerStage st;
+  const int n[3] = {g->gk.nx, g->gk.ny, g->gk.nz};
+  for (int c = 0; c < 3; ++c) {
+    st.coef[c] = r[c];
+    st.pos[c] = t->pos[c];
+    st.x0[c] = t->x0[c];
+    st.lv[c] = t->lv[c];
+    // GPart_StepRKK: dtfact = dt * xk * invdel, invdel = n / (2 pi L) (gparts_mod.f90:207-213)
+    const double pi = 3.14159265358979323846;
+    st.dtf[c] = dt * xk * ((double)n[c] / (2 * pi * g->L[c]));
+  }
+  st.nx = n[0]; st.ny = n[1]; st.nz = n[2];
+  st.np = t->n;
+  st.zcompat = t->zcompat;
+  // per particle: position and x0 read, 3 x 64 control points, position and velocity written
+  PROF(P_TRACER_STAGE, (double)t->n * (24 + 24 + 3 * 64 * 8 + 24 + 24), ctx.stream, ghx::tracer_stage(ctx.stream, st));
+  return GHX_OK;
+}
+
+extern "C" int ghx_tracers_step(ghx_tracers *t, const void *const *v, double dt, double xk) {
+  ARGCHK(t && v && v[0] && v[1] && v[2], "bad tracers_step arguments");
+  int rc = plan_scratch(&t->g->plan, 9);
+  if (rc) return rc;
+  const cplx *vv[3] = {C_(v[0]), C_(v[1]), C_(v[2])};
+  return tracers_stage(t, t->g->plan.q, vv, dt, xk, 0);
+}
+
+// GPart_EulerToLag (gparts_mod.f90:2216-2269) of one real field r[z][y][x].
+extern "C" int ghx_tracers_euler_to_lag(ghx_tracers *t, const double *field, double *out) {
+  ARGCHK(t && field && out, "bad tracers_euler_to_lag arguments");
+  ghx_grid *g = t->g;
+  int rc = grid_ctmp(g, 1);
+  if (!rc) rc = plan_scratch(&g->plan, 3);
+  if (rc) return rc;
+  void *spec = g->ctmp;
+  if ((rc = do_r2c(&g->plan, 1, &field, &spec))) return rc;
+  const FFTCtx &ctx = g->plan.ctx;
+  const size_t slot = ghx::slot_points(g->gk);
+  FieldSet in, qq, pp;
+  in.p[0] = g->ctmp;
+  qq.p[0] = g->plan.q;
+  pp.p[0] = g->plan.q + slot;
+  double *r = reinterpret_cast<double *>(g->plan.q + 2 * slot);
+  HIPCHK(ghx::zinv(ctx, 3, 1, in, qq, 0, t->spl));
+  HIPCHK(ghx::yinv(ctx, 1, qq, pp, 0));
+  HIPCHK(ghx::xinv(ctx, 1, pp, &r, 0));
+  const double *pos[3] = {t->pos[0], t->pos[1], t->pos[2]};
+  HIPCHK(ghx::spline_interp(ctx.stream, r, pos, t->n, g->gk.nx, g->gk.ny, g->gk.nz, t->zcompat, out));
+  return GHX_OK;
+}
+
+extern "C" int ghx_hd_set_tracers(ghx_hd *s, ghx_tracers *t) {
+  ARGCHK(s, "null solver");
+  ARGCHK(!t || t->g == s->g, "tracers and solver must share the grid");
+  s->tr = t;
+  return GHX_OK;
+}
+
+extern "C" int ghx_mhd_set_tracers(ghx_mhd *s, ghx_tracers *t) {
+  ARGCHK(s, "null solver");
+  ARGCHK(!t || t->g == s->g, "tracers and solver must share the grid");
+  s->tr = t;
   return GHX_OK;
 }

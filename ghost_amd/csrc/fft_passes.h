@@ -57,12 +57,15 @@ inline size_t slot_points(const GridK &g) { return (size_t)g.nz * g.ny * p_row(g
 // prune=1: the input is dealiased (zero outside tiny <= kn2 <= kmax): only
 // lines that carry kept modes are read and written (every pass below takes
 // the same flag; see the kernels for exactly what each one skips).
-hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out, int prune = 0);
+// mode 3: nf fields in, nf out, each mode scaled by aux[kx] aux[nxh+ky]
+// aux[nxh+ny+kz] (the periodic cubic-spline prefilter of the tracers).
+hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out, int prune = 0,
+                const double *aux = nullptr);
 // Y^-1: Q -> P.  Y forward: P -> Q.
 hipError_t yinv(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune = 0);
 hipError_t yfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune = 0);
-// X^-1: P -> real rows.
-hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out);
+// X^-1: P -> real rows (prune: P holds data only for kx < active_kx).
+hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out, int prune = 0);
 // X forward: real rows -> P.
 hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet &out);
 // Fused X^-1 . cross product . X on P rows: in.p[0..2] = A, in.p[3..5] = B

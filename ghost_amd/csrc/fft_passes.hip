@@ -39,7 +39,7 @@ static size_t g_max_lds = 0;
 static size_t max_lds() {
   if (!g_max_lds) {
     int dev = 0, v = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
       v = 65536;
     g_max_lds = (size_t)v;
@@ -72,7 +72,7 @@ static int tile_want(const char *env, int dflt) {
 }
 
 template <typename K> static void allow_lds(K kern, size_t bytes) {
-  if (bytes > 49152) hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (bytes > 49152) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 // ---------------------------------------------------------------------------
@@ -144,7 +144,7 @@ GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
 
 template <int N, int MODE>
 __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ tw,
-                                             int prune) {
+                                             int prune, const double *__restrict__ aux) {
   using F = LineFFT<N, +1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
@@ -162,11 +162,18 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
   cplx *row = lds + lt * stride;
   const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
   auto need = [&](int k) { return ok && (!prune || kn2_of(g, i, j, k) <= g.kmax); };
-  if constexpr (MODE == 0) {
+  if constexpr (MODE == 0 || MODE == 3) {
+    // MODE 3: periodic cubic-spline prefilter.  aux = [sx (nx/2+1) | sy (ny) |
+    // sz (nz)] and every mode is scaled by sx[kx] sy[ky] sz[kz] on load.
+    double sxy = 1.0;
+    if constexpr (MODE == 3) sxy = aux[i] * aux[g.nxh + (ok ? j : 0)];
     for (int f = 0; f < nf; ++f) {
       cplx v[E];
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = need(t + T * m) ? in.p[f][sline + t + T * m] : c_make(0.0, 0.0);
+      for (int m = 0; m < E; ++m) {
+        v[m] = need(t + T * m) ? in.p[f][sline + t + T * m] : c_make(0.0, 0.0);
+        if constexpr (MODE == 3) v[m] = c_scale(v[m], sxy * aux[g.nxh + g.ny + t + T * m]);
+      }
       zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0, lmask);
     }
   } else {
@@ -597,7 +604,8 @@ struct RealSet {
 
 template <int N>
 __global__ __launch_bounds__(256) void k_xinv(FieldSet in, RealSet out, int nf, GridK g, int nxp,
-                                             const cplx *__restrict__ twh, const cplx *__restrict__ twx) {
+                                             const cplx *__restrict__ twh, const cplx *__restrict__ twx,
+                                             int kxlim) {
   using Cfg = FFTCfgLow<N>;
   constexpr int T = Cfg::T, E = Cfg::E;
   extern __shared__ cplx lds[];
@@ -612,7 +620,7 @@ __global__ __launch_bounds__(256) void k_xinv(FieldSet in, RealSet out, int nf, 
   cplx *buf = lds + lt * lds_line_stride(N);
   for (int f = 0; f < nf; ++f) {
     cplx v[E];
-    xinv_row<N, E, Cfg>(v, buf, t, twh, twx, in.p[f] + (ok ? r : 0) * nxp, ok ? g.nxh : 0);
+    xinv_row<N, E, Cfg>(v, buf, t, twh, twx, in.p[f] + (ok ? r : 0) * nxp, ok ? kxlim : 0);
     if (ok) {
       cplx *dst = reinterpret_cast<cplx *>(out.p[f] + r * g.nx);
 #pragma unroll
@@ -735,7 +743,9 @@ int active_kx(const GridK &g) {
 #endif
 
 #if GHX_PART == 2
-hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out, int prune) {
+hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out, int prune,
+                const double *aux) {
+  if (mode == 3 && !aux) return hipErrorInvalidValue;
   const GridK &g = c.gk;
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
@@ -746,13 +756,16 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     if (mode == 0) {                                                                        \
       allow_lds(k_zinv<n_, 0>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 0>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune); \
+      hipLaunchKernelGGL((k_zinv<n_, 0>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     } else if (mode == 1) {                                                                 \
       allow_lds(k_zinv<n_, 1>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 1>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune); \
-    } else {                                                                                \
+      hipLaunchKernelGGL((k_zinv<n_, 1>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+    } else if (mode == 2) {                                                                 \
       allow_lds(k_zinv<n_, 2>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 2>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune); \
+      hipLaunchKernelGGL((k_zinv<n_, 2>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+    } else {                                                                                \
+      allow_lds(k_zinv<n_, 3>, lds);                                                        \
+      hipLaunchKernelGGL((k_zinv<n_, 3>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     }                                                                                       \
     break;                                                                                  \
   }
@@ -888,9 +901,10 @@ hipError_t yfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 
 #if GHX_PART == 6
 
-hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out) {
+hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out, int prune) {
   const GridK &g = c.gk;
   if (g.nx % 2) return hipErrorInvalidValue;
+  const int kxlim = prune ? active_kx(g) : g.nxh;
   RealSet rs;
   for (int f = 0; f < nf; ++f) rs.p[f] = out[f];
   const long nrows = (long)g.nz * g.ny;
@@ -901,7 +915,7 @@ hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out)
     constexpr int T = FFTCfgLow<n_>::T;                                                     \
     const int rpb = rows_per_block(T);                                                      \
     size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
-    hipLaunchKernelGGL((k_xinv<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, rs, nf, g, nxp, c.tw_xh, c.tw_x); \
+    hipLaunchKernelGGL((k_xinv<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, rs, nf, g, nxp, c.tw_xh, c.tw_x, kxlim); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)

@@ -23,6 +23,7 @@ class Grid:
         nz = nx if nz is None else nz
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nxh = self.nx // 2 + 1
+        self.L = tuple(float(x) for x in L) if arbsize else (1.0, 1.0, 1.0)
         L_ = _lib.load()
         h = C.c_void_p()
         _lib.check(L_.ghx_grid_create(C.byref(h), self.nx, self.ny, self.nz, float(L[0]), float(L[1]),

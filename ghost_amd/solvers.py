@@ -66,6 +66,12 @@ class HDSolver(_Resident):
     def step(self, n=1):
         _lib.check(_lib.load().ghx_hd_step(self._h, int(n)))
 
+    def set_tracers(self, parts):
+        """Advance the GPart `parts` with the flow: one tracer stage after every
+        substep (main3D.fpp:3202-3267); None detaches."""
+        _lib.check(_lib.load().ghx_hd_set_tracers(self._h, parts._h if parts is not None else None))
+        self._parts = parts  # keep the tracer handle alive while attached
+
     def phaseshift_forcing(self, cdump):
         """The rand=1 forcing update of main3D.fpp:2544-2556 on the resident f."""
         L = _lib.load()
@@ -116,3 +122,8 @@ class MHDSolver(_Resident):
 
     def step(self, n=1):
         _lib.check(_lib.load().ghx_mhd_step(self._h, int(n)))
+
+    def set_tracers(self, parts):
+        """As HDSolver.set_tracers."""
+        _lib.check(_lib.load().ghx_mhd_set_tracers(self._h, parts._h if parts is not None else None))
+        self._parts = parts

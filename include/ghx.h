@@ -189,6 +189,40 @@ int ghx_mhd_set_forcing(ghx_mhd *s, int enabled);
 int ghx_mhd_step(ghx_mhd *s, int nsteps);
 
 /* ---------------------------------------------------------------------- */
+/* Lagrangian tracers: GHOST's GPart with cubic-spline interpolation on one */
+/* rank (lagpart/gparts_mod.f90, lagpart/gpsplineint_mod.fpp).  Positions   */
+/* are in grid units, [0, n) along each axis, as GPart's px_/py_/pz_.        */
+/* ---------------------------------------------------------------------- */
+typedef struct ghx_tracers ghx_tracers;
+/* GPart_ctor (gparts_mod.f90:148-327) for n tracers on grid g (at least 4
+ * points per axis).  zcompat = 1 reproduces GHOST's z stencil: the spline is
+ * set up for 3 ghost planes (gparts_mod.f90:291) while the extended field
+ * holds 2 (gpartcomm_mod.f90:1005-1057), so the field is sampled one grid
+ * cell above the particle and the top cell is clamped
+ * (gpsplineint_mod.fpp:653-661).  zcompat = 0 interpolates at the particle.
+ * Positions start at 0; write them through ghx_tracers_positions. */
+int ghx_tracers_create(ghx_tracers **t, ghx_grid *g, long long n, int zcompat);
+int ghx_tracers_destroy(ghx_tracers *t);
+/* device pointers of the positions (GPart px_, py_, pz_; n doubles each) */
+int ghx_tracers_positions(ghx_tracers *t, double **px, double **py, double **pz);
+/* device pointers of the velocities interpolated by the last stage (lvx_, lvy_, lvz_) */
+int ghx_tracers_velocities(ghx_tracers *t, double **ux, double **uy, double **uz);
+/* GPart_SetStepRKK (gparts_mod.f90:1839-1908): x0 = x at the start of an RK step */
+int ghx_tracers_set_step(ghx_tracers *t);
+/* One RK stage, main3D.fpp:3233-3267 + GPart_StepRKK (gparts_mod.f90:2046-2145)
+ * + GPart_EndStageRKK (:1911-2043): u = spline interpolant of c2r(v/(nx ny nz))
+ * at x, then x = x0 + dt xk n/(2 pi L) u, periodised.  v: 3 spectral fields. */
+int ghx_tracers_step(ghx_tracers *t, const void *const *v, double dt, double xk);
+/* GPart_EulerToLag (gparts_mod.f90:2216-2269): out[q] = spline interpolant of
+ * the real field r[z][y][x] at tracer q (n doubles, device) */
+int ghx_tracers_euler_to_lag(ghx_tracers *t, const double *field, double *out);
+/* Advance the tracers inside ghx_hd_step / ghx_mhd_step: x0 = x at the start
+ * of each step and one tracer stage after every substep, on the velocity that
+ * substep produced (main3D.fpp:3202-3267).  t = NULL detaches. */
+int ghx_hd_set_tracers(ghx_hd *s, ghx_tracers *t);
+int ghx_mhd_set_tracers(ghx_mhd *s, ghx_tracers *t);
+
+/* ---------------------------------------------------------------------- */
 /* Pass profiler (replaces the GTStart/GTStop handles around the fftp      */
 /* phases, utils/gtimer_mod.f90:37-185 and fftp-3/fftp3D.fpp:244-312):      */
 /* HIP events recorded around every pass launch on its stream.  Slots name  */

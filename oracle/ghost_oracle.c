@@ -1012,3 +1012,235 @@ void og_mhd_step(const og_grid *g, int ord, double dt, double nu, double mu, ocp
   }
   free(W);
 }
+
+/* ======================================================================
+ * Lagrangian tracers (3D/src/lagpart/), single rank, cubic splines,
+ * voxel-database exchange (GHOST's default ilgexchtype), no internal
+ * acceleration (dopacc = 0).  Positions are in grid units, [0, n).
+ * ====================================================================== */
+
+/* Factorisation of the cyclic tridiagonal system (1/6, 2/3, 1/6) of the
+ * periodic cubic B-spline, GPSplineInt_MatInvQ (lagpart/gpsplineint_mod.fpp:
+ * 1077-1125).  Index 1..n as in GHOST; element 0 unused. */
+typedef struct {
+  int n;
+  double *bet, *p, *gam, *xx;
+  double zeta, a, c;
+} cyc_fac;
+
+static cyc_fac *cyc_new(int n) {
+  cyc_fac *f = (cyc_fac *)calloc(1, sizeof(cyc_fac));
+  f->n = n;
+  f->bet = (double *)calloc((size_t)n + 2, sizeof(double));
+  f->p = (double *)calloc((size_t)n + 2, sizeof(double));
+  f->gam = (double *)calloc((size_t)n + 2, sizeof(double));
+  f->xx = (double *)calloc((size_t)n + 2, sizeof(double));
+  const double a = 1.0 / 6.0, c = 1.0 / 6.0, b = 2.0 / 3.0; /* sub, super, diagonal */
+  f->a = a; f->c = c;
+  double beta;
+  f->bet[1] = 1. / b;
+  f->p[1] = a * f->bet[1];
+  f->xx[1] = c;
+  beta = b;
+  for (int i = 2; i <= n - 2; ++i) {
+    f->gam[i] = c * f->bet[i - 1];
+    f->bet[i] = 1. / (b - a * f->gam[i]);
+    f->p[i] = -f->p[i - 1] * a * f->bet[i];
+    beta = beta - f->xx[i - 1] * f->p[i - 1];
+    f->xx[i] = -f->xx[i - 1] * f->gam[i];
+  }
+  f->gam[n - 1] = c * f->bet[n - 2];
+  f->bet[n - 1] = 1. / (b - a * f->gam[n - 1]);
+  f->gam[n] = (c - f->p[n - 2] * a) * f->bet[n - 1];
+  f->zeta = a - f->xx[n - 2] * f->gam[n - 1];
+  beta = beta - f->xx[n - 2] * f->p[n - 2];
+  f->bet[n] = 1. / (beta - f->zeta * f->gam[n]);
+  return f;
+}
+static void cyc_free(cyc_fac *f) {
+  free(f->bet); free(f->p); free(f->gam); free(f->xx); free(f);
+}
+
+/* Solve one line in place (elements s[(i-1)*st], i = 1..n), in the order of
+ * the x-sweep of GPSplineInt_CompSpline3D (gpsplineint_mod.fpp:1200-1262).
+ * ghost != 0 keeps GHOST's row-n update with xx(n-2)*s(n-1) (:1240); ghost
+ * == 0 uses xx(n-2)*s(n-2), which makes the solve exact.  The two differ by
+ * a multiple of xx(n-2) ~ (2-sqrt 3)^(n-2): below 1e-17 for n >= 32. */
+static void cyc_solve(const cyc_fac *f, double *s0, size_t st, double *w, int ghost) {
+  const int n = f->n;
+  double *s = w; /* 1-based work copy */
+  for (int i = 1; i <= n; ++i) s[i] = s0[(size_t)(i - 1) * st];
+  double fn1 = s[n - 1];
+  s[1] = s[1] * f->bet[1];
+  /* s[n] keeps f(n) */
+  for (int i = 2; i <= n - 2; ++i) s[i] = (s[i] - f->a * s[i - 1]) * f->bet[i];
+  for (int i = 2; i <= n - 2; ++i) s[n] = s[n] - f->xx[i - 1] * s[i - 1];
+  s[n - 1] = (fn1 - f->a * s[n - 2]) * f->bet[n - 1];
+  s[n] = s[n] - f->xx[n - 2] * (ghost ? s[n - 1] : s[n - 2]);
+  s[n] = (s[n] - s[n - 1] * f->zeta) * f->bet[n];
+  s[n - 1] = s[n - 1] - f->gam[n] * s[n];
+  for (int i = n - 2; i >= 1; --i) s[i] = s[i] - f->gam[i + 1] * s[i + 1] - f->p[i] * s[n];
+  for (int i = 1; i <= n; ++i) s0[(size_t)(i - 1) * st] = s[i];
+}
+
+/* GPSplineInt_CompSpline3D (gpsplineint_mod.fpp:1158-1579): spline control
+ * points of a real field r[z][y][x], solved along x, then y, then z.  The
+ * z-sweep runs on GHOST's transposed copy there; the arithmetic per line is
+ * the same.  coef may alias field.  Needs n >= 4 along each axis. */
+void og_spline_coef(const og_grid *g, const double *field, double *coef, int ghost) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const size_t nxy = (size_t)nx * ny;
+  if (coef != field) memcpy(coef, field, sizeof(double) * nxy * nz);
+  cyc_fac *fx = cyc_new(nx), *fy = cyc_new(ny), *fz = cyc_new(nz);
+  const int nmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+#pragma omp parallel
+  {
+    double *w = (double *)malloc(sizeof(double) * ((size_t)nmax + 2));
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)ny * nz; ++l) cyc_solve(fx, coef + (size_t)l * nx, 1, w, ghost);
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)nx * nz; ++l) {
+      long k = l / nx, i = l % nx;
+      cyc_solve(fy, coef + (size_t)k * nxy + i, (size_t)nx, w, ghost);
+    }
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)nxy; ++l) cyc_solve(fz, coef + l, nxy, w, ghost);
+    free(w);
+  }
+  cyc_free(fx); cyc_free(fy); cyc_free(fz);
+}
+
+/* Interpolation weights of GPSplineInt_Interp3D (gpsplineint_mod.fpp:268-283):
+ * w[0..2] in GHOST's expressions, w[3] = 1 - w0 - w1 - w2 (:393-404). */
+static inline void spline_w(double t, double w[4]) {
+  const double sixth = 1.0 / 6.0, four = 4.0, three = 3.0, six = 6.0;
+  const double tm = 1.0 - t;
+  w[0] = sixth * tm * tm * tm;
+  w[1] = sixth * (four + t * t * (three * t - six));
+  w[2] = sixth * (four + tm * tm * (three * tm - six));
+  w[3] = 1.0 - w[0] - w[1] - w[2];
+}
+
+/* Stencil of GPSplineInt_PartUpdate3D (gpsplineint_mod.fpp:560-667) for one
+ * coordinate.  x and y: points floor-1 .. floor+2 (periodic), t = x - floor.
+ * z with zcompat: GHOST builds the spline with nzghost = intorder = 3
+ * (gparts_mod.f90:291 passes intorder where GPSplineInt_ctor expects the
+ * ghost count, gpsplineint_mod.fpp:159) while the extended field holds 2
+ * ghost planes per side (gpartcomm_mod.f90:1005-1057, :2632-2670): the
+ * control-point index is klg = clamp(int(z + 3), 1, nz + 1) into planes that
+ * map to field plane klg - 3 (0-based), i.e. points floor .. floor+3.  Below
+ * z = nz - 1 that interpolates the field at z + 1; in the top cell the clamp
+ * gives t in [1, 2).  zcompat == 0 treats z like x and y. */
+static inline void stencil_xy(double x, int n, int idx[4], double *t) {
+  int i0 = (int)x; /* Fortran INT: truncation, x >= 0 */
+  *t = x - (double)i0;
+  idx[1] = i0 % n;
+  idx[2] = (idx[1] + 1) % n;
+  idx[3] = (idx[2] + 1) % n;
+  idx[0] = (n + idx[1] - 1) % n;
+}
+static inline void stencil_z(double z, int n, int zcompat, int idx[4], double *t) {
+  if (!zcompat) { stencil_xy(z, n, idx, t); return; }
+  const double zs = z + 3.0; /* (zp - xbnds(3,1)) * dxi with xbnds(3,1) = -3, dxi = 1 */
+  int kl = (int)zs;
+  if (kl > n + 1) kl = n + 1;
+  if (kl < 1) kl = 1;
+  *t = zs - (double)kl;
+  for (int m = 0; m < 4; ++m) idx[m] = (((kl - 3 + m) % n) + n) % n;
+}
+
+/* GPSplineInt_Interp3D (gpsplineint_mod.fpp:227-480) of control points coef
+ * at np points, summed in GHOST's order (x fastest, then y, then z). */
+void og_spline_interp(const og_grid *g, const double *coef, long np, const double *px, const double *py,
+                      const double *pz, int zcompat, double *out) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const size_t nxy = (size_t)nx * ny;
+  OMP_FOR
+  for (long q = 0; q < np; ++q) {
+    int ix[4], iy[4], iz[4];
+    double tx, ty, tz, wx[4], wy[4], wz[4];
+    stencil_xy(px[q], nx, ix, &tx);
+    stencil_xy(py[q], ny, iy, &ty);
+    stencil_z(pz[q], nz, zcompat, iz, &tz);
+    spline_w(tx, wx);
+    spline_w(ty, wy);
+    spline_w(tz, wz);
+    double acc = 0.0;
+    for (int c = 0; c < 4; ++c)
+      for (int b = 0; b < 4; ++b)
+        for (int a = 0; a < 4; ++a)
+          acc = acc + coef[iz[c] * nxy + (size_t)iy[b] * nx + ix[a]] * wx[a] * wy[b] * wz[c];
+    out[q] = acc;
+  }
+}
+
+/* GPart_MakePeriodicP (gparts_mod.f90:2270-2318): p = modulo(p + 2 n, n). */
+static inline double wrap_p(double p, double n) { return fmod(p + 2.0 * n, n); }
+
+/* One tracer stage, main3D.fpp:3233-3267 + GPart_StepRKK (gparts_mod.f90:
+ * 2046-2145) + GPart_EndStageRKK (:1911-2043, voxel-database branch):
+ *   R_c = c2r(v_c / (nx ny nz)); u_c = Interp3D(CompSpline3D(R_c)) at the
+ *   current positions (all three components at the positions the stage
+ *   started from); x_c = x0_c + (dt xk n_c / (2 pi L_c)) u_c; x and y are
+ *   wrapped twice (StepRKK, then EndStageRKK), z once.
+ * lv (3*np, optional) receives the interpolated velocities. */
+void og_tracer_stage(const og_grid *g, const ocplx *vx, const ocplx *vy, const ocplx *vz, long np,
+                     double *px, double *py, double *pz, const double *x0, const double *y0,
+                     const double *z0, double dt, double xk, int zcompat, int ghost_solve, double *lv) {
+  const size_t nc = (size_t)g->nxh * g->ny * g->nz, nr = (size_t)g->nx * g->ny * g->nz;
+  const double rmp = 1.0 / ((double)g->nx * (double)g->ny * (double)g->nz);
+  ocplx *C7 = (ocplx *)malloc(sizeof(ocplx) * nc);
+  double *R = (double *)malloc(sizeof(double) * nr);
+  double *u = (double *)malloc(sizeof(double) * 3 * (size_t)np);
+  const ocplx *v[3] = {vx, vy, vz};
+  for (int c = 0; c < 3; ++c) {
+    OMP_FOR
+    for (size_t m = 0; m < nc; ++m) C7[m] = v[c][m] * rmp;
+    og_c2r(g, C7, R);
+    og_spline_coef(g, R, R, ghost_solve);
+    og_spline_interp(g, R, np, px, py, pz, zcompat, u + (size_t)c * np);
+  }
+  const double L[3] = {g->Lx, g->Ly, g->Lz};
+  const int n[3] = {g->nx, g->ny, g->nz};
+  double dtf[3];
+  for (int c = 0; c < 3; ++c) dtf[c] = dt * xk * ((double)n[c] / (2 * M_PI * L[c]));
+  OMP_FOR
+  for (long q = 0; q < np; ++q) {
+    double x = x0[q] + dtf[0] * u[q];
+    double y = y0[q] + dtf[1] * u[np + q];
+    double z = z0[q] + dtf[2] * u[2 * np + q];
+    x = wrap_p(x, (double)n[0]);
+    y = wrap_p(y, (double)n[1]);
+    x = wrap_p(x, (double)n[0]);
+    y = wrap_p(y, (double)n[1]);
+    z = wrap_p(z, (double)n[2]);
+    px[q] = x; py[q] = y; pz[q] = z;
+  }
+  if (lv) memcpy(lv, u, sizeof(double) * 3 * (size_t)np);
+  free(C7); free(R); free(u);
+}
+
+/* HD RK step with tracers: GPart_SetStepRKK (gparts_mod.f90:1839-1908)
+ * stores x0 = x, then every fluid substep o = ord..1 is followed by a tracer
+ * stage with xk = 1/o on the velocity that substep produced
+ * (main3D.fpp:3202-3267). */
+void og_hd_step_tracers(const og_grid *g, int ord, double dt, double nu, ocplx *vx, ocplx *vy, ocplx *vz,
+                        const ocplx *fx, const ocplx *fy, const ocplx *fz, long np, double *px, double *py,
+                        double *pz, int zcompat, int ghost_solve) {
+  size_t nc = ncplx(g);
+  ocplx *C = (ocplx *)malloc(sizeof(ocplx) * nc * 3);
+  double *x0 = (double *)malloc(sizeof(double) * 3 * (size_t)np);
+  memcpy(C, vx, sizeof(ocplx) * nc);
+  memcpy(C + nc, vy, sizeof(ocplx) * nc);
+  memcpy(C + 2 * nc, vz, sizeof(ocplx) * nc);
+  memcpy(x0, px, sizeof(double) * np);
+  memcpy(x0 + np, py, sizeof(double) * np);
+  memcpy(x0 + 2 * np, pz, sizeof(double) * np);
+  for (int o = ord; o >= 1; --o) {
+    og_hd_substep(g, o, dt, nu, vx, vy, vz, C, C + nc, C + 2 * nc, fx, fy, fz);
+    og_tracer_stage(g, vx, vy, vz, np, px, py, pz, x0, x0 + np, x0 + 2 * np, dt, 1.0 / (double)o, zcompat,
+                    ghost_solve, NULL);
+  }
+  free(C);
+  free(x0);
+}

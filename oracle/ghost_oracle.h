@@ -109,4 +109,19 @@ void og_mhd_step(const og_grid *g, int ord, double dt, double nu, double mu,
 /* pseudospec3D_filt.f90:200-241 */
 void og_sharp(const og_grid *g, const ocplx *a, ocplx *b, double delta);
 
+/* Lagrangian tracers (lagpart/): spline control points (GPSplineInt_CompSpline3D,
+ * ghost != 0 keeps GHOST's row-n update, see the source), interpolation at np
+ * points (GPSplineInt_Interp3D, zcompat != 0 reproduces GHOST's z stencil),
+ * one RK stage of the tracers (main3D.fpp:3233-3267 + GPart_StepRKK) and an HD
+ * step with tracers. Positions in grid units. */
+void og_spline_coef(const og_grid *g, const double *field, double *coef, int ghost);
+void og_spline_interp(const og_grid *g, const double *coef, long np, const double *px, const double *py,
+                      const double *pz, int zcompat, double *out);
+void og_tracer_stage(const og_grid *g, const ocplx *vx, const ocplx *vy, const ocplx *vz, long np,
+                     double *px, double *py, double *pz, const double *x0, const double *y0,
+                     const double *z0, double dt, double xk, int zcompat, int ghost_solve, double *lv);
+void og_hd_step_tracers(const og_grid *g, int ord, double dt, double nu, ocplx *vx, ocplx *vy, ocplx *vz,
+                        const ocplx *fx, const ocplx *fy, const ocplx *fz, long np, double *px, double *py,
+                        double *pz, int zcompat, int ghost_solve);
+
 #endif

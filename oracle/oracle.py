@@ -88,6 +88,12 @@ def _declare(L):
                                 _P, _P, _P]
     L.og_mhd_step.argtypes = [_G, C.c_int, C.c_double, C.c_double, C.c_double] + [_P] * 12
     L.og_sharp.argtypes = [_G, _P, _P, C.c_double]
+    L.og_spline_coef.argtypes = [_G, _P, _P, C.c_int]
+    L.og_spline_interp.argtypes = [_G, _P, C.c_long, _P, _P, _P, C.c_int, _P]
+    L.og_tracer_stage.argtypes = [_G, _P, _P, _P, C.c_long, _P, _P, _P, _P, _P, _P, C.c_double,
+                                  C.c_double, C.c_int, C.c_int, _P]
+    L.og_hd_step_tracers.argtypes = [_G, C.c_int, C.c_double, C.c_double] + [_P] * 6 + \
+        [C.c_long, _P, _P, _P, C.c_int, C.c_int]
 
 
 def _p(a):
@@ -268,6 +274,36 @@ class Grid:
     def hd_substep(self, v, c, o, dt, nu, f=None):
         fp = [None] * 3 if f is None else list(map(_p, f))
         lib().og_hd_substep(self._g, o, dt, nu, *map(_p, v), *map(_p, c), *fp)
+
+    # -- Lagrangian tracers (positions in grid units, float64 arrays) --------
+    def spline_coef(self, field, ghost=True):
+        """Spline control points of a real field (GPSplineInt_CompSpline3D)."""
+        out = np.empty(self.rshape)
+        lib().og_spline_coef(self._g, _p(np.ascontiguousarray(field, dtype=np.float64)), _p(out), int(ghost))
+        return out
+
+    def spline_interp(self, coef, px, py, pz, zcompat=True):
+        """GPSplineInt_Interp3D of control points coef at (px, py, pz)."""
+        px, py, pz = (np.ascontiguousarray(a, dtype=np.float64) for a in (px, py, pz))
+        out = np.empty(len(px))
+        lib().og_spline_interp(self._g, _p(np.ascontiguousarray(coef)), len(px), _p(px), _p(py), _p(pz),
+                               int(zcompat), _p(out))
+        return out
+
+    def tracer_stage(self, v, pos, x0, dt, xk, zcompat=True, ghost_solve=True):
+        """One tracer RK stage in place on pos = [px, py, pz]; returns the
+        interpolated velocities (3, np)."""
+        np_ = len(pos[0])
+        lv = np.empty((3, np_))
+        lib().og_tracer_stage(self._g, *map(_p, v), np_, *map(_p, pos), *map(_p, x0), dt, xk, int(zcompat),
+                              int(ghost_solve), _p(lv))
+        return lv
+
+    def hd_step_tracers(self, v, pos, dt, nu, f=None, ord=2, zcompat=True, ghost_solve=True):
+        """One RK step of the fluid (in place on v) and the tracers (in place on pos)."""
+        fp = [None] * 3 if f is None else list(map(_p, f))
+        lib().og_hd_step_tracers(self._g, ord, dt, nu, *map(_p, v), *fp, len(pos[0]), *map(_p, pos),
+                                 int(zcompat), int(ghost_solve))
 
     def mhd_step(self, v, a, dt, nu, mu, f=None, m=None, ord=2):
         fp = [None] * 3 if f is None else list(map(_p, f))
