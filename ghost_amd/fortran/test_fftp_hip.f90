@@ -1,0 +1,185 @@
+! test_fftp_hip.f90 -- the Fortran host side, end to end on the GPU.
+!
+! Usage: test_fftp_hip OUTDIR
+! 1. fftp3d_* on host arrays (GHOST's calls, unchanged): round trip and
+!    Taylor-Green closed forms through the host-array pseudospec drop-ins.
+! 2. Device-resident fields through pseudospec_hip: TG enstrophy, and an RK2
+!    loop written as GHOST's main3D (C = v; DO o = ord,1,-1: hd_rkstep2).
+! 3. The resident solver with tracers attached.
+! Writes OUTDIR/fortran_hd32.bin (v after the RK loop, 3 x C(nz,ny,nx/2+1))
+! and OUTDIR/fortran_tracers32.bin (initial and final positions) for the
+! comparison with the CPU oracle; prints one line per check and stops with a
+! non-zero status on the first failure.
+PROGRAM test_fftp_hip
+  USE iso_c_binding
+  USE fprecision
+  USE fftplans
+  USE ghx_c
+  USE pseudospec_hip, ONLY: ghx_pseudospec_init, ghx_pseudospec_finalize, ghx_field, field_alloc, field_free, &
+                            field_upload, field_download, ghx_grid_h, d_rotor3 => rotor3, d_energy => energy, &
+                            hd_rkstep2
+  IMPLICIT NONE
+  INTEGER, PARAMETER :: n = 32, nh = n/2 + 1, nsteps = 20, npart = 1000, psteps = 5
+  REAL(KIND=GP), PARAMETER :: pi = 3.14159265358979323846_GP
+  REAL(KIND=GP), PARAMETER :: dt = 1.0E-3_GP, nu = 1.0E-2_GP
+  TYPE(FFTPLAN) :: planrc, plancr
+  REAL(KIND=GP), ALLOCATABLE, TARGET :: ux(:,:,:), uy(:,:,:), uz(:,:,:), r(:,:,:)
+  COMPLEX(KIND=GP), ALLOCATABLE, TARGET :: c1(:,:,:), c2(:,:,:), c3(:,:,:), c4(:,:,:)
+  REAL(KIND=GP), ALLOCATABLE, TARGET :: pos0(:,:), pos1(:,:)
+  TYPE(ghx_field) :: v(3), cst(3), w(3)
+  TYPE(C_PTR) :: hd, hv(3), hf(3), tr, pp(3)
+  CHARACTER(LEN=512) :: outdir
+  DOUBLE PRECISION :: e, err, x, y, z, rn
+  INTEGER :: i, j, k, o, q, u
+  INTEGER :: ierr
+
+  CALL GET_COMMAND_ARGUMENT(1, outdir)
+  IF (LEN_TRIM(outdir) .EQ. 0) outdir = '.'
+  IF (ghx_device_ok() .EQ. 0) THEN
+    WRITE(*, '(A)') 'no GPU visible'
+    STOP 2
+  END IF
+
+  CALL fftp3d_init_threads(ierr)
+  CALL fftp3d_create_plan(planrc, (/n, n, n/), FFTW_REAL_TO_COMPLEX, FFTW_ESTIMATE)
+  CALL fftp3d_create_plan(plancr, (/n, n, n/), FFTW_COMPLEX_TO_REAL, FFTW_ESTIMATE)
+  CALL ghx_pseudospec_init(n, n, n, 1.0_GP, 1.0_GP, 1.0_GP, 0, 0.0_GP)
+
+  ALLOCATE(ux(n, n, n), uy(n, n, n), uz(n, n, n), r(n, n, n))
+  ALLOCATE(c1(n, n, nh), c2(n, n, nh), c3(n, n, nh), c4(n, n, nh))
+  DO k = 1, n
+    z = 2*pi*(k - 1)/n
+    DO j = 1, n
+      y = 2*pi*(j - 1)/n
+      DO i = 1, n
+        x = 2*pi*(i - 1)/n
+        ux(i, j, k) = SIN(x)*COS(y)*COS(z)
+        uy(i, j, k) = -COS(x)*SIN(y)*COS(z)
+        uz(i, j, k) = 0.0_GP
+      END DO
+    END DO
+  END DO
+
+  ! -- 1. GHOST's host-array calls -------------------------------------------
+  CALL fftp3d_real_to_complex(planrc, ux, c1, 0)
+  CALL fftp3d_real_to_complex(planrc, uy, c2, 0)
+  CALL fftp3d_real_to_complex(planrc, uz, c3, 0)
+  CALL fftp3d_complex_to_real(plancr, c1, r, 0)
+  err = MAXVAL(ABS(r/REAL(n, GP)**3 - ux))
+  CALL report('fftp round trip', err, 1.0E-13_GP)
+
+  CALL derivk3(c1, c4, 1)                    ! d/dx (sin x cos y cos z)
+  CALL fftp3d_complex_to_real(plancr, c4, r, 0)
+  err = 0.0
+  DO k = 1, n
+    DO j = 1, n
+      DO i = 1, n
+        err = MAX(err, ABS(r(i, j, k)/REAL(n, GP)**3 - COS(2*pi*(i - 1)/n)*COS(2*pi*(j - 1)/n)*COS(2*pi*(k - 1)/n)))
+      END DO
+    END DO
+  END DO
+  CALL report('derivk3 closed form', err, 1.0E-12_GP)
+
+  CALL energy(c1, c2, c3, e, 1)              ! <u^2> of unit TG = 1/4
+  CALL report('energy of TG', ABS(e - 0.25D0), 1.0E-13_GP)
+
+  ! -- 2. device-resident fields ---------------------------------------------
+  DO q = 1, 3
+    CALL field_alloc(v(q), .TRUE.)
+    CALL field_alloc(cst(q), .TRUE.)
+    CALL field_alloc(w(q), .TRUE.)
+  END DO
+  CALL field_upload(v(1), C_LOC(c1))
+  CALL field_upload(v(2), C_LOC(c2))
+  CALL field_upload(v(3), C_LOC(c3))
+  CALL d_rotor3(v(2), v(3), w(1), 1)
+  CALL d_rotor3(v(1), v(3), w(2), 2)
+  CALL d_rotor3(v(1), v(2), w(3), 3)
+  CALL d_energy(w(1), w(2), w(3), e, 1)      ! <omega^2> of unit TG = 3/4
+  CALL report('enstrophy of TG (device fields)', ABS(e - 0.75D0), 1.0E-12_GP)
+
+  ! GHOST's TG initial condition (examples/initialv.f90_tg, u0 = 0.5) and the
+  ! RK loop of main3D.fpp:3202-3231 on resident fields
+  CALL ghx_check(ghx_initialv_tg(ghx_grid_h, 1.0_C_DOUBLE, 1.0_C_DOUBLE, 0.5_C_DOUBLE, v(1)%p, v(2)%p, v(3)%p), &
+                 'initialv_tg')
+  DO q = 1, nsteps
+    DO k = 1, 3
+      CALL ghx_check(ghx_memcpy_d2d(cst(k)%p, v(k)%p, INT(n, C_SIZE_T)**2*nh*16), 'copy C = v')
+    END DO
+    DO o = 2, 1, -1
+      CALL hd_rkstep2(o, dt, nu, v, cst)
+    END DO
+  END DO
+  OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/fortran_hd32.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+  DO k = 1, 3
+    CALL field_download(v(k), C_LOC(c4))
+    WRITE(u) c4
+  END DO
+  CLOSE(u)
+  WRITE(*, '(A)') 'ok   RK loop written (fortran_hd32.bin)'
+
+  ! -- 3. resident solver with tracers ---------------------------------------
+  ALLOCATE(pos0(npart, 3), pos1(npart, 3))
+  DO q = 1, npart                           ! a fixed low-discrepancy set in [0, n)
+    rn = REAL(q, GP)
+    pos0(q, 1) = n*(rn*0.7548776662466927_GP - AINT(rn*0.7548776662466927_GP))
+    pos0(q, 2) = n*(rn*0.5698402909980532_GP - AINT(rn*0.5698402909980532_GP))
+    pos0(q, 3) = n*(rn*0.4301597090019468_GP - AINT(rn*0.4301597090019468_GP))
+  END DO
+  CALL ghx_check(ghx_hd_create(hd, ghx_grid_h, 2_C_INT, dt, nu), 'hd_create')
+  CALL ghx_check(ghx_hd_fields(hd, hv, hf), 'hd_fields')
+  CALL ghx_check(ghx_initialv_tg(ghx_grid_h, 1.0_C_DOUBLE, 1.0_C_DOUBLE, 0.5_C_DOUBLE, hv(1), hv(2), hv(3)), &
+                 'initialv_tg')
+  CALL ghx_check(ghx_hd_touch(hd), 'hd_touch')
+  CALL ghx_check(ghx_hd_set_forcing(hd, 0_C_INT), 'hd_set_forcing')
+  CALL ghx_check(ghx_tracers_create(tr, ghx_grid_h, INT(npart, C_LONG_LONG), 1_C_INT), 'tracers_create')
+  CALL ghx_check(ghx_tracers_positions(tr, pp(1), pp(2), pp(3)), 'tracers_positions')
+  DO q = 1, 3
+    CALL ghx_check(ghx_memcpy_h2d(pp(q), C_LOC(pos0(1, q)), INT(npart, C_SIZE_T)*8), 'positions in')
+  END DO
+  CALL ghx_check(ghx_hd_set_tracers(hd, tr), 'hd_set_tracers')
+  CALL ghx_check(ghx_hd_step(hd, INT(psteps, C_INT)), 'hd_step')
+  DO q = 1, 3
+    CALL ghx_check(ghx_memcpy_d2h(C_LOC(pos1(1, q)), pp(q), INT(npart, C_SIZE_T)*8), 'positions out')
+  END DO
+  err = MAXVAL(ABS(pos1 - pos0))
+  IF (err .LE. 0.0 .OR. MINVAL(pos1) .LT. 0.0 .OR. MAXVAL(pos1) .GE. n) THEN
+    WRITE(*, '(A)') 'FAIL tracers did not move or left the box'
+    STOP 1
+  END IF
+  OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/fortran_tracers32.bin', ACCESS='STREAM', FORM='UNFORMATTED', &
+       STATUS='REPLACE')
+  WRITE(u) pos0
+  WRITE(u) pos1
+  CLOSE(u)
+  WRITE(*, '(A)') 'ok   solver with tracers (fortran_tracers32.bin)'
+
+  CALL ghx_check(ghx_hd_set_tracers(hd, C_NULL_PTR), 'hd_set_tracers')
+  CALL ghx_check(ghx_tracers_destroy(tr), 'tracers_destroy')
+  CALL ghx_check(ghx_hd_destroy(hd), 'hd_destroy')
+  DO q = 1, 3
+    CALL field_free(v(q))
+    CALL field_free(cst(q))
+    CALL field_free(w(q))
+  END DO
+  CALL fftp3d_destroy_plan(planrc)
+  CALL fftp3d_destroy_plan(plancr)
+  CALL ghx_pseudospec_finalize()
+  WRITE(*, '(A,ES10.3,A)') 'ok   ffttime accumulated: ', ffttime, ' s'
+  WRITE(*, '(A)') 'ALL OK'
+
+CONTAINS
+
+  SUBROUTINE report(what, err, tol)
+    CHARACTER(LEN=*), INTENT(IN) :: what
+    DOUBLE PRECISION, INTENT(IN) :: err
+    REAL(KIND=GP), INTENT(IN) :: tol
+    IF (err .LE. tol) THEN
+      WRITE(*, '(A,A,A,ES10.3)') 'ok   ', what, ': ', err
+    ELSE
+      WRITE(*, '(A,A,A,ES10.3,A,ES10.3)') 'FAIL ', what, ': ', err, ' > ', tol
+      STOP 1
+    END IF
+  END SUBROUTINE report
+
+END PROGRAM test_fftp_hip
