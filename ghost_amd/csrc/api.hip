@@ -91,11 +91,11 @@ extern "C" int ghx_sync(void) {
 namespace {
 enum ProfSlot {
   P_ZINV, P_ZINV_CURL, P_ZINV_MHD, P_YINV, P_YFWD, P_XINV, P_XFWD, P_XCROSS, P_ZFWD, P_ZFWD_RK_HD, P_ZFWD_RK_MHD,
-  P_ZINV_SPLINE, P_TRACER_STAGE, P_NSLOTS
+  P_ZINV_SPLINE, P_TRACER_STAGE, P_ZINV_BOUSS, P_XDOT, P_ZFWD_RK_BOUSS, P_NSLOTS
 };
 const char *kProfNames[P_NSLOTS] = {"zinv", "zinv_curl", "zinv_mhd", "ypass_inv", "ypass_fwd", "xinv",
                                     "xfwd", "xcross", "zfwd", "zfwd_rk_hd", "zfwd_rk_mhd", "zinv_spline",
-                                    "tracer_stage"};
+                                    "tracer_stage", "zinv_bouss", "xdot", "zfwd_rk_bouss"};
 struct ProfRec {
   int slot;
   double bytes;
@@ -620,6 +620,29 @@ extern "C" int ghx_vector3(ghx_grid *g, const void *a, const void *b, const void
                            const void *f, void *x, void *y, void *z) {
   ARGCHK(g && a && b && c && d && e && f && x && y && z, "bad vector3 arguments");
   return fused_cross(g, 0, a, b, c, d, e, f, x, y, z);
+}
+
+// advect3 (pseudospec3D_phd.f90:23-115): Z^-1 of (a, b, c) and grad d in one
+// pass, Y^-1, the dot product fused into the x pass, Y, Z.
+extern "C" int ghx_advect3(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, void *e) {
+  ARGCHK(g && a && b && c && d && e, "bad advect3 arguments");
+  int rc = plan_scratch(&g->plan, 12);
+  if (rc) return rc;
+  const FFTCtx &ctx = g->plan.ctx;
+  const size_t slot = ghx::slot_points(g->gk);
+  FieldSet in, q6, p6, p1, q1, out;
+  in.p[0] = M_(a); in.p[1] = M_(b); in.p[2] = M_(c); in.p[3] = M_(d);
+  for (int k = 0; k < 6; ++k) { q6.p[k] = g->plan.q + k * slot; p6.p[k] = g->plan.q + (6 + k) * slot; }
+  p1.p[0] = g->plan.q;
+  q1.p[0] = g->plan.q + 6 * slot;
+  out.p[0] = M_(e);
+  const double sc = (double)g->nc * sizeof(cplx);
+  PROF(P_ZINV, 10 * sc, ctx.stream, ghx::zinv(ctx, 4, 4, in, q6));
+  PROF(P_YINV, 12 * sc, ctx.stream, ghx::yinv(ctx, 6, q6, p6));
+  PROF(P_XDOT, 7 * sc, ctx.stream, ghx::xdot(ctx, p6, p1));
+  PROF(P_YFWD, 2 * sc, ctx.stream, ghx::yfwd(ctx, 1, p1, q1));
+  PROF(P_ZFWD, 2 * sc, ctx.stream, ghx::zfwd(ctx, 1, q1, out));
+  return GHX_OK;
 }
 
 static int reduce(ghx_grid *g, int kind, const void *a, const void *b, const void *c, const void *d, const void *e,
@@ -1404,28 +1427,28 @@ extern "C" int ghx_tracers_set_step(ghx_tracers *t) {
 }
 
 // One stage on 9 scratch slots of q: Q (0-2) -> P (3-5) -> real control
-// points (6-8), then the interpolation and position update.
+// points of the three components interleaved (slots 6-8, contiguous), then
+// the interpolation and position update.
 static int tracers_stage(ghx_tracers *t, cplx *q, const cplx *const *v, double dt, double xk, int prune) {
   ghx_grid *g = t->g;
   const FFTCtx &ctx = g->plan.ctx;
   const size_t slot = ghx::slot_points(g->gk);
   FieldSet in, qq, pp;
-  double *r[3];
+  double *r3 = reinterpret_cast<double *>(q + 6 * slot);
   for (int c = 0; c < 3; ++c) {
     in.p[c] = const_cast<cplx *>(v[c]);
     qq.p[c] = q + c * slot;
     pp.p[c] = q + (3 + c) * slot;
-    r[c] = reinterpret_cast<double *>(q + (6 + c) * slot);
   }
   const double sc = (double)g->nc * sizeof(cplx), sr = (double)g->nr * sizeof(double);
   const PruneShares ps = (prune && g_prof.on) ? prune_shares(g) : PruneShares{1.0, 1.0, 1.0};
   PROF(P_ZINV_SPLINE, (3 * ps.fm + 3 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 3, 3, in, qq, prune, t->spl));
   PROF(P_YINV, (3 * ps.fl + 3 * ps.fx) * sc, ctx.stream, ghx::yinv(ctx, 3, qq, pp, prune));
-  PROF(P_XINV, 3 * (ps.fx * sc + sr), ctx.stream, ghx::xinv(ctx, 3, pp, r, prune));
+  PROF(P_XINV, 3 * (ps.fx * sc + sr), ctx.stream, ghx::xinv3i(ctx, pp, r3, prune));
   ghx::TracerStage st;
   const int n[3] = {g->gk.nx, g->gk.ny, g->gk.nz};
+  st.coef3 = r3;
   for (int c = 0; c < 3; ++c) {
-    st.coef[c] = r[c];
     st.pos[c] = t->pos[c];
     st.x0[c] = t->x0[c];
     st.lv[c] = t->lv[c];
@@ -1484,5 +1507,170 @@ extern "C" int ghx_mhd_set_tracers(ghx_mhd *s, ghx_tracers *t) {
   ARGCHK(s, "null solver");
   ARGCHK(!t || t->g == s->g, "tracers and solver must share the grid");
   s->tr = t;
+  return GHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Boussinesq (include/bouss_rkstep{1,2}.f90)
+// ---------------------------------------------------------------------------
+struct ghx_bouss {
+  ghx_grid *g;
+  int ord;
+  double dt, nu, kappa, xmom, xtemp;
+  cplx *mem = nullptr;  // s(4) s1(4) f(4), 18 scratch slots
+  cplx *s[4], *s1[4], *f[4], *q;
+  uint8_t *fline = nullptr;
+  bool forced = false;
+  bool clean = false;
+  ghx_tracers *tr = nullptr;
+};
+
+extern "C" int ghx_bouss_create(ghx_bouss **out, ghx_grid *g, int ord, double dt, double nu, double kappa,
+                                double xmom, double xtemp) {
+  ARGCHK(out && g, "bad bouss_create arguments");
+  if (ord < 1 || ord > 2)
+    return set_err(GHX_ERR_ARG, "Boussinesq solver supports ord 1 and 2 (the ord = 3 correction of "
+                                "bouss_rkstep2.f90:73-145 is not built)");
+  ghx_bouss *h = new (std::nothrow) ghx_bouss();
+  if (!h) return set_err(GHX_ERR_NOMEM, "host allocation failed");
+  h->g = g; h->ord = ord; h->dt = dt; h->nu = nu; h->kappa = kappa; h->xmom = xmom; h->xtemp = xtemp;
+  (void)prune_shares(g);
+  const size_t nc = g->nc;
+  hipError_t e = hipMalloc(&h->mem, (12 * nc + 18 * ghx::slot_points(g->gk)) * sizeof(cplx));
+  if (e != hipSuccess) { delete h; HIPCHK(e); }
+  for (int k = 0; k < 4; ++k) {
+    h->s[k] = h->mem + k * nc;
+    h->s1[k] = h->mem + (4 + k) * nc;
+    h->f[k] = h->mem + (8 + k) * nc;
+  }
+  h->q = h->mem + 12 * nc;
+  e = hipMalloc(&h->fline, (size_t)g->gk.nxh * g->gk.ny);
+  if (e == hipSuccess) e = ghx::op_zero(g->stream, h->mem, 12 * nc * sizeof(cplx));
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) { (void)hipFree(h->mem); (void)hipFree(h->fline); delete h; HIPCHK(e); }
+  *out = h;
+  return GHX_OK;
+}
+
+extern "C" int ghx_bouss_destroy(ghx_bouss *s) {
+  if (!s) return GHX_OK;
+  (void)hipFree(s->mem);
+  (void)hipFree(s->fline);
+  delete s;
+  return GHX_OK;
+}
+
+extern "C" int ghx_bouss_fields(ghx_bouss *s, void **v, void **th, void **f, void **fs) {
+  ARGCHK(s, "null solver");
+  s->clean = false;
+  for (int k = 0; k < 3; ++k) {
+    if (v) v[k] = s->s[k];
+    if (f) f[k] = s->f[k];
+  }
+  if (th) *th = s->s[3];
+  if (fs) *fs = s->f[3];
+  return GHX_OK;
+}
+
+extern "C" int ghx_bouss_touch(ghx_bouss *s) {
+  ARGCHK(s, "null solver");
+  s->clean = false;
+  return GHX_OK;
+}
+
+extern "C" int ghx_bouss_set_forcing(ghx_bouss *s, int enabled) {
+  ARGCHK(s, "null solver");
+  s->forced = enabled != 0;
+  if (s->forced) {
+    ghx_grid *g = s->g;
+    size_t nl = (size_t)g->gk.nxh * g->gk.ny;
+    int rc = grid_work(g, nl / 8 + 2);
+    if (rc) return rc;
+    uint8_t *tmp = (uint8_t *)g->work;
+    if ((rc = line_flags(g, s->f[0], s->f[1], s->f[2], s->fline))) return rc;
+    if ((rc = line_flags(g, s->f[3], s->f[3], s->f[3], tmp))) return rc;
+    hipLaunchKernelGGL(k_or_flags, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, g->stream, s->fline, tmp, nl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g->stream));
+  }
+  return GHX_OK;
+}
+
+extern "C" int ghx_bouss_set_tracers(ghx_bouss *s, ghx_tracers *t) {
+  ARGCHK(s, "null solver");
+  ARGCHK(!t || t->g == s->g, "tracers and solver must share the grid");
+  s->tr = t;
+  return GHX_OK;
+}
+
+// include/bouss_rkstep2.f90:1-71 fused: one inverse chain for (v, th)
+// producing (u, w, grad th); w x u and -(u . grad th) in the x pass; one
+// forward chain ending in buoyancy, projection, heat current and the updates.
+static int bouss_substep(ghx_bouss *s, int o, const cplx *const *vin, cplx *const *out, int prune) {
+  ghx_grid *g = s->g;
+  const size_t nc = g->nc, slot = ghx::slot_points(g->gk);
+  const FFTCtx &ctx = g->plan.ctx;
+  // Q9 (slots 0-8) -> P9 (9-17) -> P4 (reuses 0-3) -> Q4 (reuses 9-12)
+  FieldSet in, q9, p9, pq, o3, pd, o1, p4, q4;
+  for (int k = 0; k < 4; ++k) in.p[k] = const_cast<cplx *>(vin[k]);
+  for (int k = 0; k < 9; ++k) { q9.p[k] = s->q + k * slot; p9.p[k] = s->q + (9 + k) * slot; }
+  for (int k = 0; k < 4; ++k) { p4.p[k] = s->q + k * slot; q4.p[k] = s->q + (9 + k) * slot; }
+  const double sc = (double)nc * sizeof(cplx);
+  const PruneShares ps = (prune && g_prof.on) ? prune_shares(g) : PruneShares{1.0, 1.0, 1.0};
+  PROF(P_ZINV_BOUSS, (4 * ps.fm + 9 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 5, 4, in, q9, prune));
+  PROF(P_YINV, (9 * ps.fl + 9 * ps.fx) * sc, ctx.stream, ghx::yinv(ctx, 9, q9, p9, prune));
+  // w x u in GHOST's operand order (P = w, Q = u)
+  for (int k = 0; k < 3; ++k) { pq.p[k] = p9.p[3 + k]; pq.p[3 + k] = p9.p[k]; o3.p[k] = p4.p[k]; }
+  PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, o3, prune));
+  for (int k = 0; k < 3; ++k) { pd.p[k] = p9.p[k]; pd.p[3 + k] = p9.p[6 + k]; }
+  o1.p[0] = p4.p[3];
+  PROF(P_XDOT, 7 * ps.fx * sc, ctx.stream, ghx::xdot(ctx, pd, o1, prune));
+  PROF(P_YFWD, (4 * ps.fx + 4 * ps.fl) * sc, ctx.stream, ghx::yfwd(ctx, 4, p4, q4, prune));
+  ghx::BoussRKParams rk;
+  rk.dt = s->dt; rk.nu = s->nu; rk.kappa = s->kappa; rk.xmom = s->xmom; rk.xtemp = s->xtemp;
+  rk.rmp = 1. / (double)o;
+  for (int k = 0; k < 4; ++k) {
+    rk.v[k] = vin[k];
+    rk.c[k] = s->s[k];
+    rk.out[k] = out[k];
+    rk.f[k] = s->forced ? s->f[k] : nullptr;
+  }
+  rk.fline = s->forced ? s->fline : nullptr;
+  const int nvc = vin[0] == s->s[0] ? 4 : 8;
+  PROF(P_ZFWD_RK_BOUSS, (4 * ps.fl + nvc * ps.fm + 4 * (prune ? ps.fm : 1.0)) * sc, ctx.stream,
+       ghx::zfwd_rk_bouss(ctx, q4, rk, prune));
+  return GHX_OK;
+}
+
+extern "C" int ghx_bouss_step(ghx_bouss *s, int nsteps) {
+  ARGCHK(s && nsteps >= 0, "bad bouss_step arguments");
+  const cplx *v0[4], *v1[4];
+  for (int k = 0; k < 4; ++k) { v0[k] = s->s[k]; v1[k] = s->s1[k]; }
+  for (int n = 0; n < nsteps; ++n) {
+    if (!s->clean) {
+      double a = 0, b = 0;
+      int rc = count_unkept(s->g, s->s[0], s->s[1], s->s[2], &a);
+      if (!rc) rc = reduce(s->g, 5, s->s[3], s->s[3], s->s[3], nullptr, nullptr, nullptr, &b);
+      if (rc) return rc;
+      s->clean = (a == 0.0 && b == 0.0);
+    }
+    const int prune = s->clean && !getenv("GHX_NO_PRUNE") ? 1 : 0;
+    if (s->tr) {
+      int rc = tracers_begin_step(s->tr);
+      if (rc) return rc;
+    }
+    for (int o = s->ord; o >= 1; --o) {
+      const bool first = (o == s->ord), last = (o == 1);
+      cplx *const *out = last ? s->s : s->s1;
+      int rc = bouss_substep(s, o, first ? v0 : v1, out, prune);
+      if (rc) return rc;
+      if (s->tr) {  // main3D.fpp:3233-3267, velocity after the substep
+        const cplx *vo[3] = {out[0], out[1], out[2]};
+        rc = tracers_stage(s->tr, s->q, vo, s->dt, 1.0 / (double)o, getenv("GHX_NO_PRUNE") ? 0 : 1);
+        if (rc) return rc;
+      }
+    }
+    s->clean = true;
+  }
   return GHX_OK;
 }

@@ -178,6 +178,35 @@ template <int T> GHX_DEV void line_sync() {
 }
 
 // ---------------------------------------------------------------------------
+// Twiddle sources for the stages.  TwTable reads the root of each butterfly
+// from the forward table when the stage runs; TwCache holds the roots of one
+// lane for every stage of one schedule in registers, loaded once, for kernels
+// that run many transforms of the same length per lane.
+// ---------------------------------------------------------------------------
+struct TwTable {
+  const cplx *p;
+  template <int S> GHX_DEV cplx root(int, int idx) const { return p[idx]; }
+};
+
+template <int N, typename Cfg = FFTCfg<N>> struct TwCache {
+  static constexpr int NS = Cfg::NS, E = Cfg::E, T = Cfg::T;
+  cplx w[NS > 0 ? NS : 1][E];
+  template <int S, int Ns> GHX_DEV void fill(const cplx *tw, int t) {
+    if constexpr (S < NS) {
+      constexpr int R = Cfg::radix(S), Q = E / R;
+      if constexpr (Ns > 1) {
+        constexpr int tws = N / (Ns * R);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) w[S][q] = tw[((t + q * T) % Ns) * tws];
+      }
+      fill<S + 1, Ns * R>(tw, t);
+    }
+  }
+  GHX_DEV void load(const cplx *tw, int t) { fill<0, 1>(tw, t); }
+  template <int S> GHX_DEV cplx root(int q, int) const { return w[S][q]; }
+};
+
+// ---------------------------------------------------------------------------
 // The line transform.
 // ---------------------------------------------------------------------------
 template <int N, int DIR, typename Cfg = FFTCfg<N>> struct LineFFT {
@@ -188,8 +217,8 @@ template <int N, int DIR, typename Cfg = FFTCfg<N>> struct LineFFT {
   // One Stockham stage (radix R, Ns = product of earlier radices).  If LAST,
   // the outputs stay in v; otherwise they are written to the LDS line buffer
   // and v is reloaded from it.
-  template <int S, int NsS>
-  GHX_DEV static void stages(cplx (&v)[E], cplx *buf, int t, const cplx *tw) {
+  template <int S, int NsS, typename TW>
+  GHX_DEV static void stages(cplx (&v)[E], cplx *buf, int t, const TW &tw) {
     if constexpr (S < C::NS) {
       constexpr int R = C::radix(S);
       constexpr int Q = E / R;
@@ -206,7 +235,8 @@ template <int N, int DIR, typename Cfg = FFTCfg<N>> struct LineFFT {
           const int k = j % Ns;
           constexpr int tws = N / (Ns * R);
           cplx w[R];
-          tw_powers<R>(c_tw<DIR>(tw, k * tws), w);
+          const cplx w1 = tw.template root<S>(q, k * tws);
+          tw_powers<R>(DIR < 0 ? w1 : c_conj(w1), w);
 #pragma unroll
           for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], w[r]);
         }
@@ -225,7 +255,7 @@ template <int N, int DIR, typename Cfg = FFTCfg<N>> struct LineFFT {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = buf[lds_pad(t + T * m)];
         line_sync<T>();
-        stages<S + 1, Ns * R>(v, buf, t, tw);
+        stages<S + 1, Ns * R, TW>(v, buf, t, tw);
       }
     }
   }
@@ -235,6 +265,11 @@ template <int N, int DIR, typename Cfg = FFTCfg<N>> struct LineFFT {
   // thread of the block must call this (lines longer than a wavefront use
   // block barriers).
   GHX_DEV static void run(cplx (&v)[E], cplx *buf, int t, const cplx *tw) {
-    stages<0, 1>(v, buf, t, tw);
+    stages<0, 1, TwTable>(v, buf, t, TwTable{tw});
+  }
+  // the same with the stage roots held in registers (TwCache<N, Cfg> of the
+  // forward table, loaded by this lane)
+  GHX_DEV static void run(cplx (&v)[E], cplx *buf, int t, const TwCache<N, Cfg> &tw) {
+    stages<0, 1, TwCache<N, Cfg>>(v, buf, t, tw);
   }
 };

@@ -44,6 +44,15 @@ struct MHDRKParams {
   const uint8_t *fline;
 };
 
+// Boussinesq update parameters: fields 0..2 velocity, 3 temperature.
+struct BoussRKParams {
+  double dt, nu, kappa, xmom, xtemp, rmp;
+  const cplx *v[4], *c[4];
+  cplx *out[4];
+  const cplx *f[4];
+  const uint8_t *fline;
+};
+
 bool fft_size_supported(int n);
 
 // row length of the P layout: nx/2+1 rounded up to 8 points (128-byte rows)
@@ -53,7 +62,9 @@ inline size_t slot_points(const GridK &g) { return (size_t)g.nz * g.ny * p_row(g
 
 // Z^-1 pass, S -> Q.  mode 0: nf fields in, nf out.  mode 1 (HD): 3 in
 // (v) -> 6 out (v, curl v).  mode 2 (MHD): 6 in (v, A) -> 12 out
-// (v, curl v, b = curl A, j = curl b).
+// (v, curl v, b = curl A, j = curl b).  mode 4 (advect3): 4 in (a, b, c, d)
+// -> 6 out (a, b, c, grad d).  mode 5 (Boussinesq): 4 in (v, th) -> 9 out
+// (v, curl v, grad th).
 // prune=1: the input is dealiased (zero outside tiny <= kn2 <= kmax): only
 // lines that carry kept modes are read and written (every pass below takes
 // the same flag; see the kernels for exactly what each one skips).
@@ -66,6 +77,8 @@ hipError_t yinv(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 hipError_t yfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune = 0);
 // X^-1: P -> real rows (prune: P holds data only for kx < active_kx).
 hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out, int prune = 0);
+// X^-1 of three P fields written interleaved: out[(row * nx + x) * 3 + f].
+hipError_t xinv3i(const FFTCtx &c, const FieldSet &in, double *out, int prune = 0);
 // X forward: real rows -> P.
 hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet &out);
 // Fused X^-1 . cross product . X on P rows: in.p[0..2] = A, in.p[3..5] = B
@@ -75,6 +88,12 @@ hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet 
 hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out);
 hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, int prune = 0);
 hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &rk, int prune = 0);
+// Fused X^-1 . dot product . X for advect3: in.p[0..2] = A, in.p[3..5] = B
+// (P rows) -> out.p[0] = FFT[-(A . B)/N^6].
+hipError_t xdot(const FFTCtx &c, const FieldSet &in, const FieldSet &out, int prune = 0);
+// Z forward fused with the Boussinesq projection and update: in.p[0..2] =
+// products w x u, in.p[3] = -(u . grad th), Q layout.
+hipError_t zfwd_rk_bouss(const FFTCtx &c, const FieldSet &in, const BoussRKParams &rk, int prune = 0);
 // number of kx indices (from 0) whose kn2(kx,0,0) <= kmax
 int active_kx(const GridK &g);
 

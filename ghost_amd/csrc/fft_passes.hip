@@ -9,10 +9,10 @@
 #include "fft_engine.h"
 #include "fft_passes.h"
 
-// The launchers are split into parts (GHX_PART=1..7) so the build compiles
+// The launchers are split into parts (GHX_PART=1..8) so the build compiles
 // them in parallel; each part instantiates only the kernels it launches.
 #ifndef GHX_PART
-#error "compile with -DGHX_PART=1..7"
+#error "compile with -DGHX_PART=1..8"
 #endif
 
 namespace ghx {
@@ -178,6 +178,7 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
     }
   } else {
     // MODE 1: v -> (v, curl v).  MODE 2: (v, A) -> (v, curl v, b = curl A, curl b).
+    // MODE 4: (a, b, c, d) -> (a, b, c, grad d).  MODE 5: (v, th) -> (v, curl v, grad th).
     const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
     cplx u[3][E];
 #pragma unroll
@@ -201,11 +202,30 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
       for (int m = 0; m < E; ++m) v[m] = u[c][m];
       zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[c], g, i, j0, lmask);
     }
+    if constexpr (MODE != 4) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      cplx v[E];
-      curl(u, c, v);
-      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[3 + c], g, i, j0, lmask);
+      for (int c = 0; c < 3; ++c) {
+        cplx v[E];
+        curl(u, c, v);
+        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[3 + c], g, i, j0, lmask);
+      }
+    }
+    if constexpr (MODE == 4 || MODE == 5) {
+      // derivk3 (pseudospec3D_hd.f90:22-88) of the scalar along x, y, z
+      constexpr int base = MODE == 4 ? 3 : 6;
+      cplx d[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) d[m] = need(t + T * m) ? in.p[3][sline + t + T * m] : c_make(0.0, 0.0);
+#pragma unroll
+      for (int dir = 0; dir < 3; ++dir) {
+        cplx v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const double kd = dir == 0 ? kx : (dir == 1 ? ky : wavenum(t + T * m, g.nz, g.dkz));
+          v[m] = c_ik(kd, d[m]);
+        }
+        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[base + dir], g, i, j0, lmask);
+      }
     }
     if constexpr (MODE == 2) {
       cplx a[3][E], b[3][E];
@@ -345,6 +365,91 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const 
     }
     // keep the loads of one mode from being hoisted over the others (register pressure)
     __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Boussinesq (include/bouss_rkstep2.f90:1-71): in.p[0..2] = FFT of w x u,
+// in.p[3] = FFT of -(u . grad th).  Per kept mode: buoyancy xmom th added to
+// the z product, nonlhd3 projection, heat current xtemp v_z added to the
+// advection, then the updates of v (nu) and th (kappa).  The k = 0 mode keeps
+// the step-start mean of th; v is zero there and outside the dealias set.
+template <int N>
+__global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, const cplx *__restrict__ tw,
+                                                      BoussRKParams rk, int prune) {
+  using F = LineFFT<N, -1>;
+  constexpr int T = F::T, E = F::E;
+  extern __shared__ cplx lds[];
+  const int L = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i = by, j0 = bx * L, j = j0 + lt;
+  const unsigned lmask = prune ? tile_lines_active(g, i, j0, L) : ~0u;
+  if (lmask == 0u) return;  // uniform over the block
+  const bool ok = j < g.ny && ((lmask >> lt) & 1u);
+  const int stride = lds_line_stride(N);
+  cplx *row = lds + lt * stride;
+  const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j < g.ny ? j : 0, g.ny, g.dky);
+  const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
+  const bool forced = ok && rk.fline && rk.fline[(size_t)i * g.ny + j];
+  const cplx z = c_make(0.0, 0.0);
+  // v_z as the nonlinear term saw it, for the heat current: with ord = 1 the
+  // update writes over the input (out == v) before the temperature is done
+  cplx vz_in[E];
+  {
+    cplx P[3][E];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
+      __syncthreads();
+    }
+    if (ok) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + T * m;
+        const size_t s = sline + k;
+        const double kz = wavenum(k, g.nz, g.dkz);
+        const double kn2 = kn2_of(g, i, j, k);
+        vz_in[m] = z;
+        if (kn2 <= g.kmax && kn2 >= g.tiny) {
+          const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
+          vz_in[m] = rk.v[2][s];
+          const cplx th = rk.v[3][s];
+          const cplx nzb = c_make(P[2][m].x + rk.xmom * th.x, P[2][m].y + rk.xmom * th.y);
+          cplx n0, n1, n2;
+          project_hd(P[0][m], P[1][m], nzb, kx, ky, kz, kk2, i, j, k, n0, n1, n2);
+          rk.out[0][s] = rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+          rk.out[1][s] = rk_update(rk.c[1][s], rk.v[1][s], n1, forced ? rk.f[1][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+          rk.out[2][s] = rk_update(rk.c[2][s], rk.v[2][s], n2, forced ? rk.f[2][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+        } else if (!prune || kn2 < g.tiny) {
+          rk.out[0][s] = z;
+          rk.out[1][s] = z;
+          rk.out[2][s] = z;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  cplx A[E];
+  zfwd_load<N, E>(A, lds, row, stride, L, t, tw, in.p[3], g, i, j0, lmask);
+  if (!ok) return;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    const size_t s = sline + k;
+    const double kz = wavenum(k, g.nz, g.dkz);
+    const double kn2 = kn2_of(g, i, j, k);
+    if (kn2 <= g.kmax && kn2 >= g.tiny) {
+      const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
+      const cplx vz = vz_in[m];
+      const cplx a = c_make(A[m].x + rk.xtemp * vz.x, A[m].y + rk.xtemp * vz.y);
+      rk.out[3][s] = rk_update(rk.c[3][s], rk.v[3][s], a, forced && rk.f[3] ? rk.f[3][s] : z, kk2, rk.kappa, rk.dt,
+                               rk.rmp);
+    } else if (kn2 < g.tiny) {
+      rk.out[3][s] = rk.c[3][s];
+    } else if (!prune) {
+      rk.out[3][s] = z;
+    }
   }
 }
 
@@ -538,24 +643,56 @@ __global__ __launch_bounds__(512) void k_yfwd(FieldSet in, FieldSet out, int nf,
 // only wave-level synchronisation.  Block = RPB rows.
 // ---------------------------------------------------------------------------
 
+// Twiddles of the x pass held by one lane for the whole kernel: the stage
+// roots of the half-length transform and the packing twiddles w^k of the
+// real <-> half-complex step (k = t + T m).
+template <int N, typename Cfg> struct XTw {
+  static constexpr int E = Cfg::E, T = Cfg::T;
+  TwCache<N, Cfg> stage;
+  cplx pack[E];
+  GHX_DEV void load(const cplx *twh, const cplx *twx, int t) {
+    stage.load(twh, t);
+#pragma unroll
+    for (int m = 0; m < E; ++m) pack[m] = twx[t + T * m];
+  }
+};
+
+// The loads of one P row for X^-1: points k = t + T m and the Nyquist point.
+// Every lane loads from inside the row (points at kx >= kxlim read point 0
+// and are zeroed by a select), so the loads issue back to back and a caller
+// can start the next row's loads before transforming this one.
+template <int E> struct XRowIn {
+  cplx a[E];
+  cplx nyq;
+};
+
+template <int N, int E, typename Cfg>
+GHX_DEV void xrow_load(XRowIn<E> &r, const cplx *src, int t, int kxlim) {
+  constexpr int T = Cfg::T;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    r.a[m] = src[k < kxlim ? k : 0];
+  }
+  r.nyq = src[N < kxlim ? N : 0];
+}
+
 // Half-spectrum row -> unnormalised c2r of the row, packed: v[m] holds
 // (x[2k], x[2k+1]) * nx for k = t + T m.  GHOST's c2r ignores the imaginary
-// parts of the kx = 0 and Nyquist entries.
+// parts of the kx = 0 and Nyquist entries.  The mirrored points X[N-k] come
+// through the line's LDS buffer (slot N holds the Nyquist point).
 template <int N, int E, typename Cfg>
-GHX_DEV void xinv_row(cplx (&v)[E], cplx *buf, int t, const cplx *twh, const cplx *twx, const cplx *src,
-                      int kxlim) {
+GHX_DEV void xrow_inv(cplx (&v)[E], const XRowIn<E> &r, cplx *buf, int t, const XTw<N, Cfg> &tw, int kxlim) {
   using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T;
-  // each point is read once; the mirrored points X[N-k] come through the
-  // line's LDS buffer (slot N holds the Nyquist point)
   cplx a[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + T * m;
-    a[m] = k < kxlim ? src[k] : c_make(0.0, 0.0);
+    a[m] = k < kxlim ? r.a[m] : c_make(0.0, 0.0);
     buf[lds_pad(k)] = a[m];
   }
-  if (t == 0) buf[lds_pad(N)] = N < kxlim ? src[N] : c_make(0.0, 0.0);
+  if (t == 0) buf[lds_pad(N)] = N < kxlim ? r.nyq : c_make(0.0, 0.0);
   line_sync<T>();
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -565,19 +702,19 @@ GHX_DEV void xinv_row(cplx (&v)[E], cplx *buf, int t, const cplx *twh, const cpl
     if (k == 0) { A.y = 0.0; Bs.y = 0.0; }
     cplx B = c_conj(Bs);
     cplx Xe = c_add(A, B);
-    cplx Xo = c_mul(c_sub(A, B), c_conj(twx[k]));
+    cplx Xo = c_mul(c_sub(A, B), c_conj(tw.pack[m]));
     v[m] = c_add(Xe, c_make(-Xo.y, Xo.x));
   }
   line_sync<T>();
-  F::run(v, buf, t, twh);
+  F::run(v, buf, t, tw.stage);
 }
 
 // Packed real row -> r2c half spectrum, stored to a P row (kx < kxlim only).
 template <int N, int E, typename Cfg>
-GHX_DEV void xfwd_row(cplx (&v)[E], cplx *buf, int t, const cplx *twh, const cplx *twx, cplx *dst, int kxlim) {
+GHX_DEV void xrow_fwd(cplx (&v)[E], cplx *buf, int t, const XTw<N, Cfg> &tw, cplx *dst, int kxlim) {
   using F = LineFFT<N, -1, Cfg>;
   constexpr int T = F::T;
-  F::run(v, buf, t, twh);
+  F::run(v, buf, t, tw.stage);
 #pragma unroll
   for (int m = 0; m < E; ++m) buf[lds_pad(t + T * m)] = v[m];
   line_sync<T>();
@@ -589,7 +726,7 @@ GHX_DEV void xfwd_row(cplx (&v)[E], cplx *buf, int t, const cplx *twh, const cpl
     cplx Xe = c_scale(c_add(A, B), 0.5);
     cplx d = c_sub(A, B);
     cplx Xo = c_make(0.5 * d.y, -0.5 * d.x);  // -0.5 i (A - B)
-    if (k < kxlim) dst[k] = c_add(Xe, c_mul(twx[k], Xo));
+    if (k < kxlim) dst[k] = c_add(Xe, c_mul(tw.pack[m], Xo));
   }
   if (t == 0 && N < kxlim) {
     cplx z0 = v[0];
@@ -617,16 +754,67 @@ __global__ __launch_bounds__(256) void k_xinv(FieldSet in, RealSet out, int nf, 
   if constexpr (T <= 64) {
     if (!ok) return;  // only wave-level syncs
   }
+  const int lim = ok ? kxlim : 0;
+  const size_t off = (size_t)(ok ? r : 0) * nxp;
   cplx *buf = lds + lt * lds_line_stride(N);
+  XTw<N, Cfg> tw;
+  tw.load(twh, twx, t);
+  XRowIn<E> cur, nxt;
+  xrow_load<N, E, Cfg>(cur, in.p[0] + off, t, lim);
   for (int f = 0; f < nf; ++f) {
+    if (f + 1 < nf) xrow_load<N, E, Cfg>(nxt, in.p[f + 1] + off, t, lim);  // next field in flight
     cplx v[E];
-    xinv_row<N, E, Cfg>(v, buf, t, twh, twx, in.p[f] + (ok ? r : 0) * nxp, ok ? kxlim : 0);
+    xrow_inv<N, E, Cfg>(v, cur, buf, t, tw, lim);
     if (ok) {
       cplx *dst = reinterpret_cast<cplx *>(out.p[f] + r * g.nx);
 #pragma unroll
       for (int m = 0; m < E; ++m) dst[t + T * m] = v[m];
     }
     line_sync<T>();
+    cur = nxt;
+  }
+}
+
+// X^-1 of three fields written interleaved, out[(row * nx + x) * 3 + f]: the
+// layout the tracer gather reads (one 4-point stencil row of all three
+// components is 96 contiguous bytes).
+template <int N>
+__global__ __launch_bounds__(256) void k_xinv3i(FieldSet in, double *out, GridK g, int nxp,
+                                               const cplx *__restrict__ twh, const cplx *__restrict__ twx,
+                                               int kxlim) {
+  using Cfg = FFTCfgLow<N>;
+  constexpr int T = Cfg::T, E = Cfg::E;
+  extern __shared__ cplx lds[];
+  const int rpb = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const long nrows = (long)g.nz * g.ny;
+  const long r = (long)blockIdx.x * rpb + lt;
+  const bool ok = r < nrows;
+  if constexpr (T <= 64) {
+    if (!ok) return;
+  }
+  const int lim = ok ? kxlim : 0;
+  const size_t off = (size_t)(ok ? r : 0) * nxp;
+  cplx *buf = lds + lt * lds_line_stride(N);
+  XTw<N, Cfg> tw;
+  tw.load(twh, twx, t);
+  XRowIn<E> r0, r1;
+  xrow_load<N, E, Cfg>(r0, in.p[0] + off, t, lim);
+  xrow_load<N, E, Cfg>(r1, in.p[1] + off, t, lim);
+  cplx v[3][E];
+  xrow_inv<N, E, Cfg>(v[0], r0, buf, t, tw, lim);
+  xrow_load<N, E, Cfg>(r0, in.p[2] + off, t, lim);
+  xrow_inv<N, E, Cfg>(v[1], r1, buf, t, tw, lim);
+  xrow_inv<N, E, Cfg>(v[2], r0, buf, t, tw, lim);
+  if (!ok) return;
+  // lane point k holds x = 2k and 2k+1: six consecutive doubles, 16-byte aligned
+  double2 *dst = reinterpret_cast<double2 *>(out + (size_t)r * g.nx * 3);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    dst[3 * k + 0] = make_double2(v[0][m].x, v[1][m].x);
+    dst[3 * k + 1] = make_double2(v[2][m].x, v[0][m].y);
+    dst[3 * k + 2] = make_double2(v[1][m].y, v[2][m].y);
   }
 }
 
@@ -645,12 +833,27 @@ __global__ __launch_bounds__(256) void k_xfwd(RealSet in, FieldSet out, int nf, 
     if (!ok) return;
   }
   cplx *buf = lds + lt * lds_line_stride(N);
+  XTw<N, Cfg> tw;
+  tw.load(twh, twx, t);
+  const size_t roff = (size_t)(ok ? r : 0) * g.nx;
+  cplx cur[E], nxt[E];
+  {
+    const cplx *src = reinterpret_cast<const cplx *>(in.p[0] + roff);
+#pragma unroll
+    for (int m = 0; m < E; ++m) cur[m] = src[t + T * m];
+  }
   for (int f = 0; f < nf; ++f) {
-    const cplx *src = reinterpret_cast<const cplx *>(in.p[f] + (ok ? r : 0) * g.nx);
+    if (f + 1 < nf) {
+      const cplx *src = reinterpret_cast<const cplx *>(in.p[f + 1] + roff);
+#pragma unroll
+      for (int m = 0; m < E; ++m) nxt[m] = src[t + T * m];
+    }
     cplx v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = ok ? src[t + T * m] : c_make(0.0, 0.0);
-    xfwd_row<N, E, Cfg>(v, buf, t, twh, twx, out.p[f] + (ok ? r : 0) * nxp, ok ? g.nxh : 0);
+    for (int m = 0; m < E; ++m) v[m] = ok ? cur[m] : c_make(0.0, 0.0);
+    xrow_fwd<N, E, Cfg>(v, buf, t, tw, out.p[f] + (ok ? r : 0) * nxp, ok ? g.nxh : 0);
+#pragma unroll
+    for (int m = 0; m < E; ++m) cur[m] = nxt[m];
   }
 }
 
@@ -658,7 +861,8 @@ __global__ __launch_bounds__(256) void k_xfwd(RealSet in, FieldSet out, int nf, 
 // (pseudospec3D_hd.f90:377-389): out_x = (P_y Q_z - Q_y P_z) tmp,
 // out_y = (P_z Q_x - Q_z P_x) tmp, out_z = (P_x Q_y - Q_x P_y) tmp.
 // Fields: in.p[0..2] = P, in.p[3..5] = Q; out.p[0..2].  Only kx < kxlim is
-// read and written (dealiased inputs, dealiased use of the outputs).
+// read and written (dealiased inputs, dealiased use of the outputs).  Each
+// row's next input is loaded before the current one is transformed.
 template <int N, int W, int SB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_xcross(
     FieldSet in, FieldSet out, GridK g, int nxp, const cplx *__restrict__ twh, const cplx *__restrict__ twx,
@@ -678,17 +882,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
   const size_t off = (size_t)(ok ? r : 0) * nxp;
   cplx *buf = lds + lt * lds_line_stride(N);
   const double tmp = g.inv_n6;
+  XTw<N, Cfg> tw;
+  tw.load(twh, twx, t);
   cplx qx[E], qy[E], qz[E], ay[E], az[E];
+  XRowIn<E> ra, rb;
 #define GHX_PHASE() do { if constexpr (SB) __builtin_amdgcn_sched_barrier(0); } while (0)
-  xinv_row<N, E, Cfg>(qx, buf, t, twh, twx, in.p[3] + off, lim);
+  xrow_load<N, E, Cfg>(ra, in.p[3] + off, t, lim);
+  xrow_load<N, E, Cfg>(rb, in.p[4] + off, t, lim);
+  xrow_inv<N, E, Cfg>(qx, ra, buf, t, tw, lim);
   GHX_PHASE();
-  xinv_row<N, E, Cfg>(qy, buf, t, twh, twx, in.p[4] + off, lim);
+  xrow_load<N, E, Cfg>(ra, in.p[5] + off, t, lim);
+  xrow_inv<N, E, Cfg>(qy, rb, buf, t, tw, lim);
   GHX_PHASE();
-  xinv_row<N, E, Cfg>(qz, buf, t, twh, twx, in.p[5] + off, lim);
+  xrow_load<N, E, Cfg>(rb, in.p[0] + off, t, lim);
+  xrow_inv<N, E, Cfg>(qz, ra, buf, t, tw, lim);
   GHX_PHASE();
+  xrow_load<N, E, Cfg>(ra, in.p[1] + off, t, lim);
   {
     cplx px[E];
-    xinv_row<N, E, Cfg>(px, buf, t, twh, twx, in.p[0] + off, lim);
+    xrow_inv<N, E, Cfg>(px, rb, buf, t, tw, lim);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       ay[m] = c_make(qz[m].x * px[m].x, qz[m].y * px[m].y);  // Q_z P_x
@@ -696,21 +908,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     }
   }
   GHX_PHASE();
+  xrow_load<N, E, Cfg>(rb, in.p[2] + off, t, lim);
   {
     cplx py[E];
-    xinv_row<N, E, Cfg>(py, buf, t, twh, twx, in.p[1] + off, lim);
+    xrow_inv<N, E, Cfg>(py, ra, buf, t, tw, lim);
     cplx oz[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       oz[m] = c_make((az[m].x - qx[m].x * py[m].x) * tmp, (az[m].y - qx[m].y * py[m].y) * tmp);
       az[m] = c_make(py[m].x * qz[m].x, py[m].y * qz[m].y);  // now P_y Q_z
     }
-    xfwd_row<N, E, Cfg>(oz, buf, t, twh, twx, out.p[2] + off, lim);
+    xrow_fwd<N, E, Cfg>(oz, buf, t, tw, out.p[2] + off, lim);
   }
   GHX_PHASE();
   {
     cplx pz[E];
-    xinv_row<N, E, Cfg>(pz, buf, t, twh, twx, in.p[2] + off, lim);
+    xrow_inv<N, E, Cfg>(pz, rb, buf, t, tw, lim);
     cplx ox[E], oy[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -718,11 +931,55 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       oy[m] = c_make((pz[m].x * qx[m].x - ay[m].x) * tmp, (pz[m].y * qx[m].y - ay[m].y) * tmp);
     }
     GHX_PHASE();
-    xfwd_row<N, E, Cfg>(ox, buf, t, twh, twx, out.p[0] + off, lim);
+    xrow_fwd<N, E, Cfg>(ox, buf, t, tw, out.p[0] + off, lim);
     GHX_PHASE();
-    xfwd_row<N, E, Cfg>(oy, buf, t, twh, twx, out.p[1] + off, lim);
+    xrow_fwd<N, E, Cfg>(oy, buf, t, tw, out.p[1] + off, lim);
   }
 #undef GHX_PHASE
+}
+
+// Fused X^-1 . (A . B) tmp . X on P rows for advect3 (pseudospec3D_phd.f90:
+// 23-115): in.p[0..2] = A, in.p[3..5] = B (grad of the scalar), out.p[0] =
+// FFT[((A_x B_x + A_y B_y) + A_z B_z) tmp] with tmp = -1/N^6.
+template <int N>
+__global__ __launch_bounds__(256) void k_xdot(FieldSet in, FieldSet out, GridK g, int nxp,
+                                             const cplx *__restrict__ twh, const cplx *__restrict__ twx, int kxlim) {
+  using Cfg = FFTCfgLow<N>;
+  constexpr int T = Cfg::T, E = Cfg::E;
+  extern __shared__ cplx lds[];
+  const int rpb = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const long nrows = (long)g.nz * g.ny;
+  const long r = (long)blockIdx.x * rpb + lt;
+  const bool ok = r < nrows;
+  if constexpr (T <= 64) {
+    if (!ok) return;
+  }
+  const int lim = ok ? kxlim : 0;
+  const size_t off = (size_t)(ok ? r : 0) * nxp;
+  cplx *buf = lds + lt * lds_line_stride(N);
+  const double tmp = -g.inv_n6;
+  XTw<N, Cfg> tw;
+  tw.load(twh, twx, t);
+  XRowIn<E> ra, rb;
+  cplx acc[E];
+  xrow_load<N, E, Cfg>(ra, in.p[0] + off, t, lim);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    cplx a[E], b[E];
+    xrow_load<N, E, Cfg>(rb, in.p[3 + d] + off, t, lim);
+    xrow_inv<N, E, Cfg>(a, ra, buf, t, tw, lim);
+    if (d < 2) xrow_load<N, E, Cfg>(ra, in.p[d + 1] + off, t, lim);
+    xrow_inv<N, E, Cfg>(b, rb, buf, t, tw, lim);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const cplx p = c_make(a[m].x * b[m].x, a[m].y * b[m].y);
+      if (d == 0) acc[m] = p;
+      else if (d == 1) acc[m] = c_make(acc[m].x + p.x, acc[m].y + p.y);
+      else acc[m] = c_make((acc[m].x + p.x) * tmp, (acc[m].y + p.y) * tmp);
+    }
+  }
+  xrow_fwd<N, E, Cfg>(acc, buf, t, tw, out.p[0] + off, lim);
 }
 
 // ---------------------------------------------------------------------------
@@ -763,6 +1020,12 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
     } else if (mode == 2) {                                                                 \
       allow_lds(k_zinv<n_, 2>, lds);                                                        \
       hipLaunchKernelGGL((k_zinv<n_, 2>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+    } else if (mode == 4) {                                                                 \
+      allow_lds(k_zinv<n_, 4>, lds);                                                        \
+      hipLaunchKernelGGL((k_zinv<n_, 4>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+    } else if (mode == 5) {                                                                 \
+      allow_lds(k_zinv<n_, 5>, lds);                                                        \
+      hipLaunchKernelGGL((k_zinv<n_, 5>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     } else {                                                                                \
       allow_lds(k_zinv<n_, 3>, lds);                                                        \
       hipLaunchKernelGGL((k_zinv<n_, 3>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
@@ -926,6 +1189,29 @@ hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out,
   return hipGetLastError();
 }
 
+hipError_t xinv3i(const FFTCtx &c, const FieldSet &in, double *out, int prune) {
+  const GridK &g = c.gk;
+  if (g.nx % 2) return hipErrorInvalidValue;
+  const int kxlim = prune ? active_kx(g) : g.nxh;
+  const long nrows = (long)g.nz * g.ny;
+  const int nxp = p_row(g);
+  switch (g.nx / 2) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfgLow<n_>::T;                                                     \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    hipLaunchKernelGGL((k_xinv3i<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet &out) {
   const GridK &g = c.gk;
   if (g.nx % 2) return hipErrorInvalidValue;
@@ -984,3 +1270,51 @@ hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet 
 #endif
 
 }  // namespace ghx
+
+#if GHX_PART == 8
+namespace ghx {
+hipError_t xdot(const FFTCtx &c, const FieldSet &in, const FieldSet &out, int prune) {
+  const GridK &g = c.gk;
+  if (g.nx % 2) return hipErrorInvalidValue;
+  const int kxlim = prune ? active_kx(g) : g.nxh;
+  const long nrows = (long)g.nz * g.ny;
+  const int nxp = p_row(g);
+  switch (g.nx / 2) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfgLow<n_>::T;                                                     \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    hipLaunchKernelGGL((k_xdot<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t zfwd_rk_bouss(const FFTCtx &c, const FieldSet &in, const BoussRKParams &rk, int prune) {
+  const GridK &g = c.gk;
+  switch (g.nz) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4));                   \
+    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
+    allow_lds(k_zfwd_rk_bouss<n_>, lds);                                                    \
+    hipLaunchKernelGGL((k_zfwd_rk_bouss<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
+    break;                                                                                  \
+  }
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+}  // namespace ghx
+#endif

@@ -260,11 +260,14 @@ __global__ void k_reduce(GridK g, int kind, const cplx *a, const cplx *b, const 
       cplx dv = c_add(c_add(c_ik(kx, a[m]), c_ik(ky, b[m])), c_ik(kz, c[m]));
       s = cabs2(dv);
     } else {
-      // kind 4: count of modes outside tiny <= kn2 <= kmax holding a non-zero value
+      // kind 4: count of modes outside tiny <= kn2 <= kmax holding a non-zero
+      // value; kind 5: the same outside kn2 <= kmax only (a scalar whose mean,
+      // kn2 < tiny, is carried)
       const double kn2 = kn2_of(g, q.i, q.j, q.k);
       const cplx A = a[m], B = b[m], Cc = c[m];
       const bool nz = A.x != 0.0 || A.y != 0.0 || B.x != 0.0 || B.y != 0.0 || Cc.x != 0.0 || Cc.y != 0.0;
-      acc += (!(kn2 <= g.kmax && kn2 >= g.tiny) && nz) ? 1.0 : 0.0;
+      const bool kept = kind == 4 ? (kn2 <= g.kmax && kn2 >= g.tiny) : kn2 <= g.kmax;
+      acc += (!kept && nz) ? 1.0 : 0.0;
       continue;
     }
     acc += w * s * tmp;

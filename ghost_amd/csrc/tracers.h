@@ -7,7 +7,7 @@
 namespace ghx {
 
 struct TracerStage {
-  const double *coef[3];  // spline control points of u_x, u_y, u_z, r[z][y][x]
+  const double *coef3;    // spline control points of u_x, u_y, u_z interleaved: c[((z ny + y) nx + x) 3 + comp]
   double *pos[3];         // positions, updated in place
   const double *x0[3];    // positions at the start of the RK step
   double *lv[3];          // interpolated velocities (may be null)

@@ -97,6 +97,26 @@ GHX_DEV double gather(const double *__restrict__ c, const Stencil &s, int nx, si
   return acc;
 }
 
+// The same sums for three interleaved components (c[(row nx + x) 3 + comp]):
+// each stencil row of the three components is 96 contiguous bytes.
+GHX_DEV void gather3(const double *__restrict__ c, const Stencil &s, int nx, size_t nxy, double acc[3]) {
+  acc[0] = acc[1] = acc[2] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double *plane = c + (size_t)s.iz[k] * nxy * 3;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double *row = plane + (size_t)s.iy[j] * nx * 3;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double *p = row + 3 * s.ix[i];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q] = add(acc[q], mul(mul(mul(p[q], s.wx[i]), s.wy[j]), s.wz[k]));
+      }
+    }
+  }
+}
+
 // GPart_MakePeriodicP (gparts_mod.f90:2270-2318): modulo(p + 2 n, n).
 GHX_DEV double wrap(double p, double n) { return fmod(add(p, mul(2.0, n)), n); }
 
@@ -107,8 +127,7 @@ __global__ __launch_bounds__(256) void k_tracer_stage(TracerStage p) {
   Stencil s;
   make_stencil(s, p.pos[0][q], p.pos[1][q], p.pos[2][q], p.nx, p.ny, p.nz, p.zcompat);
   double u[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) u[c] = gather(p.coef[c], s, p.nx, nxy);
+  gather3(p.coef3, s, p.nx, nxy, u);
   // GPart_StepRKK (gparts_mod.f90:2046-2145): x = x0 + dtfact u, then x and
   // y wrapped there and again in GPart_EndStageRKK (:1911-2043), z once.
   double x = add(p.x0[0][q], mul(p.dtf[0], u[0]));

@@ -144,6 +144,12 @@ class Grid:
         _lib.check(_lib.load().ghx_sharp(self._h, self._c(a), self._c(b), float(delta)))
         return b
 
+    # ---- pseudospec3D_phd.f90 ---------------------------------------------
+    def advect3(self, a, b, c, d, e):
+        """e = FFT[-(A . grad d)] / N^6 (pseudospec3D_phd.f90:23-115)."""
+        _lib.check(_lib.load().ghx_advect3(self._h, *map(self._c, (a, b, c, d, e))))
+        return e
+
     # ---- pseudospec3D_mhd.f90 ---------------------------------------------
     def vector3(self, a, b, c, d, e, f, x, y, z):
         _lib.check(_lib.load().ghx_vector3(self._h, *map(self._c, (a, b, c, d, e, f, x, y, z))))

@@ -127,3 +127,60 @@ class MHDSolver(_Resident):
         """As HDSolver.set_tracers."""
         _lib.check(_lib.load().ghx_mhd_set_tracers(self._h, parts._h if parts is not None else None))
         self._parts = parts
+
+
+class BoussSolver(_Resident):
+    """Boussinesq state: velocity v (3), temperature th, forcings f (3) and fs
+    (include/bouss_rkstep{1,2}.f90).  xmom and xtemp are the buoyancy and
+    heat-current factors already multiplied by bvfreq."""
+
+    def __init__(self, grid: Grid, dt, nu, kappa, xmom, xtemp, ord=2):
+        self.grid, self.dt, self.nu, self.kappa, self.ord = grid, dt, nu, kappa, ord
+        h = C.c_void_p()
+        _lib.check(_lib.load().ghx_bouss_create(C.byref(h), grid._h, ord, float(dt), float(nu), float(kappa),
+                                                float(xmom), float(xtemp)))
+        self._h = h
+        v, f = (C.c_void_p * 3)(), (C.c_void_p * 3)()
+        th, fs = C.c_void_p(), C.c_void_p()
+        _lib.check(_lib.load().ghx_bouss_fields(h, C.cast(v, C.POINTER(C.c_void_p)), C.byref(th),
+                                                C.cast(f, C.POINTER(C.c_void_p)), C.byref(fs)))
+        self._v = [v[k] for k in range(3)]
+        self._f = [f[k] for k in range(3)]
+        self._th, self._fs = th.value, fs.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _lib.load().ghx_bouss_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def set_state(self, v, th):
+        torch.cuda.synchronize()
+        for k in range(3):
+            self._copy_in(self._v[k], v[k])
+        self._copy_in(self._th, th)
+        _lib.check(_lib.load().ghx_bouss_touch(self._h))
+
+    def set_forcing(self, f=None, fs=None):
+        torch.cuda.synchronize()
+        if f is None and fs is None:
+            _lib.check(_lib.load().ghx_bouss_set_forcing(self._h, 0))
+            return
+        zero = self.grid.czeros()
+        for k in range(3):
+            self._copy_in(self._f[k], f[k] if f is not None else zero)
+        self._copy_in(self._fs, fs if fs is not None else zero)
+        _lib.check(_lib.load().ghx_bouss_set_forcing(self._h, 1))
+
+    def state(self):
+        return [self._copy_out(p) for p in self._v], self._copy_out(self._th)
+
+    def step(self, n=1):
+        _lib.check(_lib.load().ghx_bouss_step(self._h, int(n)))
+
+    def set_tracers(self, parts):
+        """As HDSolver.set_tracers."""
+        _lib.check(_lib.load().ghx_bouss_set_tracers(self._h, parts._h if parts is not None else None))
+        self._parts = parts

@@ -138,6 +138,10 @@ int ghx_spectrumc(ghx_grid *g, const void *a, const void *b, const void *c, int 
 int ghx_phaseshift(ghx_grid *g, void *a, double cdump_re, double cdump_im);                   /* hd :2084 */
 int ghx_normalize(ghx_grid *g, void *fx, void *fy, void *fz, double f0, int kin);             /* module_dns.f90:13 */
 int ghx_sharp(ghx_grid *g, const void *a, void *b, double delta);                             /* filt :200 */
+/* advect3 (pseudo/pseudospec3D_phd.f90:23-115): e = FFT[-(A . grad d)] / N^6
+ * with A = (a, b, c); the products of GHOST's three c2r pairs are fused into
+ * one inverse chain, the x pass and one forward chain. */
+int ghx_advect3(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, void *e);
 
 /* ---------------------------------------------------------------------- */
 /* Initial conditions / forcing (examples/initialv.f90_tg,                 */
@@ -188,6 +192,22 @@ int ghx_mhd_touch(ghx_mhd *s);
 int ghx_mhd_set_forcing(ghx_mhd *s, int enabled);
 int ghx_mhd_step(ghx_mhd *s, int nsteps);
 
+typedef struct ghx_bouss ghx_bouss;
+/* Boussinesq solver (include/bouss_rkstep1.f90, bouss_rkstep2.f90:1-71 in the
+ * loop of main3D.fpp:3202-3231): state v (3) and th, forcing f (3) and fs.
+ * kappa: diffusivity; xmom, xtemp: the buoyancy and heat-current factors
+ * already multiplied by bvfreq (main3D.fpp:973-974).  ord 1 or 2; the ord = 3
+ * correction of bouss_rkstep2.f90:73-145 is not built (GHX_ERR_ARG). */
+int ghx_bouss_create(ghx_bouss **s, ghx_grid *g, int ord, double dt, double nu, double kappa, double xmom,
+                     double xtemp);
+int ghx_bouss_destroy(ghx_bouss *s);
+/* device pointers: v[3], th, f[3], fs */
+int ghx_bouss_fields(ghx_bouss *s, void **v, void **th, void **f, void **fs);
+/* as ghx_hd_touch; the mean of th (kn2 < tiny) is carried and does not count */
+int ghx_bouss_touch(ghx_bouss *s);
+int ghx_bouss_set_forcing(ghx_bouss *s, int enabled);
+int ghx_bouss_step(ghx_bouss *s, int nsteps);
+
 /* ---------------------------------------------------------------------- */
 /* Lagrangian tracers: GHOST's GPart with cubic-spline interpolation on one */
 /* rank (lagpart/gparts_mod.f90, lagpart/gpsplineint_mod.fpp).  Positions   */
@@ -221,6 +241,7 @@ int ghx_tracers_euler_to_lag(ghx_tracers *t, const double *field, double *out);
  * substep produced (main3D.fpp:3202-3267).  t = NULL detaches. */
 int ghx_hd_set_tracers(ghx_hd *s, ghx_tracers *t);
 int ghx_mhd_set_tracers(ghx_mhd *s, ghx_tracers *t);
+int ghx_bouss_set_tracers(ghx_bouss *s, ghx_tracers *t);
 
 /* ---------------------------------------------------------------------- */
 /* Pass profiler (replaces the GTStart/GTStop handles around the fftp      */

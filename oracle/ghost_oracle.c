@@ -1244,3 +1244,141 @@ void og_hd_step_tracers(const og_grid *g, int ord, double dt, double nu, ocplx *
   free(C);
   free(x0);
 }
+
+/* ======================================================================
+ * Boussinesq (pseudo/pseudospec3D_phd.f90, include/bouss_rkstep{1,2}.f90)
+ * ====================================================================== */
+
+/* advect3 (pseudospec3D_phd.f90:23-115): e = FFT[-(a . grad d)] / N^6, the
+ * three products accumulated in GHOST's order (x, then y, then z, then the
+ * scale). */
+void og_advect3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                ocplx *e) {
+  const size_t nr = nreal(g), nc = ncplx(g);
+  double *r1 = (double *)malloc(sizeof(double) * nr);
+  double *r2 = (double *)malloc(sizeof(double) * nr);
+  double *r3 = (double *)malloc(sizeof(double) * nr);
+  ocplx *c2 = (ocplx *)malloc(sizeof(ocplx) * nc);
+  const ocplx *comp[3] = {a, b, c};
+  const double t = (double)g->nx * (double)g->ny * (double)g->nz;
+  const double tmp = -1.0 / (t * t);
+  for (int dir = 1; dir <= 3; ++dir) {
+    og_derivk3(g, d, c2, dir);
+    og_c2r(g, comp[dir - 1], r1);
+    og_c2r(g, c2, r2);
+    OMP_FOR
+    for (size_t m = 0; m < nr; ++m) {
+      if (dir == 1) r3[m] = r1[m] * r2[m];
+      else if (dir == 2) r3[m] = r3[m] + r1[m] * r2[m];
+      else r3[m] = (r3[m] + r1[m] * r2[m]) * tmp;
+    }
+  }
+  og_r2c(g, r3, e);
+  free(r1); free(r2); free(r3); free(c2);
+}
+
+/* One Boussinesq substep o, include/bouss_rkstep2.f90:1-71 (the ord = 3
+ * correction of :73-145 is og_bouss_ord3_correction).  State (vx,vy,vz,th)
+ * updated in place from the step-start copies (c1,c2,c3,c20); forcing
+ * (fx,fy,fz,fs) may be NULL. */
+void og_bouss_substep(const og_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
+                      ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *c1, const ocplx *c2,
+                      const ocplx *c3, const ocplx *c20, const ocplx *fx, const ocplx *fy, const ocplx *fz,
+                      const ocplx *fs) {
+  size_t nc = ncplx(g);
+  ocplx *W = (ocplx *)malloc(sizeof(ocplx) * nc * 5);
+  ocplx *C4 = W, *C5 = W + nc, *C6 = W + 2 * nc, *C7 = W + 3 * nc, *C8 = W + 4 * nc;
+  og_prodre3(g, vx, vy, vz, C4, C5, C6);
+  OMP_FOR
+  for (size_t m = 0; m < nc; ++m) C6[m] = C6[m] + xmom * th[m]; /* gravity */
+  og_nonlhd3(g, C4, C5, C6, C7, 1);
+  og_nonlhd3(g, C4, C5, C6, C8, 2);
+  og_nonlhd3(g, C4, C5, C6, C4, 3);
+  og_advect3(g, vx, vy, vz, th, C5);
+  OMP_FOR
+  for (size_t m = 0; m < nc; ++m) C5[m] = C5[m] + xtemp * vz[m]; /* heat current */
+  og_laplak3(g, vx, vx);
+  og_laplak3(g, vy, vy);
+  og_laplak3(g, vz, vz);
+  og_laplak3(g, th, th);
+  const double rmp = 1. / (double)o;
+  OMP_FOR
+  for (size_t m = 0; m < nc; ++m) {
+    const double k2 = g->kn2[m];
+    if (k2 <= g->kmax && k2 >= g->tiny) {
+      ocplx f1 = fx ? fx[m] : 0, f2 = fy ? fy[m] : 0, f3 = fz ? fz[m] : 0, f4 = fs ? fs[m] : 0;
+      vx[m] = c1[m] + dt * (nu * vx[m] + C7[m] + f1) * rmp;
+      vy[m] = c2[m] + dt * (nu * vy[m] + C8[m] + f2) * rmp;
+      vz[m] = c3[m] + dt * (nu * vz[m] + C4[m] + f3) * rmp;
+      th[m] = c20[m] + dt * (kappa * th[m] + C5[m] + f4) * rmp;
+    } else if (k2 > g->kmax) {
+      vx[m] = 0; vy[m] = 0; vz[m] = 0; th[m] = 0;
+    } else { /* kn2 < tiny: the mean of th is kept */
+      vx[m] = 0; vy[m] = 0; vz[m] = 0; th[m] = c20[m];
+    }
+  }
+  free(W);
+}
+
+/* include/bouss_rkstep2.f90:73-145, applied after the last substep when
+ * ord == 3: u += 2 N(L(C) + N(C)) dt^3/24 with N the projected nonlinear term
+ * including the buoyancy of the step-start temperature c20, L the Laplacian
+ * (without nu, as written there).  c1..c3 are overwritten. */
+static void bouss_ord3_correction(const og_grid *g, double dt, double xmom, ocplx *vx, ocplx *vy, ocplx *vz,
+                                  ocplx *c1, ocplx *c2, ocplx *c3, const ocplx *c20) {
+  size_t nc = ncplx(g);
+  ocplx *W = (ocplx *)malloc(sizeof(ocplx) * nc * 5);
+  ocplx *C4 = W, *C5 = W + nc, *C6 = W + 2 * nc, *C7 = W + 3 * nc, *C8 = W + 4 * nc;
+  og_prodre3(g, c1, c2, c3, C4, C5, C6);
+  OMP_FOR
+  for (size_t m = 0; m < nc; ++m) C6[m] = C6[m] + xmom * c20[m];
+  og_nonlhd3(g, C4, C5, C6, C7, 1);
+  og_nonlhd3(g, C4, C5, C6, C8, 2);
+  og_nonlhd3(g, C4, C5, C6, C4, 3);
+  og_laplak3(g, c1, c1);
+  og_laplak3(g, c2, c2);
+  og_laplak3(g, c3, c3);
+  OMP_FOR
+  for (size_t m = 0; m < nc; ++m) {
+    if (g->kn2[m] <= g->kmax && g->kn2[m] >= g->tiny) {
+      c1[m] = c1[m] + C7[m];
+      c2[m] = c2[m] + C8[m];
+      c3[m] = c3[m] + C4[m];
+    } else {
+      c1[m] = 0; c2[m] = 0; c3[m] = 0;
+    }
+  }
+  og_prodre3(g, c1, c2, c3, C4, C5, C6);
+  OMP_FOR
+  for (size_t m = 0; m < nc; ++m) C6[m] = C6[m] + xmom * c20[m];
+  og_nonlhd3(g, C4, C5, C6, C7, 1);
+  og_nonlhd3(g, C4, C5, C6, C8, 2);
+  og_nonlhd3(g, C4, C5, C6, C4, 3);
+  const double rmp = dt * dt * dt / 24.0;
+  OMP_FOR
+  for (size_t m = 0; m < nc; ++m) {
+    vx[m] = vx[m] + 2.0 * C7[m] * rmp;
+    vy[m] = vy[m] + 2.0 * C8[m] * rmp;
+    vz[m] = vz[m] + 2.0 * C4[m] * rmp;
+  }
+  free(W);
+}
+
+/* main3D.fpp:3202-3231 with include/bouss_rkstep1.f90 and bouss_rkstep2.f90 */
+void og_bouss_step(const og_grid *g, int ord, double dt, double nu, double kappa, double xmom, double xtemp,
+                   ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *fx, const ocplx *fy,
+                   const ocplx *fz, const ocplx *fs) {
+  size_t nc = ncplx(g);
+  ocplx *C = (ocplx *)malloc(sizeof(ocplx) * nc * 4);
+  memcpy(C, vx, sizeof(ocplx) * nc);
+  memcpy(C + nc, vy, sizeof(ocplx) * nc);
+  memcpy(C + 2 * nc, vz, sizeof(ocplx) * nc);
+  memcpy(C + 3 * nc, th, sizeof(ocplx) * nc);
+  for (int o = ord; o >= 1; --o) {
+    og_bouss_substep(g, o, dt, nu, kappa, xmom, xtemp, vx, vy, vz, th, C, C + nc, C + 2 * nc, C + 3 * nc, fx,
+                     fy, fz, fs);
+    if (ord == 3 && o <= 1)
+      bouss_ord3_correction(g, dt, xmom, vx, vy, vz, C, C + nc, C + 2 * nc, C + 3 * nc);
+  }
+  free(C);
+}

@@ -124,4 +124,17 @@ void og_hd_step_tracers(const og_grid *g, int ord, double dt, double nu, ocplx *
                         const ocplx *fx, const ocplx *fy, const ocplx *fz, long np, double *px, double *py,
                         double *pz, int zcompat, int ghost_solve);
 
+/* Boussinesq: advect3 (pseudospec3D_phd.f90:23-115) and the RK step of
+ * include/bouss_rkstep{1,2}.f90 (state v, th; kappa diffusivity, xmom and
+ * xtemp already multiplied by bvfreq as main3D.fpp:973-974 does). */
+void og_advect3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                ocplx *e);
+void og_bouss_substep(const og_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
+                      ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *c1, const ocplx *c2,
+                      const ocplx *c3, const ocplx *c20, const ocplx *fx, const ocplx *fy, const ocplx *fz,
+                      const ocplx *fs);
+void og_bouss_step(const og_grid *g, int ord, double dt, double nu, double kappa, double xmom, double xtemp,
+                   ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *fx, const ocplx *fy,
+                   const ocplx *fz, const ocplx *fs);
+
 #endif

@@ -88,6 +88,8 @@ def _declare(L):
                                 _P, _P, _P]
     L.og_mhd_step.argtypes = [_G, C.c_int, C.c_double, C.c_double, C.c_double] + [_P] * 12
     L.og_sharp.argtypes = [_G, _P, _P, C.c_double]
+    L.og_advect3.argtypes = [_G, _P, _P, _P, _P, _P]
+    L.og_bouss_step.argtypes = [_G, C.c_int] + [C.c_double] * 5 + [_P] * 8
     L.og_spline_coef.argtypes = [_G, _P, _P, C.c_int]
     L.og_spline_interp.argtypes = [_G, _P, C.c_long, _P, _P, _P, C.c_int, _P]
     L.og_tracer_stage.argtypes = [_G, _P, _P, _P, C.c_long, _P, _P, _P, _P, _P, _P, C.c_double,
@@ -274,6 +276,18 @@ class Grid:
     def hd_substep(self, v, c, o, dt, nu, f=None):
         fp = [None] * 3 if f is None else list(map(_p, f))
         lib().og_hd_substep(self._g, o, dt, nu, *map(_p, v), *map(_p, c), *fp)
+
+    # -- Boussinesq -----------------------------------------------------------
+    def advect3(self, a, b, c, d):
+        """FFT[-(A . grad d)] / N^6 (pseudospec3D_phd.f90:23-115)."""
+        e = self.czeros()
+        lib().og_advect3(self._g, _p(a), _p(b), _p(c), _p(d), _p(e))
+        return e
+
+    def bouss_step(self, v, th, dt, nu, kappa, xmom, xtemp, f=None, fs=None, ord=2):
+        """One Boussinesq RK step in place on v = [vx, vy, vz] and th."""
+        fp = [None] * 3 if f is None else list(map(_p, f))
+        lib().og_bouss_step(self._g, ord, dt, nu, kappa, xmom, xtemp, *map(_p, v), _p(th), *fp, _p(fs))
 
     # -- Lagrangian tracers (positions in grid units, float64 arrays) --------
     def spline_coef(self, field, ghost=True):
