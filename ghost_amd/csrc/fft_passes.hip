@@ -130,7 +130,7 @@ GHX_DEV unsigned tile_lines_active(const GridK &g, int i, int j0, int L) {
 // Z^-1: S -> Q.  Block = (kx index i, ky chunk j0..j0+L-1).
 // ---------------------------------------------------------------------------
 template <int N, int E>
-GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *tw,
+GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N> &tw,
                        cplx *dst, const GridK &g, int i, int j0, unsigned lmask) {
   using F = LineFFT<N, +1>;
   constexpr int T = F::T;
@@ -143,13 +143,15 @@ GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
 }
 
 template <int N, int MODE>
-__global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ tw,
+__global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twt,
                                              int prune, const double *__restrict__ aux) {
   using F = LineFFT<N, +1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  tw.load(twt, t);
   int bx, by;
   xcd_tile(bx, by);
   const int i = by, j0 = bx * L, j = j0 + lt;
@@ -162,6 +164,14 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
   cplx *row = lds + lt * stride;
   const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
   auto need = [&](int k) { return ok && (!prune || kn2_of(g, i, j, k) <= g.kmax); };
+  // every lane loads from inside its line (skipped modes read the line's
+  // first point, which the line's own loads fetch anyway) and a select zeroes
+  // what is not needed, so the loads of a field issue back to back
+  auto load = [&](const cplx *p, int k) {
+    const bool nd = need(k);
+    const cplx x = p[sline + (nd ? k : 0)];
+    return nd ? x : c_make(0.0, 0.0);
+  };
   if constexpr (MODE == 0 || MODE == 3) {
     // MODE 3: periodic cubic-spline prefilter.  aux = [sx (nx/2+1) | sy (ny) |
     // sz (nz)] and every mode is scaled by sx[kx] sy[ky] sz[kz] on load.
@@ -171,7 +181,7 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
       cplx v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        v[m] = need(t + T * m) ? in.p[f][sline + t + T * m] : c_make(0.0, 0.0);
+        v[m] = load(in.p[f], t + T * m);
         if constexpr (MODE == 3) v[m] = c_scale(v[m], sxy * aux[g.nxh + g.ny + t + T * m]);
       }
       zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0, lmask);
@@ -184,7 +194,7 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int m = 0; m < E; ++m) u[c][m] = need(t + T * m) ? in.p[c][sline + t + T * m] : c_make(0.0, 0.0);
+      for (int m = 0; m < E; ++m) u[c][m] = load(in.p[c], t + T * m);
     // rotor3 (pseudospec3D_hd.f90:124-200) on (a0,a1,a2) -> component c
     auto curl = [&](const cplx (&a)[3][E], int c, cplx (&o)[E]) {
 #pragma unroll
@@ -215,7 +225,7 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
       constexpr int base = MODE == 4 ? 3 : 6;
       cplx d[E];
 #pragma unroll
-      for (int m = 0; m < E; ++m) d[m] = need(t + T * m) ? in.p[3][sline + t + T * m] : c_make(0.0, 0.0);
+      for (int m = 0; m < E; ++m) d[m] = load(in.p[3], t + T * m);
 #pragma unroll
       for (int dir = 0; dir < 3; ++dir) {
         cplx v[E];
@@ -232,7 +242,7 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int m = 0; m < E; ++m) a[c][m] = need(t + T * m) ? in.p[3 + c][sline + t + T * m] : c_make(0.0, 0.0);
+        for (int m = 0; m < E; ++m) a[c][m] = load(in.p[3 + c], t + T * m);
       curl(a, 0, b[0]);
       curl(a, 1, b[1]);
       curl(a, 2, b[2]);
@@ -257,7 +267,7 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
 // Z forward: Q -> S.  Block = (kx index i, ky chunk).  Transposed load.
 // ---------------------------------------------------------------------------
 template <int N, int E>
-GHX_DEV void zfwd_load(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const cplx *tw,
+GHX_DEV void zfwd_load(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N> &tw,
                        const cplx *src, const GridK &g, int i, int j0, unsigned lmask = ~0u) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T;
@@ -270,12 +280,14 @@ GHX_DEV void zfwd_load(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
 }
 
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ tw) {
+__global__ __launch_bounds__(512) void k_zfwd(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twt) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  tw.load(twt, t);
   int bx, by;
   xcd_tile(bx, by);
   const int i = by, j0 = bx * L, j = j0 + lt;
@@ -312,13 +324,15 @@ GHX_DEV cplx rk_update(cplx cc, cplx vv, cplx nl, cplx ff, double kk2, double nu
 }
 
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const cplx *__restrict__ tw, RKParams rk,
+__global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const cplx *__restrict__ twt, RKParams rk,
                                                    int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  tw.load(twt, t);
   int bx, by;
   xcd_tile(bx, by);
   const int i = by, j0 = bx * L, j = j0 + lt;
@@ -340,28 +354,24 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const 
   const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
   const size_t sline = ((size_t)i * g.ny + j) * N;
   const bool forced = rk.fline && rk.fline[(size_t)i * g.ny + j];
+  const cplx z = c_make(0.0, 0.0);
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + T * m;
     const size_t s = sline + k;
     const double kz = wavenum(k, g.nz, g.dkz);
     const double kn2 = kn2_of(g, i, j, k);
-    cplx o0 = c_make(0.0, 0.0), o1 = o0, o2 = o0;
     if (kn2 <= g.kmax && kn2 >= g.tiny) {
       const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
       cplx n0, n1, n2;
       project_hd(P[0][m], P[1][m], P[2][m], kx, ky, kz, kk2, i, j, k, n0, n1, n2);
-      const cplx z = c_make(0.0, 0.0);
-      o0 = rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
-      o1 = rk_update(rk.c[1][s], rk.v[1][s], n1, forced ? rk.f[1][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
-      o2 = rk_update(rk.c[2][s], rk.v[2][s], n2, forced ? rk.f[2][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
-      rk.out[0][s] = o0;
-      rk.out[1][s] = o1;
-      rk.out[2][s] = o2;
+      rk.out[0][s] = rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+      rk.out[1][s] = rk_update(rk.c[1][s], rk.v[1][s], n1, forced ? rk.f[1][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+      rk.out[2][s] = rk_update(rk.c[2][s], rk.v[2][s], n2, forced ? rk.f[2][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
     } else if (!prune) {
-      rk.out[0][s] = o0;
-      rk.out[1][s] = o1;
-      rk.out[2][s] = o2;
+      rk.out[0][s] = z;
+      rk.out[1][s] = z;
+      rk.out[2][s] = z;
     }
     // keep the loads of one mode from being hoisted over the others (register pressure)
     __builtin_amdgcn_sched_barrier(0);
@@ -374,13 +384,15 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const 
 // advection, then the updates of v (nu) and th (kappa).  The k = 0 mode keeps
 // the step-start mean of th; v is zero there and outside the dealias set.
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, const cplx *__restrict__ tw,
+__global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, const cplx *__restrict__ twt,
                                                       BoussRKParams rk, int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  tw.load(twt, t);
   int bx, by;
   xcd_tile(bx, by);
   const int i = by, j0 = bx * L, j = j0 + lt;
@@ -456,13 +468,15 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, con
 // MHD: nonlin3 (pseudospec3D_mhd.f90:108-227) for the velocity from
 // (w x u, j x b) and gauge3 (:230-336) for the potential from u x b.
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const cplx *__restrict__ tw, MHDRKParams rk,
+__global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const cplx *__restrict__ twt, MHDRKParams rk,
                                                     int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  tw.load(twt, t);
   int bx, by;
   xcd_tile(bx, by);
   const int i = by, j0 = bx * L, j = j0 + lt;
@@ -560,12 +574,14 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ __launch_bounds__(512) void k_yinv(FieldSet in, FieldSet out, int nf, GridK g, int nxp,
-                                             const cplx *__restrict__ tw, int kxlim, int prune) {
+                                             const cplx *__restrict__ twt, int kxlim, int prune) {
   using F = LineFFT<N, +1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  tw.load(twt, t);
   int bx, by;
   xcd_tile(bx, by);
   const int z = by, kx0 = bx * L, kx = kx0 + lt;
@@ -581,7 +597,8 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in, FieldSet out, int nf,
       const int j = t + T * m;
       // dealiased input: Q holds data only on the lines (kx,ky) of the dealias disc
       const bool need = ok && (!prune || kn2_of(g, kx, j, 0) <= g.kmax);
-      v[m] = need ? src[j] : c_make(0.0, 0.0);
+      const cplx x = src[need ? j : 0];  // unconditional in-line load, see k_zinv
+      v[m] = need ? x : c_make(0.0, 0.0);
     }
     F::run(v, row, t, tw);
 #pragma unroll
@@ -598,12 +615,14 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in, FieldSet out, int nf,
 
 template <int N>
 __global__ __launch_bounds__(512) void k_yfwd(FieldSet in, FieldSet out, int nf, GridK g, int nxp,
-                                             const cplx *__restrict__ tw, int kxlim, int prune) {
+                                             const cplx *__restrict__ twt, int kxlim, int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  tw.load(twt, t);
   int bx, by;
   xcd_tile(bx, by);
   const int z = by, kx0 = bx * L, kx = kx0 + lt;
