@@ -107,6 +107,7 @@ def _declare(L):
         "ghx_mhd_set_forcing": (_I, [_P, _I]),
         "ghx_mhd_step": (_I, [_P, _I]),
         "ghx_advect3": (_I, [_P, _P, _P, _P, _P, _P]),
+        "ghx_truncate": (_I, [_P, _P, _P, _P]),
         "ghx_bouss_create": (_I, [_PP, _P, _I, _D, _D, _D, _D, _D]),
         "ghx_bouss_destroy": (_I, [_P]),
         "ghx_bouss_fields": (_I, [_P, _PP, _PP, _PP, _PP]),

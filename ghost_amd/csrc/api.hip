@@ -622,6 +622,18 @@ extern "C" int ghx_vector3(ghx_grid *g, const void *a, const void *b, const void
   return fused_cross(g, 0, a, b, c, d, e, f, x, y, z);
 }
 
+// analysis/trunc3D.fpp:190-320: c (spectrum on g) truncated onto the smaller
+// grid t; a c2r on t then gives the truncated field on t's points.
+extern "C" int ghx_truncate(ghx_grid *g, const void *c, ghx_grid *t, void *out) {
+  ARGCHK(g && c && t && out, "bad truncate arguments");
+  const GridK &a = g->gk, &b = t->gk;
+  if (b.nx > a.nx || b.ny > a.ny || b.nz > a.nz || b.nx % 2 || b.ny % 2 || b.nz % 2)
+    return set_err(GHX_ERR_SIZE, "truncated grid %dx%dx%d must be even and no larger than %dx%dx%d", b.nx, b.ny,
+                   b.nz, a.nx, a.ny, a.nz);
+  HIPCHK(ghx::op_truncate(a, b, t->stream, C_(c), M_(out)));
+  return GHX_OK;
+}
+
 // advect3 (pseudospec3D_phd.f90:23-115): Z^-1 of (a, b, c) and grad d in one
 // pass, Y^-1, the dot product fused into the x pass, Y, Z.
 extern "C" int ghx_advect3(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, void *e) {

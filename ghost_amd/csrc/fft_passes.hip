@@ -107,13 +107,25 @@ GHX_DEV void tile_store_q(cplx *lds, int stride, int L, int NP, cplx *dst, size_
       dst[base + (size_t)p * z_or_p_stride + j0 + l] = lds[l * stride + lds_pad(p)];
   }
 }
-GHX_DEV void tile_load_q(cplx *lds, int stride, int L, int NP, const cplx *src, size_t pstride,
-                         size_t base, int j0, int ny, unsigned lmask = ~0u, int pmax = 1 << 30) {
-  for (int idx = threadIdx.x; idx < L * NP; idx += blockDim.x) {
-    int l = idx % L, p = idx / L;
-    cplx v = c_make(0.0, 0.0);
-    if (j0 + l < ny && ((lmask >> l) & 1u) && p < pmax) v = src[base + (size_t)p * pstride + j0 + l];
-    lds[l * stride + lds_pad(p)] = v;
+// Block = L lines x T lanes, E = N / T points per thread.  The E loads of a
+// thread issue back to back (lines outside the mask or past ny read line j0,
+// inside the tile, and are zeroed by a select), then go to LDS.
+template <int E>
+GHX_DEV void tile_load_q(cplx *lds, int stride, int L, const cplx *src, size_t pstride, size_t base, int j0, int ny,
+                         unsigned lmask = ~0u) {
+  cplx w[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * blockDim.x;
+    const int l = idx % L, p = idx / L;
+    const bool inside = j0 + l < ny && ((lmask >> l) & 1u);
+    const cplx x = src[base + (size_t)p * pstride + j0 + (inside ? l : 0)];
+    w[q] = inside ? x : c_make(0.0, 0.0);
+  }
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * blockDim.x;
+    lds[(idx % L) * stride + lds_pad(idx / L)] = w[q];
   }
 }
 
@@ -271,7 +283,7 @@ GHX_DEV void zfwd_load(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
                        const cplx *src, const GridK &g, int i, int j0, unsigned lmask = ~0u) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T;
-  tile_load_q(lds, stride, L, N, src, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny, lmask);
+  tile_load_q<E>(lds, stride, L, src, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny, lmask);
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = row[lds_pad(t + T * m)];
@@ -406,8 +418,9 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, con
   const bool forced = ok && rk.fline && rk.fline[(size_t)i * g.ny + j];
   const cplx z = c_make(0.0, 0.0);
   // v_z as the nonlinear term saw it, for the heat current: with ord = 1 the
-  // update writes over the input (out == v) before the temperature is done
-  cplx vz_in[E];
+  // update writes over the input (out == v) before the temperature is done.
+  // Kept in LDS past the line tiles (each lane rereads only its own modes).
+  cplx *vz_in = lds + L * stride + lt * N;
   {
     cplx P[3][E];
 #pragma unroll
@@ -422,10 +435,9 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, con
         const size_t s = sline + k;
         const double kz = wavenum(k, g.nz, g.dkz);
         const double kn2 = kn2_of(g, i, j, k);
-        vz_in[m] = z;
         if (kn2 <= g.kmax && kn2 >= g.tiny) {
           const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
-          vz_in[m] = rk.v[2][s];
+          vz_in[k] = rk.v[2][s];
           const cplx th = rk.v[3][s];
           const cplx nzb = c_make(P[2][m].x + rk.xmom * th.x, P[2][m].y + rk.xmom * th.y);
           cplx n0, n1, n2;
@@ -453,7 +465,7 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, con
     const double kn2 = kn2_of(g, i, j, k);
     if (kn2 <= g.kmax && kn2 >= g.tiny) {
       const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
-      const cplx vz = vz_in[m];
+      const cplx vz = vz_in[k];
       const cplx a = c_make(A[m].x + rk.xtemp * vz.x, A[m].y + rk.xtemp * vz.y);
       rk.out[3][s] = rk_update(rk.c[3][s], rk.v[3][s], a, forced && rk.f[3] ? rk.f[3][s] : z, kk2, rk.kappa, rk.dt,
                                rk.rmp);
@@ -589,17 +601,25 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in, FieldSet out, int nf,
   const bool ok = kx < kxlim;
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
-  for (int f = 0; f < nf; ++f) {
+  // dealiased input: Q holds data only on the lines (kx,ky) of the dealias
+  // disc; every lane loads inside its line and a select drops the rest
+  auto load = [&](int f, cplx (&v)[E]) {
     const cplx *src = in.p[f] + ((size_t)z * g.nxh + (ok ? kx : 0)) * N;
-    cplx v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int j = t + T * m;
-      // dealiased input: Q holds data only on the lines (kx,ky) of the dealias disc
       const bool need = ok && (!prune || kn2_of(g, kx, j, 0) <= g.kmax);
-      const cplx x = src[need ? j : 0];  // unconditional in-line load, see k_zinv
+      const cplx x = src[need ? j : 0];
       v[m] = need ? x : c_make(0.0, 0.0);
     }
+  };
+  cplx nxt[E];
+  load(0, nxt);
+  for (int f = 0; f < nf; ++f) {
+    cplx v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = nxt[m];
+    if (f + 1 < nf) load(f + 1, nxt);  // the next field's line in flight during this one
     F::run(v, row, t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
@@ -630,13 +650,31 @@ __global__ __launch_bounds__(512) void k_yfwd(FieldSet in, FieldSet out, int nf,
   const bool ok = kx < kxlim;
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
-  for (int f = 0; f < nf; ++f) {
+  // The block's tile of a field: E points per thread (L*N points over L*T
+  // threads), loaded into registers with in-tile addresses (columns at or
+  // beyond kxlim read column kx0 and are zeroed), so the loads issue back to
+  // back and the next field's tile is in flight while this one is transformed.
+  auto tload = [&](int f, cplx (&w)[E]) {
     const cplx *src = in.p[f] + (size_t)z * g.ny * nxp;
-    for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * blockDim.x;
       const int l = idx % L, p = idx / L;
-      lds[l * stride + lds_pad(p)] = kx0 + l < kxlim ? src[(size_t)p * nxp + kx0 + l] : c_make(0.0, 0.0);
+      const bool inside = kx0 + l < kxlim;
+      const cplx x = src[(size_t)p * nxp + kx0 + (inside ? l : 0)];
+      w[q] = inside ? x : c_make(0.0, 0.0);
+    }
+  };
+  cplx nxt[E];
+  tload(0, nxt);
+  for (int f = 0; f < nf; ++f) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * blockDim.x;
+      lds[(idx % L) * stride + lds_pad(idx / L)] = nxt[q];
     }
     __syncthreads();
+    if (f + 1 < nf) tload(f + 1, nxt);
     cplx v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = row[lds_pad(t + T * m)];
@@ -1321,8 +1359,8 @@ hipError_t zfwd_rk_bouss(const FFTCtx &c, const FieldSet &in, const BoussRKParam
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4));                   \
-    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    int L = pick_L(T, lds_line_stride(n_) + n_, g.ny, tile_want("GHX_LZ", 4));              \
+    size_t lds = (size_t)L * (lds_line_stride(n_) + n_) * sizeof(cplx);                     \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd_rk_bouss<n_>, lds);                                                    \
     hipLaunchKernelGGL((k_zfwd_rk_bouss<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \

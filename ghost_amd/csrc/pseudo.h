@@ -17,6 +17,8 @@ hipError_t op_scale(const GridK &g, hipStream_t st, cplx *a, const cplx *src, do
 // a = a*f0/sqrt(e) (module_dns.f90 normalize) for three fields
 hipError_t op_normalize_apply(const GridK &g, hipStream_t st, cplx *x, cplx *y, cplx *z, double f0, double e);
 hipError_t op_sharp(const GridK &g, hipStream_t st, const cplx *a, cplx *b, double kc2);
+// trunc3D's spectral truncation of c (grid g) onto the smaller grid t
+hipError_t op_truncate(const GridK &g, const GridK &t, hipStream_t st, const cplx *c, cplx *out);
 hipError_t op_zero(hipStream_t st, void *p, size_t bytes);
 // real-space cross product r7=(r2 r6-r5 r3)tmp ... in place (prodre3's loop)
 hipError_t op_cross_real(const GridK &g, hipStream_t st, double *r1, double *r2, double *r3,

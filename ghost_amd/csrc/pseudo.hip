@@ -167,6 +167,26 @@ __global__ void k_sharp(GridK g, const cplx *a, cplx *b, double kc2) {
   }
 }
 
+// analysis/trunc3D.fpp:190-320: out (grid t, smaller) = the modes of c
+// (grid g) with kx^2/nxt^2 + ky^2/nyt^2 + kz^2/nzt^2 <= 1/9, scaled by
+// 1/(nx ny nz), ky and kz re-indexed onto t's axes.
+__global__ void k_truncate(GridK g, GridK t, const cplx *c, cplx *out) {
+  const double rmp = 1.0 / ((double)t.nx * (double)t.nx), rmq = 1.0 / ((double)t.ny * (double)t.ny),
+               rms = 1.0 / ((double)t.nz * (double)t.nz);
+  const double fact = 1.0 / ((double)g.nx * (double)g.ny * (double)g.nz);
+  GRID_STRIDE(m, ncplx(t)) {
+    Idx q = decompose(m, t);
+    const int j = q.j < t.ny / 2 ? q.j : q.j - t.ny + g.ny;
+    const int k = q.k < t.nz / 2 ? q.k : q.k - t.nz + g.nz;
+    const double kx = (double)(q.i < g.nx / 2 ? q.i : q.i - g.nx), ky = (double)(j < g.ny / 2 ? j : j - g.ny),
+                 kz = (double)(k < g.nz / 2 ? k : k - g.nz);
+    const double kn2 = __dadd_rn(__dadd_rn(__dmul_rn(rmp, __dmul_rn(kx, kx)), __dmul_rn(rmq, __dmul_rn(ky, ky))),
+                                 __dmul_rn(rms, __dmul_rn(kz, kz)));
+    const cplx v = c[((size_t)q.i * g.ny + j) * g.nz + k];
+    out[m] = kn2 > 1.0 / 9.0 ? c_make(0.0, 0.0) : c_make(v.x * fact, v.y * fact);
+  }
+}
+
 __global__ void k_zero(size_t n, double *p) {
   GRID_STRIDE(m, n) p[m] = 0.0;
 }
@@ -379,6 +399,10 @@ hipError_t op_scale(const GridK &g, hipStream_t st, cplx *a, const cplx *src, do
 hipError_t op_normalize_apply(const GridK &g, hipStream_t st, cplx *x, cplx *y, cplx *z, double f0, double e) {
   LAUNCH(k_normalize, ncplx(g), st, ncplx(g), x, y, z, f0, sqrt(e));
 }
+hipError_t op_truncate(const GridK &g, const GridK &t, hipStream_t st, const cplx *c, cplx *out) {
+  LAUNCH(k_truncate, ncplx(t), st, g, t, c, out);
+}
+
 hipError_t op_sharp(const GridK &g, hipStream_t st, const cplx *a, cplx *b, double kc2) {
   LAUNCH(k_sharp, ncplx(g), st, g, a, b, kc2);
 }

@@ -144,6 +144,13 @@ class Grid:
         _lib.check(_lib.load().ghx_sharp(self._h, self._c(a), self._c(b), float(delta)))
         return b
 
+    # ---- analysis/trunc3D.fpp ---------------------------------------------
+    def truncate(self, c, small, out=None):
+        """Spectral truncation of c onto the smaller Grid `small` (trunc3D)."""
+        out = small.czeros() if out is None else out
+        _lib.check(_lib.load().ghx_truncate(self._h, self._c(c), small._h, small._c(out)))
+        return out
+
     # ---- pseudospec3D_phd.f90 ---------------------------------------------
     def advect3(self, a, b, c, d, e):
         """e = FFT[-(A . grad d)] / N^6 (pseudospec3D_phd.f90:23-115)."""

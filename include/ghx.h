@@ -138,6 +138,11 @@ int ghx_spectrumc(ghx_grid *g, const void *a, const void *b, const void *c, int 
 int ghx_phaseshift(ghx_grid *g, void *a, double cdump_re, double cdump_im);                   /* hd :2084 */
 int ghx_normalize(ghx_grid *g, void *fx, void *fy, void *fz, double f0, int kin);             /* module_dns.f90:13 */
 int ghx_sharp(ghx_grid *g, const void *a, void *b, double delta);                             /* filt :200 */
+/* trunc3D (analysis/trunc3D.fpp:190-320) spectral core: out (grid t, no
+ * larger than g, even sizes) = the modes of c (spectrum on g) with
+ * kx^2/nxt^2 + ky^2/nyt^2 + kz^2/nzt^2 <= 1/9, scaled by 1/(nx ny nz); the c2r
+ * of out on t is the truncated field sampled on t's points. */
+int ghx_truncate(ghx_grid *g, const void *c, ghx_grid *t, void *out);
 /* advect3 (pseudo/pseudospec3D_phd.f90:23-115): e = FFT[-(A . grad d)] / N^6
  * with A = (a, b, c); the products of GHOST's three c2r pairs are fused into
  * one inverse chain, the x pass and one forward chain. */

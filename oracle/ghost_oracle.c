@@ -1382,3 +1382,32 @@ void og_bouss_step(const og_grid *g, int ord, double dt, double nu, double kappa
   }
   free(C);
 }
+
+/* analysis/trunc3D.fpp:190-320 spectral core: the spectrum c of a field on
+ * grid g (r2c, unnormalised) is truncated to the modes with
+ * kx^2/nxt^2 + ky^2/nyt^2 + kz^2/nzt^2 <= 1/9 (wavenumbers of g, normalised by
+ * the new sizes), scaled by 1/(nx ny nz) and stored on the smaller grid t
+ * (kz and ky re-indexed as trunc3D's T1 fill), so that og_c2r on t gives the
+ * truncated field sampled on t's points. */
+void og_truncate(const og_grid *g, const ocplx *c, const og_grid *t, ocplx *out) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const int nxt = t->nx, nyt = t->ny, nzt = t->nz, nxht = t->nxh;
+  const double kmax = 1.0 / 9.0;
+  const double rmp = 1.0 / ((double)nxt * (double)nxt), rmq = 1.0 / ((double)nyt * (double)nyt),
+               rms = 1.0 / ((double)nzt * (double)nzt);
+  const double fact = 1.0 / ((double)nx * (double)ny * (double)nz);
+  OMP_FOR
+  for (long i = 0; i < nxht; ++i)
+    for (int jt = 0; jt < nyt; ++jt)
+      for (int kt = 0; kt < nzt; ++kt) {
+        const int j = jt < nyt / 2 ? jt : jt - nyt + ny;
+        const int k = kt < nzt / 2 ? kt : kt - nzt + nz;
+        /* trunc3D's kx, ky, kz: the integer wavenumbers of the old grid */
+        const double kx = (double)(i < nx / 2 ? i : i - nx), ky = (double)(j < ny / 2 ? j : j - ny),
+                     kz = (double)(k < nz / 2 ? k : k - nz);
+        const double kn2 = rmp * (kx * kx) + rmq * (ky * ky) + rms * (kz * kz);
+        ocplx v = c[((size_t)i * ny + j) * nz + k];
+        if (kn2 > kmax) v = 0;
+        out[((size_t)i * nyt + jt) * nzt + kt] = v * fact;
+      }
+}

@@ -137,4 +137,7 @@ void og_bouss_step(const og_grid *g, int ord, double dt, double nu, double kappa
                    ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *fx, const ocplx *fy,
                    const ocplx *fz, const ocplx *fs);
 
+/* analysis/trunc3D.fpp: spectral truncation of c (grid g) onto the smaller grid t */
+void og_truncate(const og_grid *g, const ocplx *c, const og_grid *t, ocplx *out);
+
 #endif

@@ -89,6 +89,7 @@ def _declare(L):
     L.og_mhd_step.argtypes = [_G, C.c_int, C.c_double, C.c_double, C.c_double] + [_P] * 12
     L.og_sharp.argtypes = [_G, _P, _P, C.c_double]
     L.og_advect3.argtypes = [_G, _P, _P, _P, _P, _P]
+    L.og_truncate.argtypes = [_G, _P, _G, _P]
     L.og_bouss_step.argtypes = [_G, C.c_int] + [C.c_double] * 5 + [_P] * 8
     L.og_spline_coef.argtypes = [_G, _P, _P, C.c_int]
     L.og_spline_interp.argtypes = [_G, _P, C.c_long, _P, _P, _P, C.c_int, _P]
@@ -276,6 +277,12 @@ class Grid:
     def hd_substep(self, v, c, o, dt, nu, f=None):
         fp = [None] * 3 if f is None else list(map(_p, f))
         lib().og_hd_substep(self._g, o, dt, nu, *map(_p, v), *map(_p, c), *fp)
+
+    def truncate(self, c, small):
+        """trunc3D's spectral truncation of c onto the smaller Grid `small`."""
+        out = small.czeros()
+        lib().og_truncate(self._g, _p(np.ascontiguousarray(c)), small._g, _p(out))
+        return out
 
     # -- Boussinesq -----------------------------------------------------------
     def advect3(self, a, b, c, d):

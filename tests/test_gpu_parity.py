@@ -306,3 +306,18 @@ def test_hd_pruned_equals_full_path(gpu):
             os.environ.pop("GHX_NO_PRUNE", None)
     for k in range(3):
         assert rel(outs[0][k], outs[1][k]) < 1e-14
+
+
+@pytest.mark.parametrize("big,small", [((32, 32, 32), (16, 16, 16)), ((64, 48, 32), (32, 24, 16)),
+                                       ((128, 128, 128), (48, 48, 48))])
+def test_truncate_matches_oracle(gpu, big, small):
+    """trunc3D's spectral truncation onto a smaller grid, then the c2r there."""
+    from ghost_amd.pseudospec import Grid
+    from oracle import oracle as O
+    g, og = Grid(*big), O.Grid(*big)
+    t, ot = Grid(*small), O.Grid(*small)
+    c = og.r2c(rand_real(og, 9))
+    want = og.truncate(c, ot)
+    got = g.truncate(to_gpu(c), t)
+    assert rel(to_np(got), want) < 1e-15
+    assert rel(to_np(t.c2r(got)), ot.c2r(want)) < 1e-13
