@@ -37,11 +37,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(kernel, n):
+def read_traffic(kernel, n, npart):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if any.
 
     The summary carries one line per pass: `traffic <pass> <bytes per launch>`."""
-    path = os.path.join(ROOT, "profiles", f"hd{n}_rk2_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"hd{n}_rk2_tracers_hbm_traffic.md")
+    if npart != 1_000_000:
+        return None  # the summary covers the workload with 10^6 tracers
     try:
         with open(path) as fh:
             for line in fh:
@@ -195,7 +197,7 @@ def main():
     dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
     cnt, ms, by = prof[dom]
     achieved = by / cnt / (ms / cnt * 1e-3) / 1e9
-    traffic = read_traffic(dom, n)
+    traffic = read_traffic(dom, n, npart)
 
     # algorithmic bytes of a step (18 transforms read once + written once, and the
     # 30 spectral field reads/writes of the two updates), plus per tracer stage

@@ -91,11 +91,12 @@ extern "C" int ghx_sync(void) {
 namespace {
 enum ProfSlot {
   P_ZINV, P_ZINV_CURL, P_ZINV_MHD, P_YINV, P_YFWD, P_XINV, P_XFWD, P_XCROSS, P_ZFWD, P_ZFWD_RK_HD, P_ZFWD_RK_MHD,
-  P_ZINV_SPLINE, P_TRACER_STAGE, P_ZINV_BOUSS, P_XDOT, P_ZFWD_RK_BOUSS, P_NSLOTS
+  P_ZINV_SPLINE, P_TRACER_STAGE, P_ZINV_BOUSS, P_XDOT, P_ZFWD_RK_BOUSS, P_YINV_TRACER, P_XINV_TRACER, P_NSLOTS
 };
 const char *kProfNames[P_NSLOTS] = {"zinv", "zinv_curl", "zinv_mhd", "ypass_inv", "ypass_fwd", "xinv",
                                     "xfwd", "xcross", "zfwd", "zfwd_rk_hd", "zfwd_rk_mhd", "zinv_spline",
-                                    "tracer_stage", "zinv_bouss", "xdot", "zfwd_rk_bouss"};
+                                    "tracer_stage", "zinv_bouss", "xdot", "zfwd_rk_bouss", "ypass_inv_tracer",
+                                    "xinv_tracer"};
 struct ProfRec {
   int slot;
   double bytes;
@@ -1455,8 +1456,8 @@ static int tracers_stage(ghx_tracers *t, cplx *q, const cplx *const *v, double d
   const double sc = (double)g->nc * sizeof(cplx), sr = (double)g->nr * sizeof(double);
   const PruneShares ps = (prune && g_prof.on) ? prune_shares(g) : PruneShares{1.0, 1.0, 1.0};
   PROF(P_ZINV_SPLINE, (3 * ps.fm + 3 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 3, 3, in, qq, prune, t->spl));
-  PROF(P_YINV, (3 * ps.fl + 3 * ps.fx) * sc, ctx.stream, ghx::yinv(ctx, 3, qq, pp, prune));
-  PROF(P_XINV, 3 * (ps.fx * sc + sr), ctx.stream, ghx::xinv3i(ctx, pp, r3, prune));
+  PROF(P_YINV_TRACER, (3 * ps.fl + 3 * ps.fx) * sc, ctx.stream, ghx::yinv(ctx, 3, qq, pp, prune));
+  PROF(P_XINV_TRACER, 3 * (ps.fx * sc + sr), ctx.stream, ghx::xinv3i(ctx, pp, r3, prune));
   ghx::TracerStage st;
   const int n[3] = {g->gk.nx, g->gk.ny, g->gk.nz};
   st.coef3 = r3;
