@@ -13,6 +13,10 @@ particles resident in HBM.
 
 Prints one JSON line on stdout.  Everything else goes to stderr.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--tracers 1000000] [--no-cpu-baseline]
+
+--workload mhd measures the MHD configuration instead (512x512x256 box with
+Lz = pi, nu = eta = 3e-4, dt = 2e-4, velocity and vector potential from the
+k^4 exp IC at unit energy ratio, mechanical forcing in 2 <= |k| <= 3).
 """
 import argparse
 import json
@@ -62,6 +66,120 @@ def tracer_positions(n, npart):
     return [rng.uniform(0.0, float(n), npart) for _ in range(3)]
 
 
+# MHD configuration: 512 x 512 x 256, Lx = Ly = 2 pi, Lz = pi
+MHD_SHAPE, MHD_L = (512, 512, 256), (1.0, 1.0, 0.5)
+MHD_NU, MHD_DT = 3e-4, 2e-4
+
+
+def time_steps(one_step, steps, warmup, lib, torch):
+    """warmup untimed steps, then steps timed with a sync on both sides and
+    the pass profiler's HIP events on; returns (elapsed s, profile)."""
+    for _ in range(warmup):
+        one_step()
+    torch.cuda.synchronize()
+    lib.prof_reset()
+    lib.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one_step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lib.prof_enable(False)
+    return t1 - t0, lib.prof_read()
+
+
+def pass_table(prof, elapsed):
+    passes = {}
+    for name, (cnt, ms, by) in prof.items():
+        avg_s = ms / cnt * 1e-3
+        passes[name] = {"launches": cnt, "avg_ms": round(ms / cnt, 4),
+                        "algorithmic_GB_per_launch": round(by / cnt / 1e9, 4),
+                        "achieved_GBps": round(by / cnt / avg_s / 1e9, 1),
+                        "share_of_step": round(ms * 1e-3 / elapsed, 4)}
+        log(f"  {name:12s} x{cnt:4d}  avg {ms / cnt:8.3f} ms  {by / cnt / 1e9:7.3f} GB  "
+            f"{by / cnt / avg_s / 1e9:8.1f} GB/s")
+    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    cnt, ms, by = prof[dom]
+    return passes, dom, by / cnt / (ms / cnt * 1e-3) / 1e9
+
+
+def run_mhd(args, torch):
+    """The MHD configuration; returns the JSON record."""
+    from ghost_amd import _lib
+    from ghost_amd.pseudospec import Grid, randu
+    from ghost_amd.solvers import MHDSolver
+    nx, ny, nz = MHD_SHAPE
+    g = Grid(nx, ny, nz, L=MHD_L, arbsize=True)
+    s = MHDSolver(g, dt=MHD_DT, nu=MHD_NU, mu=MHD_NU, ord=2)
+    v, _ = g.initialv_k4exp(K0, U0, IC_SEED)
+    a, _ = g.initialv_k4exp(K0, U0, IC_SEED + 1)
+    g.normalize(*a, U0, kin=0)  # <b^2> = <u^2>: unit energy ratio
+    f, fseed = g.initialfv_random(KDN, KUP, F0, F_SEED)
+    s.set_state(v, a)
+    s.set_forcing(f, None)
+    ev, eb = g.energy(*v, 1), g.energy(*a, 0)
+    del v, a, f
+    torch.cuda.empty_cache()
+    log(f"MHD grid {nx}x{ny}x{nz}, L = {MHD_L}, <u^2> = {ev:.6f}, <b^2> = {eb:.6f}")
+    state = {"timef": FSTEP, "seed": fseed}
+
+    def one_step():
+        if state["timef"] == FSTEP:
+            state["timef"] = 0
+            r, state["seed"] = randu(state["seed"])
+            phase = 2 * math.pi * r
+            s.phaseshift_forcing(complex(math.cos(phase), math.sin(phase)))
+        s.step(1)
+        state["timef"] += 1
+
+    elapsed, prof = time_steps(one_step, args.steps, args.warmup, _lib, torch)
+    steps_per_s = args.steps / elapsed
+    vv, aa = s.state()
+    e1 = g.energy(*vv, 1)
+    assert math.isfinite(e1), "energy is not finite"
+    log(f"{args.steps} steps in {elapsed:.4f} s -> {steps_per_s:.3f} steps/s; <u^2> = {e1:.6f}")
+    passes, dom, achieved = pass_table(prof, elapsed)
+    # per substep: 12 c2r (u, w, b, j) and 9 r2c (three products), the update
+    # reads N (6), C (6), v (6), f (3) and writes 6 fields
+    s_r, s_c = nx * ny * nz * 8.0, (nx // 2 + 1) * ny * nz * 16.0
+    b_step = 2 * (21 * (s_r + s_c) + 27 * s_c)
+    out = {
+        "metric": "MHD3D 512x512x256 FP64 steps/s",
+        "value": round(steps_per_s, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded random-phase k^4 exp velocity and vector potential, shell-2..3 forcing)",
+        "config": {"workload": f"MHD3D {nx}x{ny}x{nz} FP64, Lx=Ly=2pi, Lz=pi, RK2, dt={MHD_DT}, "
+                               f"nu=eta={MHD_NU}, unit energy ratio, mechanical forcing f0={F0} in 2<=|k|<=3",
+                   "grid": [nx, ny, nz], "ord": 2, "dt": MHD_DT, "nu": MHD_NU, "eta": MHD_NU},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "fft_passes": passes,
+        "step_algorithmic_GB": round(b_step / 1e9, 3),
+        "step_effective_GBps": round(b_step * steps_per_s / 1e9, 1),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        del s, vv, aa
+        torch.cuda.empty_cache()
+        O.set_threads(threads)
+        og = O.Grid(nx, ny, nz, L=MHD_L, arbsize=True)
+        v, _ = og.initialv_k4exp(K0, U0, IC_SEED)
+        a, _ = og.initialv_k4exp(K0, U0, IC_SEED + 1)
+        a = og.normalize(*a, U0, kin=0)
+        f, _ = og.initialfv_random(KDN, KUP, F0, F_SEED)
+        t0 = time.perf_counter()
+        og.mhd_step(v, a, MHD_DT, MHD_NU, MHD_NU, f=f, ord=2)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / t, "unit": "steps/s", "cores": int(threads), "kind": "port",
+                               "sample": f"one RK2 step of the MHD workload with the C/OpenMP oracle on "
+                                         f"{threads} host threads ({t:.2f} s)"}
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.5f} steps/s")
+    return out
+
+
 def cpu_baseline(n, threads, npart):
     """Time one RK substep and one tracer stage of the same workload with the
     CPU oracle (test infrastructure)."""
@@ -98,6 +216,7 @@ def main():
     ap.add_argument("--tracers", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", choices=("hd", "mhd"), default="hd")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel for the rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE scale factors) and exit")
@@ -131,6 +250,9 @@ def main():
         torch.cuda.synchronize()
         log("copied 1 GiB device to device")
         return
+    if args.workload == "mhd":
+        print(json.dumps(run_mhd(args, torch)), flush=True)
+        return
 
     g = Grid(n)
     s = HDSolver(g, dt=DT, nu=NU, ord=2)
@@ -161,21 +283,7 @@ def main():
         s.step(1)
         state["timef"] += 1
 
-    for _ in range(args.warmup):
-        one_step()
-    torch.cuda.synchronize()
-
-    _lib.prof_reset()
-    _lib.prof_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one_step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    _lib.prof_enable(False)
-    prof = _lib.prof_read()
-    elapsed = t1 - t0
+    elapsed, prof = time_steps(one_step, args.steps, args.warmup, _lib, torch)
     steps_per_s = args.steps / elapsed
     v = s.velocity()
     e1 = g.energy(*v, 1)
@@ -185,18 +293,7 @@ def main():
             assert bool(((p >= 0) & (p < n)).all()), "tracer outside the box"
     log(f"{args.steps} steps in {elapsed:.4f} s -> {steps_per_s:.3f} steps/s; <u^2> = {e1:.6f}")
 
-    passes = {}
-    for name, (cnt, ms, by) in prof.items():
-        avg_s = ms / cnt * 1e-3
-        passes[name] = {"launches": cnt, "avg_ms": round(ms / cnt, 4),
-                        "algorithmic_GB_per_launch": round(by / cnt / 1e9, 4),
-                        "achieved_GBps": round(by / cnt / avg_s / 1e9, 1),
-                        "share_of_step": round(ms * 1e-3 / elapsed, 4)}
-        log(f"  {name:12s} x{cnt:4d}  avg {ms / cnt:8.3f} ms  {by / cnt / 1e9:7.3f} GB  "
-            f"{by / cnt / avg_s / 1e9:8.1f} GB/s")
-    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
-    cnt, ms, by = prof[dom]
-    achieved = by / cnt / (ms / cnt * 1e-3) / 1e9
+    passes, dom, achieved = pass_table(prof, elapsed)
     traffic = read_traffic(dom, n, npart)
 
     # algorithmic bytes of a step (18 transforms read once + written once, and the

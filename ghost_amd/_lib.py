@@ -115,6 +115,16 @@ def _declare(L):
         "ghx_bouss_set_forcing": (_I, [_P, _I]),
         "ghx_bouss_step": (_I, [_P, _I]),
         "ghx_bouss_set_tracers": (_I, [_P, _P]),
+        "ghx_ensemble_create": (_I, [_PP, _I, C.POINTER(_I), _DP, _D, _I]),
+        "ghx_ensemble_destroy": (_I, [_P]),
+        "ghx_ensemble_fields": (_I, [_P, _I, _PP]),
+        "ghx_ensemble_grid": (_I, [_P, _I, _PP]),
+        "ghx_ensemble_touch": (_I, [_P]),
+        "ghx_ensemble_step": (_I, [_P, _I]),
+        "ghx_ensemble_set_graph": (_I, [_P, _I]),
+        "ghx_ensemble_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I)]),
+        "ghx_ensemble_groups": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+        "ghx_ensemble_sync": (_I, [_P]),
         "ghx_tracers_create": (_I, [_PP, _P, C.c_longlong, _I]),
         "ghx_tracers_destroy": (_I, [_P]),
         "ghx_tracers_positions": (_I, [_P, _PP, _PP, _PP]),

@@ -1687,3 +1687,299 @@ extern "C" int ghx_bouss_step(ghx_bouss *s, int nsteps) {
   }
   return GHX_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Ensembles: many independent HD runs advanced together.  Members of equal
+// grid size form a group whose fields are laid out member after member, so
+// each pass of the fused substep is one launch for the whole group (the
+// member index is a grid dimension of the launch).  After the first step the
+// whole ensemble step is captured in a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+struct EnsGroup {
+  int n = 0;
+  ghx_grid *g = nullptr;
+  std::vector<int> members;
+  cplx *mem = nullptr;    // v [member][3][nc], v1 [member][3][nc], scratch [member][12][slot]
+  double *nu = nullptr;   // per-member viscosity (device)
+  size_t nc = 0, slot = 0;
+  int tz = 0, ty = 0;     // tile widths of the z and y passes (0: built-in)
+  bool clean = false;
+  cplx *v() const { return mem; }
+  cplx *v1() const { return mem + members.size() * 3 * nc; }
+  cplx *q() const { return mem + members.size() * 6 * nc; }
+};
+
+struct ghx_ensemble {
+  std::vector<EnsGroup> groups;
+  std::vector<std::pair<int, int>> where;  // member -> (group, index in group)
+  int ord = 2;
+  double dt = 0.0;
+  hipStream_t stream = nullptr;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void ens_drop_graph(ghx_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+}
+
+extern "C" int ghx_ensemble_destroy(ghx_ensemble *e) {
+  if (!e) return GHX_OK;
+  ens_drop_graph(e);
+  for (EnsGroup &G : e->groups) {
+    if (G.mem) (void)hipFree(G.mem);
+    if (G.nu) (void)hipFree(G.nu);
+    ghx_grid_destroy(G.g);
+  }
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return GHX_OK;
+}
+
+extern "C" int ghx_ensemble_create(ghx_ensemble **out, int nmembers, const int *n, const double *nu, double dt,
+                                   int ord) {
+  ARGCHK(out && nmembers > 0 && n && nu && ord >= 1, "bad ensemble_create arguments");
+  ghx_ensemble *e = new (std::nothrow) ghx_ensemble();
+  if (!e) return set_err(GHX_ERR_NOMEM, "host allocation failed");
+  e->ord = ord;
+  e->dt = dt;
+  e->where.resize(nmembers);
+  int rc = GHX_OK;
+  hipError_t he = hipStreamCreate(&e->stream);  // default priority
+  if (he != hipSuccess) { delete e; HIPCHK(he); }
+  for (int m = 0; m < nmembers && !rc; ++m) {
+    int gi = -1;
+    for (size_t k = 0; k < e->groups.size(); ++k)
+      if (e->groups[k].n == n[m]) gi = (int)k;
+    if (gi < 0) {
+      EnsGroup G;
+      G.n = n[m];
+      rc = ghx_grid_create(&G.g, n[m], n[m], n[m], 1.0, 1.0, 1.0, 0, 0.0);
+      if (rc) break;
+      ghx_grid_set_stream(G.g, e->stream);
+      e->groups.push_back(G);
+      gi = (int)e->groups.size() - 1;
+    }
+    e->where[m] = {gi, (int)e->groups[gi].members.size()};
+    e->groups[gi].members.push_back(m);
+  }
+  for (EnsGroup &G : e->groups) {
+    if (rc) break;
+    const size_t cnt = G.members.size();
+    G.nc = G.g->nc;
+    G.slot = ghx::slot_points(G.g->gk);
+    (void)prune_shares(G.g);
+    he = hipMalloc(&G.mem, (cnt * 6 * G.nc + cnt * 12 * G.slot) * sizeof(cplx));
+    if (he == hipSuccess) he = hipMalloc(&G.nu, cnt * sizeof(double));
+    std::vector<double> h(cnt);
+    for (size_t b = 0; b < cnt; ++b) h[b] = nu[G.members[b]];
+    if (he == hipSuccess) he = hipMemcpy(G.nu, h.data(), cnt * sizeof(double), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = ghx::op_zero(e->stream, G.mem, cnt * 6 * G.nc * sizeof(cplx));
+    if (he != hipSuccess) rc = set_err(he == hipErrorOutOfMemory ? GHX_ERR_NOMEM : GHX_ERR_HIP, "ensemble: %s",
+                                       hipGetErrorString(he));
+  }
+  if (!rc && (he = hipStreamSynchronize(e->stream)) != hipSuccess)
+    rc = set_err(GHX_ERR_HIP, "ensemble: %s", hipGetErrorString(he));
+  if (rc) { ghx_ensemble_destroy(e); return rc; }
+  *out = e;
+  return GHX_OK;
+}
+
+extern "C" int ghx_ensemble_fields(ghx_ensemble *e, int member, void **v) {
+  ARGCHK(e && v && member >= 0 && member < (int)e->where.size(), "bad ensemble_fields arguments");
+  EnsGroup &G = e->groups[e->where[member].first];
+  const size_t b = e->where[member].second;
+  for (int k = 0; k < 3; ++k) v[k] = G.v() + (b * 3 + k) * G.nc;
+  G.clean = false;
+  return GHX_OK;
+}
+
+extern "C" int ghx_ensemble_grid(ghx_ensemble *e, int member, ghx_grid **g) {
+  ARGCHK(e && g && member >= 0 && member < (int)e->where.size(), "bad ensemble_grid arguments");
+  *g = e->groups[e->where[member].first].g;
+  return GHX_OK;
+}
+
+extern "C" int ghx_ensemble_touch(ghx_ensemble *e) {
+  ARGCHK(e, "null ensemble");
+  for (EnsGroup &G : e->groups) G.clean = false;
+  return GHX_OK;
+}
+
+extern "C" int ghx_ensemble_set_graph(ghx_ensemble *e, int enabled) {
+  ARGCHK(e, "null ensemble");
+  e->use_graph = enabled != 0;
+  ens_drop_graph(e);
+  return GHX_OK;
+}
+
+// One substep o of every member of G (include/hd_rkstep2.f90, unforced), one
+// launch per pass for the whole group.
+static int ens_substep(ghx_ensemble *e, EnsGroup &G, int o, const cplx *vin, cplx *out, int prune) {
+  FFTCtx ctx = G.g->plan.ctx;
+  ctx.stream = e->stream;
+  ctx.batch = (int)G.members.size();
+  ctx.tile_z = G.tz;
+  ctx.tile_y = G.ty;
+  const size_t nc = G.nc, slot = G.slot, sstride = 12 * slot;
+  Chain ch = chain_slots(G.q(), slot);
+  ch.q6.bstride = ch.p6.bstride = ch.p3.bstride = ch.q3.bstride = sstride;
+  FieldSet in, pq;
+  for (int k = 0; k < 3; ++k) in.p[k] = const_cast<cplx *>(vin) + k * nc;
+  in.bstride = 3 * nc;
+  for (int k = 0; k < 3; ++k) { pq.p[k] = ch.p6.p[3 + k]; pq.p[3 + k] = ch.p6.p[k]; }
+  pq.bstride = sstride;
+  const double sc = (double)nc * sizeof(cplx) * ctx.batch;
+  const PruneShares ps = (prune && g_prof.on) ? prune_shares(G.g) : PruneShares{1.0, 1.0, 1.0};
+  PROF(P_ZINV_CURL, (3 * ps.fm + 6 * ps.fl) * sc, ctx.stream, ghx::zinv(ctx, 1, 3, in, ch.q6, prune));
+  PROF(P_YINV, (6 * ps.fl + 6 * ps.fx) * sc, ctx.stream, ghx::yinv(ctx, 6, ch.q6, ch.p6, prune));
+  PROF(P_XCROSS, 9 * ps.fx * sc, ctx.stream, ghx::xcross(ctx, 0, pq, ch.p3, prune));
+  PROF(P_YFWD, (3 * ps.fx + 3 * ps.fl) * sc, ctx.stream, ghx::yfwd(ctx, 3, ch.p3, ch.q3, prune));
+  ghx::RKParams rk;
+  rk.dt = e->dt; rk.nu = 0.0; rk.rmp = 1. / (double)o;
+  for (int k = 0; k < 3; ++k) {
+    rk.v[k] = vin + k * nc;
+    rk.c[k] = G.v() + k * nc;
+    rk.out[k] = out + k * nc;
+    rk.f[k] = nullptr;
+  }
+  rk.fline = nullptr;
+  rk.bstride = 3 * nc;
+  rk.nu_b = G.nu;
+  const int nvc = vin == G.v() ? 3 : 6;
+  PROF(P_ZFWD_RK_HD, (3 * ps.fl + nvc * ps.fm + 3 * (prune ? ps.fm : 1.0)) * sc, ctx.stream,
+       ghx::zfwd_rk_hd(ctx, ch.q3, rk, prune));
+  return GHX_OK;
+}
+
+static int ens_step_group(ghx_ensemble *e, EnsGroup &G, int prune) {
+  for (int o = e->ord; o >= 1; --o) {
+    const bool first = (o == e->ord), last = (o == 1);
+    int rc = ens_substep(e, G, o, first ? G.v() : G.v1(), last ? G.v() : G.v1(), prune);
+    if (rc) return rc;
+  }
+  return GHX_OK;
+}
+
+// The members' states are dealiased once checked; every step keeps them so.
+static int ens_check_clean(ghx_ensemble *e, EnsGroup &G) {
+  bool clean = true;
+  for (size_t b = 0; b < G.members.size() && clean; ++b) {
+    const cplx *v = G.v() + b * 3 * G.nc;
+    double n = 0;
+    int rc = count_unkept(G.g, v, v + G.nc, v + 2 * G.nc, &n);
+    if (rc) return rc;
+    clean = (n == 0.0);
+  }
+  G.clean = clean;
+  return GHX_OK;
+}
+
+extern "C" int ghx_ensemble_step(ghx_ensemble *e, int nsteps) {
+  ARGCHK(e && nsteps >= 0, "bad ensemble_step arguments");
+  const bool allow_prune = !getenv("GHX_NO_PRUNE");
+  for (int s = 0; s < nsteps; ++s) {
+    bool all_clean = true;
+    for (EnsGroup &G : e->groups) {
+      if (!G.clean) {
+        ens_drop_graph(e);
+        int rc = ens_check_clean(e, G);
+        if (rc) return rc;
+      }
+      all_clean = all_clean && G.clean;
+    }
+    if (e->use_graph && all_clean && allow_prune) {
+      if (!e->exec) {
+        const bool prof_was = g_prof.on;
+        g_prof.on = false;  // no profiler events inside the graph
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = GHX_OK;
+        for (EnsGroup &G : e->groups)
+          if (!rc) rc = ens_step_group(e, G, 1);
+        hipGraph_t graph = nullptr;
+        hipError_t he = hipStreamEndCapture(e->stream, &graph);
+        g_prof.on = prof_was;
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        HIPCHK(he);
+        e->graph = graph;
+        HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(e->exec, e->stream));
+    } else {
+      for (EnsGroup &G : e->groups) {
+        int rc = ens_step_group(e, G, G.clean && allow_prune ? 1 : 0);
+        if (rc) return rc;
+      }
+    }
+    for (EnsGroup &G : e->groups) G.clean = true;
+  }
+  return GHX_OK;
+}
+
+extern "C" int ghx_ensemble_sync(ghx_ensemble *e) {
+  ARGCHK(e, "null ensemble");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return GHX_OK;
+}
+
+// Tile widths per grid size chosen by timing: for every group, each candidate
+// (z tile, y tile) runs `steps` steps of that group alone, from a saved copy
+// of its state that is put back afterwards; the fastest is kept.
+extern "C" int ghx_ensemble_autotune(ghx_ensemble *e, int steps, int *tz_out, int *ty_out) {
+  ARGCHK(e && steps > 0, "bad ensemble_autotune arguments");
+  static const int cand[][2] = {{4, 8}, {2, 4}, {4, 4}, {8, 8}, {8, 4}, {2, 8}, {4, 16}};
+  ens_drop_graph(e);
+  hipEvent_t a = nullptr, b = nullptr;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  int rc = GHX_OK;
+  for (size_t gi = 0; gi < e->groups.size() && !rc; ++gi) {
+    EnsGroup &G = e->groups[gi];
+    if (!G.clean && (rc = ens_check_clean(e, G))) break;
+    const size_t bytes = G.members.size() * 3 * G.nc * sizeof(cplx);
+    cplx *save = nullptr;
+    hipError_t he = hipMalloc(&save, bytes);
+    if (he == hipSuccess) he = hipMemcpyAsync(save, G.v(), bytes, hipMemcpyDeviceToDevice, e->stream);
+    if (he != hipSuccess) { rc = set_err(GHX_ERR_HIP, "autotune: %s", hipGetErrorString(he)); break; }
+    double best = 1e30;
+    int bz = 0, by = 0;
+    for (const auto &c : cand) {
+      G.tz = c[0];
+      G.ty = c[1];
+      (void)ens_step_group(e, G, G.clean ? 1 : 0);  // warm up this shape
+      (void)hipEventRecord(a, e->stream);
+      for (int s = 0; s < steps && !rc; ++s) rc = ens_step_group(e, G, G.clean ? 1 : 0);
+      (void)hipEventRecord(b, e->stream);
+      float ms = 0.f;
+      if ((he = hipEventSynchronize(b)) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        rc = set_err(GHX_ERR_HIP, "autotune: %s", hipGetErrorString(he));
+        break;
+      }
+      if (ms < best) { best = ms; bz = c[0]; by = c[1]; }
+      (void)hipMemcpyAsync(G.v(), save, bytes, hipMemcpyDeviceToDevice, e->stream);
+    }
+    G.tz = bz;
+    G.ty = by;
+    if (tz_out) tz_out[gi] = bz;
+    if (ty_out) ty_out[gi] = by;
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipFree(save);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+extern "C" int ghx_ensemble_groups(ghx_ensemble *e, int *ngroups, int *sizes, int *counts) {
+  ARGCHK(e && ngroups, "bad ensemble_groups arguments");
+  *ngroups = (int)e->groups.size();
+  for (size_t k = 0; k < e->groups.size(); ++k) {
+    if (sizes) sizes[k] = e->groups[k].n;
+    if (counts) counts[k] = (int)e->groups[k].members.size();
+  }
+  return GHX_OK;
+}

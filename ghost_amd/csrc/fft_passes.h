@@ -19,11 +19,19 @@ struct FFTCtx {
   const cplx *tw_y;   // ny table
   const cplx *tw_z;   // nz table
   hipStream_t stream;
+  // Batched launches: the fused-substep passes (zinv, yinv, xcross, yfwd,
+  // zfwd_rk_hd) run `batch` independent fields of this grid in one launch;
+  // member b's pointers are the FieldSet's plus b * bstride.  Other passes
+  // take batch == 1.
+  int batch = 1;
+  // tile widths (lines per block) of the z and y passes; 0 = built-in choice
+  int tile_z = 0, tile_y = 0;
 };
 
 constexpr int kMaxFields = 12;
 struct FieldSet {
   cplx *p[kMaxFields];
+  size_t bstride = 0;  // elements between batch members
 };
 
 // Fused-epilogue parameters for the Z forward pass of an RK substep.
@@ -34,6 +42,8 @@ struct RKParams {
   cplx *out[3];        // updated state (may alias c or v element-wise)
   const cplx *f[3];    // forcing (may be null)
   const uint8_t *fline;  // per (kx,ky) line: 1 if the forcing is non-zero there
+  size_t bstride = 0;       // elements between batch members (v, c, out, f)
+  const double *nu_b = nullptr;  // per-member viscosity of a batched launch (else nu)
 };
 
 struct MHDRKParams {

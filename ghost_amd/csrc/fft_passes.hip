@@ -71,6 +71,13 @@ static int tile_want(const char *env, int dflt) {
   return v > 0 ? v : dflt;
 }
 
+// tile width: the environment override (tuning sweeps), else the context's
+// choice, else the built-in default
+static int tile_pick(const char *env, int ctx, int dflt) {
+  const int e = tile_want(env, 0);
+  return e > 0 ? e : (ctx > 0 ? ctx : dflt);
+}
+
 template <typename K> static void allow_lds(K kern, size_t bytes) {
   if (bytes > 49152) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
@@ -83,6 +90,14 @@ template <typename K> static void allow_lds(K kern, size_t bytes) {
 // (b and b+8) are given neighbouring tiles.  Tiles stay spread over the XCDs
 // in their natural order, which keeps pruned passes load-balanced.
 // ---------------------------------------------------------------------------
+// Member b of a batched launch: every field pointer moves by b * bstride.
+GHX_DEV void batch_shift(FieldSet &fs, unsigned b) {
+  if (b == 0 || fs.bstride == 0) return;
+#pragma unroll
+  for (int f = 0; f < kMaxFields; ++f)
+    if (fs.p[f]) fs.p[f] += (size_t)b * fs.bstride;
+}
+
 GHX_DEV void xcd_tile(int &bx, int &by) {
   const int nb = gridDim.x * gridDim.y;
   int b = blockIdx.y * gridDim.x + blockIdx.x;
@@ -164,6 +179,8 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
+  batch_shift(in, blockIdx.z);
+  batch_shift(out, blockIdx.z);
   int bx, by;
   xcd_tile(bx, by);
   const int i = by, j0 = bx * L, j = j0 + lt;
@@ -345,6 +362,18 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const 
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
+  batch_shift(in, blockIdx.z);
+  if (blockIdx.z && rk.bstride) {
+    const size_t o = (size_t)blockIdx.z * rk.bstride;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rk.v[c] += o;
+      rk.c[c] += o;
+      rk.out[c] += o;
+      if (rk.f[c]) rk.f[c] += o;
+    }
+  }
+  if (rk.nu_b) rk.nu = rk.nu_b[blockIdx.z];
   int bx, by;
   xcd_tile(bx, by);
   const int i = by, j0 = bx * L, j = j0 + lt;
@@ -594,6 +623,8 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in, FieldSet out, int nf,
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
+  batch_shift(in, blockIdx.z);
+  batch_shift(out, blockIdx.z);
   int bx, by;
   xcd_tile(bx, by);
   const int z = by, kx0 = bx * L, kx = kx0 + lt;
@@ -643,6 +674,8 @@ __global__ __launch_bounds__(512) void k_yfwd(FieldSet in, FieldSet out, int nf,
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
+  batch_shift(in, blockIdx.z);
+  batch_shift(out, blockIdx.z);
   int bx, by;
   xcd_tile(bx, by);
   const int z = by, kx0 = bx * L, kx = kx0 + lt;
@@ -939,6 +972,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
   const size_t off = (size_t)(ok ? r : 0) * nxp;
   cplx *buf = lds + lt * lds_line_stride(N);
   const double tmp = g.inv_n6;
+  batch_shift(in, blockIdx.y);
+  batch_shift(out, blockIdx.y);
   XTw<N, Cfg> tw;
   tw.load(twh, twx, t);
   cplx qx[E], qy[E], qz[E], ay[E], az[E];
@@ -1065,9 +1100,9 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", mode == 0 ? 8 : 4));                       \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_pick("GHX_LZ", c.tile_z, mode == 0 ? 8 : 4));              \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
-    dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
+    dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);                                          \
     if (mode == 0) {                                                                        \
       allow_lds(k_zinv<n_, 0>, lds);                                                        \
       hipLaunchKernelGGL((k_zinv<n_, 0>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
@@ -1130,9 +1165,9 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, i
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4));                                        \
+    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_pick("GHX_LZ", c.tile_z, 4));         \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
-    dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
+    dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);                                          \
     allow_lds(k_zfwd_rk_hd<n_>, lds);                                                       \
     hipLaunchKernelGGL((k_zfwd_rk_hd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
     break;                                                                                  \
@@ -1180,9 +1215,9 @@ hipError_t yinv(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.nxh, tile_want("GHX_LY", 8));                  \
+    int L = pick_L(T, lds_line_stride(n_), g.nxh, tile_pick("GHX_LY", c.tile_y, 8));        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
-    dim3 grid((g.nxh + L - 1) / L, g.nz);                                                   \
+    dim3 grid((g.nxh + L - 1) / L, g.nz, c.batch);                                          \
     allow_lds(k_yinv<n_>, lds);                                                             \
     hipLaunchKernelGGL((k_yinv<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune); \
     break;                                                                                  \
@@ -1203,9 +1238,9 @@ hipError_t yfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.nxh, tile_want("GHX_LY", 4));                  \
+    int L = pick_L(T, lds_line_stride(n_), g.nxh, tile_pick("GHX_LY", c.tile_y, 4));        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
-    dim3 grid((g.nxh + L - 1) / L, g.nz);                                                   \
+    dim3 grid((g.nxh + L - 1) / L, g.nz, c.batch);                                          \
     allow_lds(k_yfwd<n_>, lds);                                                             \
     hipLaunchKernelGGL((k_yfwd<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune); \
     break;                                                                                  \
@@ -1308,10 +1343,8 @@ hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet 
     constexpr int T = FFTCfgLow<n_>::T;                                                     \
     const int rpb = rows_per_block(T);                                                      \
     size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
-    const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                         \
+    const dim3 gr((unsigned)((nrows + rpb - 1) / rpb), c.batch), bl(rpb * T);               \
     switch (tile_want("GHX_XV", 1)) {                                                       \
-    case 2: hipLaunchKernelGGL((k_xcross<n_, 3, 0>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
-    case 3: hipLaunchKernelGGL((k_xcross<n_, 3, 1>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
     case 4: hipLaunchKernelGGL((k_xcross<n_, 2, 1>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
     default: hipLaunchKernelGGL((k_xcross<n_, 2, 0>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
     }                                                                                       \

@@ -120,6 +120,12 @@ class MHDSolver(_Resident):
     def state(self):
         return [self._copy_out(p) for p in self._v], [self._copy_out(p) for p in self._a]
 
+    def phaseshift_forcing(self, cdump):
+        """The rand=1 update of the mechanical forcing (main3D.fpp:2544-2556)."""
+        L = _lib.load()
+        for p in self._f:
+            _lib.check(L.ghx_phaseshift(self.grid._h, C.c_void_p(p), float(cdump.real), float(cdump.imag)))
+
     def step(self, n=1):
         _lib.check(_lib.load().ghx_mhd_step(self._h, int(n)))
 
@@ -184,3 +190,71 @@ class BoussSolver(_Resident):
         """As HDSolver.set_tracers."""
         _lib.check(_lib.load().ghx_bouss_set_tracers(self._h, parts._h if parts is not None else None))
         self._parts = parts
+
+
+class Ensemble:
+    """Independent unforced HD runs advanced together: members of one grid
+    size share each pass launch, and after the first step the ensemble step
+    is a replayed HIP graph."""
+
+    def __init__(self, sizes, nus, dt, ord=2):
+        n = len(sizes)
+        self.sizes = [int(x) for x in sizes]
+        arr_n = (C.c_int * n)(*self.sizes)
+        arr_nu = (C.c_double * n)(*[float(x) for x in nus])
+        h = C.c_void_p()
+        _lib.check(_lib.load().ghx_ensemble_create(C.byref(h), n, arr_n, arr_nu, float(dt), int(ord)))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _lib.load().ghx_ensemble_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _ptrs(self, m):
+        v = (C.c_void_p * 3)()
+        _lib.check(_lib.load().ghx_ensemble_fields(self._h, m, C.cast(v, C.POINTER(C.c_void_p))))
+        return [v[k] for k in range(3)]
+
+    def set_velocity(self, m, v):
+        n = self.sizes[m]
+        torch.cuda.synchronize()
+        for p, t in zip(self._ptrs(m), v):
+            t = t.contiguous()
+            assert t.numel() == (n // 2 + 1) * n * n
+            _lib.check(_lib.load().ghx_memcpy_d2d(C.c_void_p(p), C.c_void_p(t.data_ptr()), t.numel() * 16))
+        _lib.check(_lib.load().ghx_ensemble_touch(self._h))
+
+    def velocity(self, m):
+        n = self.sizes[m]
+        out = []
+        _lib.check(_lib.load().ghx_ensemble_sync(self._h))
+        for p in self._ptrs(m):
+            t = torch.empty((n // 2 + 1, n, n), dtype=torch.complex128, device="cuda")
+            _lib.check(_lib.load().ghx_memcpy_d2d(C.c_void_p(t.data_ptr()), C.c_void_p(p), t.numel() * 16))
+            out.append(t)
+        return out
+
+    def step(self, n=1):
+        _lib.check(_lib.load().ghx_ensemble_step(self._h, int(n)))
+
+    def sync(self):
+        _lib.check(_lib.load().ghx_ensemble_sync(self._h))
+
+    def set_graph(self, enabled):
+        _lib.check(_lib.load().ghx_ensemble_set_graph(self._h, int(bool(enabled))))
+
+    def groups(self):
+        ng = C.c_int()
+        sizes, counts = (C.c_int * 64)(), (C.c_int * 64)()
+        _lib.check(_lib.load().ghx_ensemble_groups(self._h, C.byref(ng), sizes, counts))
+        return [(sizes[k], counts[k]) for k in range(ng.value)]
+
+    def autotune(self, steps=3):
+        ng = len(self.groups())
+        tz, ty = (C.c_int * ng)(), (C.c_int * ng)()
+        _lib.check(_lib.load().ghx_ensemble_autotune(self._h, int(steps), tz, ty))
+        return [(tz[k], ty[k]) for k in range(ng)]

@@ -214,6 +214,34 @@ int ghx_bouss_set_forcing(ghx_bouss *s, int enabled);
 int ghx_bouss_step(ghx_bouss *s, int nsteps);
 
 /* ---------------------------------------------------------------------- */
+/* Ensembles: many independent unforced HD runs (cubic 2 pi boxes of any     */
+/* supported sizes, each with its own viscosity) advanced together.  Members */
+/* of one size share every pass launch of the fused substep (include/        */
+/* hd_rkstep2.f90 in the loop of main3D.fpp:3202-3231, once per member);     */
+/* after the first step the ensemble step is a captured HIP graph.  No        */
+/* counterpart in GHOST, which runs one solver per process.                  */
+/* ---------------------------------------------------------------------- */
+typedef struct ghx_ensemble ghx_ensemble;
+/* n[m]: grid size of member m (n^3), nu[m]: its viscosity; RK order ord, step dt */
+int ghx_ensemble_create(ghx_ensemble **e, int nmembers, const int *n, const double *nu, double dt, int ord);
+int ghx_ensemble_destroy(ghx_ensemble *e);
+/* device pointers of member m's velocity v[3] (spectral, on its grid) */
+int ghx_ensemble_fields(ghx_ensemble *e, int member, void **v);
+/* the grid of member m (for initial conditions and diagnostics) */
+int ghx_ensemble_grid(ghx_ensemble *e, int member, ghx_grid **g);
+/* call after writing member states through ghx_ensemble_fields */
+int ghx_ensemble_touch(ghx_ensemble *e);
+int ghx_ensemble_step(ghx_ensemble *e, int nsteps);
+/* 0: launch every pass eagerly each step; 1 (default): replay a captured graph */
+int ghx_ensemble_set_graph(ghx_ensemble *e, int enabled);
+/* choose the z and y tile widths of each size group by timing `steps` steps
+ * per candidate (member states are restored); tz/ty receive one value per group */
+int ghx_ensemble_autotune(ghx_ensemble *e, int steps, int *tz, int *ty);
+/* size groups: ngroups, and per group its size and member count */
+int ghx_ensemble_groups(ghx_ensemble *e, int *ngroups, int *sizes, int *counts);
+int ghx_ensemble_sync(ghx_ensemble *e);
+
+/* ---------------------------------------------------------------------- */
 /* Lagrangian tracers: GHOST's GPart with cubic-spline interpolation on one */
 /* rank (lagpart/gparts_mod.f90, lagpart/gpsplineint_mod.fpp).  Positions   */
 /* are in grid units, [0, n) along each axis, as GPart's px_/py_/pz_.        */

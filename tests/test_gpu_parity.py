@@ -321,3 +321,28 @@ def test_truncate_matches_oracle(gpu, big, small):
     got = g.truncate(to_gpu(c), t)
     assert rel(to_np(got), want) < 1e-15
     assert rel(to_np(t.c2r(got)), ot.c2r(want)) < 1e-13
+
+
+def test_mhd_512x512x256_step_matches_oracle(gpu):
+    """The MHD configuration at its full size (512x512x256, Lz = pi): one
+    forced RK2 step from k^4 exp velocity and vector potential."""
+    from ghost_amd.solvers import MHDSolver
+    from oracle import oracle as O
+    O.set_threads(16)
+    L = (1.0, 1.0, 0.5)
+    g, og = grids((512, 512, 256), True, L)
+    v0, _ = og.initialv_k4exp(3.0, 1.0, 1000)
+    a0, _ = og.initialv_k4exp(3.0, 1.0, 1001)
+    a0 = list(og.normalize(*a0, 1.0, kin=0))
+    f0, _ = og.initialfv_random(2.0, 3.0, 0.4, 2000)
+    s = MHDSolver(g, dt=2e-4, nu=3e-4, mu=3e-4, ord=2)
+    s.set_state([to_gpu(x) for x in v0], [to_gpu(x) for x in a0])
+    s.set_forcing([to_gpu(x) for x in f0], None)
+    s.step(1)
+    gv, ga = s.state()
+    gv, ga = [to_np(x) for x in gv], [to_np(x) for x in ga]
+    del s
+    og.mhd_step(v0, a0, 2e-4, 3e-4, 3e-4, f=f0, ord=2)
+    for k in range(3):
+        assert rel(gv[k], v0[k]) < 1e-12
+        assert rel(ga[k], a0[k]) < 1e-12
