@@ -17,6 +17,9 @@ Prints one JSON line on stdout.  Everything else goes to stderr.
 --workload mhd measures the MHD configuration instead (512x512x256 box with
 Lz = pi, nu = eta = 3e-4, dt = 2e-4, velocity and vector potential from the
 k^4 exp IC at unit energy ratio, mechanical forcing in 2 <= |k| <= 3).
+--workload ensemble measures the batched ensemble: 96 unforced HD runs, 24
+each at 32^3, 48^3, 64^3 and 96^3, k^4 exp(-2 (k/k0)^2) ICs with k0 drawn
+uniformly in [2, 4], nu set so that k_max eta = 1.5 at the start, RK2.
 """
 import argparse
 import json
@@ -180,6 +183,94 @@ def run_mhd(args, torch):
     return out
 
 
+ENS_SIZES, ENS_PER_SIZE, ENS_DT = (32, 48, 64, 96), 24, 2e-3
+
+
+def ensemble_members(grids):
+    """(size, k0, nu, v) of the 96 members, ICs generated on the GPU.  nu gives
+    k_max eta = 1.5 with k_max = n/3, eta = (nu^3/eps)^(1/4) and eps = nu <|w|^2>
+    of the IC (<u^2> = 1): nu = (1.5/k_max)^2 <|w|^2>^(1/2)."""
+    import numpy as np
+    rng = np.random.default_rng(4000)
+    out, m = [], 0
+    for n in ENS_SIZES:
+        g = grids[n]
+        for _ in range(ENS_PER_SIZE):
+            k0 = float(rng.uniform(2.0, 4.0))
+            v, _ = g.initialv_k4exp(k0, 1.0, 5000 + m)
+            omega2 = g.energy(*v, 0)
+            nu = (1.5 / (n / 3.0)) ** 2 * math.sqrt(omega2)
+            out.append((n, k0, nu, v))
+            m += 1
+    return out
+
+
+def run_ensemble(args, torch):
+    from ghost_amd import _lib
+    from ghost_amd.pseudospec import Grid
+    from ghost_amd.solvers import Ensemble
+    grids = {n: Grid(n) for n in ENS_SIZES}
+    mem = ensemble_members(grids)
+    e = Ensemble([m[0] for m in mem], [m[2] for m in mem], dt=ENS_DT, ord=2)
+    for i, (n, k0, nu, v) in enumerate(mem):
+        e.set_velocity(i, v)
+    mem = [m[:3] for m in mem]  # drop the IC copies
+    torch.cuda.empty_cache()
+    tiles = e.autotune(steps=3)
+    log(f"ensemble of {len(mem)} members, groups {e.groups()}, tiles (z, y) {tiles}")
+
+    def one_step():
+        e.step(1)
+
+    elapsed, prof = time_steps(one_step, args.steps, args.warmup, _lib, torch)
+    e.sync()
+    steps_per_s = args.steps / elapsed
+    for i in (0, len(mem) - 1):
+        v = e.velocity(i)
+        en = grids[mem[i][0]].energy(*v, 1)
+        assert math.isfinite(en), "energy is not finite"
+    log(f"{args.steps} ensemble steps in {elapsed:.4f} s -> {steps_per_s:.3f} steps/s "
+        f"({steps_per_s * len(mem):.1f} member-steps/s)")
+    b_step = 0.0
+    for n in ENS_SIZES:
+        s_r, s_c = n ** 3 * 8.0, (n // 2 + 1) * n * n * 16.0
+        b_step += ENS_PER_SIZE * (18 * (s_r + s_c) + 30 * s_c)
+    out = {
+        "metric": "HD3D ensemble (96 runs, 24 each of 32^3/48^3/64^3/96^3) FP64 RK2 ensemble steps/s",
+        "value": round(steps_per_s, 4), "unit": "ensemble steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded k^4 exp ICs, k0 uniform in [2, 4])",
+        "config": {"workload": "96 unforced HD3D runs, 24 each at 32^3, 48^3, 64^3, 96^3, RK2, "
+                               f"dt={ENS_DT}, nu from k_max eta = 1.5, one launch per pass per size, "
+                               "captured HIP graph, autotuned tile widths",
+                   "members": len(mem), "sizes": list(ENS_SIZES), "tiles": tiles},
+        "member_steps_per_s": round(steps_per_s * len(mem), 1),
+        "step_algorithmic_GB": round(b_step / 1e9, 3),
+        "step_effective_GBps": round(b_step * steps_per_s / 1e9, 1),
+        "roofline": {"bound": "hbm", "kernel": "ensemble step", "achieved": round(b_step * steps_per_s / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(b_step * steps_per_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        t_total = 0.0
+        for n in ENS_SIZES:
+            og = O.Grid(n)
+            v, _ = og.initialv_k4exp(3.0, 1.0, 1)
+            t0 = time.perf_counter()
+            og.hd_step(v, ENS_DT, 1e-2, ord=2)
+            t_total += ENS_PER_SIZE * (time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": 1.0 / t_total, "unit": "ensemble steps/s", "cores": int(threads),
+                               "kind": "port",
+                               "sample": "one RK2 step of one member per size with the C/OpenMP oracle, times "
+                                         f"24 members per size (96 sequential runs), {threads} host threads"}
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.4f} ensemble steps/s")
+    return out
+
+
 def cpu_baseline(n, threads, npart):
     """Time one RK substep and one tracer stage of the same workload with the
     CPU oracle (test infrastructure)."""
@@ -216,7 +307,7 @@ def main():
     ap.add_argument("--tracers", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=("hd", "mhd"), default="hd")
+    ap.add_argument("--workload", choices=("hd", "mhd", "ensemble"), default="hd")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel for the rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE scale factors) and exit")
@@ -252,6 +343,9 @@ def main():
         return
     if args.workload == "mhd":
         print(json.dumps(run_mhd(args, torch)), flush=True)
+        return
+    if args.workload == "ensemble":
+        print(json.dumps(run_ensemble(args, torch)), flush=True)
         return
 
     g = Grid(n)

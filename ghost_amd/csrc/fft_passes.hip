@@ -91,12 +91,18 @@ template <typename K> static void allow_lds(K kern, size_t bytes) {
 // in their natural order, which keeps pruned passes load-balanced.
 // ---------------------------------------------------------------------------
 // Member b of a batched launch: every field pointer moves by b * bstride.
-GHX_DEV void batch_shift(FieldSet &fs, unsigned b) {
-  if (b == 0 || fs.bstride == 0) return;
-#pragma unroll
-  for (int f = 0; f < kMaxFields; ++f)
-    if (fs.p[f]) fs.p[f] += (size_t)b * fs.bstride;
-}
+// The view reads the pointers from the kernel arguments on each use and adds
+// the offset; a shifted copy of the argument that is indexed by a run-time
+// field number would be placed in scratch memory.
+struct FieldRef {
+  const FieldSet &fs;
+  size_t off;
+  GHX_DEV cplx *operator[](int f) const { return fs.p[f] + off; }
+};
+struct FieldView {
+  FieldRef p;
+};
+GHX_DEV FieldView batch_view(const FieldSet &fs, unsigned b) { return FieldView{FieldRef{fs, (size_t)b * fs.bstride}}; }
 
 GHX_DEV void xcd_tile(int &bx, int &by) {
   const int nb = gridDim.x * gridDim.y;
@@ -170,7 +176,7 @@ GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
 }
 
 template <int N, int MODE>
-__global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twt,
+__global__ __launch_bounds__(512) void k_zinv(FieldSet in_, FieldSet out_, int nf, GridK g, const cplx *__restrict__ twt,
                                              int prune, const double *__restrict__ aux) {
   using F = LineFFT<N, +1>;
   constexpr int T = F::T, E = F::E;
@@ -179,8 +185,8 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in, FieldSet out, int nf,
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
-  batch_shift(in, blockIdx.z);
-  batch_shift(out, blockIdx.z);
+  const FieldView in = batch_view(in_, blockIdx.z);
+  const FieldView out = batch_view(out_, blockIdx.z);
   int bx, by;
   xcd_tile(bx, by);
   const int i = by, j0 = bx * L, j = j0 + lt;
@@ -353,7 +359,7 @@ GHX_DEV cplx rk_update(cplx cc, cplx vv, cplx nl, cplx ff, double kk2, double nu
 }
 
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const cplx *__restrict__ twt, RKParams rk,
+__global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in_, GridK g, const cplx *__restrict__ twt, RKParams rk,
                                                    int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
@@ -362,7 +368,7 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in, GridK g, const 
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
-  batch_shift(in, blockIdx.z);
+  const FieldView in = batch_view(in_, blockIdx.z);
   if (blockIdx.z && rk.bstride) {
     const size_t o = (size_t)blockIdx.z * rk.bstride;
 #pragma unroll
@@ -530,13 +536,22 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
   const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
   const bool forced = ok && rk.fline && rk.fline[(size_t)i * g.ny + j];
   const cplx z0 = c_make(0.0, 0.0);
-  // velocity: fields 0..2 = w x u, 3..5 = j x b
+  // velocity: fields 0..2 = w x u, 3..5 = j x b; nonlin3 needs only their
+  // difference, accumulated field by field (D = FFT(w x u) - FFT(j x b))
   {
-    cplx P[6][E];
+    cplx D[3][E];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-      zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
+    for (int f = 0; f < 3; ++f) {
+      zfwd_load<N, E>(D[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
       __syncthreads();
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      cplx Q[E];
+      zfwd_load<N, E>(Q, lds, row, stride, L, t, tw, in.p[3 + f], g, i, j0, lmask);
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < E; ++m) D[f][m] = c_sub(D[f][m], Q[m]);
     }
     if (ok) {
 #pragma unroll
@@ -548,13 +563,12 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
         cplx o[3] = {z0, z0, z0};
         if (kn2 <= g.kmax && kn2 >= g.tiny) {
           const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
-          cplx d0 = c_sub(P[0][m], P[3][m]), d1 = c_sub(P[1][m], P[4][m]), d2 = c_sub(P[2][m], P[5][m]);
-          cplx dot = c_add(c_add(c_scale(d0, kx), c_scale(d1, ky)), c_scale(d2, kz));
+          cplx dot = c_add(c_add(c_scale(D[0][m], kx), c_scale(D[1][m], ky)), c_scale(D[2][m], kz));
           const double kd[3] = {kx, ky, kz};
           const int pl[3] = {i, j, k};
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            cplx base = c_add(c_make(-P[c][m].x, -P[c][m].y), P[3 + c][m]);
+            cplx base = c_make(-D[c][m].x, -D[c][m].y);  // -(P_c - P_{3+c}), exactly (-P_c) + P_{3+c}
             cplx nl = pl[c] == 0 ? base : c_add(base, c_make(kd[c] * dot.x / kk2, kd[c] * dot.y / kk2));
             o[c] = rk_update(rk.c[c][s], rk.v[c][s], nl, forced && rk.f[c] ? rk.f[c][s] : z0, kk2, rk.nu,
                              rk.dt, rk.rmp);
@@ -614,7 +628,7 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
 // touched (kxlim < nx/2+1 when the fields are dealiased).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(512) void k_yinv(FieldSet in, FieldSet out, int nf, GridK g, int nxp,
+__global__ __launch_bounds__(512) void k_yinv(FieldSet in_, FieldSet out_, int nf, GridK g, int nxp,
                                              const cplx *__restrict__ twt, int kxlim, int prune) {
   using F = LineFFT<N, +1>;
   constexpr int T = F::T, E = F::E;
@@ -623,8 +637,8 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in, FieldSet out, int nf,
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
-  batch_shift(in, blockIdx.z);
-  batch_shift(out, blockIdx.z);
+  const FieldView in = batch_view(in_, blockIdx.z);
+  const FieldView out = batch_view(out_, blockIdx.z);
   int bx, by;
   xcd_tile(bx, by);
   const int z = by, kx0 = bx * L, kx = kx0 + lt;
@@ -665,7 +679,7 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in, FieldSet out, int nf,
 }
 
 template <int N>
-__global__ __launch_bounds__(512) void k_yfwd(FieldSet in, FieldSet out, int nf, GridK g, int nxp,
+__global__ __launch_bounds__(512) void k_yfwd(FieldSet in_, FieldSet out_, int nf, GridK g, int nxp,
                                              const cplx *__restrict__ twt, int kxlim, int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
@@ -674,8 +688,8 @@ __global__ __launch_bounds__(512) void k_yfwd(FieldSet in, FieldSet out, int nf,
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
   TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
-  batch_shift(in, blockIdx.z);
-  batch_shift(out, blockIdx.z);
+  const FieldView in = batch_view(in_, blockIdx.z);
+  const FieldView out = batch_view(out_, blockIdx.z);
   int bx, by;
   xcd_tile(bx, by);
   const int z = by, kx0 = bx * L, kx = kx0 + lt;
@@ -955,7 +969,7 @@ __global__ __launch_bounds__(256) void k_xfwd(RealSet in, FieldSet out, int nf, 
 // row's next input is loaded before the current one is transformed.
 template <int N, int W, int SB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_xcross(
-    FieldSet in, FieldSet out, GridK g, int nxp, const cplx *__restrict__ twh, const cplx *__restrict__ twx,
+    FieldSet in_, FieldSet out_, GridK g, int nxp, const cplx *__restrict__ twh, const cplx *__restrict__ twx,
     int kxlim) {
   using Cfg = FFTCfgLow<N>;
   constexpr int T = Cfg::T, E = Cfg::E;
@@ -972,8 +986,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
   const size_t off = (size_t)(ok ? r : 0) * nxp;
   cplx *buf = lds + lt * lds_line_stride(N);
   const double tmp = g.inv_n6;
-  batch_shift(in, blockIdx.y);
-  batch_shift(out, blockIdx.y);
+  const FieldView in = batch_view(in_, blockIdx.y);
+  const FieldView out = batch_view(out_, blockIdx.y);
   XTw<N, Cfg> tw;
   tw.load(twh, twx, t);
   cplx qx[E], qy[E], qz[E], ay[E], az[E];
