@@ -134,6 +134,16 @@ def _declare(L):
         "ghx_tracers_euler_to_lag": (_I, [_P, _P, _P]),
         "ghx_hd_set_tracers": (_I, [_P, _P]),
         "ghx_mhd_set_tracers": (_I, [_P, _P]),
+        "ghx_io_init": (_I, [_PP, _I, C.POINTER(_I), _I, _I]),
+        "ghx_io_initc": (_I, [_PP, _I, C.POINTER(_I), _I, _I]),
+        "ghx_io_destroy": (_I, [_P]),
+        "ghx_io_set_options": (_I, [_P, _I, _I, C.c_longlong]),
+        "ghx_io_read": (_I, [_P, C.c_char_p, C.c_char_p, C.c_char_p, _P]),
+        "ghx_io_write": (_I, [_P, C.c_char_p, C.c_char_p, C.c_char_p, _P]),
+        "ghx_io_readc": (_I, [_P, C.c_char_p, C.c_char_p, C.c_char_p, _P]),
+        "ghx_io_writec": (_I, [_P, C.c_char_p, C.c_char_p, C.c_char_p, _P]),
+        "ghx_hd_output": (_I, [_P, _P, C.c_char_p, C.c_char_p, _I, _P, _P, _P]),
+        "ghx_hd_restart": (_I, [_P, _P, C.c_char_p, C.c_char_p, _P, _P, _P]),
         "ghx_prof_enable": (_I, [_I]),
         "ghx_prof_reset": (_I, []),
         "ghx_prof_read": (_I, [_I, C.POINTER(C.c_char_p), C.POINTER(_I), _DP, _DP]),

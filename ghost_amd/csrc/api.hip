@@ -45,6 +45,9 @@ static int set_err(int code, const char *fmt, ...) {
 
 extern "C" const char *ghx_last_error(void) { return g_err; }
 
+// for the other translation units of the library (binio.hip)
+int ghx_internal_set_err(int code, const char *msg) { return set_err(code, "%s", msg); }
+
 extern "C" int ghx_device_ok(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -779,6 +782,63 @@ extern "C" int ghx_sharp(ghx_grid *g, const void *a, void *b, double delta) {
   ARGCHK(g && a && b && delta > 0, "bad sharp arguments");
   const double pi = 3.14159265358979323846;
   HIPCHK(ghx::op_sharp(g->gk, g->stream, C_(a), M_(b), (pi / delta) * (pi / delta)));
+  return GHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// field output and restart (binary files of binio.hip)
+// ---------------------------------------------------------------------------
+// main3D.fpp:2675-2710 (VELOC_): C = v * rmp with rmp = 1/(nx ny nz); with
+// outs >= 1 the vorticity rotor3(C) is transformed and written as wx, wy, wz
+// first; then vx, vy, vz = c2r(C).
+extern "C" int ghx_hd_output(ghx_grid *g, ghx_ioplan *io, const char *odir, const char *ext, int outs,
+                             const void *vx, const void *vy, const void *vz) {
+  ARGCHK(g && io && odir && ext && vx && vy && vz, "bad hd_output arguments");
+  int rc = grid_ctmp(g, outs >= 1 ? 6 : 3);
+  if (!rc) rc = grid_rtmp(g, 1);
+  if (rc) return rc;
+  const double rmp = 1.0 / ((double)g->gk.nx * (double)g->gk.ny * (double)g->gk.nz);
+  cplx *C[6];
+  for (int f = 0; f < 6; ++f) C[f] = g->ctmp + (size_t)f * g->nc;
+  const void *v[3] = {vx, vy, vz};
+  for (int c = 0; c < 3; ++c) HIPCHK(ghx::op_scale(g->gk, g->stream, C[c], C_(v[c]), rmp));
+  double *R = g->rtmp;
+  static const char *const wn[3] = {"wx", "wy", "wz"}, *const vn[3] = {"vx", "vy", "vz"};
+  if (outs >= 1) {
+    HIPCHK(ghx::op_rotor3(g->gk, g->stream, C[1], C[2], C[3], 1));
+    HIPCHK(ghx::op_rotor3(g->gk, g->stream, C[0], C[2], C[4], 2));
+    HIPCHK(ghx::op_rotor3(g->gk, g->stream, C[0], C[1], C[5], 3));
+    for (int c = 0; c < 3; ++c) {
+      const void *src = C[3 + c];
+      if ((rc = do_c2r(&g->plan, 1, &src, &R))) return rc;
+      if ((rc = ghx_io_write(io, odir, wn[c], ext, R))) return rc;
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    const void *src = C[c];
+    if ((rc = do_c2r(&g->plan, 1, &src, &R))) return rc;
+    if ((rc = ghx_io_write(io, odir, vn[c], ext, R))) return rc;
+  }
+  return GHX_OK;
+}
+
+// main3D.fpp:2143-2169 (VELOC_, stat > 0): vx, vy, vz read from their files
+// and transformed with r2c.
+extern "C" int ghx_hd_restart(ghx_grid *g, ghx_ioplan *io, const char *idir, const char *ext, void *vx, void *vy,
+                              void *vz) {
+  ARGCHK(g && io && idir && ext && vx && vy && vz, "bad hd_restart arguments");
+  int rc = grid_rtmp(g, 1);
+  if (rc) return rc;
+  static const char *const vn[3] = {"vx", "vy", "vz"};
+  void *v[3] = {vx, vy, vz};
+  double *R = g->rtmp;
+  for (int c = 0; c < 3; ++c) {
+    HIPCHK(hipStreamSynchronize(g->stream));  // the previous transform has read R
+    if ((rc = ghx_io_read(io, idir, vn[c], ext, R))) return rc;
+    const double *src = R;
+    if ((rc = do_r2c(&g->plan, 1, &src, &v[c]))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(g->stream));
   return GHX_OK;
 }
 

@@ -78,6 +78,15 @@ static int tile_pick(const char *env, int ctx, int dflt) {
   return e > 0 ? e : (ctx > 0 ? ctx : dflt);
 }
 
+// Row lengths with a two-for-one x kernel (a full-length schedule of E = 8
+// points per lane and at most 64 lanes); GHX_XC2=2 selects the half-length
+// kernels instead, for comparisons.
+#define GHX_X2_SIZES(X) X(32) X(64) X(128) X(256) X(512)
+static bool use_x2(int nx) {
+  if (tile_want("GHX_XC2", 1) != 1) return false;
+  return nx == 32 || nx == 64 || nx == 128 || nx == 256 || nx == 512;
+}
+
 template <typename K> static void allow_lds(K kern, size_t bytes) {
   if (bytes > 49152) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
@@ -961,6 +970,294 @@ __global__ __launch_bounds__(256) void k_xfwd(RealSet in, FieldSet out, int nf, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-for-one x passes.  Two Hermitian half rows A, B (kx = 0..NX/2) make one
+// complex line of NX points, z[k] = A[k] + i B[k] with A[NX-k] = conj A[k];
+// its inverse transform is x_a + i x_b, the two real rows at once.  Forward,
+// z = x_a + i x_b gives A[k] = (Z[k] + conj Z[NX-k])/2 and
+// B[k] = (Z[k] - conj Z[NX-k])/(2i).  Against one half-length transform per
+// row plus its packing step, a pair costs one full-length transform (two LDS
+// exchanges for NX = 512 with radix 8^3) and no mirror exchange on the
+// inverse side: the mirrored points are read from the row itself.
+// ---------------------------------------------------------------------------
+// Loads of one pair: lane t takes the points k = t + T m of the full line,
+// the upper half from the mirrored kx.  Points with kx >= kxlim read kx = 0
+// (kept in-row so the loads issue back to back) and are zeroed when the line
+// is formed.
+template <int NX, int E, int T>
+GHX_DEV void xpair_load(cplx (&A)[E], cplx (&B)[E], const cplx *sa, const cplx *sb, int t, int kxlim) {
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    const int kk = k <= NX / 2 ? k : NX - k;
+    const int i = kk < kxlim ? kk : 0;
+    A[m] = sa[i];
+    B[m] = sb[i];
+  }
+}
+
+// z[m] = A[k] + i B[k] over the full line.  GHOST's c2r ignores the
+// imaginary parts of the kx = 0 and Nyquist entries.
+// With hasb false the line carries A alone (B taken as zero).
+template <int NX, int E, int T>
+GHX_DEV void xpair_make(cplx (&z)[E], const cplx (&A)[E], const cplx (&B)[E], int t, int kxlim, bool hasb = true) {
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    const int kk = k <= NX / 2 ? k : NX - k;
+    const bool live = kk < kxlim;
+    const bool realpt = (kk == 0) || (kk == NX / 2);
+    const double s = k > NX / 2 ? -1.0 : 1.0;  // upper half: conjugate
+    const bool lb = live && hasb;
+    const double ar = live ? A[m].x : 0.0, br = lb ? B[m].x : 0.0;
+    const double ai = (live && !realpt) ? s * A[m].y : 0.0, bi = (lb && !realpt) ? s * B[m].y : 0.0;
+    z[m] = c_make(ar - bi, ai + br);
+  }
+}
+
+// Z = forward transform of x_a + i x_b in natural order (z[m] = Z[t + T m]);
+// stores A[k], B[k] for k < kxlim (k <= NX/2) to the rows da, db.  The
+// mirrored Z[NX-k] come through the line's LDS buffer.
+template <int NX, int E, int T>
+GHX_DEV void xpair_store2(const cplx (&z)[E], cplx *buf, int t, cplx *da, cplx *db, int kxlim) {
+#pragma unroll
+  for (int m = 0; m < E; ++m) buf[lds_pad(t + T * m)] = z[m];
+  line_sync<T>();
+#pragma unroll
+  for (int m = 0; m <= E / 2; ++m) {
+    const int k = t + T * m;
+    if (k <= NX / 2) {
+      const cplx Zc = c_conj(buf[lds_pad((NX - k) % NX)]);
+      const cplx a = c_scale(c_add(z[m], Zc), 0.5);
+      const cplx d = c_sub(z[m], Zc);
+      const cplx b = c_make(0.5 * d.y, -0.5 * d.x);  // -0.5 i (Z - conj Z')
+      if (k < kxlim) {
+        da[k] = a;
+        db[k] = b;
+      }
+    }
+  }
+  line_sync<T>();
+}
+
+// single row: z = forward transform of x_a (imaginary input zero) is A itself
+template <int NX, int E, int T>
+GHX_DEV void xpair_store1(const cplx (&z)[E], int t, cplx *da, int kxlim) {
+#pragma unroll
+  for (int m = 0; m <= E / 2; ++m) {
+    const int k = t + T * m;
+    if (k <= NX / 2 && k < kxlim) da[k] = z[m];
+  }
+}
+
+// x_a + i x_b -> the line's samples n = t + T m; the single-row form leaves
+// the imaginary input zero
+template <int E>
+GHX_DEV void xpair_real(cplx (&z)[E], const double *sa, const double *sb, int t, int T) {
+#pragma unroll
+  for (int m = 0; m < E; ++m) z[m] = c_make(sa[t + T * m], sb ? sb[t + T * m] : 0.0);
+}
+
+// X^-1 of nf fields, two rows per transform (plain c2r; the last field of an
+// odd count goes alone).
+template <int NX>
+__global__ __launch_bounds__(256) void k_xinv2(FieldSet in, RealSet out, int nf, GridK g, int nxp,
+                                              const cplx *__restrict__ twt, int kxlim) {
+  using Cfg = FFTCfg<NX>;
+  using FI = LineFFT<NX, +1>;
+  constexpr int T = Cfg::T, E = Cfg::E;
+  static_assert(T <= 64, "one row per wavefront or less");
+  extern __shared__ cplx lds[];
+  const int rpb = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const long r = (long)blockIdx.x * rpb + lt;
+  if (r >= (long)g.nz * g.ny) return;
+  const size_t off = (size_t)r * nxp, roff = (size_t)r * NX;
+  cplx *buf = lds + lt * lds_line_stride(NX);
+  TwCache<NX> tw;
+  tw.load(twt, t);
+  cplx a[E], b[E], z[E];
+  const int np = (nf + 1) / 2;
+  xpair_load<NX, E, T>(a, b, in.p[0] + off, in.p[nf > 1 ? 1 : 0] + off, t, kxlim);
+  for (int p = 0; p < np; ++p) {
+    const bool two = 2 * p + 1 < nf;
+    xpair_make<NX, E, T>(z, a, b, t, kxlim, two);
+    if (p + 1 < np) {  // next pair in flight
+      const int fa = 2 * p + 2, fb = fa + 1 < nf ? fa + 1 : fa;
+      xpair_load<NX, E, T>(a, b, in.p[fa] + off, in.p[fb] + off, t, kxlim);
+    }
+    FI::run(z, buf, t, tw);
+    double *da = out.p[2 * p] + roff;
+#pragma unroll
+    for (int m = 0; m < E; ++m) da[t + T * m] = z[m].x;
+    if (two) {
+      double *db = out.p[2 * p + 1] + roff;
+#pragma unroll
+      for (int m = 0; m < E; ++m) db[t + T * m] = z[m].y;
+    }
+  }
+}
+
+// X^-1 of three fields written interleaved (the tracer layout of k_xinv3i),
+// pairs (0, 1) and (2, alone).
+template <int NX>
+__global__ __launch_bounds__(256) void k_xinv3i2(FieldSet in, double *out, GridK g, int nxp,
+                                                const cplx *__restrict__ twt, int kxlim) {
+  using Cfg = FFTCfg<NX>;
+  using FI = LineFFT<NX, +1>;
+  constexpr int T = Cfg::T, E = Cfg::E;
+  static_assert(T <= 64, "one row per wavefront or less");
+  extern __shared__ cplx lds[];
+  const int rpb = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const long r = (long)blockIdx.x * rpb + lt;
+  if (r >= (long)g.nz * g.ny) return;
+  const size_t off = (size_t)r * nxp;
+  cplx *buf = lds + lt * lds_line_stride(NX);
+  TwCache<NX> tw;
+  tw.load(twt, t);
+  cplx a[E], b[E], z0[E], z1[E];
+  xpair_load<NX, E, T>(a, b, in.p[0] + off, in.p[1] + off, t, kxlim);
+  xpair_make<NX, E, T>(z0, a, b, t, kxlim);
+  xpair_load<NX, E, T>(a, b, in.p[2] + off, in.p[2] + off, t, kxlim);
+  FI::run(z0, buf, t, tw);
+  xpair_make<NX, E, T>(z1, a, b, t, kxlim, false);
+  FI::run(z1, buf, t, tw);
+  double *dst = out + (size_t)r * NX * 3;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = t + T * m;
+    dst[3 * n + 0] = z0[m].x;
+    dst[3 * n + 1] = z0[m].y;
+    dst[3 * n + 2] = z1[m].x;
+  }
+}
+
+// X of nf real rows, two rows per transform.
+template <int NX>
+__global__ __launch_bounds__(256) void k_xfwd2(RealSet in, FieldSet out, int nf, GridK g, int nxp,
+                                              const cplx *__restrict__ twt) {
+  using Cfg = FFTCfg<NX>;
+  using FF = LineFFT<NX, -1>;
+  constexpr int T = Cfg::T, E = Cfg::E;
+  static_assert(T <= 64, "one row per wavefront or less");
+  extern __shared__ cplx lds[];
+  const int rpb = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const long r = (long)blockIdx.x * rpb + lt;
+  if (r >= (long)g.nz * g.ny) return;
+  const size_t off = (size_t)r * nxp, roff = (size_t)r * NX;
+  cplx *buf = lds + lt * lds_line_stride(NX);
+  TwCache<NX> tw;
+  tw.load(twt, t);
+  cplx z[E], nxt[E];
+  const int np = (nf + 1) / 2;
+  xpair_real<E>(z, in.p[0] + roff, nf > 1 ? in.p[1] + roff : nullptr, t, T);
+  for (int p = 0; p < np; ++p) {
+    const bool two = 2 * p + 1 < nf;
+    if (p + 1 < np) {
+      const int fa = 2 * p + 2;
+      xpair_real<E>(nxt, in.p[fa] + roff, fa + 1 < nf ? in.p[fa + 1] + roff : nullptr, t, T);
+    }
+    FF::run(z, buf, t, tw);
+    if (two)
+      xpair_store2<NX, E, T>(z, buf, t, out.p[2 * p] + off, out.p[2 * p + 1] + off, NX / 2 + 1);
+    else
+      xpair_store1<NX, E, T>(z, t, out.p[2 * p] + off, NX / 2 + 1);
+#pragma unroll
+    for (int m = 0; m < E; ++m) z[m] = nxt[m];
+  }
+}
+
+// Fused X^-1 . ((A_x B_x + A_y B_y) + A_z B_z) tmp . X (advect3) with pairs
+// (A_x, A_y), (A_z, B_x), (B_y, B_z); fields as k_xdot.
+template <int NX>
+__global__ __launch_bounds__(256) void k_xdot2(FieldSet in, FieldSet out, GridK g, int nxp,
+                                              const cplx *__restrict__ twt, int kxlim) {
+  using Cfg = FFTCfg<NX>;
+  using FI = LineFFT<NX, +1>;
+  using FF = LineFFT<NX, -1>;
+  constexpr int T = Cfg::T, E = Cfg::E;
+  static_assert(T <= 64, "one row per wavefront or less");
+  extern __shared__ cplx lds[];
+  const int rpb = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const long r = (long)blockIdx.x * rpb + lt;
+  if (r >= (long)g.nz * g.ny) return;
+  const size_t off = (size_t)r * nxp;
+  cplx *buf = lds + lt * lds_line_stride(NX);
+  const double tmp = -g.inv_n6;
+  TwCache<NX> tw;
+  tw.load(twt, t);
+  cplx a[E], b[E], z0[E], z1[E], z2[E];
+  xpair_load<NX, E, T>(a, b, in.p[0] + off, in.p[1] + off, t, kxlim);
+  xpair_make<NX, E, T>(z0, a, b, t, kxlim);
+  xpair_load<NX, E, T>(a, b, in.p[2] + off, in.p[3] + off, t, kxlim);
+  FI::run(z0, buf, t, tw);
+  xpair_make<NX, E, T>(z1, a, b, t, kxlim);
+  xpair_load<NX, E, T>(a, b, in.p[4] + off, in.p[5] + off, t, kxlim);
+  FI::run(z1, buf, t, tw);
+  xpair_make<NX, E, T>(z2, a, b, t, kxlim);
+  FI::run(z2, buf, t, tw);
+  // samples: A_x = z0.x, A_y = z0.y, A_z = z1.x, B_x = z1.y, B_y = z2.x, B_z = z2.y
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+    z0[m] = c_make(((z0[m].x * z1[m].y + z0[m].y * z2[m].x) + z1[m].x * z2[m].y) * tmp, 0.0);
+  FF::run(z0, buf, t, tw);
+  xpair_store1<NX, E, T>(z0, t, out.p[0] + off, kxlim);
+}
+
+// Fused X^-1 . (P x Q)/N^6 . X with two-for-one transforms: pairs (P_x, P_y),
+// (P_z, Q_x), (Q_y, Q_z) inverse, (out_x, out_y) and out_z forward.  Same
+// fields, products and operand order as k_xcross.  One row per line of
+// T <= 64 lanes.
+template <int NX>
+__global__ __launch_bounds__(256) void k_xcross2(FieldSet in_, FieldSet out_, GridK g, int nxp,
+                                                const cplx *__restrict__ twt, int kxlim) {
+  using Cfg = FFTCfg<NX>;
+  using FI = LineFFT<NX, +1>;
+  using FF = LineFFT<NX, -1>;
+  constexpr int T = Cfg::T, E = Cfg::E;
+  static_assert(T <= 64, "one row per wavefront or less");
+  extern __shared__ cplx lds[];
+  const int rpb = blockDim.x / T;
+  const int lt = threadIdx.x / T, t = threadIdx.x % T;
+  const long nrows = (long)g.nz * g.ny;
+  const long r = (long)blockIdx.x * rpb + lt;
+  if (r >= nrows) return;  // whole lines only: wave-level syncs
+  const size_t off = (size_t)r * nxp;
+  cplx *buf = lds + lt * lds_line_stride(NX);
+  const double tmp = g.inv_n6;
+  const FieldView in = batch_view(in_, blockIdx.y);
+  const FieldView out = batch_view(out_, blockIdx.y);
+  TwCache<NX> tw;
+  tw.load(twt, t);
+  cplx a[E], b[E], z0[E], z1[E], z2[E];
+  xpair_load<NX, E, T>(a, b, in.p[0] + off, in.p[1] + off, t, kxlim);
+  xpair_make<NX, E, T>(z0, a, b, t, kxlim);
+  xpair_load<NX, E, T>(a, b, in.p[2] + off, in.p[3] + off, t, kxlim);
+  FI::run(z0, buf, t, tw);
+  xpair_make<NX, E, T>(z1, a, b, t, kxlim);
+  xpair_load<NX, E, T>(a, b, in.p[4] + off, in.p[5] + off, t, kxlim);
+  FI::run(z1, buf, t, tw);
+  xpair_make<NX, E, T>(z2, a, b, t, kxlim);
+  FI::run(z2, buf, t, tw);
+  // samples: P_x = z0.x, P_y = z0.y, P_z = z1.x, Q_x = z1.y, Q_y = z2.x, Q_z = z2.y
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const double px = z0[m].x, py = z0[m].y, pz = z1[m].x, qx = z1[m].y, qy = z2[m].x, qz = z2[m].y;
+    const double ox = (py * qz - qy * pz) * tmp;
+    const double oy = (pz * qx - qz * px) * tmp;
+    const double oz = (px * qy - qx * py) * tmp;
+    z0[m] = c_make(ox, oy);
+    z1[m] = c_make(oz, 0.0);
+  }
+  FF::run(z0, buf, t, tw);
+  xpair_store2<NX, E, T>(z0, buf, t, out.p[0] + off, out.p[1] + off, kxlim);
+  FF::run(z1, buf, t, tw);
+  xpair_store1<NX, E, T>(z1, t, out.p[2] + off, kxlim);
+}
+
 // Fused X^-1 . (P x Q)/N^6 . X on P rows with GHOST's operand order
 // (pseudospec3D_hd.f90:377-389): out_x = (P_y Q_z - Q_y P_z) tmp,
 // out_y = (P_z Q_x - Q_z P_x) tmp, out_z = (P_x Q_y - Q_x P_y) tmp.
@@ -1278,6 +1575,23 @@ hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out,
   for (int f = 0; f < nf; ++f) rs.p[f] = out[f];
   const long nrows = (long)g.nz * g.ny;
   const int nxp = p_row(g);
+  if (use_x2(g.nx)) {
+    switch (g.nx) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                      \
+    hipLaunchKernelGGL((k_xinv2<n_>), gr, bl, lds, c.stream, in, rs, nf, g, nxp, c.tw_x, kxlim);                                                                 \
+    return hipGetLastError();                                                               \
+  }
+      GHX_X2_SIZES(GHX_CASE)
+#undef GHX_CASE
+      default:
+        break;
+    }
+  }
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
@@ -1301,6 +1615,23 @@ hipError_t xinv3i(const FFTCtx &c, const FieldSet &in, double *out, int prune) {
   const int kxlim = prune ? active_kx(g) : g.nxh;
   const long nrows = (long)g.nz * g.ny;
   const int nxp = p_row(g);
+  if (use_x2(g.nx)) {
+    switch (g.nx) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                      \
+    hipLaunchKernelGGL((k_xinv3i2<n_>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_x, kxlim);                                                                 \
+    return hipGetLastError();                                                               \
+  }
+      GHX_X2_SIZES(GHX_CASE)
+#undef GHX_CASE
+      default:
+        break;
+    }
+  }
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
@@ -1325,6 +1656,23 @@ hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet
   for (int f = 0; f < nf; ++f) rs.p[f] = const_cast<double *>(in[f]);
   const long nrows = (long)g.nz * g.ny;
   const int nxp = p_row(g);
+  if (use_x2(g.nx)) {
+    switch (g.nx) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                      \
+    hipLaunchKernelGGL((k_xfwd2<n_>), gr, bl, lds, c.stream, rs, out, nf, g, nxp, c.tw_x);                                                                 \
+    return hipGetLastError();                                                               \
+  }
+      GHX_X2_SIZES(GHX_CASE)
+#undef GHX_CASE
+      default:
+        break;
+    }
+  }
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
@@ -1351,6 +1699,23 @@ hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet 
   const int kxlim = prune ? active_kx(g) : g.nxh;
   const long nrows = (long)g.nz * g.ny;
   const int nxp = p_row(g);
+  if (use_x2(g.nx)) {
+    switch (g.nx) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const dim3 gr((unsigned)((nrows + rpb - 1) / rpb), c.batch), bl(rpb * T);                      \
+    hipLaunchKernelGGL((k_xcross2<n_>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_x, kxlim);                                                                 \
+    return hipGetLastError();                                                               \
+  }
+      GHX_X2_SIZES(GHX_CASE)
+#undef GHX_CASE
+      default:
+        break;
+    }
+  }
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
@@ -1383,6 +1748,23 @@ hipError_t xdot(const FFTCtx &c, const FieldSet &in, const FieldSet &out, int pr
   const int kxlim = prune ? active_kx(g) : g.nxh;
   const long nrows = (long)g.nz * g.ny;
   const int nxp = p_row(g);
+  if (use_x2(g.nx)) {
+    switch (g.nx) {
+#define GHX_CASE(n_)                                                                        \
+  case n_: {                                                                                \
+    constexpr int T = FFTCfg<n_>::T;                                                        \
+    const int rpb = rows_per_block(T);                                                      \
+    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                      \
+    hipLaunchKernelGGL((k_xdot2<n_>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_x, kxlim);                                                                 \
+    return hipGetLastError();                                                               \
+  }
+      GHX_X2_SIZES(GHX_CASE)
+#undef GHX_CASE
+      default:
+        break;
+    }
+  }
   switch (g.nx / 2) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \

@@ -284,6 +284,61 @@ MODULE ghx_c
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: t, field, out
     END FUNCTION
+
+    ! binary field files (mpiio/binary_io.f90); strings are NUL-terminated
+    INTEGER(C_INT) FUNCTION ghx_io_init(plan, myrank, n, ksta, kend) BIND(C, name="ghx_io_init")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR) :: plan
+      INTEGER(C_INT), VALUE :: myrank, ksta, kend
+      INTEGER(C_INT) :: n(3)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_io_initc(plan, myrank, n, ksta, kend) BIND(C, name="ghx_io_initc")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR) :: plan
+      INTEGER(C_INT), VALUE :: myrank, ksta, kend
+      INTEGER(C_INT) :: n(3)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_io_destroy(plan) BIND(C, name="ghx_io_destroy")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: plan
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_io_set_options(plan, iswap, bmangle, disp) BIND(C, name="ghx_io_set_options")
+      IMPORT :: C_INT, C_PTR, C_LONG_LONG
+      TYPE(C_PTR), VALUE :: plan
+      INTEGER(C_INT), VALUE :: iswap, bmangle
+      INTEGER(C_LONG_LONG), VALUE :: disp
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_io_read(plan, dir, fname, nmb, var) BIND(C, name="ghx_io_read")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: plan, var
+      CHARACTER(KIND=C_CHAR) :: dir(*), fname(*), nmb(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_io_write(plan, dir, fname, nmb, var) BIND(C, name="ghx_io_write")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: plan, var
+      CHARACTER(KIND=C_CHAR) :: dir(*), fname(*), nmb(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_io_readc(plan, dir, fname, nmb, var) BIND(C, name="ghx_io_readc")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: plan, var
+      CHARACTER(KIND=C_CHAR) :: dir(*), fname(*), nmb(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_io_writec(plan, dir, fname, nmb, var) BIND(C, name="ghx_io_writec")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: plan, var
+      CHARACTER(KIND=C_CHAR) :: dir(*), fname(*), nmb(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_hd_output(grid, io, odir, ext, outs, vx, vy, vz) BIND(C, name="ghx_hd_output")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: grid, io, vx, vy, vz
+      CHARACTER(KIND=C_CHAR) :: odir(*), ext(*)
+      INTEGER(C_INT), VALUE :: outs
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_hd_restart(grid, io, idir, ext, vx, vy, vz) BIND(C, name="ghx_hd_restart")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: grid, io, vx, vy, vz
+      CHARACTER(KIND=C_CHAR) :: idir(*), ext(*)
+    END FUNCTION
   END INTERFACE
 
 CONTAINS

@@ -18,6 +18,7 @@ PROGRAM test_fftp_hip
   USE pseudospec_hip, ONLY: ghx_pseudospec_init, ghx_pseudospec_finalize, ghx_field, field_alloc, field_free, &
                             field_upload, field_download, ghx_grid_h, d_rotor3 => rotor3, d_energy => energy, &
                             hd_rkstep2
+  USE iovar, ONLY: IOPLAN
   IMPLICIT NONE
   INTEGER, PARAMETER :: n = 32, nh = n/2 + 1, nsteps = 20, npart = 1000, psteps = 5
   REAL(KIND=GP), PARAMETER :: pi = 3.14159265358979323846_GP
@@ -29,6 +30,8 @@ PROGRAM test_fftp_hip
   TYPE(ghx_field) :: v(3), cst(3), w(3)
   TYPE(C_PTR) :: hd, hv(3), hf(3), tr, pp(3)
   CHARACTER(LEN=512) :: outdir
+  CHARACTER(LEN=100) :: iodir
+  TYPE(IOPLAN) :: planio
   DOUBLE PRECISION :: e, err, x, y, z, rn
   INTEGER :: i, j, k, o, q, u
   INTEGER :: ierr
@@ -117,6 +120,21 @@ PROGRAM test_fftp_hip
   END DO
   CLOSE(u)
   WRITE(*, '(A)') 'ok   RK loop written (fortran_hd32.bin)'
+
+  ! GHOST's binary output of v (main3D.fpp:2675-2710) from the resident
+  ! fields, and the same file written from host arrays through GHOST's own
+  ! calls (C = v*rmp, fftp3d_complex_to_real, io_write), then read back
+  iodir = outdir
+  CALL io_init(0, (/n, n, n/), 1, n, planio)
+  CALL ghx_check(ghx_hd_output(ghx_grid_h, planio%h, TRIM(iodir)//C_NULL_CHAR, '0001'//C_NULL_CHAR, 0_C_INT, &
+                               v(1)%p, v(2)%p, v(3)%p), 'hd_output')
+  CALL field_download(v(1), C_LOC(c4))
+  c4 = c4*(1.0_GP/(REAL(n, GP)*REAL(n, GP)*REAL(n, GP)))
+  CALL fftp3d_complex_to_real(plancr, c4, r, 0)
+  CALL io_write(1, iodir, 'hx', '0001', planio, r)
+  CALL io_read(1, iodir, 'vx', '0001', planio, ux)
+  CALL report('io_write/io_read: device output == host output', MAXVAL(ABS(ux - r)), 0.0_GP)
+  WRITE(*, '(A)') 'ok   binary output written (vx.0001.out, hx.0001.out)'
 
   ! -- 3. resident solver with tracers ---------------------------------------
   ALLOCATE(pos0(npart, 3), pos1(npart, 3))

@@ -277,6 +277,37 @@ int ghx_mhd_set_tracers(ghx_mhd *s, ghx_tracers *t);
 int ghx_bouss_set_tracers(ghx_bouss *s, ghx_tracers *t);
 
 /* ---------------------------------------------------------------------- */
+/* Binary field files (mpiio/binary_io.f90, module iovar/iompi in           */
+/* mpiio/binary_mod.f90:11-32): the global array in Fortran order, raw native     */
+/* FP64 (complex: re, im pairs), no header, at byte offset disp; a slab     */
+/* (n1, n2, ksta:kend) sits at disp + (ksta-1) n1 n2 esize.  The file is    */
+/* dir/fname.nmb.out when bmangle = 1 (the default); otherwise dir/fname,   */
+/* except io_write, which opens fname alone.  var may be device memory      */
+/* (staged through pinned buffers; all streams are synchronised first) or   */
+/* host memory.  Reading a file shorter than the slab fails.                */
+/* ---------------------------------------------------------------------- */
+typedef struct ghx_ioplan ghx_ioplan;
+/* io_init (mpiio/binary_io.f90:13-59): real arrays var(n1, n2, ksta:kend) */
+int ghx_io_init(ghx_ioplan **plan, int myrank, const int n[3], int ksta, int kend);
+/* io_initc (mpiio/binary_io.f90:189-233): complex arrays */
+int ghx_io_initc(ghx_ioplan **plan, int myrank, const int n[3], int ksta, int kend);
+int ghx_io_destroy(ghx_ioplan *plan);
+/* iovar's iswap (byte-reverse every real after io_read, gutils_mod.fpp:114),
+ * iompi's bmangle and disp (mpiio/binary_mod.f90:11-30) */
+int ghx_io_set_options(ghx_ioplan *plan, int iswap, int bmangle, long long disp);
+int ghx_io_read(ghx_ioplan *plan, const char *dir, const char *fname, const char *nmb, double *var);       /* :62-129 */
+int ghx_io_write(ghx_ioplan *plan, const char *dir, const char *fname, const char *nmb, const double *var); /* :132-186 */
+int ghx_io_readc(ghx_ioplan *plan, const char *dir, const char *fname, const char *nmb, void *var);         /* :236-282 */
+int ghx_io_writec(ghx_ioplan *plan, const char *dir, const char *fname, const char *nmb, const void *var);  /* :285-331 */
+/* Field output of main3D.fpp:2675-2710: vx, vy, vz = c2r(v/(nx ny nz)) and,
+ * with outs >= 1, wx, wy, wz = c2r(rotor3(v/(nx ny nz))) written first;
+ * ext is the time label (fmtext, i4.4). */
+int ghx_hd_output(ghx_grid *g, ghx_ioplan *io, const char *odir, const char *ext, int outs, const void *vx,
+                  const void *vy, const void *vz);
+/* Restart of main3D.fpp:2143-2169: vx, vy, vz read and transformed with r2c */
+int ghx_hd_restart(ghx_grid *g, ghx_ioplan *io, const char *idir, const char *ext, void *vx, void *vy, void *vz);
+
+/* ---------------------------------------------------------------------- */
 /* Pass profiler (replaces the GTStart/GTStop handles around the fftp      */
 /* phases, utils/gtimer_mod.f90:37-185 and fftp-3/fftp3D.fpp:244-312):      */
 /* HIP events recorded around every pass launch on its stream.  Slots name  */

@@ -29,6 +29,15 @@ def test_fortran_host_side(gpu, tmp_path):
         og.hd_step(v, 1e-3, 1e-2, ord=2)
     for k in range(3):
         assert np.linalg.norm(got[k] - v[k]) <= 1e-12 * max(np.linalg.norm(v[k]), np.linalg.norm(v[0]))
+    # GHOST's binary output of that v: the resident path (vx, vy, vz) and the
+    # host-array path through io_write (hx) write the same bytes; the file is
+    # c2r(v/N^3) in Fortran order
+    vx_file = (tmp_path / "vx.0001.out").read_bytes()
+    assert vx_file == (tmp_path / "hx.0001.out").read_bytes()
+    for k, name in enumerate(("vx", "vy", "vz")):
+        r = np.fromfile(tmp_path / f"{name}.0001.out").reshape((n, n, n), order="F").transpose(2, 1, 0)
+        want = og.c2r(got[k] * (1.0 / n ** 3))
+        assert np.linalg.norm(r - want) <= 1e-13 * max(np.linalg.norm(want), 1e-300) + 1e-16
     # resident solver with 1000 tracers attached, 5 steps, GHOST's z stencil
     raw = np.fromfile(tmp_path / "fortran_tracers32.bin", dtype=np.float64)
     p0 = raw[:3000].reshape(3, 1000)  # Fortran pos(npart, 3): column c is axis c
