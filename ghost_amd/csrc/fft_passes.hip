@@ -323,6 +323,26 @@ GHX_DEV void zfwd_load(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
   F::run(v, row, t, tw);
 }
 
+// The transform of a difference of two Q fields, FFT(a - b) = FFT(a) - FFT(b):
+// the lines are subtracted before one transform.
+template <int N, int E>
+GHX_DEV void zfwd_load_diff(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N> &tw,
+                            const cplx *srca, const cplx *srcb, const GridK &g, int i, int j0, unsigned lmask) {
+  using F = LineFFT<N, -1>;
+  constexpr int T = F::T;
+  tile_load_q<E>(lds, stride, L, srca, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny, lmask);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = row[lds_pad(t + T * m)];
+  __syncthreads();
+  tile_load_q<E>(lds, stride, L, srcb, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny, lmask);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = c_sub(v[m], row[lds_pad(t + T * m)]);
+  __syncthreads();
+  F::run(v, row, t, tw);
+}
+
 template <int N>
 __global__ __launch_bounds__(512) void k_zfwd(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twt) {
   using F = LineFFT<N, -1>;
@@ -522,8 +542,10 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, con
 }
 
 // MHD: nonlin3 (pseudospec3D_mhd.f90:108-227) for the velocity from
-// (w x u, j x b) and gauge3 (:230-336) for the potential from u x b.
-template <int N>
+// (w x u, j x b) (PART 0) and gauge3 (:230-336) for the potential from u x b
+// (PART 1).  The two parts are separate launches so that each keeps its
+// fields in registers.
+template <int N, int PART>
 __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const cplx *__restrict__ twt, MHDRKParams rk,
                                                     int prune) {
   using F = LineFFT<N, -1>;
@@ -541,91 +563,56 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
   const bool ok = j < g.ny && ((lmask >> lt) & 1u);
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
-  const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
-  const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
-  const bool forced = ok && rk.fline && rk.fline[(size_t)i * g.ny + j];
-  const cplx z0 = c_make(0.0, 0.0);
-  // velocity: fields 0..2 = w x u, 3..5 = j x b; nonlin3 needs only their
-  // difference, accumulated field by field (D = FFT(w x u) - FFT(j x b))
-  {
-    cplx D[3][E];
+  cplx D[3][E];
+  if constexpr (PART == 0) {
+    // velocity: fields 0..2 = w x u, 3..5 = j x b; nonlin3 needs only the
+    // difference of their transforms, D = FFT(w x u - j x b)
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      zfwd_load<N, E>(D[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
+      zfwd_load_diff<N, E>(D[f], lds, row, stride, L, t, tw, in.p[f], in.p[3 + f], g, i, j0, lmask);
       __syncthreads();
     }
+  } else {
+    // vector potential: fields 6..8 = u x b
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      cplx Q[E];
-      zfwd_load<N, E>(Q, lds, row, stride, L, t, tw, in.p[3 + f], g, i, j0, lmask);
+      zfwd_load<N, E>(D[f], lds, row, stride, L, t, tw, in.p[6 + f], g, i, j0, lmask);
       __syncthreads();
-#pragma unroll
-      for (int m = 0; m < E; ++m) D[f][m] = c_sub(D[f][m], Q[m]);
-    }
-    if (ok) {
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int k = t + T * m;
-        const size_t s = sline + k;
-        const double kz = wavenum(k, g.nz, g.dkz);
-        const double kn2 = kn2_of(g, i, j, k);
-        cplx o[3] = {z0, z0, z0};
-        if (kn2 <= g.kmax && kn2 >= g.tiny) {
-          const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
-          cplx dot = c_add(c_add(c_scale(D[0][m], kx), c_scale(D[1][m], ky)), c_scale(D[2][m], kz));
-          const double kd[3] = {kx, ky, kz};
-          const int pl[3] = {i, j, k};
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            cplx base = c_make(-D[c][m].x, -D[c][m].y);  // -(P_c - P_{3+c}), exactly (-P_c) + P_{3+c}
-            cplx nl = pl[c] == 0 ? base : c_add(base, c_make(kd[c] * dot.x / kk2, kd[c] * dot.y / kk2));
-            o[c] = rk_update(rk.c[c][s], rk.v[c][s], nl, forced && rk.f[c] ? rk.f[c][s] : z0, kk2, rk.nu,
-                             rk.dt, rk.rmp);
-          }
-        }
-        if (!prune || (kn2 <= g.kmax && kn2 >= g.tiny)) {
-          rk.out[0][s] = o[0];
-          rk.out[1][s] = o[1];
-          rk.out[2][s] = o[2];
-        }
-      }
     }
   }
-  // vector potential: fields 6..8 = u x b
-  {
-    cplx P[3][E];
+  if (!ok) return;
+  constexpr int c0 = PART == 0 ? 0 : 3;
+  const double coef = PART == 0 ? rk.nu : rk.mu;
+  const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
+  const size_t sline = ((size_t)i * g.ny + j) * N;
+  const bool forced = rk.fline && rk.fline[(size_t)i * g.ny + j];
+  const cplx z = c_make(0.0, 0.0);
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[6 + f], g, i, j0, lmask);
-      __syncthreads();
+  for (int m = 0; m < E; ++m) {
+    const int k = t + T * m;
+    const size_t s = sline + k;
+    const double kz = wavenum(k, g.nz, g.dkz);
+    const double kn2 = kn2_of(g, i, j, k);
+    if (kn2 <= g.kmax && kn2 >= g.tiny) {
+      const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
+      cplx n0, n1, n2;
+      // nonlin3 is the nonlhd3 projection of D; gauge3 is its negative
+      // applied to u x b (negation is exact, so both match the routines)
+      project_hd(D[0][m], D[1][m], D[2][m], kx, ky, kz, kk2, i, j, k, n0, n1, n2);
+      auto upd = [&](int c, cplx nl) {
+        if constexpr (PART == 1) nl = c_make(-nl.x, -nl.y);
+        const cplx ff = forced && rk.f[c0 + c] ? rk.f[c0 + c][s] : z;
+        rk.out[c0 + c][s] = rk_update(rk.c[c0 + c][s], rk.v[c0 + c][s], nl, ff, kk2, coef, rk.dt, rk.rmp);
+      };
+      upd(0, n0);
+      upd(1, n1);
+      upd(2, n2);
+    } else if (!prune) {
+      rk.out[c0][s] = z;
+      rk.out[c0 + 1][s] = z;
+      rk.out[c0 + 2][s] = z;
     }
-    if (ok) {
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int k = t + T * m;
-        const size_t s = sline + k;
-        const double kz = wavenum(k, g.nz, g.dkz);
-        const double kn2 = kn2_of(g, i, j, k);
-        cplx o[3] = {z0, z0, z0};
-        if (kn2 <= g.kmax && kn2 >= g.tiny) {
-          const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
-          cplx dot = c_add(c_add(c_scale(P[0][m], kx), c_scale(P[1][m], ky)), c_scale(P[2][m], kz));
-          const double kd[3] = {kx, ky, kz};
-          const int pl[3] = {i, j, k};
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            cplx nl = pl[c] == 0 ? P[c][m] : c_sub(P[c][m], c_make(kd[c] * dot.x / kk2, kd[c] * dot.y / kk2));
-            o[c] = rk_update(rk.c[3 + c][s], rk.v[3 + c][s], nl,
-                             forced && rk.f[3 + c] ? rk.f[3 + c][s] : z0, kk2, rk.mu, rk.dt, rk.rmp);
-          }
-        }
-        if (!prune || (kn2 <= g.kmax && kn2 >= g.tiny)) {
-          rk.out[3][s] = o[0];
-          rk.out[4][s] = o[1];
-          rk.out[5][s] = o[2];
-        }
-      }
-    }
+    __builtin_amdgcn_sched_barrier(0);  // one mode's loads at a time (register pressure)
   }
 }
 
@@ -1503,8 +1490,10 @@ hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &r
     int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4));                                        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
-    allow_lds(k_zfwd_rk_mhd<n_>, lds);                                                      \
-    hipLaunchKernelGGL((k_zfwd_rk_mhd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
+    allow_lds(k_zfwd_rk_mhd<n_, 0>, lds);                                                   \
+    allow_lds(k_zfwd_rk_mhd<n_, 1>, lds);                                                   \
+    hipLaunchKernelGGL((k_zfwd_rk_mhd<n_, 0>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
+    hipLaunchKernelGGL((k_zfwd_rk_mhd<n_, 1>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
