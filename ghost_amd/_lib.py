@@ -144,6 +144,20 @@ def _declare(L):
         "ghx_io_writec": (_I, [_P, C.c_char_p, C.c_char_p, C.c_char_p, _P]),
         "ghx_hd_output": (_I, [_P, _P, C.c_char_p, C.c_char_p, _I, _P, _P, _P]),
         "ghx_hd_restart": (_I, [_P, _P, C.c_char_p, C.c_char_p, _P, _P, _P]),
+        "ghx_mlp_nparams": (_I, []),
+        "ghx_mlp_create": (_I, [_PP, C.c_ulonglong]),
+        "ghx_mlp_destroy": (_I, [_P]),
+        "ghx_mlp_set_stream": (_I, [_P, _P]),
+        "ghx_mlp_params": (_I, [_P, _PP]),
+        "ghx_mlp_set_norm": (_I, [_P, C.POINTER(C.c_float)]),
+        "ghx_mlp_get_norm": (_I, [_P, C.POINTER(C.c_float)]),
+        "ghx_mlp_fit_norm": (_I, [_P, C.c_longlong, _PP, _D, _PP]),
+        "ghx_mlp_pack": (_I, [_P, C.c_longlong, _PP, _D, _PP, _P, _P]),
+        "ghx_mlp_forward": (_I, [_P, C.c_longlong, _P, _P]),
+        "ghx_mlp_grad": (_I, [_P, C.c_longlong, _P, _P, _P, _DP]),
+        "ghx_mlp_train": (_I, [_P, C.c_longlong, _P, _P, C.c_longlong, _I, C.c_float, C.c_ulonglong,
+                               C.POINTER(C.c_float)]),
+        "ghx_mlp_stress": (_I, [_P, C.c_longlong, _PP, _D, _PP]),
         "ghx_prof_enable": (_I, [_I]),
         "ghx_prof_reset": (_I, []),
         "ghx_prof_read": (_I, [_I, C.POINTER(C.c_char_p), C.POINTER(_I), _DP, _DP]),

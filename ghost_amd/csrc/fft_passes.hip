@@ -282,25 +282,32 @@ __global__ __launch_bounds__(512) void k_zinv(FieldSet in_, FieldSet out_, int n
       }
     }
     if constexpr (MODE == 2) {
-      cplx a[3][E], b[3][E];
+      cplx a[3][E];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int m = 0; m < E; ++m) a[c][m] = load(in.p[3 + c], t + T * m);
-      curl(a, 0, b[0]);
-      curl(a, 1, b[1]);
-      curl(a, 2, b[2]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         cplx v[E];
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = b[c][m];
+        curl(a, c, v);
         zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[6 + c], g, i, j0, lmask);
       }
+      // j = curl b with the two components of b = curl A it needs recomputed
+      // per mode (the same operations as rotor3 of the stored b)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         cplx v[E];
-        curl(b, c, v);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const double kz = wavenum(t + T * m, g.nz, g.dkz);
+          const cplx b0 = c_sub(c_ik(ky, a[2][m]), c_ik(kz, a[1][m]));
+          const cplx b1 = c_sub(c_ik(kz, a[0][m]), c_ik(kx, a[2][m]));
+          const cplx b2 = c_sub(c_ik(kx, a[1][m]), c_ik(ky, a[0][m]));
+          if (c == 0) v[m] = c_sub(c_ik(ky, b2), c_ik(kz, b1));
+          else if (c == 1) v[m] = c_sub(c_ik(kz, b0), c_ik(kx, b2));
+          else v[m] = c_sub(c_ik(kx, b1), c_ik(ky, b0));
+        }
         zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[9 + c], g, i, j0, lmask);
       }
     }

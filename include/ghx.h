@@ -308,6 +308,46 @@ int ghx_hd_output(ghx_grid *g, ghx_ioplan *io, const char *odir, const char *ext
 int ghx_hd_restart(ghx_grid *g, ghx_ioplan *io, const char *idir, const char *ext, void *vx, void *vy, void *vz);
 
 /* ---------------------------------------------------------------------- */
+/* Learned subgrid closure (no counterpart in GHOST; GHOST's own closures  */
+/* are the alpha/LES/EDQNM models of pseudospec3D_{lles,lhd,lahd,edqnm}.f90,*/
+/* which this build does not provide).  A 10 -> 64 -> 64 -> 6 perceptron    */
+/* with GELU maps the resolved velocity-gradient tensor (d u_i/d x_j, i     */
+/* major) and the filter width to the stress components tau_xx, tau_yy,     */
+/* tau_zz, tau_xy, tau_xz, tau_yz.  FP32 master parameters (5254 floats:     */
+/* W1[64][10], b1[64], W2[64][64], b2[64], W3[6][64], b3[6]) enter the     */
+/* matrix cores as BF16 with FP32 accumulation.  Inputs and outputs are     */
+/* normalised: norm = mean_x[10], sd_x[10], mean_y[6], sd_y[6].  Arrays X   */
+/* (n x 10) and T, Y (n x 6) are row-major FP32 device arrays.             */
+/* ---------------------------------------------------------------------- */
+typedef struct ghx_mlp ghx_mlp;
+int ghx_mlp_nparams(void);
+/* Glorot-uniform weights from seed, zero biases, identity normalisation */
+int ghx_mlp_create(ghx_mlp **m, unsigned long long seed);
+int ghx_mlp_destroy(ghx_mlp *m);
+int ghx_mlp_set_stream(ghx_mlp *m, void *stream);
+/* device pointer to the FP32 master parameters (read or overwrite in place) */
+int ghx_mlp_params(ghx_mlp *m, float **params);
+int ghx_mlp_set_norm(ghx_mlp *m, const float *norm);  /* host array of 32 */
+int ghx_mlp_get_norm(ghx_mlp *m, float *norm);
+/* normalisation from the moments of n points of the fields (the filter-width
+ * input is centred on delta with unit scale) */
+int ghx_mlp_fit_norm(ghx_mlp *m, long long n, const double *const *grad9, double delta, const double *const *tau6);
+/* X (and T when tau6 and T are given) from fields, normalised */
+int ghx_mlp_pack(ghx_mlp *m, long long n, const double *const *grad9, double delta, const double *const *tau6,
+                 float *X, float *T);
+/* Y = network(X), normalised units */
+int ghx_mlp_forward(ghx_mlp *m, long long n, const float *X, float *Y);
+/* loss = mean over the n x 6 outputs of (Y - T)^2 and its gradient (grad:
+ * device array of ghx_mlp_nparams() floats, may be NULL) */
+int ghx_mlp_grad(ghx_mlp *m, long long n, const float *X, const float *T, float *grad, double *loss);
+/* steps of Adam (beta 0.9, 0.999, eps 1e-8) on batches of the data set taken
+ * in a seeded permutation; losses (host, steps floats, may be NULL) */
+int ghx_mlp_train(ghx_mlp *m, long long n, const float *X, const float *T, long long batch, int steps, float lr,
+                  unsigned long long seed, float *losses);
+/* stress fields from gradient fields (n points, FP64 device fields) */
+int ghx_mlp_stress(ghx_mlp *m, long long n, const double *const *grad9, double delta, double *const *tau6);
+
+/* ---------------------------------------------------------------------- */
 /* Pass profiler (replaces the GTStart/GTStop handles around the fftp      */
 /* phases, utils/gtimer_mod.f90:37-185 and fftp-3/fftp3D.fpp:244-312):      */
 /* HIP events recorded around every pass launch on its stream.  Slots name  */

@@ -1411,3 +1411,108 @@ void og_truncate(const og_grid *g, const ocplx *c, const og_grid *t, ocplx *out)
         out[((size_t)i * nyt + jt) * nzt + kt] = v * fact;
       }
 }
+
+/* ------------------------------------------------------------------------
+ * The closure network of the learned LES (no counterpart in GHOST): a
+ * 10 -> 64 -> 64 -> 6 perceptron, h = GELU(W x + b) with the exact GELU
+ * x Phi(x), parameters W1[64][10], b1[64], W2[64][64], b2[64], W3[6][64],
+ * b3[6] in one float array.  With bf16 != 0 every operand of a matrix product
+ * is first rounded to bfloat16 (round to nearest, ties to even), as the matrix
+ * cores see it: the inputs, the weights, the hidden activations and, in the
+ * backward pass, the output error and the pre-activation errors of the
+ * hidden layers; sums are taken in double.  Biases and their gradients are
+ * not rounded.  bf16 = 0 is plain double arithmetic.
+ * ------------------------------------------------------------------------ */
+enum { MNI = 10, MNH = 64, MNO = 6 };
+enum { MW1 = 0, MB1 = MW1 + MNH * MNI, MW2 = MB1 + MNH, MB2 = MW2 + MNH * MNH, MW3 = MB2 + MNH,
+       MB3 = MW3 + MNO * MNH, MNPAR = MB3 + MNO };
+
+static double bf16_round(double x, int on) {
+  if (!on) return x;
+  float f = (float)x; /* the GPU holds these values in FP32 before packing */
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return f; /* inf / nan unchanged */
+  const uint32_t lsb = (u >> 16) & 1u;
+  u = (u + 0x7fffu + lsb) & 0xffff0000u;
+  memcpy(&f, &u, 4);
+  return f;
+}
+static double gelu_o(double x) { return 0.5 * x * (1.0 + erf(x * 0.70710678118654752440)); }
+static double gelu_do(double x) {
+  return 0.5 * (1.0 + erf(x * 0.70710678118654752440)) + x * 0.39894228040143267794 * exp(-0.5 * x * x);
+}
+
+/* one point: z1, h1, z2, h2 and y (y, z1, z2 in double) */
+static void mlp_point(const float *P, const float *x, int bf, double *z1, double *h1, double *z2, double *h2,
+                      double *y) {
+  double xb[MNI];
+  for (int k = 0; k < MNI; ++k) xb[k] = bf16_round(x[k], bf);
+  for (int m = 0; m < MNH; ++m) {
+    double s = 0.0;
+    for (int k = 0; k < MNI; ++k) s += bf16_round(P[MW1 + m * MNI + k], bf) * xb[k];
+    z1[m] = s + P[MB1 + m];
+    h1[m] = bf16_round(gelu_o(z1[m]), bf);
+  }
+  for (int m = 0; m < MNH; ++m) {
+    double s = 0.0;
+    for (int k = 0; k < MNH; ++k) s += bf16_round(P[MW2 + m * MNH + k], bf) * h1[k];
+    z2[m] = s + P[MB2 + m];
+    h2[m] = bf16_round(gelu_o(z2[m]), bf);
+  }
+  for (int j = 0; j < MNO; ++j) {
+    double s = 0.0;
+    for (int k = 0; k < MNH; ++k) s += bf16_round(P[MW3 + j * MNH + k], bf) * h2[k];
+    y[j] = s + P[MB3 + j];
+  }
+}
+
+void og_mlp_forward(const float *P, long n, const float *X, int bf16, double *Y) {
+  OMP_FOR
+  for (long p = 0; p < n; ++p) {
+    double z1[MNH], h1[MNH], z2[MNH], h2[MNH];
+    mlp_point(P, X + p * MNI, bf16, z1, h1, z2, h2, Y + p * MNO);
+  }
+}
+
+/* loss = mean over the n points and 6 outputs of (y - t)^2; grad (MNPAR
+ * doubles) receives d loss / d parameter */
+double og_mlp_grad(const float *P, long n, const float *X, const float *T, int bf16, double *grad) {
+  memset(grad, 0, sizeof(double) * MNPAR);
+  double loss = 0.0;
+  const double sc = 2.0 / ((double)n * MNO);
+  for (long p = 0; p < n; ++p) {
+    double z1[MNH], h1[MNH], z2[MNH], h2[MNH], y[MNO], dy[MNO], dyb[MNO], dz2[MNH], dz2b[MNH], dz1[MNH];
+    const float *x = X + p * MNI;
+    mlp_point(P, x, bf16, z1, h1, z2, h2, y);
+    for (int j = 0; j < MNO; ++j) {
+      const double e = y[j] - T[p * MNO + j];
+      loss += e * e;
+      dy[j] = (double)(float)(sc * e); /* the GPU forms this in FP32 */
+      dyb[j] = bf16_round(dy[j], bf16);
+    }
+    for (int m = 0; m < MNH; ++m) {
+      double s = 0.0;
+      for (int j = 0; j < MNO; ++j) s += bf16_round(P[MW3 + j * MNH + m], bf16) * dyb[j];
+      dz2[m] = s * gelu_do(z2[m]);
+      dz2b[m] = bf16_round(dz2[m], bf16);
+    }
+    for (int m = 0; m < MNH; ++m) {
+      double s = 0.0;
+      for (int u = 0; u < MNH; ++u) s += bf16_round(P[MW2 + u * MNH + m], bf16) * dz2b[u];
+      dz1[m] = s * gelu_do(z1[m]);
+    }
+    for (int j = 0; j < MNO; ++j) {
+      grad[MB3 + j] += dy[j];
+      for (int k = 0; k < MNH; ++k) grad[MW3 + j * MNH + k] += dyb[j] * h2[k];
+    }
+    for (int m = 0; m < MNH; ++m) {
+      grad[MB2 + m] += dz2[m];
+      for (int k = 0; k < MNH; ++k) grad[MW2 + m * MNH + k] += dz2b[m] * h1[k];
+      grad[MB1 + m] += dz1[m];
+      const double d1 = bf16_round(dz1[m], bf16);
+      for (int k = 0; k < MNI; ++k) grad[MW1 + m * MNI + k] += d1 * bf16_round(x[k], bf16);
+    }
+  }
+  return loss / ((double)n * MNO);
+}

@@ -140,4 +140,11 @@ void og_bouss_step(const og_grid *g, int ord, double dt, double nu, double kappa
 /* analysis/trunc3D.fpp: spectral truncation of c (grid g) onto the smaller grid t */
 void og_truncate(const og_grid *g, const ocplx *c, const og_grid *t, ocplx *out);
 
+/* The closure network of the learned LES (no counterpart in GHOST):
+ * 10 -> 64 -> 64 -> 6, GELU, parameters W1[64][10], b1[64], W2[64][64],
+ * b2[64], W3[6][64], b3[6] (5254 floats).  bf16 != 0 rounds every matrix
+ * product operand to bfloat16 as the matrix cores do; sums in double. */
+void og_mlp_forward(const float *P, long n, const float *X, int bf16, double *Y);
+double og_mlp_grad(const float *P, long n, const float *X, const float *T, int bf16, double *grad);
+
 #endif

@@ -97,6 +97,9 @@ def _declare(L):
                                   C.c_double, C.c_int, C.c_int, _P]
     L.og_hd_step_tracers.argtypes = [_G, C.c_int, C.c_double, C.c_double] + [_P] * 6 + \
         [C.c_long, _P, _P, _P, C.c_int, C.c_int]
+    L.og_mlp_forward.argtypes = [_P, C.c_long, _P, C.c_int, _P]
+    L.og_mlp_grad.restype = C.c_double
+    L.og_mlp_grad.argtypes = [_P, C.c_long, _P, _P, C.c_int, _P]
 
 
 def _p(a):
@@ -330,3 +333,26 @@ class Grid:
         fp = [None] * 3 if f is None else list(map(_p, f))
         mp = [None] * 3 if m is None else list(map(_p, m))
         lib().og_mhd_step(self._g, ord, dt, nu, mu, *map(_p, v), *map(_p, a), *fp, *mp)
+
+
+# ---- the closure network of the learned LES (no GHOST counterpart) ----------
+MLP_NPAR = 5254
+
+
+def mlp_forward(params, X, bf16=True):
+    """Outputs (n x 6, double) of the 10-64-64-6 GELU network for inputs X (n x 10)."""
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.zeros((X.shape[0], 6))
+    lib().og_mlp_forward(_p(P), X.shape[0], _p(X), int(bool(bf16)), _p(Y))
+    return Y
+
+
+def mlp_grad(params, X, T, bf16=True):
+    """(loss, gradient) of the mean squared error over the n x 6 outputs."""
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    g = np.zeros(MLP_NPAR)
+    loss = lib().og_mlp_grad(_p(P), X.shape[0], _p(X), _p(T), int(bool(bf16)), _p(g))
+    return loss, g
