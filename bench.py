@@ -17,6 +17,9 @@ Prints one JSON line on stdout.  Everything else goes to stderr.
 --workload mhd measures the MHD configuration instead (512x512x256 box with
 Lz = pi, nu = eta = 3e-4, dt = 2e-4, velocity and vector potential from the
 k^4 exp IC at unit energy ratio, mechanical forcing in 2 <= |k| <= 3).
+--workload les measures the learned-closure LES: the 512^3 DNS above (without
+tracers) trains a 10-64-64-6 closure on its own filtered states (kc = 42,
+128^3), and a 2000-step 128^3 LES is judged against the filtered DNS spectrum.
 --workload ensemble measures the batched ensemble: 96 unforced HD runs, 24
 each at 32^3, 48^3, 64^3 and 96^3, k^4 exp(-2 (k/k0)^2) ICs with k0 drawn
 uniformly in [2, 4], nu set so that k_max eta = 1.5 at the start, RK2.
@@ -271,6 +274,72 @@ def run_ensemble(args, torch):
     return out
 
 
+# learned-closure LES: the 512^3 DNS of the HD workload (no tracers) filtered
+# at kc = 42 onto a 128^3 LES that runs 2000 steps of twice the DNS step
+LES_N, LES_KC, LES_SPIN, LES_WINDOW = 128, 42, 4000, 4000
+
+
+def run_les(args, torch):
+    from ghost_amd.les import Experiment
+    N = args.n
+    ex = Experiment(nfine=N, nles=LES_N, kc=LES_KC, nu=NU, dt=DT, spin=LES_SPIN,
+                    snaps=[LES_SPIN - 1000, LES_SPIN - 500, LES_SPIN], window=LES_WINDOW, ratio=2, samples=8,
+                    f0=F0, fstep=FSTEP, k0=K0, u0=U0, ic_seed=IC_SEED, f_seed=F_SEED, log=log)
+    ex.run_dns()
+    ex.fit(steps=3000, batch=65536, lr=1e-3)
+    with_c = ex.run_les(closure=True)
+    without = ex.run_les(closure=False)
+    # the closure's inference kernel alone on the LES grid, for its roofline
+    import math as _m
+    from ghost_amd import les as L
+    grad = L.velocity_gradients(ex.small, ex.v_start)
+    tau = [ex.small.rzeros() for _ in range(6)]
+    ex.mlp.stress(grad, _m.pi / LES_KC, tau)
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        ex.mlp.stress(grad, _m.pi / LES_KC, tau)
+    e1.record()
+    torch.cuda.synchronize()
+    t_mlp = e0.elapsed_time(e1) / reps * 1e-3
+    npts = LES_N ** 3
+    mlp_bytes = npts * (9 + 6) * 8.0
+    mlp_flops = npts * 2.0 * (10 * 64 + 64 * 64 + 64 * 6)
+    speedup = ex.t_dns / with_c["seconds"]
+    within = with_c["finite"] and with_c["spectrum_error_mean"] <= 0.05
+    log(f"speedup {speedup:.1f}x over the DNS window; spectrum error {with_c['spectrum_error_mean']:.4f} "
+        f"(no closure {without['spectrum_error_mean']:.4f}); closure kernel {t_mlp * 1e3:.3f} ms, "
+        f"{mlp_bytes / t_mlp / 1e9:.0f} GB/s, {mlp_flops / t_mlp / 1e12:.1f} TFLOP/s")
+    return {
+        "metric": "learned-LES speedup at <=5% spectrum err (128^3 LES vs 512^3 FP64 DNS)",
+        "value": round(speedup if within else 1.0, 3), "unit": "x (DNS GPU-seconds / LES GPU-seconds)",
+        "n_gpus": 1, "steps": with_c["steps"], "warmup": 0,
+        "ms_per_step": round(1e3 * with_c["seconds"] / with_c["steps"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (closure: bf16 operands, f32 accumulate)",
+        "data": "synthetic (the 512^3 DNS of the HD workload generates its own training states)",
+        "config": {"workload": f"HD3D {N}^3 FP64 DNS (nu={NU}, dt={DT}, forcing f0={F0} in 2<=|k|<=3, "
+                               f"phase shift every {FSTEP} steps) spun up {LES_SPIN} steps; sharp filter kc={LES_KC} "
+                               f"and truncation onto {LES_N}^3; 10-64-64-6 GELU closure trained on the filtered states "
+                               f"at steps {LES_SPIN - 1000}, {LES_SPIN - 500}, {LES_SPIN}; LES {with_c['steps']} steps "
+                               f"of dt={2 * DT} against the DNS's next {LES_WINDOW} steps",
+                   "grid_dns": N, "grid_les": LES_N, "kc": LES_KC},
+        "within_5pct": bool(within),
+        "fallback": None if within else "spectrum error above 5%: the plain DNS is the result (speedup 1)",
+        "les_closure": with_c, "les_no_closure": without,
+        "dns_window_seconds": round(ex.t_dns, 3), "dns_spinup_seconds": round(ex.t_spin, 3),
+        "training": {"points": ex.n_train, "steps": len(ex.losses), "seconds": round(ex.t_train, 3),
+                     "loss_first": float(ex.losses[0]), "loss_last": float(ex.losses[-1]),
+                     "dissipation_scale": round(float(ex.scale), 4)},
+        "raw_speedup": round(speedup, 3),
+        "roofline": {"bound": "hbm", "kernel": "closure inference (k_mlp_fields)",
+                     "achieved": round(mlp_bytes / t_mlp / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(mlp_bytes / t_mlp / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "mfma_tflops": round(mlp_flops / t_mlp / 1e12, 2), "mfma_peak_tflops_bf16": 2500.0},
+    }
+
+
 def cpu_baseline(n, threads, npart):
     """Time one RK substep and one tracer stage of the same workload with the
     CPU oracle (test infrastructure)."""
@@ -307,7 +376,7 @@ def main():
     ap.add_argument("--tracers", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=("hd", "mhd", "ensemble"), default="hd")
+    ap.add_argument("--workload", choices=("hd", "mhd", "ensemble", "les"), default="hd")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel for the rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE scale factors) and exit")
@@ -346,6 +415,9 @@ def main():
         return
     if args.workload == "ensemble":
         print(json.dumps(run_ensemble(args, torch)), flush=True)
+        return
+    if args.workload == "les":
+        print(json.dumps(run_les(args, torch)), flush=True)
         return
 
     g = Grid(n)

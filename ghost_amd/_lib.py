@@ -158,6 +158,9 @@ def _declare(L):
         "ghx_mlp_train": (_I, [_P, C.c_longlong, _P, _P, C.c_longlong, _I, C.c_float, C.c_ulonglong,
                                C.POINTER(C.c_float)]),
         "ghx_mlp_stress": (_I, [_P, C.c_longlong, _PP, _D, _PP]),
+        "ghx_velocity_gradients": (_I, [_P, _PP, _PP]),
+        "ghx_sgs_targets": (_I, [_P, _P, _PP, _D, _PP, _PP]),
+        "ghx_hd_set_sgs": (_I, [_P, _P, _D]),
         "ghx_prof_enable": (_I, [_I]),
         "ghx_prof_reset": (_I, []),
         "ghx_prof_read": (_I, [_I, C.POINTER(C.c_char_p), C.POINTER(_I), _DP, _DP]),

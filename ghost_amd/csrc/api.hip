@@ -1115,6 +1115,11 @@ struct ghx_hd {
   bool forced = false;
   bool clean = false;  // v known to be zero outside the kept modes
   ghx_tracers *tr = nullptr;  // Lagrangian tracers advanced with the flow
+  // learned subgrid closure (LES): network, filter width and its buffers
+  ghx_mlp *sgs = nullptr;
+  double sgs_delta = 0.0;
+  void *sgs_mem = nullptr;  // grad 9 + tau 6 real, tau 6 + force 3 spectral
+  uint8_t *sgs_fline = nullptr;
 };
 
 static int tracers_begin_step(ghx_tracers *t);
@@ -1153,6 +1158,8 @@ extern "C" int ghx_hd_destroy(ghx_hd *s) {
   if (!s) return GHX_OK;
   (void)hipFree(s->mem);
   (void)hipFree(s->fline);
+  if (s->sgs_mem) (void)hipFree(s->sgs_mem);
+  if (s->sgs_fline) (void)hipFree(s->sgs_fline);
   delete s;
   return GHX_OK;
 }
@@ -1187,6 +1194,136 @@ extern "C" int ghx_hd_touch(ghx_hd *s) {
   return GHX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// learned LES (no counterpart in GHOST)
+// ---------------------------------------------------------------------------
+// Physical velocity gradients d u_i / d x_j (i major) of a spectral state:
+// derivk3 scaled by 1/(nx ny nz), then c2r of the nine fields.
+static int velocity_gradients(ghx_grid *g, const void *const *v, cplx *tmp9, double *const *grad9) {
+  const double s = 1.0 / ((double)g->gk.nx * (double)g->gk.ny * (double)g->gk.nz);
+  const void *src[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      cplx *d = tmp9 + (size_t)(3 * i + j) * g->nc;
+      HIPCHK(ghx::op_derivk3(g->gk, g->stream, C_(v[i]), d, j + 1));
+      HIPCHK(ghx::op_scale(g->gk, g->stream, d, d, s));
+      src[3 * i + j] = d;
+    }
+  return do_c2r(&g->plan, 9, src, grad9);
+}
+
+extern "C" int ghx_velocity_gradients(ghx_grid *g, const void *const *v, double *const *grad9) {
+  ARGCHK(g && v && grad9 && v[0] && v[1] && v[2], "bad velocity_gradients arguments");
+  for (int k = 0; k < 9; ++k) ARGCHK(grad9[k], "null gradient field");
+  int rc = grid_ctmp(g, 9);
+  if (rc) return rc;
+  return velocity_gradients(g, v, g->ctmp, grad9);
+}
+
+// Training pairs of the closure from a fine-grid state v (spectral, GHOST's
+// normalisation): ub = sharp(v, kc); on the small grid t, vsmall = the
+// truncation of ub in t's normalisation (the LES state) and tau6 = the
+// physical subgrid stress F(u_i u_j) - F(ub_i ub_j) sampled on t's points,
+// F = sharp(kc) then trunc3D's truncation onto t.
+extern "C" int ghx_sgs_targets(ghx_grid *g, ghx_grid *t, const void *const *v, double kc, double *const *tau6,
+                               void *const *vsmall) {
+  ARGCHK(g && t && v && tau6 && vsmall && kc > 0, "bad sgs_targets arguments");
+  const GridK &a = g->gk, &b = t->gk;
+  if (b.nx > a.nx || b.ny > a.ny || b.nz > a.nz)
+    return set_err(GHX_ERR_SIZE, "the LES grid must not be larger than the fine grid");
+  int rc = grid_ctmp(g, 6);
+  if (!rc) rc = grid_rtmp(g, 12);
+  if (!rc) rc = grid_ctmp(t, 6);
+  if (rc) return rc;
+  const size_t nc = g->nc, nr = g->nr;
+  cplx *ub[3];
+  double *r[12];
+  for (int k = 0; k < 3; ++k) ub[k] = g->ctmp + k * nc;
+  for (int k = 0; k < 12; ++k) r[k] = g->rtmp + k * nr;
+  for (int k = 0; k < 3; ++k) HIPCHK(ghx::op_sharp(a, g->stream, C_(v[k]), ub[k], kc * kc));
+  const void *srcv[3] = {v[0], v[1], v[2]}, *srcb[3] = {ub[0], ub[1], ub[2]};
+  if ((rc = do_c2r(&g->plan, 3, srcv, r))) return rc;
+  if ((rc = do_c2r(&g->plan, 3, srcb, r + 3))) return rc;
+  // the LES state: truncation (which scales by 1/(nx ny nz)) times t's size
+  const double st = (double)b.nx * (double)b.ny * (double)b.nz;
+  for (int k = 0; k < 3; ++k) {
+    HIPCHK(ghx::op_truncate(a, b, g->stream, ub[k], M_(vsmall[k])));
+    HIPCHK(ghx::op_scale(b, g->stream, M_(vsmall[k]), C_(vsmall[k]), st));
+  }
+  const double s = 1.0 / ((double)a.nx * (double)a.ny * (double)a.nz);
+  const double *u[3] = {r[0], r[1], r[2]}, *uf[3] = {r[3], r[4], r[5]};
+  HIPCHK(ghx::op_sgs_products(a, g->stream, u, uf, s * s, r + 6));
+  const double *srcp[6] = {r[6], r[7], r[8], r[9], r[10], r[11]};
+  void *pc[6];
+  for (int k = 0; k < 6; ++k) pc[k] = g->ctmp + k * nc;
+  if ((rc = do_r2c(&g->plan, 6, srcp, pc))) return rc;
+  const void *tc[6];
+  for (int k = 0; k < 6; ++k) {
+    cplx *c = M_(pc[k]);
+    HIPCHK(ghx::op_sharp(a, g->stream, c, c, kc * kc));
+    cplx *d = t->ctmp + k * t->nc;
+    HIPCHK(ghx::op_truncate(a, b, g->stream, c, d));
+    tc[k] = d;
+  }
+  HIPCHK(hipStreamSynchronize(g->stream));  // t's transforms may run on another stream
+  if ((rc = do_c2r(&t->plan, 6, tc, tau6))) return rc;
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return GHX_OK;
+}
+
+// The subgrid forcing of one substep: gradients of vin, tau from the network,
+// its transform, and f + P(-div tau) into the solver's buffers.
+static int sgs_force(ghx_hd *s, const cplx *const *vin, const cplx **fs) {
+  ghx_grid *g = s->g;
+  const size_t nc = g->nc, nr = g->nr;
+  double *gr = static_cast<double *>(s->sgs_mem);
+  double *tr = gr + 9 * nr;
+  cplx *tc = reinterpret_cast<cplx *>(tr + 6 * nr), *fc = tc + 6 * nc;
+  double *grad9[9], *tau6[6];
+  for (int k = 0; k < 9; ++k) grad9[k] = gr + k * nr;
+  for (int k = 0; k < 6; ++k) tau6[k] = tr + k * nr;
+  int rc = grid_ctmp(g, 9);
+  if (rc) return rc;
+  const void *v[3] = {vin[0], vin[1], vin[2]};
+  if ((rc = velocity_gradients(g, v, g->ctmp, grad9))) return rc;
+  if ((rc = ghx_mlp_set_stream(s->sgs, g->stream))) return rc;
+  const double *cg[9];
+  for (int k = 0; k < 9; ++k) cg[k] = grad9[k];
+  if ((rc = ghx_mlp_stress(s->sgs, (long long)nr, cg, s->sgs_delta, tau6))) return rc;
+  const double *ct[6];
+  void *tcv[6];
+  for (int k = 0; k < 6; ++k) { ct[k] = tau6[k]; tcv[k] = tc + k * nc; }
+  if ((rc = do_r2c(&g->plan, 6, ct, tcv))) return rc;
+  const cplx *tau[6];
+  for (int k = 0; k < 6; ++k) tau[k] = tc + k * nc;
+  const cplx *f0[3] = {s->f[0], s->f[1], s->f[2]};
+  cplx *out[3] = {fc, fc + nc, fc + 2 * nc};
+  HIPCHK(ghx::op_sgs_force(g->gk, g->stream, tau, s->forced ? f0 : nullptr, out));
+  for (int k = 0; k < 3; ++k) fs[k] = out[k];
+  return GHX_OK;
+}
+
+// Attach the closure (m = NULL detaches): before every substep the forcing
+// becomes f + P(-div tau) with tau = m(grad u, delta).
+extern "C" int ghx_hd_set_sgs(ghx_hd *s, ghx_mlp *m, double delta) {
+  ARGCHK(s, "null solver");
+  if (!m) {
+    s->sgs = nullptr;
+    return GHX_OK;
+  }
+  ARGCHK(delta > 0, "filter width must be positive");
+  ghx_grid *g = s->g;
+  if (!s->sgs_mem) {
+    HIPCHK(hipMalloc(&s->sgs_mem, 15 * g->nr * sizeof(double) + 9 * g->nc * sizeof(cplx)));
+    const size_t nl = (size_t)g->gk.nxh * g->gk.ny;
+    HIPCHK(hipMalloc(&s->sgs_fline, nl));
+    HIPCHK(hipMemset(s->sgs_fline, 1, nl));  // the subgrid force reaches every line of the dealias set
+  }
+  s->sgs = m;
+  s->sgs_delta = delta;
+  return GHX_OK;
+}
+
 static int hd_step_once(ghx_hd *s) {
   if (!s->clean) {
     double n = 0;
@@ -1207,7 +1344,16 @@ static int hd_step_once(ghx_hd *s) {
     const bool first = (o == s->ord), last = (o == 1);
     const cplx *const *vin = first ? v0 : v1;
     cplx *const *out = last ? s->v : s->v1;
-    int rc = hd_substep(s->g, s->q, o, s->dt, s->nu, vin, v0, f, s->fline, out, prune);
+    const cplx *const *fo = f;
+    const uint8_t *fl = s->fline;
+    const cplx *fs[3];
+    if (s->sgs) {  // the forcing of this substep is f + P(-div tau(vin))
+      int rc = sgs_force(s, vin, fs);
+      if (rc) return rc;
+      fo = fs;
+      fl = s->sgs_fline;
+    }
+    int rc = hd_substep(s->g, s->q, o, s->dt, s->nu, vin, v0, fo, fl, out, prune);
     if (rc) return rc;
     // main3D.fpp:3233-3267: the tracers take a stage on the velocity this
     // substep produced, which is dealiased whatever the input was

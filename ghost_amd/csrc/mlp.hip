@@ -49,10 +49,19 @@ DEV f4 mma32(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf1
 DEV f4 mma16(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 DEV short bfbits(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 
-// GELU(x) = x Phi(x) with the exact normal distribution function
-DEV float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU(x) = x Phi(x), Phi(x) = (1 + erf(x/sqrt 2))/2.  erf from Abramowitz &
+// Stegun 7.1.26 (absolute error <= 1.5e-7, far below the bfloat16 rounding
+// the activations then receive): erf(|x|) = 1 - t (a1 + t (a2 + ...)) e^-x^2,
+// t = 1/(1 + p |x|).
+DEV float erf_as(float x) {
+  const float ax = fabsf(x), t = __frcp_rn(1.0f + 0.3275911f * ax);
+  const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  const float e = 1.0f - poly * __expf(-ax * ax);
+  return copysignf(e, x);
+}
+DEV float gelu(float x) { return 0.5f * x * (1.0f + erf_as(x * 0.70710678118654752f)); }
 DEV float gelu_d(float x) {
-  return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
+  return 0.5f * (1.0f + erf_as(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
 }
 
 // unit held in k-slot (s, q, r) of the 64-wide layers
@@ -164,19 +173,33 @@ __global__ __launch_bounds__(BLOCK) void k_mlp_fields(Ptr9 g, double delta, cons
   load_fwd(W, P, l);
   const long long wave = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6), nw = (long long)gridDim.x * WAVES;
   const long long tiles = (n + 15) / 16;
+  // the features of lane (point c, q): fields 8q .. 8q+7 (lanes q = 0, 1)
+  auto fetch = [&](long long t, double (&raw)[8]) {
+    const long long p = t * 16 + c;
+    const long long pp = p < n ? p : 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int f = 8 * q + r;
+      raw[r] = f < 9 ? g.p[f][pp] : 0.0;
+    }
+  };
+  double cur[8], nxt[8];
+  if (wave < tiles) fetch(wave, cur);
   for (long long t = wave; t < tiles; t += nw) {
     const long long p = t * 16 + c;
     const bool ok = p < n;
-    const long long pp = ok ? p : 0;
+    if (t + nw < tiles) fetch(t + nw, nxt);  // next tile's loads in flight
     bf8 xb;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int f = 8 * q + r;
       float v = 0.0f;
-      if (f < 9) v = (float)(g.p[f][pp] - (double)nm.mx[f]) / nm.sx[f];
+      if (f < 9) v = (float)(cur[r] - (double)nm.mx[f]) / nm.sx[f];
       else if (f == 9) v = (float)(delta - (double)nm.mx[9]) / nm.sx[9];
       xb[r] = (__bf16)v;
     }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) cur[r] = nxt[r];
     Acts a;
     forward(W, xb, a);
     if (ok) {
@@ -509,6 +532,13 @@ struct ghx_mlp {
   hipStream_t stream = nullptr;
 };
 
+// inference grids: enough waves to hide the feature loads (several per SIMD),
+// each wave taking at least two tiles so its weight loads are amortised
+static unsigned infer_blocks(long long n) {
+  const long long tiles = (n + 15) / 16, b = (tiles + 2 * WAVES - 1) / (2 * WAVES);
+  return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
 static int check_ptr(const void *p, const char *what) {
   return p ? GHX_OK : fail(GHX_ERR_ARG, std::string("mlp: null ") + what);
 }
@@ -644,7 +674,7 @@ extern "C" int ghx_mlp_pack(ghx_mlp *m, long long n, const double *const *grad9,
 extern "C" int ghx_mlp_forward(ghx_mlp *m, long long n, const float *X, float *Y) {
   if (!m || !X || !Y || n < 0) return fail(GHX_ERR_ARG, "mlp_forward: bad arguments");
   if (n == 0) return GHX_OK;
-  hipLaunchKernelGGL(k_mlp_forward, dim3(GRID), dim3(BLOCK), 0, m->stream, X, m->P, Y, n);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(infer_blocks(n)), dim3(BLOCK), 0, m->stream, X, m->P, Y, n);
   MHIP(hipGetLastError());
   return GHX_OK;
 }
@@ -677,12 +707,16 @@ extern "C" int ghx_mlp_train(ghx_mlp *m, long long n, const float *X, const floa
   if (!m || !X || !T || n < 1 || batch < 1 || steps < 0 || !(lr > 0.0f)) return fail(GHX_ERR_ARG, "mlp_train: bad arguments");
   if (batch > n) batch = n;
   uint64_t s = seed;
-  const uint64_t mul = next_prime_above((uint64_t)n) % (uint64_t)n;  // a prime above n is coprime to n
+  // a prime multiplier between n/2 and n never divides n, so i -> mul i + add
+  // (mod n) is a permutation; near the golden section of n it scatters
+  // consecutive batch entries over the whole data set
+  uint64_t mul = n > 2 ? next_prime_above((uint64_t)(0.6180339887498949 * (double)n)) : 1;
+  if (mul >= (uint64_t)n) mul = 1;
   std::vector<float> lh(steps > 0 ? steps : 1);
   unsigned long long base = 0;
   const unsigned long long add = splitmix(s) % (uint64_t)n;
   for (int it = 0; it < steps; ++it) {
-    Perm pm{mul == 0 ? 1 : mul, add, base};
+    Perm pm{mul, add, base};
     int rc = grad_batch(m, n, X, T, batch, pm);
     if (rc) return rc;
     ++m->step;
@@ -703,8 +737,8 @@ extern "C" int ghx_mlp_stress(ghx_mlp *m, long long n, const double *const *grad
   for (int i = 0; i < 9; ++i) if (!grad9[i]) return fail(GHX_ERR_ARG, "mlp_stress: null gradient field");
   for (int j = 0; j < 6; ++j) if (!tau6[j]) return fail(GHX_ERR_ARG, "mlp_stress: null stress field");
   if (n == 0) return GHX_OK;
-  hipLaunchKernelGGL(k_mlp_fields, dim3(GRID), dim3(BLOCK), 0, m->stream, ptr9(grad9), delta, m->P, m->norm,
-                     ptr6(tau6), n);
+  hipLaunchKernelGGL(k_mlp_fields, dim3(infer_blocks(n)), dim3(BLOCK), 0, m->stream, ptr9(grad9), delta, m->P,
+                     m->norm, ptr6(tau6), n);
   MHIP(hipGetLastError());
   return GHX_OK;
 }

@@ -18,6 +18,12 @@ hipError_t op_scale(const GridK &g, hipStream_t st, cplx *a, const cplx *src, do
 hipError_t op_normalize_apply(const GridK &g, hipStream_t st, cplx *x, cplx *y, cplx *z, double f0, double e);
 hipError_t op_sharp(const GridK &g, hipStream_t st, const cplx *a, cplx *b, double kc2);
 // trunc3D's spectral truncation of c (grid g) onto the smaller grid t
+// learned LES: subgrid stress products on the fine grid and the projected
+// subgrid force (f0 may be null)
+hipError_t op_sgs_products(const GridK &g, hipStream_t st, const double *const *u, const double *const *ub, double s2,
+                           double *const *out);
+hipError_t op_sgs_force(const GridK &g, hipStream_t st, const cplx *const *tau, const cplx *const *f0,
+                        cplx *const *out);
 hipError_t op_truncate(const GridK &g, const GridK &t, hipStream_t st, const cplx *c, cplx *out);
 hipError_t op_zero(hipStream_t st, void *p, size_t bytes);
 // real-space cross product r7=(r2 r6-r5 r3)tmp ... in place (prodre3's loop)

@@ -167,6 +167,59 @@ __global__ void k_sharp(GridK g, const cplx *a, cplx *b, double kc2) {
   }
 }
 
+// Subgrid stress on the fine grid (learned LES, no counterpart in GHOST):
+// p_ij = (u_i u_j - ub_i ub_j) s^2 from the unnormalised c2r outputs of the
+// velocity u and of its filtered part ub (s = 1/(nx ny nz)); components in
+// the order xx, yy, zz, xy, xz, yz.
+struct R6 {
+  double *p[6];
+};
+struct CR3 {
+  const double *p[3];
+};
+__global__ void k_sgs_products(size_t n, CR3 u, CR3 ub, double s2, R6 out) {
+  constexpr int I[6] = {0, 1, 2, 0, 0, 1}, J[6] = {0, 1, 2, 1, 2, 2};
+  GRID_STRIDE(m, n) {
+    const double a[3] = {u.p[0][m], u.p[1][m], u.p[2][m]};
+    const double b[3] = {ub.p[0][m], ub.p[1][m], ub.p[2][m]};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) out.p[c][m] = (a[I[c]] * a[J[c]] - b[I[c]] * b[J[c]]) * s2;
+  }
+}
+
+// Subgrid force of the LES: out = f0 + P(-div tau) on the kept modes
+// (tiny <= kn2 <= kmax), f0 elsewhere (zero without f0).  tau: six spectral
+// fields xx, yy, zz, xy, xz, yz; P removes the component along k, as the
+// projections of nonlhd3 do (the pressure of the subgrid force).
+struct C6 {
+  const cplx *p[6];
+};
+__global__ void k_sgs_force(GridK g, C6 t, const cplx *f0x, const cplx *f0y, const cplx *f0z, cplx *ox, cplx *oy,
+                            cplx *oz) {
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    const cplx z = c_make(0.0, 0.0);
+    cplx fx = f0x ? f0x[m] : z, fy = f0y ? f0y[m] : z, fz = f0z ? f0z[m] : z;
+    const double kn2 = kn2_of(g, q.i, q.j, q.k);
+    if (kn2 <= g.kmax && kn2 >= g.tiny) {
+      const double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky), kz = wavenum(q.k, g.nz, g.dkz);
+      const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
+      const cplx txx = t.p[0][m], tyy = t.p[1][m], tzz = t.p[2][m], txy = t.p[3][m], txz = t.p[4][m], tyz = t.p[5][m];
+      // -i k_j tau_ij
+      const cplx sx = c_make(-(-kx * txx.y - ky * txy.y - kz * txz.y), -(kx * txx.x + ky * txy.x + kz * txz.x));
+      const cplx sy = c_make(-(-kx * txy.y - ky * tyy.y - kz * tyz.y), -(kx * txy.x + ky * tyy.x + kz * tyz.x));
+      const cplx sz = c_make(-(-kx * txz.y - ky * tyz.y - kz * tzz.y), -(kx * txz.x + ky * tyz.x + kz * tzz.x));
+      const cplx dot = c_add(c_add(c_scale(sx, kx), c_scale(sy, ky)), c_scale(sz, kz));
+      fx = c_add(fx, c_sub(sx, c_scale(dot, kx / kk2)));
+      fy = c_add(fy, c_sub(sy, c_scale(dot, ky / kk2)));
+      fz = c_add(fz, c_sub(sz, c_scale(dot, kz / kk2)));
+    }
+    ox[m] = fx;
+    oy[m] = fy;
+    oz[m] = fz;
+  }
+}
+
 // analysis/trunc3D.fpp:190-320: out (grid t, smaller) = the modes of c
 // (grid g) with kx^2/nxt^2 + ky^2/nyt^2 + kz^2/nzt^2 <= 1/9, scaled by
 // 1/(nx ny nz), ky and kz re-indexed onto t's axes.
@@ -403,6 +456,21 @@ hipError_t op_truncate(const GridK &g, const GridK &t, hipStream_t st, const cpl
   LAUNCH(k_truncate, ncplx(t), st, g, t, c, out);
 }
 
+hipError_t op_sgs_products(const GridK &g, hipStream_t st, const double *const *u, const double *const *ub, double s2,
+                           double *const *out) {
+  CR3 a, b;
+  R6 o;
+  for (int c = 0; c < 3; ++c) { a.p[c] = u[c]; b.p[c] = ub[c]; }
+  for (int c = 0; c < 6; ++c) o.p[c] = out[c];
+  LAUNCH(k_sgs_products, nreal(g), st, nreal(g), a, b, s2, o);
+}
+hipError_t op_sgs_force(const GridK &g, hipStream_t st, const cplx *const *tau, const cplx *const *f0,
+                        cplx *const *out) {
+  C6 t;
+  for (int c = 0; c < 6; ++c) t.p[c] = tau[c];
+  LAUNCH(k_sgs_force, ncplx(g), st, g, t, f0 ? f0[0] : nullptr, f0 ? f0[1] : nullptr, f0 ? f0[2] : nullptr, out[0],
+         out[1], out[2]);
+}
 hipError_t op_sharp(const GridK &g, hipStream_t st, const cplx *a, cplx *b, double kc2) {
   LAUNCH(k_sharp, ncplx(g), st, g, a, b, kc2);
 }

@@ -1,0 +1,283 @@
+"""Learned-closure LES on the gfx950 path (no counterpart in GHOST): training
+pairs from sharp-spectrally filtered DNS states that the GPU DNS generates,
+training of the closure network on the GPU, and the LES itself, an HD solver
+on the coarse grid whose forcing gains the projected divergence of the
+network's subgrid stress before every substep.
+
+The filter is GHOST's sharp cut-off (pseudospec3D_filt.f90:200-241,
+delta = pi/kc) followed by trunc3D's truncation onto the LES grid
+(analysis/trunc3D.fpp:190-320).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .closure import MLP
+from .pseudospec import Grid
+
+
+def _pp(ts):
+    arr = (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+    return C.cast(arr, _lib._PP), arr
+
+
+def velocity_gradients(grid: Grid, v):
+    """Nine real fields d u_i / d x_j (i major) of the spectral state v."""
+    out = [grid.rzeros() for _ in range(9)]
+    vp, ka = _pp([x.contiguous() for x in v])
+    gp, kb = _pp(out)
+    torch.cuda.synchronize()
+    _lib.check(_lib.load().ghx_velocity_gradients(grid._h, vp, gp))
+    torch.cuda.synchronize()
+    return out
+
+
+def sgs_targets(fine: Grid, les: Grid, v, kc):
+    """(tau6, vsmall): the physical subgrid stress on the LES grid's points
+    (xx, yy, zz, xy, xz, yz) and the filtered state in the LES grid's
+    normalisation, from the fine-grid spectral state v."""
+    tau = [les.rzeros() for _ in range(6)]
+    vs = [les.czeros() for _ in range(3)]
+    vp, ka = _pp([x.contiguous() for x in v])
+    tp, kb = _pp(tau)
+    sp, kc_ = _pp(vs)
+    torch.cuda.synchronize()
+    _lib.check(_lib.load().ghx_sgs_targets(fine._h, les._h, vp, float(kc), tp, sp))
+    torch.cuda.synchronize()
+    return tau, vs
+
+
+def filter_forcing(fine: Grid, les: Grid, f, kc):
+    """The fine-grid forcing carried to the LES grid: sharp(kc), truncated,
+    in the LES grid's normalisation."""
+    out = []
+    scale = float(les.nx * les.ny * les.nz)
+    for x in f:
+        b = fine.czeros()
+        fine.sharp(x, b, math.pi / kc)
+        t = les.czeros()
+        fine.truncate(b, les, out=t)
+        out.append(t * scale)
+    return out
+
+
+def spectrum(grid: Grid, v):
+    """E(k) of shells k = 1..nmax/2+1 at indices k - 1 (GHOST's spectrumc, kin = 1)"""
+    e, _ = grid.spectrumc(*v, kin=1)
+    return np.asarray(e)
+
+
+def spectrum_error(e, eref, kmax):
+    """Mean over the shells k = 1..kmax of |E(k) - E_ref(k)| / E_ref(k)."""
+    i = np.arange(0, int(kmax))
+    return float(np.mean(np.abs(e[i] - eref[i]) / eref[i]))
+
+
+def dissipation(grad, tau):
+    """Mean subgrid dissipation -<tau_ij S_ij> with S_ij = (d_j u_i + d_i u_j)/2
+    (stress components xx, yy, zz, xy, xz, yz; gradients i major)."""
+    s = lambda i, j: 0.5 * (grad[3 * i + j] + grad[3 * j + i])
+    pairs = [(0, 0, 1.0), (1, 1, 1.0), (2, 2, 1.0), (0, 1, 2.0), (0, 2, 2.0), (1, 2, 2.0)]
+    tot = 0.0
+    for c, (i, j, w) in enumerate(pairs):
+        tot += w * float(torch.sum(tau[c] * s(i, j)))
+    return -tot / grad[0].numel()
+
+
+class TrainingSet:
+    """Normalised features and targets of the closure on the LES grid,
+    accumulated from several filtered DNS states."""
+
+    def __init__(self, les: Grid, kc):
+        self.les, self.kc, self.delta = les, kc, math.pi / kc
+        self.snaps = []  # (grad9, tau6) per state
+
+    def add(self, fine: Grid, v):
+        tau, vs = sgs_targets(fine, self.les, v, self.kc)
+        grad = velocity_gradients(self.les, vs)
+        self.snaps.append((grad, tau))
+        return vs
+
+    def arrays(self, mlp: MLP):
+        """Fit the network's normalisation on all states (the pooled moments of
+        the per-state fits), then pack every state: (X, T) as n x 10 and n x 6
+        FP32 device arrays."""
+        mu, m2 = [], []
+        for g, t in self.snaps:
+            mlp.fit_norm(g, self.delta, t)
+            nm = mlp.norm().astype(np.float64)
+            mean = np.concatenate([nm[0:10], nm[20:26]])
+            sd = np.concatenate([nm[10:20], nm[26:32]])
+            mu.append(mean)
+            m2.append(sd * sd + mean * mean)
+        mean = np.mean(mu, axis=0)
+        sd = np.sqrt(np.maximum(np.mean(m2, axis=0) - mean * mean, 0.0))
+        sd[sd == 0] = 1.0
+        sd[9] = 1.0  # the filter-width input: one value, centred
+        mlp.set_norm(np.concatenate([mean[0:10], sd[0:10], mean[10:16], sd[10:16]]).astype(np.float32))
+        xs, ts = [], []
+        for g, t in self.snaps:
+            X, T = mlp.pack(g, self.delta, t)
+            xs.append(X)
+            ts.append(T)
+        return torch.cat(xs), torch.cat(ts)
+
+    def match_dissipation(self, mlp: MLP):
+        """Scale the network's outputs so that its mean subgrid dissipation on
+        the training states equals the filtered DNS's: the regression towards
+        the mean that a weakly correlated target imposes on the fit shrinks
+        the predicted stress, and with it the energy drained at the cut-off.
+        Returns the factor applied."""
+        true_d = pred_d = 0.0
+        for g, t in self.snaps:
+            p = [torch.empty_like(x) for x in t]
+            mlp.stress(g, self.delta, p)
+            true_d += dissipation(g, t)
+            pred_d += dissipation(g, p)
+        if not (pred_d > 0.0 and true_d > 0.0):
+            return 1.0
+        a = true_d / pred_d
+        nm = mlp.norm()
+        nm[20:26] *= a
+        nm[26:32] *= a
+        mlp.set_norm(nm)
+        return a
+
+
+class Experiment:
+    """The a-posteriori comparison: a forced DNS on the fine grid (GHOST's
+    rand=1 forcing, phase shift every fstep steps) spins up, leaves filtered
+    training states behind, then runs a window while its filtered spectrum is
+    sampled; LES runs on the coarse grid over the same window from the
+    filtered state, with the trained closure and without it, with the same
+    forcing and phase schedule and ratio times the DNS time step.  Errors are
+    the mean over shells 1..kc of the relative error of the time-averaged
+    spectra (and of the spectra at the end of the window)."""
+
+    def __init__(self, nfine, nles, kc, nu, dt, spin, snaps, window, ratio=2, f0=0.4, fstep=20, k0=3.0, u0=1.0,
+                 ic_seed=1000, f_seed=2000, samples=8, log=print):
+        self.__dict__.update(dict(nfine=nfine, nles=nles, kc=kc, nu=nu, dt=dt, spin=spin, snaps=set(snaps),
+                                  window=window, ratio=ratio, f0=f0, fstep=fstep, k0=k0, u0=u0, ic_seed=ic_seed,
+                                  f_seed=f_seed, samples=samples, log=log))
+        assert window % (ratio * samples) == 0, "window must hold whole LES steps per sample"
+
+    def run_dns(self):
+        from .solvers import HDSolver
+        from .pseudospec import randu
+        import time
+        self.fine, self.small = Grid(self.nfine), Grid(self.nles)
+        s = HDSolver(self.fine, dt=self.dt, nu=self.nu, ord=2)
+        v, _ = self.fine.initialv_k4exp(self.k0, self.u0, self.ic_seed)
+        f, fseed = self.fine.initialfv_random(2.0, 3.0, self.f0, self.f_seed)
+        s.set_velocity(v)
+        s.set_forcing(f)
+        del v, f
+        st = {"timef": self.fstep, "seed": fseed}
+        self.phases = []
+
+        def step():
+            if st["timef"] == self.fstep:
+                st["timef"] = 0
+                r, st["seed"] = randu(st["seed"])
+                ph = 2 * math.pi * r
+                self.phases.append(ph)
+                s.phaseshift_forcing(complex(math.cos(ph), math.sin(ph)))
+            s.step(1)
+            st["timef"] += 1
+
+        self.train = TrainingSet(self.small, self.kc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for it in range(1, self.spin + 1):
+            step()
+            if it in self.snaps:
+                self.train.add(self.fine, s.velocity())
+        torch.cuda.synchronize()
+        self.t_spin = time.perf_counter() - t0
+        _, self.v_start = sgs_targets(self.fine, self.small, s.velocity(), self.kc)
+        self.f_start = filter_forcing(self.fine, self.small, s.forcing(), self.kc)
+        self.phase0, self.timef0 = len(self.phases), st["timef"]
+        every = self.window // self.samples
+        self.ref = []
+        t_sample = 0.0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for it in range(1, self.window + 1):
+            step()
+            if it % every == 0:
+                torch.cuda.synchronize()
+                ta = time.perf_counter()
+                _, vf = sgs_targets(self.fine, self.small, s.velocity(), self.kc)
+                self.ref.append(spectrum(self.small, vf))
+                torch.cuda.synchronize()
+                t_sample += time.perf_counter() - ta
+        torch.cuda.synchronize()
+        self.t_dns = time.perf_counter() - t0 - t_sample  # GPU-seconds of the DNS over the window
+        self.log(f"DNS {self.nfine}^3: spin-up {self.spin} steps {self.t_spin:.1f} s, window {self.window} steps "
+                 f"{self.t_dns:.2f} s")
+        del s
+        torch.cuda.empty_cache()
+
+    def fit(self, steps=3000, batch=65536, lr=1e-3, seed=7, match=True):
+        import time
+        self.mlp = MLP(seed)
+        X, T = self.train.arrays(self.mlp)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.losses = self.mlp.train(X, T, batch=min(batch, X.shape[0]), steps=steps, lr=lr, seed=seed)
+        torch.cuda.synchronize()
+        self.t_train = time.perf_counter() - t0
+        self.n_train = int(X.shape[0])
+        del X, T
+        self.scale = self.train.match_dissipation(self.mlp) if match else 1.0
+        self.log(f"closure trained on {self.n_train} points, {steps} Adam steps in {self.t_train:.2f} s: "
+                 f"loss {self.losses[0]:.4f} -> {self.losses[-1]:.4f}; dissipation-matching scale {self.scale:.3f}")
+
+    def run_les(self, closure=True):
+        from .solvers import HDSolver
+        import time
+        L = HDSolver(self.small, dt=self.dt * self.ratio, nu=self.nu, ord=2)
+        L.set_velocity(self.v_start)
+        L.set_forcing(self.f_start)
+        if closure:
+            L.set_sgs(self.mlp, math.pi / self.kc)
+        st = {"timef": self.timef0, "pi": self.phase0}
+        every = self.window // self.samples // self.ratio
+        spectra = []
+        t_sample = 0.0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for it in range(1, self.window // self.ratio + 1):
+            for _ in range(self.ratio):  # the DNS's phase shifts that fall in this step
+                if st["timef"] == self.fstep:
+                    st["timef"] = 0
+                    ph = self.phases[st["pi"]]
+                    st["pi"] += 1
+                    L.phaseshift_forcing(complex(math.cos(ph), math.sin(ph)))
+                st["timef"] += 1
+            L.step(1)
+            if it % every == 0:
+                torch.cuda.synchronize()
+                ta = time.perf_counter()
+                # the same sharp filter as the reference: the LES grid keeps
+                # modes up to its own dealias bound, slightly above kc
+                vel = [self.small.sharp(x, self.small.czeros(), math.pi / self.kc) for x in L.velocity()]
+                spectra.append(spectrum(self.small, vel))
+                t_sample += time.perf_counter() - ta
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0 - t_sample
+        ok = all(np.all(np.isfinite(e)) for e in spectra)
+        mean_err = spectrum_error(np.mean(spectra, axis=0), np.mean(self.ref, axis=0), self.kc) if ok else math.inf
+        end_err = spectrum_error(spectra[-1], self.ref[-1], self.kc) if ok else math.inf
+        name = "closure" if closure else "no closure"
+        self.log(f"LES {self.nles}^3 {name}: {self.window // self.ratio} steps {elapsed:.2f} s, spectrum error "
+                 f"(time average) {mean_err:.4f}, at the end {end_err:.4f}")
+        if ok:
+            ea, er = np.mean(spectra, axis=0), np.mean(self.ref, axis=0)
+            shells = [k for k in (1, 2, 3, 5, 8, 12, 16, 20, 25, 30, 35, 40, int(self.kc)) if k <= self.kc]
+            self.log("  E_les/E_dns at k = " + ", ".join(f"{k}: {ea[k - 1] / er[k - 1]:.3f}" for k in shells))
+        return {"seconds": elapsed, "steps": self.window // self.ratio, "spectrum_error_mean": mean_err,
+                "spectrum_error_end": end_err, "finite": bool(ok)}

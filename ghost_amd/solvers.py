@@ -78,6 +78,12 @@ class HDSolver(_Resident):
         for p in self._f:
             _lib.check(L.ghx_phaseshift(self.grid._h, C.c_void_p(p), float(cdump.real), float(cdump.imag)))
 
+    def set_sgs(self, mlp, delta):
+        """Attach the learned closure (ghost_amd.closure.MLP; None detaches): before
+        each substep the forcing becomes f + P(-div tau), tau = mlp(grad u, delta)."""
+        _lib.check(_lib.load().ghx_hd_set_sgs(self._h, mlp._h if mlp is not None else None, float(delta)))
+        self._sgs = mlp  # keep the network alive while attached
+
 
 class MHDSolver(_Resident):
     """MHD3D state: velocity v, vector potential a, forcings f (mechanical) and m (electromotive)."""

@@ -346,6 +346,20 @@ int ghx_mlp_train(ghx_mlp *m, long long n, const float *X, const float *T, long 
                   unsigned long long seed, float *losses);
 /* stress fields from gradient fields (n points, FP64 device fields) */
 int ghx_mlp_stress(ghx_mlp *m, long long n, const double *const *grad9, double delta, double *const *tau6);
+/* physical velocity gradients d u_i / d x_j (i major, nine real fields) of a
+ * spectral state in GHOST's normalisation */
+int ghx_velocity_gradients(ghx_grid *g, const void *const *v, double *const *grad9);
+/* training pairs from a fine-grid state v: ub = sharp(v, kc) (filt.f90:200
+ * with delta = pi/kc); vsmall = the LES state on the small grid t (trunc3D's
+ * truncation of ub, in t's normalisation) and tau6 = the physical subgrid
+ * stress F(u_i u_j) - F(ub_i ub_j) on t's points (F = sharp then truncate),
+ * order xx, yy, zz, xy, xz, yz */
+int ghx_sgs_targets(ghx_grid *g, ghx_grid *t, const void *const *v, double kc, double *const *tau6,
+                    void *const *vsmall);
+/* Attach a closure to an HD solver (m = NULL detaches): before each substep
+ * the forcing becomes f + P(-div tau), tau = m(grad u, delta), P the
+ * projection of nonlhd3 (pseudospec3D_hd.f90:399-509). */
+int ghx_hd_set_sgs(ghx_hd *s, ghx_mlp *m, double delta);
 
 /* ---------------------------------------------------------------------- */
 /* Pass profiler (replaces the GTStart/GTStop handles around the fftp      */
