@@ -161,6 +161,11 @@ def _declare(L):
         "ghx_velocity_gradients": (_I, [_P, _PP, _PP]),
         "ghx_sgs_targets": (_I, [_P, _P, _PP, _D, _PP, _PP]),
         "ghx_hd_set_sgs": (_I, [_P, _P, _D]),
+        "ghx_entrans": (_I, [_P] * 7 + [_I, _DP]),
+        "ghx_heltrans": (_I, [_P] * 7 + [_DP]),
+        "ghx_crosspecc": (_I, [_P] * 7 + [_D, _DP]),
+        "ghx_spectr1d": (_I, [_P, _P, _I, _DP]),
+        "ghx_mhdcheck": (_I, [_P] * 7 + [_I, _I, _I, _DP]),
         "ghx_prof_enable": (_I, [_I]),
         "ghx_prof_reset": (_I, []),
         "ghx_prof_read": (_I, [_I, C.POINTER(C.c_char_p), C.POINTER(_I), _DP, _DP]),

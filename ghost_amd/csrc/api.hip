@@ -763,6 +763,104 @@ extern "C" int ghx_spectrumc(ghx_grid *g, const void *a, const void *b, const vo
   return GHX_OK;
 }
 
+// entrans (pseudospec3D_hd.f90:1629-1820): shell sums of
+// Re(a d* + b e* + c f*) / N^6, times kk2 when kin = 0.  Ek: nmax/2+1 host
+// doubles, the reduced totals (GHOST's files hold Ek/Dkk).
+extern "C" int ghx_entrans(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                           const void *f, int kin, double *Ek) {
+  ARGCHK(g && a && b && c && d && e && f && Ek && kin >= 0, "bad entrans arguments");
+  const int nb = g->nmax / 2 + 1;
+  int rc = grid_work(g, ghx::spectrum_work_doubles(g->gk, nb));
+  if (rc) return rc;
+  HIPCHK(ghx::op_spectrum(g->gk, g->stream, kin == 0 ? 2 : 1, g->Dkk, nb, C_(a), C_(b), C_(c), C_(d), C_(e), C_(f),
+                          g->work, Ek));
+  return GHX_OK;
+}
+
+// heltrans (:1823-1949): curl(a,b,c) against (d,e,f)
+extern "C" int ghx_heltrans(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                            const void *f, double *Hk) {
+  ARGCHK(g && a && b && c && d && e && f && Hk, "bad heltrans arguments");
+  const int nb = g->nmax / 2 + 1;
+  int rc = grid_work(g, ghx::spectrum_work_doubles(g->gk, nb));
+  if (!rc) rc = grid_ctmp(g, 3);
+  if (!rc) rc = curl3(g, a, b, c, g->ctmp);
+  if (rc) return rc;
+  const cplx *t = g->ctmp;
+  HIPCHK(ghx::op_spectrum(g->gk, g->stream, 1, g->Dkk, nb, t, t + g->nc, t + 2 * g->nc, C_(d), C_(e), C_(f), g->work,
+                          Hk));
+  return GHX_OK;
+}
+
+// crosspecc (pseudospec3D_mhd.f90:603-712): Re(v . conj(curl A)) per shell,
+// shells int(sqrt(kk2)/Dkk + shift + .501) with the offset in working precision
+extern "C" int ghx_crosspecc(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                             const void *f, double shift, double *Ck) {
+  ARGCHK(g && a && b && c && d && e && f && Ck, "bad crosspecc arguments");
+  const int nb = g->nmax / 2 + 1;
+  int rc = grid_work(g, ghx::spectrum_work_doubles(g->gk, nb));
+  if (!rc) rc = grid_ctmp(g, 3);
+  if (!rc) rc = curl3(g, d, e, f, g->ctmp);
+  if (rc) return rc;
+  const cplx *t = g->ctmp;
+  HIPCHK(ghx::op_spectrum(g->gk, g->stream, 1, g->Dkk, nb, C_(a), C_(b), C_(c), t, t + g->nc, t + 2 * g->nc, g->work,
+                          Ck, shift + 0.501));
+  return GHX_OK;
+}
+
+// spectr1d (pseudospec3D_hd.f90:1418-1626): the 1D spectrum of one component
+// along k_dir, bins 1..n_dir/2+1 of Ek (nmax/2+1 host doubles; GHOST's
+// files hold 0.5 Ek / Dk_dir)
+extern "C" int ghx_spectr1d(ghx_grid *g, const void *a, int dir, double *Ek) {
+  ARGCHK(g && a && Ek && dir >= 1 && dir <= 3, "bad spectr1d arguments");
+  const int nb = g->nmax / 2 + 1;
+  int rc = grid_work(g, ghx::spectrum_work_doubles(g->gk, nb));
+  if (rc) return rc;
+  for (int q = 0; q < nb; ++q) Ek[q] = 0.0;
+  HIPCHK(ghx::op_spectr1d(g->gk, g->stream, dir, g->L[dir - 1], nb, C_(a), g->work, Ek));
+  return GHX_OK;
+}
+
+// mhdcheck (pseudospec3D_mhd.f90:339-547) without its file output: out =
+// {eng, ens, cur, engk, engm, helk, helm, crh, asq, divk, divm}; entries that
+// hel/crs/chk do not ask for are 0.  crs = 2 (the Hall generalised helicity)
+// is not provided.
+extern "C" int ghx_mhdcheck(ghx_grid *g, const void *a, const void *b, const void *c, const void *ma, const void *mb,
+                            const void *mc, int hel, int crs, int chk, double *out) {
+  ARGCHK(g && a && b && c && ma && mb && mc && out, "bad mhdcheck arguments");
+  ARGCHK(crs >= 0 && crs <= 1, "mhdcheck: crs = 2 (Hall generalised helicity) is not provided");
+  for (int q = 0; q < 11; ++q) out[q] = 0.0;
+  int rc = GHX_OK;
+  if (chk == 1) {
+    if ((rc = ghx_divergence(g, a, b, c, &out[9]))) return rc;
+    if ((rc = ghx_divergence(g, ma, mb, mc, &out[10]))) return rc;
+  }
+  double engk = 0, ens = 0, engm = 0, cur = 0;
+  if ((rc = ghx_energy(g, a, b, c, 1, &engk))) return rc;
+  if ((rc = ghx_energy(g, a, b, c, 0, &ens))) return rc;
+  // the magnetic field curl A lives past the three temporaries energy uses
+  if ((rc = grid_ctmp(g, 6))) return rc;
+  cplx *bm = g->ctmp + 3 * g->nc;
+  if ((rc = curl3(g, ma, mb, mc, bm))) return rc;
+  const cplx *b0 = bm, *b1 = bm + g->nc, *b2 = bm + 2 * g->nc;
+  if ((rc = ghx_energy(g, b0, b1, b2, 1, &engm))) return rc;
+  if ((rc = ghx_energy(g, b0, b1, b2, 0, &cur))) return rc;
+  out[0] = engk + engm;
+  out[1] = ens;
+  out[2] = cur;
+  out[3] = engk;
+  out[4] = engm;
+  if (hel == 1) {
+    if ((rc = ghx_helicity(g, a, b, c, &out[5]))) return rc;
+    if ((rc = ghx_helicity(g, ma, mb, mc, &out[6]))) return rc;
+  }
+  if (crs >= 1) {
+    if ((rc = ghx_energy(g, ma, mb, mc, 1, &out[8]))) return rc;
+    if ((rc = ghx_cross(g, a, b, c, b0, b1, b2, 1, &out[7]))) return rc;
+  }
+  return GHX_OK;
+}
+
 extern "C" int ghx_phaseshift(ghx_grid *g, void *a, double cre, double cim) {
   ARGCHK(g && a, "bad phaseshift arguments");
   HIPCHK(ghx::op_phaseshift(g->gk, g->stream, M_(a), cre, cim));

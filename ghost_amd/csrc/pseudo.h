@@ -48,7 +48,9 @@ hipError_t op_reduce(const GridK &g, hipStream_t st, int kind, const cplx *a, co
 // kind 1: helicity Re(a d* + b e* + c f*).  nb bins; out[nb] host.
 size_t spectrum_work_doubles(const GridK &g, int nb);
 hipError_t op_spectrum(const GridK &g, hipStream_t st, int kind, double dkk, int nb, const cplx *a, const cplx *b,
-                       const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *work, double *out);
+                       const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *work, double *out, double off = (double)0.501f);
+hipError_t op_spectr1d(const GridK &g, hipStream_t st, int dir, double ldir, int nb, const cplx *a, double *work,
+                       double *out);
 // max over real points of sqrt(r1^2+r2^2+r3^2) (maxabs)
 hipError_t op_maxnorm(const GridK &g, hipStream_t st, const double *r1, const double *r2, const double *r3,
                       double *work, double *out);

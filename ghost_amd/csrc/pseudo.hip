@@ -356,15 +356,16 @@ __global__ void k_sum_partials(const double *part, int n, double *out) {
   if (threadIdx.x == 0) *out = r;
 }
 
-// spectrumc: shell kmn = int(sqrt(kk2)/Dkk + .501) (single-precision literal),
-// bins 1..nb.
-__global__ void k_spectrum(GridK g, int kind, double dkk, int nb, const cplx *a, const cplx *b, const cplx *c,
-                           const cplx *d, const cplx *e, const cplx *f, double *part) {
+// spectrumc, entrans, heltrans, crosspecc: shell kmn = int(sqrt(kk2)/Dkk +
+// off), bins 1..nb; off is .501 as a single-precision literal, except in
+// crosspecc (shift + .501 in working precision).  kind 0: |a|^2+|b|^2+|c|^2;
+// 1: Re(a d*) + Re(b e*) + Re(c f*); 2: kind 1 times kk2 (entrans, kin = 0).
+__global__ void k_spectrum(GridK g, int kind, double dkk, int nb, double off, const cplx *a, const cplx *b,
+                           const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *part) {
   extern __shared__ double bins[];
   for (int q = threadIdx.x; q < nb; q += blockDim.x) bins[q] = 0.0;
   __syncthreads();
   const double tmp = g.inv_n6;
-  const double off = (double)0.501f;
   GRID_STRIDE(m, ncplx(g)) {
     Idx q = decompose(m, g);
     double kk2 = kk2_at(g, q);
@@ -374,6 +375,7 @@ __global__ void k_spectrum(GridK g, int kind, double dkk, int nb, const cplx *a,
       double s;
       if (kind == 0) s = cabs2(a[m]) + cabs2(b[m]) + cabs2(c[m]);
       else s = cre_dot(a[m], d[m]) + cre_dot(b[m], e[m]) + cre_dot(c[m], f[m]);
+      if (kind == 2) s = kk2 * s;
       atomicAdd(&bins[kmn - 1], w * s * tmp);
     }
   }
@@ -387,6 +389,27 @@ __global__ void k_spectrum_sum(const double *part, int nblk, int nb, double *out
     for (int b = 0; b < nblk; ++b) acc += part[(size_t)b * nb + q];
     out[q] = acc;
   }
+}
+
+// spectr1d (pseudospec3D_hd.f90:1418-1626): planes of k_dir, bin
+// int(|k_dir| L_dir + 1) in 1..n_dir/2+1 (the box-scaled wavenumber times the
+// box length, GHOST's arithmetic), |a|^2 / N^6 with the Hermitian weight.
+__global__ void k_spectr1d(GridK g, int dir, double ldir, int nb, const cplx *a, double *part) {
+  extern __shared__ double bins[];
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) bins[q] = 0.0;
+  __syncthreads();
+  const double tmp = g.inv_n6;
+  const int lim = dir == 1 ? g.nx / 2 + 1 : dir == 2 ? g.ny / 2 + 1 : g.nz / 2 + 1;
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    const double kd = dir == 1 ? fabs(wavenum(q.i, g.nx, g.dkx)) * ldir
+                      : dir == 2 ? fabs(wavenum(q.j, g.ny, g.dky)) * ldir
+                                 : fabs(wavenum(q.k, g.nz, g.dkz)) * ldir;
+    const int kmn = (int)(kd + 1);
+    if (kmn <= lim && kmn <= nb) atomicAdd(&bins[kmn - 1], (q.i == 0 ? 1.0 : 2.0) * cabs2(a[m]) * tmp);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) part[(size_t)blockIdx.x * nb + q] = bins[q];
 }
 
 __global__ void k_maxnorm(size_t n, const double *r1, const double *r2, const double *r3, double *part) {
@@ -512,10 +535,24 @@ static unsigned spectrum_blocks(const GridK &g) {
 size_t spectrum_work_doubles(const GridK &g, int nb) { return (size_t)spectrum_blocks(g) * nb + nb; }
 
 hipError_t op_spectrum(const GridK &g, hipStream_t st, int kind, double dkk, int nb, const cplx *a, const cplx *b,
-                       const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *work, double *out) {
+                       const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *work, double *out,
+                       double off) {
   unsigned nblk = spectrum_blocks(g);
-  hipLaunchKernelGGL(k_spectrum, dim3(nblk), dim3(kBlock), nb * sizeof(double), st, g, kind, dkk, nb, a, b, c, d,
-                     e, f, work);
+  hipLaunchKernelGGL(k_spectrum, dim3(nblk), dim3(kBlock), nb * sizeof(double), st, g, kind, dkk, nb, off, a, b, c,
+                     d, e, f, work);
+  double *res = work + (size_t)nblk * nb;
+  hipLaunchKernelGGL(k_spectrum_sum, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, st, work, (int)nblk, nb, res);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipMemcpyAsync(out, res, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(st);
+}
+
+hipError_t op_spectr1d(const GridK &g, hipStream_t st, int dir, double ldir, int nb, const cplx *a, double *work,
+                       double *out) {
+  unsigned nblk = spectrum_blocks(g);
+  hipLaunchKernelGGL(k_spectr1d, dim3(nblk), dim3(kBlock), nb * sizeof(double), st, g, dir, ldir, nb, a, work);
   double *res = work + (size_t)nblk * nb;
   hipLaunchKernelGGL(k_spectrum_sum, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, st, work, (int)nblk, nb, res);
   hipError_t err = hipGetLastError();

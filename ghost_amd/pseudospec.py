@@ -133,6 +133,45 @@ class Grid:
                                              C.cast(Ek, C.POINTER(C.c_double)), C.cast(Hk, C.POINTER(C.c_double))))
         return list(Ek), list(Hk)
 
+    def _bins(self):
+        return (C.c_double * self.nbins)()
+
+    def entrans(self, a, b, c, d, e, f, kin=1):
+        """Energy transfer per shell (entrans, pseudospec3D_hd.f90:1629-1820), the
+        reduced totals; kin = 0 weights each mode by kk2."""
+        out = self._bins()
+        _lib.check(_lib.load().ghx_entrans(self._h, *map(self._c, (a, b, c, d, e, f)), int(kin),
+                                           C.cast(out, C.POINTER(C.c_double))))
+        return list(out)
+
+    def heltrans(self, a, b, c, d, e, f):
+        """Helicity transfer per shell (heltrans, :1823-1949)."""
+        out = self._bins()
+        _lib.check(_lib.load().ghx_heltrans(self._h, *map(self._c, (a, b, c, d, e, f)),
+                                            C.cast(out, C.POINTER(C.c_double))))
+        return list(out)
+
+    def crosspecc(self, a, b, c, d, e, f, shift=0.0):
+        """Cross-helicity spectrum of v and curl A (crosspecc, pseudospec3D_mhd.f90:603-712)."""
+        out = self._bins()
+        _lib.check(_lib.load().ghx_crosspecc(self._h, *map(self._c, (a, b, c, d, e, f)), float(shift),
+                                             C.cast(out, C.POINTER(C.c_double))))
+        return list(out)
+
+    def spectr1d(self, a, dir):
+        """1D spectrum of one component along k_dir (spectr1d, :1418-1626)."""
+        out = self._bins()
+        _lib.check(_lib.load().ghx_spectr1d(self._h, self._c(a), int(dir), C.cast(out, C.POINTER(C.c_double))))
+        return list(out)
+
+    def mhdcheck(self, a, b, c, ma, mb, mc, hel=1, crs=1, chk=1):
+        """mhdcheck (pseudospec3D_mhd.f90:339-547): {eng, ens, cur, engk, engm, helk,
+        helm, crh, asq, divk, divm}."""
+        out = (C.c_double * 11)()
+        _lib.check(_lib.load().ghx_mhdcheck(self._h, *map(self._c, (a, b, c, ma, mb, mc)), int(hel), int(crs),
+                                            int(chk), C.cast(out, C.POINTER(C.c_double))))
+        return list(out)
+
     def phaseshift(self, a, cdump):
         _lib.check(_lib.load().ghx_phaseshift(self._h, self._c(a), float(cdump.real), float(cdump.imag)))
         return a

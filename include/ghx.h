@@ -135,6 +135,23 @@ int ghx_divergence(ghx_grid *g, const void *a, const void *b, const void *c, dou
 /* spectrumc (hd :1190-1415): Ek, Hk host arrays of nmax/2+1 doubles (either may be NULL) */
 int ghx_spectrumc(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
                   double *Hk);
+/* Transfer and 1D spectra (reduced shell totals; GHOST's files hold them
+ * divided by Dkk, or 0.5/Dk_dir for spectr1d).  Arrays of nmax/2+1 host doubles. */
+/* entrans (hd :1629-1820): Re(a d* + b e* + c f*)/N^6 per shell, times kk2 when kin = 0 */
+int ghx_entrans(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e, const void *f,
+                int kin, double *Ek);
+/* heltrans (hd :1823-1949): curl(a,b,c) against (d,e,f) */
+int ghx_heltrans(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                 const void *f, double *Hk);
+/* crosspecc (mhd :603-712): Re(v . conj(curl A)) per shell, offset shift + .501 */
+int ghx_crosspecc(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                  const void *f, double shift, double *Ck);
+/* spectr1d (hd :1418-1626): 1D spectrum of one component along k_dir, bins 1..n_dir/2+1 */
+int ghx_spectr1d(ghx_grid *g, const void *a, int dir, double *Ek);
+/* mhdcheck (mhd :339-547) without file output: out[11] = {eng, ens, cur, engk,
+ * engm, helk, helm, crh, asq, divk, divm}; crs = 2 (Hall) is refused */
+int ghx_mhdcheck(ghx_grid *g, const void *a, const void *b, const void *c, const void *ma, const void *mb,
+                 const void *mc, int hel, int crs, int chk, double *out);
 int ghx_phaseshift(ghx_grid *g, void *a, double cdump_re, double cdump_im);                   /* hd :2084 */
 int ghx_normalize(ghx_grid *g, void *fx, void *fy, void *fz, double f0, int kin);             /* module_dns.f90:13 */
 int ghx_sharp(ghx_grid *g, const void *a, void *b, double delta);                             /* filt :200 */

@@ -1516,3 +1516,101 @@ double og_mlp_grad(const float *P, long n, const float *X, const float *T, int b
   }
   return loss / ((double)n * MNO);
 }
+
+/* ------------------------------------------------------------------------
+ * Transfer and 1D spectra, MHD checks (pseudospec3D_hd.f90:1418-1949,
+ * pseudospec3D_mhd.f90:339-712).  The shell sums return the reduced totals
+ * (GHOST writes them divided by Dkk, or 0.5/Dn for spectr1d, to its files).
+ * ------------------------------------------------------------------------ */
+/* sum over shells int(sqrt(kk2)/Dkk + off) in 1..nmax/2+1 of
+ * w (Re(x d*) + Re(y e*) + Re(z f*)) tmp, times kk2 when kk2w */
+static void shell_dot(const og_grid *g, const ocplx *x, const ocplx *y, const ocplx *z, const ocplx *d,
+                      const ocplx *e, const ocplx *f, int kk2w, double off, double *out) {
+  const double tmp = nsix(g);
+  const int nb = g->nmax / 2 + 1;
+  for (int q = 0; q < nb; ++q) out[q] = 0.0;
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        size_t m = IDX(g, i, j, k);
+        int kmn = (int)(sqrt(g->kk2[m]) / g->Dkk + off);
+        if (kmn > 0 && kmn <= nb) {
+          double s = creal(x[m] * conj(d[m])) + creal(y[m] * conj(e[m])) + creal(z[m] * conj(f[m]));
+          if (kk2w) s = g->kk2[m] * s;
+          out[kmn - 1] += (i == 0 ? 1.0 : 2.0) * s * tmp;
+        }
+      }
+}
+
+/* entrans (:1629-1820): kin = 0 weights each mode by kk2 */
+void og_entrans(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                const ocplx *e, const ocplx *f, int kin, double *Ek) {
+  shell_dot(g, a, b, c, d, e, f, kin == 0, (double)0.501f, Ek);
+}
+
+/* heltrans (:1823-1949): curl(a,b,c) against (d,e,f) */
+void og_heltrans(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                 const ocplx *e, const ocplx *f, double *Hk) {
+  size_t nc = ncplx(g);
+  ocplx *cu = (ocplx *)malloc(sizeof(ocplx) * nc * 3);
+  og_rotor3(g, b, c, cu, 1); og_rotor3(g, a, c, cu + nc, 2); og_rotor3(g, a, b, cu + 2 * nc, 3);
+  shell_dot(g, cu, cu + nc, cu + 2 * nc, d, e, f, 0, (double)0.501f, Hk);
+  free(cu);
+}
+
+/* crosspecc (pseudospec3D_mhd.f90:603-712): Re(v . conj(curl A)), shells
+ * with the double-precision offset shift + .501 */
+void og_crosspecc(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                  const ocplx *e, const ocplx *f, double shift, double *Ck) {
+  size_t nc = ncplx(g);
+  ocplx *cu = (ocplx *)malloc(sizeof(ocplx) * nc * 3);
+  og_rotor3(g, e, f, cu, 1); og_rotor3(g, d, f, cu + nc, 2); og_rotor3(g, d, e, cu + 2 * nc, 3);
+  shell_dot(g, a, b, c, cu, cu + nc, cu + 2 * nc, 0, shift + 0.501, Ck);
+  free(cu);
+}
+
+/* spectr1d (:1418-1626): planes of k_dir, bin int(|k_dir| L_dir + 1) in
+ * 1..n_dir/2+1, |a|^2 / N^6 with the Hermitian weight; Ek has nmax/2+1 */
+void og_spectr1d(const og_grid *g, const ocplx *a, int dir, double *Ek) {
+  const double tmp = nsix(g);
+  for (int q = 0; q < g->nmax / 2 + 1; ++q) Ek[q] = 0.0;
+  const int lim = dir == 1 ? g->nx / 2 + 1 : dir == 2 ? g->ny / 2 + 1 : g->nz / 2 + 1;
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        const double kd = dir == 1 ? fabs(g->kx[i]) * g->Lx : dir == 2 ? fabs(g->ky[j]) * g->Ly : fabs(g->kz[k]) * g->Lz;
+        const int kmn = (int)(kd + 1);
+        if (kmn <= lim) Ek[kmn - 1] += (i == 0 ? 1.0 : 2.0) * abs2(a[IDX(g, i, j, k)]) * tmp;
+      }
+}
+
+/* mhdcheck (pseudospec3D_mhd.f90:339-547) without its file output and with
+ * crs <= 1: out = {eng, ens, cur, engk, engm, helk, helm, crh, asq, divk, divm}
+ * (entries not requested by hel/crs/chk are 0) */
+void og_mhdcheck(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *ma,
+                 const ocplx *mb, const ocplx *mc, int hel, int crs, int chk, double *out) {
+  size_t nc = ncplx(g);
+  for (int q = 0; q < 11; ++q) out[q] = 0.0;
+  if (chk == 1) {
+    out[9] = og_divergence(g, a, b, c);
+    out[10] = og_divergence(g, ma, mb, mc);
+  }
+  const double engk = og_energy(g, a, b, c, 1), ens = og_energy(g, a, b, c, 0);
+  ocplx *cu = (ocplx *)malloc(sizeof(ocplx) * nc * 3);
+  og_rotor3(g, mb, mc, cu, 1); og_rotor3(g, ma, mc, cu + nc, 2); og_rotor3(g, ma, mb, cu + 2 * nc, 3);
+  const double engm = og_energy(g, cu, cu + nc, cu + 2 * nc, 1), cur = og_energy(g, cu, cu + nc, cu + 2 * nc, 0);
+  out[0] = engk + engm;
+  out[1] = ens;
+  out[2] = cur;
+  out[3] = engk;
+  out[4] = engm;
+  if (hel == 1) {
+    out[5] = og_helicity(g, a, b, c);
+    out[6] = og_helicity(g, ma, mb, mc);
+  }
+  if (crs >= 1) {
+    out[8] = og_energy(g, ma, mb, mc, 1);
+    out[7] = og_cross(g, a, b, c, cu, cu + nc, cu + 2 * nc, 1);
+  }
+  free(cu);
+}

@@ -147,4 +147,17 @@ void og_truncate(const og_grid *g, const ocplx *c, const og_grid *t, ocplx *out)
 void og_mlp_forward(const float *P, long n, const float *X, int bf16, double *Y);
 double og_mlp_grad(const float *P, long n, const float *X, const float *T, int bf16, double *grad);
 
+/* transfer and 1D spectra, MHD checks (pseudospec3D_hd.f90:1418-1949,
+ * pseudospec3D_mhd.f90:339-712); arrays of nmax/2+1 shells, the reduced
+ * totals before the division GHOST applies when writing its files */
+void og_entrans(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                const ocplx *e, const ocplx *f, int kin, double *Ek);
+void og_heltrans(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                 const ocplx *e, const ocplx *f, double *Hk);
+void og_crosspecc(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+                  const ocplx *e, const ocplx *f, double shift, double *Ck);
+void og_spectr1d(const og_grid *g, const ocplx *a, int dir, double *Ek);
+void og_mhdcheck(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *ma,
+                 const ocplx *mb, const ocplx *mc, int hel, int crs, int chk, double *out);
+
 #endif

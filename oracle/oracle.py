@@ -97,6 +97,11 @@ def _declare(L):
                                   C.c_double, C.c_int, C.c_int, _P]
     L.og_hd_step_tracers.argtypes = [_G, C.c_int, C.c_double, C.c_double] + [_P] * 6 + \
         [C.c_long, _P, _P, _P, C.c_int, C.c_int]
+    L.og_entrans.argtypes = [_G] + [_P] * 6 + [C.c_int, _P]
+    L.og_heltrans.argtypes = [_G] + [_P] * 6 + [_P]
+    L.og_crosspecc.argtypes = [_G] + [_P] * 6 + [C.c_double, _P]
+    L.og_spectr1d.argtypes = [_G, _P, C.c_int, _P]
+    L.og_mhdcheck.argtypes = [_G] + [_P] * 6 + [C.c_int, C.c_int, C.c_int, _P]
     L.og_mlp_forward.argtypes = [_P, C.c_long, _P, C.c_int, _P]
     L.og_mlp_grad.restype = C.c_double
     L.og_mlp_grad.argtypes = [_P, C.c_long, _P, _P, C.c_int, _P]
@@ -239,6 +244,31 @@ class Grid:
         Hk = np.zeros(self.nbins)
         lib().og_spectrumc(self._g, _p(a), _p(b), _p(c), kin, hel, _p(Ek), _p(Hk))
         return Ek, Hk
+
+    def entrans(self, a, b, c, d, e, f, kin=1):
+        out = np.zeros(self.nbins)
+        lib().og_entrans(self._g, *map(_p, (a, b, c, d, e, f)), int(kin), _p(out))
+        return out
+
+    def heltrans(self, a, b, c, d, e, f):
+        out = np.zeros(self.nbins)
+        lib().og_heltrans(self._g, *map(_p, (a, b, c, d, e, f)), _p(out))
+        return out
+
+    def crosspecc(self, a, b, c, d, e, f, shift=0.0):
+        out = np.zeros(self.nbins)
+        lib().og_crosspecc(self._g, *map(_p, (a, b, c, d, e, f)), float(shift), _p(out))
+        return out
+
+    def spectr1d(self, a, dir):
+        out = np.zeros(self.nbins)
+        lib().og_spectr1d(self._g, _p(a), int(dir), _p(out))
+        return out
+
+    def mhdcheck(self, a, b, c, ma, mb, mc, hel=1, crs=1, chk=1):
+        out = np.zeros(11)
+        lib().og_mhdcheck(self._g, *map(_p, (a, b, c, ma, mb, mc)), int(hel), int(crs), int(chk), _p(out))
+        return out
 
     def phaseshift(self, a, phase_c):
         a = a.copy()
