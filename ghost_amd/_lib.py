@@ -165,6 +165,7 @@ def _declare(L):
         "ghx_heltrans": (_I, [_P] * 7 + [_DP]),
         "ghx_crosspecc": (_I, [_P] * 7 + [_D, _DP]),
         "ghx_spectr1d": (_I, [_P, _P, _I, _DP]),
+        "ghx_hdcheck": (_I, [_P] * 7 + [_I, _I, _DP]),
         "ghx_mhdcheck": (_I, [_P] * 7 + [_I, _I, _I, _DP]),
         "ghx_prof_enable": (_I, [_I]),
         "ghx_prof_reset": (_I, []),

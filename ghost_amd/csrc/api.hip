@@ -821,6 +821,22 @@ extern "C" int ghx_spectr1d(ghx_grid *g, const void *a, int dir, double *Ek) {
   return GHX_OK;
 }
 
+// hdcheck (pseudospec3D_hd.f90:984-1105) without its file output: out =
+// {eng, ens, pot, khe, div}: <v^2>, <omega^2>, the injection rate <v.f>,
+// the kinetic helicity (hel = 1) and <(div v)^2> (chk = 1); entries that
+// hel/chk do not ask for are 0.
+extern "C" int ghx_hdcheck(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                           const void *f, int hel, int chk, double *out) {
+  ARGCHK(g && a && b && c && d && e && f && out, "bad hdcheck arguments");
+  for (int q = 0; q < 5; ++q) out[q] = 0.0;
+  int rc = GHX_OK;
+  if (chk == 1 && (rc = ghx_divergence(g, a, b, c, &out[4]))) return rc;
+  if ((rc = ghx_energy(g, a, b, c, 1, &out[0]))) return rc;
+  if ((rc = ghx_energy(g, a, b, c, 0, &out[1]))) return rc;
+  if (hel == 1 && (rc = ghx_helicity(g, a, b, c, &out[3]))) return rc;
+  return ghx_cross(g, a, b, c, d, e, f, 1, &out[2]);
+}
+
 // mhdcheck (pseudospec3D_mhd.f90:339-547) without its file output: out =
 // {eng, ens, cur, engk, engm, helk, helm, crh, asq, divk, divm}; entries that
 // hel/crs/chk do not ask for are 0.  crs = 2 (the Hall generalised helicity)

@@ -37,6 +37,7 @@ constexpr int NI = 10, NH = 64, NO = 6;
 constexpr int OW1 = 0, OB1 = OW1 + NH * NI, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH, OB3 = OW3 + NO * NH;
 constexpr int NPAR = OB3 + NO;  // 5254
 constexpr int NNORM = 2 * NI + 2 * NO;
+static_assert(NNORM == 32, "the C ABI passes the normalisation as 32 floats: mx, sx, my, sy");
 constexpr int WAVES = 4, BLOCK = 64 * WAVES, GRID = 256, NWAVES = GRID * WAVES;
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));

@@ -168,6 +168,47 @@ MODULE ghx_c
       INTEGER(C_INT), VALUE :: kin, hel
       REAL(C_DOUBLE) :: Ek(*), Hk(*)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_grid_info(g, kmax, tiny, nmax, Dkk, anis) BIND(C, name="ghx_grid_info")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g
+      REAL(C_DOUBLE) :: kmax, tiny, Dkk
+      INTEGER(C_INT) :: nmax, anis
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_entrans(g, a, b, c, d, e, f, kin, Ek) BIND(C, name="ghx_entrans")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c, d, e, f
+      INTEGER(C_INT), VALUE :: kin
+      REAL(C_DOUBLE) :: Ek(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_heltrans(g, a, b, c, d, e, f, Hk) BIND(C, name="ghx_heltrans")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c, d, e, f
+      REAL(C_DOUBLE) :: Hk(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_crosspecc(g, a, b, c, d, e, f, shift, Ck) BIND(C, name="ghx_crosspecc")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c, d, e, f
+      REAL(C_DOUBLE), VALUE :: shift
+      REAL(C_DOUBLE) :: Ck(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_spectr1d(g, a, dir, Ek) BIND(C, name="ghx_spectr1d")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a
+      INTEGER(C_INT), VALUE :: dir
+      REAL(C_DOUBLE) :: Ek(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_hdcheck(g, a, b, c, d, e, f, hel, chk, out) BIND(C, name="ghx_hdcheck")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c, d, e, f
+      INTEGER(C_INT), VALUE :: hel, chk
+      REAL(C_DOUBLE) :: out(5)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_mhdcheck(g, a, b, c, ma, mb, mc, hel, crs, chk, out) BIND(C, name="ghx_mhdcheck")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c, ma, mb, mc
+      INTEGER(C_INT), VALUE :: hel, crs, chk
+      REAL(C_DOUBLE) :: out(11)
+    END FUNCTION
     INTEGER(C_INT) FUNCTION ghx_phaseshift(g, a, cre, cim) BIND(C, name="ghx_phaseshift")
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: g, a

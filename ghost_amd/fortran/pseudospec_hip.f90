@@ -22,12 +22,17 @@ MODULE pseudospec_hip
   TYPE(C_PTR), SAVE :: ghx_grid_h = C_NULL_PTR
   INTEGER, SAVE :: gnx = 0, gny = 0, gnz = 0
   INTEGER(C_SIZE_T), SAVE :: gnc = 0, gnr = 0
+  ! shell width and count of the 3D spectra, wavenumber steps of the 1D ones
+  INTEGER, SAVE :: gnmax = 0
+  DOUBLE PRECISION, SAVE :: gDkk = 1.0D0, gDk(3) = 1.0D0
 
 CONTAINS
 
   SUBROUTINE ghx_pseudospec_init(nx, ny, nz, Lx, Ly, Lz, arbsize, Dkk)
     INTEGER, INTENT(IN) :: nx, ny, nz, arbsize
     REAL(KIND=GP), INTENT(IN) :: Lx, Ly, Lz, Dkk
+    REAL(C_DOUBLE) :: kmax, tiny, dk
+    INTEGER(C_INT) :: nmax, anis
     IF (C_ASSOCIATED(ghx_grid_h)) CALL ghx_check(ghx_grid_destroy(ghx_grid_h), 'grid destroy')
     CALL ghx_check(ghx_grid_create(ghx_grid_h, INT(nx, C_INT), INT(ny, C_INT), INT(nz, C_INT), &
                                    REAL(Lx, C_DOUBLE), REAL(Ly, C_DOUBLE), REAL(Lz, C_DOUBLE), &
@@ -37,6 +42,11 @@ CONTAINS
     gnz = nz
     gnc = INT(nx/2 + 1, C_SIZE_T) * INT(ny, C_SIZE_T) * INT(nz, C_SIZE_T)
     gnr = INT(nx, C_SIZE_T) * INT(ny, C_SIZE_T) * INT(nz, C_SIZE_T)
+    CALL ghx_check(ghx_grid_info(ghx_grid_h, kmax, tiny, nmax, dk, anis), 'ghx_pseudospec_init')
+    gnmax = nmax
+    gDkk = dk
+    gDk = 1.0D0
+    IF (arbsize .EQ. 1) gDk = (/1.0D0/Lx, 1.0D0/Ly, 1.0D0/Lz/)
   END SUBROUTINE ghx_pseudospec_init
 
   SUBROUTINE ghx_pseudospec_finalize()
@@ -170,6 +180,160 @@ CONTAINS
     CALL ghx_check(ghx_spectrumc(ghx_grid_h, a%p, b%p, c%p, INT(kin, C_INT), INT(hel, C_INT), Ektot, Hktot), &
                    'spectrumc')
   END SUBROUTINE spectrumc
+
+  ! The routines below write GHOST's text files on this rank, in the working
+  ! directory, with GHOST's names and formats.
+
+  ! spectrum (:1108-1187): kspectrum/mspectrum and k/m/ghelicity.nmb.txt
+  SUBROUTINE spectrum(a, b, c, nmb, kin, hel)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c
+    INTEGER, INTENT(IN) :: kin, hel
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    DOUBLE PRECISION :: Ek(gnmax/2 + 1), Hk(gnmax/2 + 1)
+    CALL spectrumc(a, b, c, kin, hel, Ek, Hk)
+    IF (kin .EQ. 1) CALL shell_file('kspectrum.'//nmb//'.txt', Ek, 0.5D0)
+    IF (kin .EQ. 0) CALL shell_file('mspectrum.'//nmb//'.txt', Ek, 0.5D0)
+    IF (hel .EQ. 1) THEN
+      IF (kin .EQ. 1) THEN
+        CALL shell_file('khelicity.'//nmb//'.txt', Hk, 1.0D0)
+      ELSE IF (kin .EQ. 0) THEN
+        CALL shell_file('mhelicity.'//nmb//'.txt', Hk, 1.0D0)
+      ELSE
+        CALL shell_file('ghelicity.'//nmb//'.txt', Hk, 1.0D0)
+      END IF
+    END IF
+  END SUBROUTINE spectrum
+
+  ! entrans (:1629-1820): kin = 0 m-, 1 k-, 2 j-transfer, 3 mcrostran, 4 kcrostran
+  SUBROUTINE entrans(a, b, c, d, e, f, nmb, kin)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c, d, e, f
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: kin
+    DOUBLE PRECISION :: Ek(gnmax/2 + 1)
+    CALL ghx_check(ghx_entrans(ghx_grid_h, a%p, b%p, c%p, d%p, e%p, f%p, INT(kin, C_INT), Ek), 'entrans')
+    SELECT CASE (kin)
+    CASE (0)
+      CALL shell_file('mtransfer.'//nmb//'.txt', Ek, 1.0D0)
+    CASE (1)
+      CALL shell_file('ktransfer.'//nmb//'.txt', Ek, 1.0D0)
+    CASE (2)
+      CALL shell_file('jtransfer.'//nmb//'.txt', Ek, 1.0D0)
+    CASE (3)
+      CALL shell_file('mcrostran.'//nmb//'.txt', Ek, 1.0D0)
+    CASE (4)
+      CALL shell_file('kcrostran.'//nmb//'.txt', Ek, 1.0D0)
+    END SELECT
+  END SUBROUTINE entrans
+
+  ! heltrans (:1823-1949): hmtransfer (kin = 0) or hktransfer.nmb.txt
+  SUBROUTINE heltrans(a, b, c, d, e, f, nmb, kin)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c, d, e, f
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: kin
+    DOUBLE PRECISION :: Hk(gnmax/2 + 1)
+    CALL ghx_check(ghx_heltrans(ghx_grid_h, a%p, b%p, c%p, d%p, e%p, f%p, Hk), 'heltrans')
+    IF (kin .EQ. 0) THEN
+      CALL shell_file('hmtransfer.'//nmb//'.txt', Hk, 1.0D0)
+    ELSE
+      CALL shell_file('hktransfer.'//nmb//'.txt', Hk, 1.0D0)
+    END IF
+  END SUBROUTINE heltrans
+
+  ! spectr1d (:1418-1626): spref//'spec1d'//IJ.nmb.txt, k_dir and E_comp(k_dir)
+  SUBROUTINE spectr1d(a, nmb, spref, cmp, dir)
+    TYPE(ghx_field), INTENT(IN) :: a
+    CHARACTER(len=*), INTENT(IN) :: nmb, spref
+    INTEGER, INTENT(IN) :: cmp, dir
+    CHARACTER(len=3), PARAMETER :: coord = 'xyz'
+    DOUBLE PRECISION :: Ek(gnmax/2 + 1)
+    INTEGER :: n(3), i, u
+    CALL ghx_check(ghx_spectr1d(ghx_grid_h, a%p, INT(dir, C_INT), Ek), 'spectr1d')
+    n = (/gnx, gny, gnz/)
+    OPEN(NEWUNIT=u, FILE=TRIM(spref)//'spec1d'//coord(cmp:cmp)//coord(dir:dir)//'.'//nmb//'.txt')
+    DO i = 1, n(dir)/2 + 1
+      WRITE(u, FMT='(E13.6,E23.15)') REAL(gDk(dir)*(i - 1), GP), REAL(0.5D0*Ek(i)/gDk(dir), GP)
+    END DO
+    CLOSE(u)
+  END SUBROUTINE spectr1d
+
+  ! crosspecc (pseudospec3D_mhd.f90:603-712): reduced shell sums, no file
+  SUBROUTINE crosspecc(a, b, c, d, e, f, Cktot, shift)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c, d, e, f
+    DOUBLE PRECISION, INTENT(OUT) :: Cktot(*)
+    REAL(KIND=GP), INTENT(IN) :: shift
+    CALL ghx_check(ghx_crosspecc(ghx_grid_h, a%p, b%p, c%p, d%p, e%p, f%p, REAL(shift, C_DOUBLE), Cktot), &
+                   'crosspecc')
+  END SUBROUTINE crosspecc
+
+  ! hdcheck (:984-1105): appends balance.txt, helicity.txt, divergence.txt
+  SUBROUTINE hdcheck(a, b, c, d, e, f, t, dt, hel, chk)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c, d, e, f
+    INTEGER, INTENT(IN) :: t, hel, chk
+    REAL(KIND=GP), INTENT(IN) :: dt
+    REAL(C_DOUBLE) :: r(5)
+    INTEGER :: u
+    CALL ghx_check(ghx_hdcheck(ghx_grid_h, a%p, b%p, c%p, d%p, e%p, f%p, INT(hel, C_INT), INT(chk, C_INT), r), &
+                   'hdcheck')
+    OPEN(NEWUNIT=u, FILE='balance.txt', POSITION='append')
+    WRITE(u, '(E13.6,E26.18,E26.18,E26.18)') (t - 1)*dt, r(1), r(2), r(3)
+    CLOSE(u)
+    IF (hel .EQ. 1) THEN
+      OPEN(NEWUNIT=u, FILE='helicity.txt', POSITION='append')
+      WRITE(u, FMT='(E13.6,E26.18)') (t - 1)*dt, r(4)
+      CLOSE(u)
+    END IF
+    IF (chk .EQ. 1) THEN
+      OPEN(NEWUNIT=u, FILE='divergence.txt', POSITION='append')
+      WRITE(u, FMT='(E13.6,E26.18)') (t - 1)*dt, r(5)
+      CLOSE(u)
+    END IF
+  END SUBROUTINE hdcheck
+
+  ! mhdcheck (pseudospec3D_mhd.f90:339-547): appends balance, energy, helicity,
+  ! cross and divergence.txt; crs = 2 (Hall) stops with an error
+  SUBROUTINE mhdcheck(a, b, c, ma, mb, mc, t, dt, hel, crs, chk)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c, ma, mb, mc
+    INTEGER, INTENT(IN) :: t, hel, crs, chk
+    REAL(KIND=GP), INTENT(IN) :: dt
+    CHARACTER(len=*), PARAMETER :: f10 = '(E13.6,E22.14,E22.14,E22.14)'
+    REAL(C_DOUBLE) :: r(11)
+    INTEGER :: u
+    CALL ghx_check(ghx_mhdcheck(ghx_grid_h, a%p, b%p, c%p, ma%p, mb%p, mc%p, INT(hel, C_INT), INT(crs, C_INT), &
+                                INT(chk, C_INT), r), 'mhdcheck')
+    OPEN(NEWUNIT=u, FILE='balance.txt', POSITION='append')
+    WRITE(u, f10) (t - 1)*dt, r(1), r(2), r(3)
+    CLOSE(u)
+    OPEN(NEWUNIT=u, FILE='energy.txt', POSITION='append')
+    WRITE(u, f10) (t - 1)*dt, r(4), r(5)
+    CLOSE(u)
+    IF (hel .EQ. 1) THEN
+      OPEN(NEWUNIT=u, FILE='helicity.txt', POSITION='append')
+      WRITE(u, f10) (t - 1)*dt, r(6), r(7)
+      CLOSE(u)
+    END IF
+    IF (crs .EQ. 1) THEN
+      OPEN(NEWUNIT=u, FILE='cross.txt', POSITION='append')
+      WRITE(u, f10) (t - 1)*dt, r(8), r(9)
+      CLOSE(u)
+    END IF
+    IF (chk .EQ. 1) THEN
+      OPEN(NEWUNIT=u, FILE='divergence.txt', POSITION='append')
+      WRITE(u, f10) (t - 1)*dt, r(10), r(11)
+      CLOSE(u)
+    END IF
+  END SUBROUTINE mhdcheck
+
+  ! shells 1..nmax/2+1 as GHOST writes them: Dkk*i, scale*S(i)/Dkk
+  SUBROUTINE shell_file(fname, S, scale)
+    CHARACTER(len=*), INTENT(IN) :: fname
+    DOUBLE PRECISION, INTENT(IN) :: S(*), scale
+    INTEGER :: i, u
+    OPEN(NEWUNIT=u, FILE=fname)
+    DO i = 1, gnmax/2 + 1
+      WRITE(u, FMT='(E13.6,E23.15)') REAL(gDkk*i, GP), scale*S(i)/gDkk
+    END DO
+    CLOSE(u)
+  END SUBROUTINE shell_file
 
   SUBROUTINE phaseshift(a, cdump)
     TYPE(ghx_field), INTENT(INOUT) :: a

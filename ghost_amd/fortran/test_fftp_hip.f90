@@ -6,6 +6,8 @@
 ! 2. Device-resident fields through pseudospec_hip: TG enstrophy, and an RK2
 !    loop written as GHOST's main3D (C = v; DO o = ord,1,-1: hd_rkstep2).
 ! 3. The resident solver with tracers attached.
+! 2b. GHOST's diagnostic text files (spectrum, entrans, heltrans, spectr1d,
+!    hdcheck) from the resident fields, written to the working directory.
 ! Writes OUTDIR/fortran_hd32.bin (v after the RK loop, 3 x C(nz,ny,nx/2+1))
 ! and OUTDIR/fortran_tracers32.bin (initial and final positions) for the
 ! comparison with the CPU oracle; prints one line per check and stops with a
@@ -17,7 +19,7 @@ PROGRAM test_fftp_hip
   USE ghx_c
   USE pseudospec_hip, ONLY: ghx_pseudospec_init, ghx_pseudospec_finalize, ghx_field, field_alloc, field_free, &
                             field_upload, field_download, ghx_grid_h, d_rotor3 => rotor3, d_energy => energy, &
-                            hd_rkstep2
+                            hd_rkstep2, spectrum, entrans, heltrans, spectr1d, hdcheck
   USE iovar, ONLY: IOPLAN
   IMPLICIT NONE
   INTEGER, PARAMETER :: n = 32, nh = n/2 + 1, nsteps = 20, npart = 1000, psteps = 5
@@ -35,6 +37,7 @@ PROGRAM test_fftp_hip
   DOUBLE PRECISION :: e, err, x, y, z, rn
   INTEGER :: i, j, k, o, q, u
   INTEGER :: ierr
+  INTEGER(C_INT32_T) :: seed
 
   CALL GET_COMMAND_ARGUMENT(1, outdir)
   IF (LEN_TRIM(outdir) .EQ. 0) outdir = '.'
@@ -120,6 +123,23 @@ PROGRAM test_fftp_hip
   END DO
   CLOSE(u)
   WRITE(*, '(A)') 'ok   RK loop written (fortran_hd32.bin)'
+
+  ! GHOST's diagnostic files from resident fields, in the working directory:
+  ! a k^4 exp(-k^2/k0^2) field (k0 = 3, seed 11) against a random forcing
+  ! field (shells 1..6, seed 29)
+  seed = 11
+  CALL ghx_check(ghx_initialv_k4exp(ghx_grid_h, 3.0_C_DOUBLE, 1.0_C_DOUBLE, seed, w(1)%p, w(2)%p, w(3)%p), &
+                 'initialv_k4exp')
+  seed = 29
+  CALL ghx_check(ghx_initialfv_random(ghx_grid_h, 1.0_C_DOUBLE, 6.0_C_DOUBLE, 1.0_C_DOUBLE, seed, cst(1)%p, &
+                                      cst(2)%p, cst(3)%p), 'initialfv_random')
+  CALL spectrum(w(1), w(2), w(3), '0001', 1, 1)
+  CALL entrans(w(1), w(2), w(3), cst(1), cst(2), cst(3), '0001', 1)
+  CALL entrans(w(1), w(2), w(3), cst(1), cst(2), cst(3), '0001', 0)
+  CALL heltrans(w(1), w(2), w(3), cst(1), cst(2), cst(3), '0001', 1)
+  CALL spectr1d(w(1), '0001', 'k', 1, 3)
+  CALL hdcheck(w(1), w(2), w(3), cst(1), cst(2), cst(3), nsteps + 1, dt, 1, 1)
+  WRITE(*, '(A)') 'ok   diagnostic files written'
 
   ! GHOST's binary output of v (main3D.fpp:2675-2710) from the resident
   ! fields, and the same file written from host arrays through GHOST's own

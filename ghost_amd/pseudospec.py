@@ -164,6 +164,13 @@ class Grid:
         _lib.check(_lib.load().ghx_spectr1d(self._h, self._c(a), int(dir), C.cast(out, C.POINTER(C.c_double))))
         return list(out)
 
+    def hdcheck(self, a, b, c, d, e, f, hel=1, chk=1):
+        """hdcheck (pseudospec3D_hd.f90:984-1105): {eng, ens, pot, khe, div}."""
+        out = (C.c_double * 5)()
+        _lib.check(_lib.load().ghx_hdcheck(self._h, *map(self._c, (a, b, c, d, e, f)), int(hel), int(chk),
+                                           C.cast(out, C.POINTER(C.c_double))))
+        return list(out)
+
     def mhdcheck(self, a, b, c, ma, mb, mc, hel=1, crs=1, chk=1):
         """mhdcheck (pseudospec3D_mhd.f90:339-547): {eng, ens, cur, engk, engm, helk,
         helm, crh, asq, divk, divm}."""

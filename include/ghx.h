@@ -148,6 +148,10 @@ int ghx_crosspecc(ghx_grid *g, const void *a, const void *b, const void *c, cons
                   const void *f, double shift, double *Ck);
 /* spectr1d (hd :1418-1626): 1D spectrum of one component along k_dir, bins 1..n_dir/2+1 */
 int ghx_spectr1d(ghx_grid *g, const void *a, int dir, double *Ek);
+/* hdcheck (hd :984-1105) without file output: out[5] = {eng, ens, pot, khe, div}
+ * (<v^2>, <omega^2>, <v.f>, kinetic helicity if hel = 1, <(div v)^2> if chk = 1) */
+int ghx_hdcheck(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                const void *f, int hel, int chk, double *out);
 /* mhdcheck (mhd :339-547) without file output: out[11] = {eng, ens, cur, engk,
  * engm, helk, helm, crh, asq, divk, divm}; crs = 2 (Hall) is refused */
 int ghx_mhdcheck(ghx_grid *g, const void *a, const void *b, const void *c, const void *ma, const void *mb,

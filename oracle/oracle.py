@@ -265,6 +265,11 @@ class Grid:
         lib().og_spectr1d(self._g, _p(a), int(dir), _p(out))
         return out
 
+    def hdcheck(self, a, b, c, d, e, f, hel=1, chk=1):
+        """hdcheck (pseudospec3D_hd.f90:984-1105) from its parts: {eng, ens, pot, khe, div}."""
+        return np.array([self.energy(a, b, c, 1), self.energy(a, b, c, 0), self.cross(a, b, c, d, e, f, 1),
+                         self.helicity(a, b, c) if hel == 1 else 0.0, self.divergence(a, b, c) if chk == 1 else 0.0])
+
     def mhdcheck(self, a, b, c, ma, mb, mc, hel=1, crs=1, chk=1):
         out = np.zeros(11)
         lib().og_mhdcheck(self._g, *map(_p, (a, b, c, ma, mb, mc)), int(hel), int(crs), int(chk), _p(out))

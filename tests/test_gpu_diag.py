@@ -49,3 +49,17 @@ def test_mhdcheck(gpu, shape, arb, L):
         scale = max(abs(x) for x in want)
         for x, y in zip(got, want):
             assert x == pytest.approx(y, rel=1e-12, abs=1e-13 * scale)
+
+
+@pytest.mark.parametrize("shape,arb,L", CASES[:3])
+def test_hdcheck(gpu, shape, arb, L):
+    g, og = grids(shape, arb, L)
+    v = [rand_spec(og, s) for s in (61, 62, 63)]
+    f = [rand_spec(og, s) for s in (64, 65, 66)]
+    V, F = [to_gpu(x) for x in v], [to_gpu(x) for x in f]
+    for hel, chk in ((1, 1), (0, 0)):
+        got = g.hdcheck(*V, *F, hel=hel, chk=chk)
+        want = og.hdcheck(*v, *f, hel=hel, chk=chk)
+        scale = max(abs(x) for x in want)
+        for x, y in zip(got, want):
+            assert x == pytest.approx(y, rel=1e-12, abs=1e-13 * scale)

@@ -16,7 +16,7 @@ EXE = os.path.join(ROOT, "ghost_amd", "fortran", "test_fftp_hip")
 def test_fortran_host_side(gpu, tmp_path):
     from oracle import oracle as O
     assert os.path.exists(EXE), "build the Fortran host side first (__graft_entry__.build)"
-    run = subprocess.run([EXE, str(tmp_path)], capture_output=True, text=True, timeout=180)
+    run = subprocess.run([EXE, str(tmp_path)], capture_output=True, text=True, timeout=180, cwd=tmp_path)
     print(run.stdout)
     assert run.returncode == 0, run.stdout + run.stderr
     assert "ALL OK" in run.stdout
@@ -29,6 +29,39 @@ def test_fortran_host_side(gpu, tmp_path):
         og.hd_step(v, 1e-3, 1e-2, ord=2)
     for k in range(3):
         assert np.linalg.norm(got[k] - v[k]) <= 1e-12 * max(np.linalg.norm(v[k]), np.linalg.norm(v[0]))
+    # GHOST's text diagnostics (spectrum, entrans kin 1 and 0, heltrans,
+    # spectr1d of the x component along z, hdcheck) of a k^4 exp field
+    # against a random forcing field, both drawn in the Fortran program
+    v, _ = og.initialv_k4exp(3.0, 1.0, 11)
+    w, _ = og.initialfv_random(1.0, 6.0, 1.0, 29)
+    nb = n // 2 + 1
+
+    def table(name):
+        return np.loadtxt(tmp_path / name)
+
+    def close(a, b):
+        a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+        return np.all(np.abs(a - b) <= 1e-13 * np.max(np.abs(b)) + 1e-300)
+
+    Ek, Hk = og.spectrumc(*v, kin=1, hel=1)
+    for name, col in (("kspectrum.0001.txt", 0.5 * Ek), ("khelicity.0001.txt", Hk),
+                      ("ktransfer.0001.txt", og.entrans(*v, *w, kin=1)),
+                      ("mtransfer.0001.txt", og.entrans(*v, *w, kin=0)),
+                      ("hktransfer.0001.txt", og.heltrans(*v, *w))):
+        t = table(name)
+        assert t.shape == (nb, 2), name
+        assert np.array_equal(t[:, 0], np.arange(1, nb + 1, dtype=float)), name
+        assert close(t[:, 1], col), name
+    t = table("kspec1dxz.0001.txt")
+    assert np.array_equal(t[:, 0], np.arange(nb, dtype=float))
+    assert close(t[:, 1], 0.5 * og.spectr1d(v[0], 3))
+    want = og.hdcheck(*v, *w, hel=1, chk=1)
+    bal = table("balance.txt")
+    assert bal[0] == pytest.approx(20 * 1e-3, rel=1e-6)
+    assert min(abs(want[:4])) > 1e-8 * want[0]
+    assert np.allclose(bal[1:], want[:3], rtol=1e-13, atol=1e-13 * max(abs(want[:3])))
+    assert table("helicity.txt")[1] == pytest.approx(want[3], rel=1e-13, abs=1e-13 * want[0])
+    assert table("divergence.txt")[1] == pytest.approx(want[4], abs=1e-13 * want[0])
     # GHOST's binary output of that v: the resident path (vx, vy, vz) and the
     # host-array path through io_write (hx) write the same bytes; the file is
     # c2r(v/N^3) in Fortran order
