@@ -94,10 +94,12 @@ def _declare(L):
         "ghx_initialfv_random": (_I, [_P, _D, _D, _D, C.POINTER(C.c_int32), _P, _P, _P]),
         "ghx_initialv_k4exp": (_I, [_P, _D, _D, C.POINTER(C.c_int32), _P, _P, _P]),
         "ghx_hd_rkstep2": (_I, [_P, _I, _D, _D, _PP, _PP, _PP, _PP]),
+        "ghx_roth_rkstep2": (_I, [_P, _I, _D, _D, _D, _PP, _PP, _PP, _PP]),
         "ghx_hd_create": (_I, [_PP, _P, _I, _D, _D]),
         "ghx_hd_destroy": (_I, [_P]),
         "ghx_hd_fields": (_I, [_P, _PP, _PP]),
         "ghx_hd_set_forcing": (_I, [_P, _I]),
+        "ghx_hd_set_rotation": (_I, [_P, _D]),
         "ghx_hd_step": (_I, [_P, _I]),
         "ghx_hd_touch": (_I, [_P]),
         "ghx_mhd_touch": (_I, [_P]),
@@ -113,6 +115,7 @@ def _declare(L):
         "ghx_bouss_fields": (_I, [_P, _PP, _PP, _PP, _PP]),
         "ghx_bouss_touch": (_I, [_P]),
         "ghx_bouss_set_forcing": (_I, [_P, _I]),
+        "ghx_bouss_set_rotation": (_I, [_P, _D]),
         "ghx_bouss_step": (_I, [_P, _I]),
         "ghx_bouss_set_tracers": (_I, [_P, _P]),
         "ghx_ensemble_create": (_I, [_PP, _I, C.POINTER(_I), _DP, _D, _I]),

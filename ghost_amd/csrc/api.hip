@@ -1144,8 +1144,11 @@ static PruneShares prune_shares(ghx_grid *g) {
 }
 
 // Fused HD substep: q holds 9 Q-layout fields of scratch.
+// omegaz != 0: the rotating-frame step of include/roth_rkstep2.f90 (Coriolis
+// force added to the products in the last pass).
 static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const cplx *const *v, const cplx *const *c,
-                      const cplx *const *f, const uint8_t *fline, cplx *const *out, int prune = 0) {
+                      const cplx *const *f, const uint8_t *fline, cplx *const *out, int prune = 0,
+                      double omegaz = 0.0) {
   const size_t nc = g->nc;
   const FFTCtx &ctx = g->plan.ctx;
   Chain ch = chain_slots(q, ghx::slot_points(g->gk));
@@ -1162,6 +1165,7 @@ static int hd_substep(ghx_grid *g, cplx *q, int o, double dt, double nu, const c
   PROF(P_YFWD, (3 * ps.fx + 3 * ps.fl) * sc, ctx.stream, ghx::yfwd(ctx, 3, ch.p3, ch.q3, prune));
   ghx::RKParams rk;
   rk.dt = dt; rk.nu = nu; rk.rmp = 1. / (double)o;
+  rk.om2 = 2 * omegaz;
   for (int k = 0; k < 3; ++k) {
     rk.v[k] = v[k]; rk.c[k] = c[k]; rk.out[k] = out[k];
     rk.f[k] = f ? f[k] : nullptr;
@@ -1197,9 +1201,9 @@ static int line_flags(ghx_grid *g, const cplx *a, const cplx *b, const cplx *c, 
   return GHX_OK;
 }
 
-extern "C" int ghx_hd_rkstep2(ghx_grid *g, int o, double dt, double nu, const void *const *v_in, const void *const *C,
-                              const void *const *f, void *const *v_out) {
-  ARGCHK(g && o >= 1 && v_in && C && v_out, "bad hd_rkstep2 arguments");
+static int rkstep2(ghx_grid *g, int o, double dt, double nu, double omegaz, const void *const *v_in,
+                   const void *const *C, const void *const *f, void *const *v_out) {
+  ARGCHK(g && o >= 1 && v_in && C && v_out, "bad rkstep2 arguments");
   int rc = plan_scratch(&g->plan, 12);
   if (rc) return rc;
   const cplx *v[3], *c[3], *ff[3];
@@ -1216,7 +1220,18 @@ extern "C" int ghx_hd_rkstep2(ghx_grid *g, int o, double dt, double nu, const vo
     fl = (uint8_t *)g->work;
     if ((rc = line_flags(g, ff[0], ff[1], ff[2], fl))) return rc;
   }
-  return hd_substep(g, g->plan.q, o, dt, nu, v, c, f ? ff : nullptr, fl, out);
+  return hd_substep(g, g->plan.q, o, dt, nu, v, c, f ? ff : nullptr, fl, out, 0, omegaz);
+}
+
+extern "C" int ghx_hd_rkstep2(ghx_grid *g, int o, double dt, double nu, const void *const *v_in, const void *const *C,
+                              const void *const *f, void *const *v_out) {
+  return rkstep2(g, o, dt, nu, 0.0, v_in, C, f, v_out);
+}
+
+// include/roth_rkstep2.f90:1-49: the HD substep in a frame rotating about z
+extern "C" int ghx_roth_rkstep2(ghx_grid *g, int o, double dt, double nu, double omegaz, const void *const *v_in,
+                                const void *const *C, const void *const *f, void *const *v_out) {
+  return rkstep2(g, o, dt, nu, omegaz, v_in, C, f, v_out);
 }
 
 struct ghx_hd {
@@ -1234,6 +1249,7 @@ struct ghx_hd {
   double sgs_delta = 0.0;
   void *sgs_mem = nullptr;  // grad 9 + tau 6 real, tau 6 + force 3 spectral
   uint8_t *sgs_fline = nullptr;
+  double omegaz = 0.0;  // rotation rate about z (include/roth_rkstep2.f90)
 };
 
 static int tracers_begin_step(ghx_tracers *t);
@@ -1285,6 +1301,12 @@ extern "C" int ghx_hd_fields(ghx_hd *s, void **v, void **f) {
     if (v) v[k] = s->v[k];
     if (f) f[k] = s->f[k];
   }
+  return GHX_OK;
+}
+
+extern "C" int ghx_hd_set_rotation(ghx_hd *s, double omegaz) {
+  ARGCHK(s, "null solver");
+  s->omegaz = omegaz;
   return GHX_OK;
 }
 
@@ -1467,7 +1489,7 @@ static int hd_step_once(ghx_hd *s) {
       fo = fs;
       fl = s->sgs_fline;
     }
-    int rc = hd_substep(s->g, s->q, o, s->dt, s->nu, vin, v0, fo, fl, out, prune);
+    int rc = hd_substep(s->g, s->q, o, s->dt, s->nu, vin, v0, fo, fl, out, prune, s->omegaz);
     if (rc) return rc;
     // main3D.fpp:3233-3267: the tracers take a stage on the velocity this
     // substep produced, which is dealiased whatever the input was
@@ -1856,14 +1878,16 @@ struct ghx_bouss {
   bool forced = false;
   bool clean = false;
   ghx_tracers *tr = nullptr;
+  // the rotating solver (include/rotbouss_rkstep2.f90): Coriolis force of
+  // rotation omegaz about z, and no ord = 3 correction
+  bool rot = false;
+  double omegaz = 0.0;
 };
 
 extern "C" int ghx_bouss_create(ghx_bouss **out, ghx_grid *g, int ord, double dt, double nu, double kappa,
                                 double xmom, double xtemp) {
   ARGCHK(out && g, "bad bouss_create arguments");
-  if (ord < 1 || ord > 2)
-    return set_err(GHX_ERR_ARG, "Boussinesq solver supports ord 1 and 2 (the ord = 3 correction of "
-                                "bouss_rkstep2.f90:73-145 is not built)");
+  ARGCHK(ord >= 1 && ord <= 3, "Boussinesq solver: ord 1 to 3");
   ghx_bouss *h = new (std::nothrow) ghx_bouss();
   if (!h) return set_err(GHX_ERR_NOMEM, "host allocation failed");
   h->g = g; h->ord = ord; h->dt = dt; h->nu = nu; h->kappa = kappa; h->xmom = xmom; h->xtemp = xtemp;
@@ -1908,6 +1932,13 @@ extern "C" int ghx_bouss_fields(ghx_bouss *s, void **v, void **th, void **f, voi
 extern "C" int ghx_bouss_touch(ghx_bouss *s) {
   ARGCHK(s, "null solver");
   s->clean = false;
+  return GHX_OK;
+}
+
+extern "C" int ghx_bouss_set_rotation(ghx_bouss *s, double omegaz) {
+  ARGCHK(s, "null solver");
+  s->rot = true;
+  s->omegaz = omegaz;
   return GHX_OK;
 }
 
@@ -1962,6 +1993,7 @@ static int bouss_substep(ghx_bouss *s, int o, const cplx *const *vin, cplx *cons
   ghx::BoussRKParams rk;
   rk.dt = s->dt; rk.nu = s->nu; rk.kappa = s->kappa; rk.xmom = s->xmom; rk.xtemp = s->xtemp;
   rk.rmp = 1. / (double)o;
+  rk.om2 = s->rot ? 2 * s->omegaz : 0.0;
   for (int k = 0; k < 4; ++k) {
     rk.v[k] = vin[k];
     rk.c[k] = s->s[k];
@@ -1977,6 +2009,9 @@ static int bouss_substep(ghx_bouss *s, int o, const cplx *const *vin, cplx *cons
 
 extern "C" int ghx_bouss_step(ghx_bouss *s, int nsteps) {
   ARGCHK(s && nsteps >= 0, "bad bouss_step arguments");
+  if (s->ord == 3 && !s->rot)
+    return set_err(GHX_ERR_ARG, "Boussinesq solver without rotation: ord = 3 needs the correction of "
+                                "bouss_rkstep2.f90:73-145, which is not built (ord 1 and 2 are)");
   const cplx *v0[4], *v1[4];
   for (int k = 0; k < 4; ++k) { v0[k] = s->s[k]; v1[k] = s->s1[k]; }
   for (int n = 0; n < nsteps; ++n) {

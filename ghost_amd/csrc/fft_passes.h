@@ -44,6 +44,7 @@ struct RKParams {
   const uint8_t *fline;  // per (kx,ky) line: 1 if the forcing is non-zero there
   size_t bstride = 0;       // elements between batch members (v, c, out, f)
   const double *nu_b = nullptr;  // per-member viscosity of a batched launch (else nu)
+  double om2 = 0.0;  // 2 omegaz: Coriolis force of a frame rotating about z (0: none)
 };
 
 struct MHDRKParams {
@@ -61,6 +62,7 @@ struct BoussRKParams {
   cplx *out[4];
   const cplx *f[4];
   const uint8_t *fline;
+  double om2 = 0.0;  // 2 omegaz (rotating frame), 0: none
 };
 
 bool fft_size_supported(int n);

@@ -386,6 +386,14 @@ GHX_DEV void project_hd(const cplx &a, const cplx &b, const cplx &c, double kx, 
   gz = k == 0 ? c_make(-c.x, -c.y) : c_add(c_make(-c.x, -c.y), c_make(kz * dot.x / kk2, kz * dot.y / kk2));
 }
 
+// Coriolis force of include/roth_rkstep2.f90:6-14 added to the products
+// (a, b) = FFT of (w x u)_x,y before the projection: a - 2 omegaz v_y,
+// b + 2 omegaz v_x, with om2 = 2 omegaz.
+GHX_DEV void coriolis(cplx &a, cplx &b, const cplx &vx, const cplx &vy, double om2) {
+  a = c_make(a.x - om2 * vy.x, a.y - om2 * vy.y);
+  b = c_make(b.x + om2 * vx.x, b.y + om2 * vx.y);
+}
+
 // RK update of include/hd_rkstep2.f90:12-32 for one component:
 // v = C + dt*(nu*(-kk2 v) + N + f)*rmp
 GHX_DEV cplx rk_update(cplx cc, cplx vv, cplx nl, cplx ff, double kk2, double nu, double dt, double rmp) {
@@ -447,6 +455,7 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in_, GridK g, const
     if (kn2 <= g.kmax && kn2 >= g.tiny) {
       const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
       cplx n0, n1, n2;
+      if (rk.om2 != 0.0) coriolis(P[0][m], P[1][m], rk.v[0][s], rk.v[1][s], rk.om2);
       project_hd(P[0][m], P[1][m], P[2][m], kx, ky, kz, kk2, i, j, k, n0, n1, n2);
       rk.out[0][s] = rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
       rk.out[1][s] = rk_update(rk.c[1][s], rk.v[1][s], n1, forced ? rk.f[1][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
@@ -511,6 +520,8 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, con
           vz_in[k] = rk.v[2][s];
           const cplx th = rk.v[3][s];
           const cplx nzb = c_make(P[2][m].x + rk.xmom * th.x, P[2][m].y + rk.xmom * th.y);
+          // rotbouss_rkstep2.f90:6-15: Coriolis on x, y beside the gravity on z
+          if (rk.om2 != 0.0) coriolis(P[0][m], P[1][m], rk.v[0][s], rk.v[1][s], rk.om2);
           cplx n0, n1, n2;
           project_hd(P[0][m], P[1][m], nzb, kx, ky, kz, kk2, i, j, k, n0, n1, n2);
           rk.out[0][s] = rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp);

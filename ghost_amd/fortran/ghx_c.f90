@@ -251,6 +251,15 @@ MODULE ghx_c
     END FUNCTION
 
     ! RK substep and resident solvers (include/hd_rkstep2.f90, main3D.fpp:3202-3267)
+    INTEGER(C_INT) FUNCTION ghx_roth_rkstep2(g, o, dt, nu, omegaz, v_in, c, f, v_out) &
+        BIND(C, name="ghx_roth_rkstep2")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g
+      INTEGER(C_INT), VALUE :: o
+      REAL(C_DOUBLE), VALUE :: dt, nu, omegaz
+      TYPE(C_PTR) :: v_in(3), c(3), v_out(3)
+      TYPE(C_PTR), VALUE :: f
+    END FUNCTION
     INTEGER(C_INT) FUNCTION ghx_hd_rkstep2(g, o, dt, nu, v_in, c, f, v_out) BIND(C, name="ghx_hd_rkstep2")
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: g
@@ -278,6 +287,11 @@ MODULE ghx_c
     INTEGER(C_INT) FUNCTION ghx_hd_touch(s) BIND(C, name="ghx_hd_touch")
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: s
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_hd_set_rotation(s, omegaz) BIND(C, name="ghx_hd_set_rotation")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: s
+      REAL(C_DOUBLE), VALUE :: omegaz
     END FUNCTION
     INTEGER(C_INT) FUNCTION ghx_hd_set_forcing(s, enabled) BIND(C, name="ghx_hd_set_forcing")
       IMPORT :: C_INT, C_PTR

@@ -406,4 +406,29 @@ CONTAINS
     END IF
   END SUBROUTINE hd_rkstep2
 
+  ! -- include/roth_rkstep2.f90: hd_rkstep2 in a frame rotating about z ---------
+  SUBROUTINE roth_rkstep2(o, dt, nu, omegaz, v, c, f)
+    INTEGER, INTENT(IN) :: o
+    REAL(KIND=GP), INTENT(IN) :: dt, nu, omegaz
+    TYPE(ghx_field), INTENT(INOUT) :: v(3)
+    TYPE(ghx_field), INTENT(IN) :: c(3)
+    TYPE(ghx_field), INTENT(IN), OPTIONAL :: f(3)
+    TYPE(C_PTR), TARGET :: pv(3), pc(3), pf(3)
+    INTEGER :: k
+    DO k = 1, 3
+      pv(k) = v(k)%p
+      pc(k) = c(k)%p
+    END DO
+    IF (PRESENT(f)) THEN
+      DO k = 1, 3
+        pf(k) = f(k)%p
+      END DO
+      CALL ghx_check(ghx_roth_rkstep2(ghx_grid_h, INT(o, C_INT), REAL(dt, C_DOUBLE), REAL(nu, C_DOUBLE), &
+                                      REAL(omegaz, C_DOUBLE), pv, pc, C_LOC(pf), pv), 'roth_rkstep2')
+    ELSE
+      CALL ghx_check(ghx_roth_rkstep2(ghx_grid_h, INT(o, C_INT), REAL(dt, C_DOUBLE), REAL(nu, C_DOUBLE), &
+                                      REAL(omegaz, C_DOUBLE), pv, pc, C_NULL_PTR, pv), 'roth_rkstep2')
+    END IF
+  END SUBROUTINE roth_rkstep2
+
 END MODULE pseudospec_hip

@@ -235,8 +235,9 @@ class Grid:
         return v, s.value
 
     # ---- RK ----------------------------------------------------------------
-    def hd_rkstep2(self, o, dt, nu, v_in, C_, f, v_out):
-        """include/hd_rkstep2.f90 for one value of o, fused on the GPU."""
+    def hd_rkstep2(self, o, dt, nu, v_in, C_, f, v_out, omegaz=None):
+        """include/hd_rkstep2.f90 for one value of o, fused on the GPU; with
+        omegaz, include/roth_rkstep2.f90 (a frame rotating about z)."""
         L_ = _lib.load()
         pv, _a = _lib.ptr_array([t.data_ptr() for t in map(lambda t: t, v_in)])
         pc, _b = _lib.ptr_array([t.data_ptr() for t in C_])
@@ -247,7 +248,10 @@ class Grid:
             pf, _c = _lib.ptr_array([t.data_ptr() for t in f])
         else:
             pf = None
-        _lib.check(L_.ghx_hd_rkstep2(self._h, int(o), float(dt), float(nu), pv, pc, pf, po))
+        if omegaz is None:
+            _lib.check(L_.ghx_hd_rkstep2(self._h, int(o), float(dt), float(nu), pv, pc, pf, po))
+        else:
+            _lib.check(L_.ghx_roth_rkstep2(self._h, int(o), float(dt), float(nu), float(omegaz), pv, pc, pf, po))
 
 
 def randu(seed):

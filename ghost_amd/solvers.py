@@ -78,6 +78,11 @@ class HDSolver(_Resident):
         for p in self._f:
             _lib.check(L.ghx_phaseshift(self.grid._h, C.c_void_p(p), float(cdump.real), float(cdump.imag)))
 
+    def set_rotation(self, omegaz):
+        """The ROTH solver (include/roth_rkstep2.f90): a frame rotating about z
+        at rate omegaz."""
+        _lib.check(_lib.load().ghx_hd_set_rotation(self._h, float(omegaz)))
+
     def set_sgs(self, mlp, delta):
         """Attach the learned closure (ghost_amd.closure.MLP; None detaches): before
         each substep the forcing becomes f + P(-div tau), tau = mlp(grad u, delta)."""
@@ -196,6 +201,19 @@ class BoussSolver(_Resident):
         """As HDSolver.set_tracers."""
         _lib.check(_lib.load().ghx_bouss_set_tracers(self._h, parts._h if parts is not None else None))
         self._parts = parts
+
+    def set_rotation(self, omegaz):
+        """The ROTBOUSS solver (include/rotbouss_rkstep2.f90): Coriolis force of
+        rotation omegaz about z; any RK order."""
+        _lib.check(_lib.load().ghx_bouss_set_rotation(self._h, float(omegaz)))
+
+
+class PHDSolver(BoussSolver):
+    """Passive scalar th advected by the flow (include/phd_rkstep{1,2}.f90):
+    the Boussinesq step without buoyancy and heat current."""
+
+    def __init__(self, grid: Grid, dt, nu, kappa, ord=2):
+        super().__init__(grid, dt, nu, kappa, 0.0, 0.0, ord=ord)
 
 
 class Ensemble:

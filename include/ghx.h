@@ -192,6 +192,12 @@ int ghx_initialv_k4exp(ghx_grid *g, double k0, double u0, int32_t *seed, void *v
 int ghx_hd_rkstep2(ghx_grid *g, int o, double dt, double nu, const void *const *v_in, const void *const *C,
                    const void *const *f, void *const *v_out);
 
+/* include/roth_rkstep2.f90:1-49: the same pass in a frame rotating about z
+ * at rate omegaz; the Coriolis force enters the products before the
+ * projection (x: - 2 omegaz v_y, y: + 2 omegaz v_x). */
+int ghx_roth_rkstep2(ghx_grid *g, int o, double dt, double nu, double omegaz, const void *const *v_in,
+                     const void *const *C, const void *const *f, void *const *v_out);
+
 typedef struct ghx_hd ghx_hd;
 /* HD solver state: v (3), forcing f (3), ping-pong and pass scratch, all on device. */
 int ghx_hd_create(ghx_hd **s, ghx_grid *g, int ord, double dt, double nu);
@@ -207,6 +213,8 @@ int ghx_hd_touch(ghx_hd *s);
 int ghx_hd_set_forcing(ghx_hd *s, int enabled);
 /* nsteps full RK steps (main3D.fpp:3202-3231 without output/diagnostics) */
 int ghx_hd_step(ghx_hd *s, int nsteps);
+/* the ROTH solver (include/roth_rkstep{1,2}.f90): rotation rate omegaz about z */
+int ghx_hd_set_rotation(ghx_hd *s, double omegaz);
 
 typedef struct ghx_mhd ghx_mhd;
 /* MHD solver state: v (3), a (3, vector potential), f (3), m (3). */
@@ -222,8 +230,10 @@ typedef struct ghx_bouss ghx_bouss;
 /* Boussinesq solver (include/bouss_rkstep1.f90, bouss_rkstep2.f90:1-71 in the
  * loop of main3D.fpp:3202-3231): state v (3) and th, forcing f (3) and fs.
  * kappa: diffusivity; xmom, xtemp: the buoyancy and heat-current factors
- * already multiplied by bvfreq (main3D.fpp:973-974).  ord 1 or 2; the ord = 3
- * correction of bouss_rkstep2.f90:73-145 is not built (GHX_ERR_ARG). */
+ * already multiplied by bvfreq (main3D.fpp:973-974).  ord 1 to 3; ord = 3
+ * without rotation needs the correction of bouss_rkstep2.f90:73-145, which
+ * is not built (ghx_bouss_step returns GHX_ERR_ARG).  With xmom = xtemp = 0
+ * th is a passive scalar: the PHD solver (include/phd_rkstep{1,2}.f90). */
 int ghx_bouss_create(ghx_bouss **s, ghx_grid *g, int ord, double dt, double nu, double kappa, double xmom,
                      double xtemp);
 int ghx_bouss_destroy(ghx_bouss *s);
@@ -233,6 +243,9 @@ int ghx_bouss_fields(ghx_bouss *s, void **v, void **th, void **f, void **fs);
 int ghx_bouss_touch(ghx_bouss *s);
 int ghx_bouss_set_forcing(ghx_bouss *s, int enabled);
 int ghx_bouss_step(ghx_bouss *s, int nsteps);
+/* the ROTBOUSS solver (include/rotbouss_rkstep{1,2}.f90): Coriolis force of
+ * rotation omegaz about z, and RK of any order without a correction */
+int ghx_bouss_set_rotation(ghx_bouss *s, double omegaz);
 
 /* ---------------------------------------------------------------------- */
 /* Ensembles: many independent unforced HD runs (cubic 2 pi boxes of any     */

@@ -925,14 +925,27 @@ void og_initialv_k4exp(const og_grid *g, double k0, double u0, int32_t *seed, oc
 /* Runge-Kutta steps                                                        */
 /* ------------------------------------------------------------------------ */
 
-/* include/hd_rkstep2.f90:3-32 for one value of o */
-void og_hd_substep(const og_grid *g, int o, double dt, double nu, ocplx *vx, ocplx *vy, ocplx *vz,
-                   const ocplx *c1, const ocplx *c2, const ocplx *c3, const ocplx *fx,
-                   const ocplx *fy, const ocplx *fz) {
+/* Coriolis force of include/roth_rkstep2.f90:6-14 added to the products
+ * before the projection: C4 - 2 omegaz vy, C5 + 2 omegaz vx */
+static void coriolis(const og_grid *g, double omegaz, const ocplx *vx, const ocplx *vy, ocplx *C4, ocplx *C5) {
+  const size_t nc = ncplx(g);
+  OMP_FOR
+  for (size_t m = 0; m < nc; ++m) {
+    C4[m] = C4[m] - 2 * omegaz * vy[m];
+    C5[m] = C5[m] + 2 * omegaz * vx[m];
+  }
+}
+
+/* include/hd_rkstep2.f90:3-32 for one value of o; with omegaz != 0 the
+ * rotating-frame step include/roth_rkstep2.f90:1-49 */
+static void hd_substep_rot(const og_grid *g, int o, double dt, double nu, double omegaz, ocplx *vx, ocplx *vy,
+                           ocplx *vz, const ocplx *c1, const ocplx *c2, const ocplx *c3, const ocplx *fx,
+                           const ocplx *fy, const ocplx *fz) {
   size_t nc = ncplx(g);
   ocplx *W = (ocplx *)malloc(sizeof(ocplx) * nc * 5);
   ocplx *C4 = W, *C5 = W + nc, *C6 = W + 2 * nc, *C7 = W + 3 * nc, *C8 = W + 4 * nc;
   og_prodre3(g, vx, vy, vz, C4, C5, C6);
+  if (omegaz != 0.0) coriolis(g, omegaz, vx, vy, C4, C5);
   og_nonlhd3(g, C4, C5, C6, C7, 1);
   og_nonlhd3(g, C4, C5, C6, C8, 2);
   og_nonlhd3(g, C4, C5, C6, C4, 3);
@@ -952,6 +965,31 @@ void og_hd_substep(const og_grid *g, int o, double dt, double nu, ocplx *vx, ocp
     }
   }
   free(W);
+}
+
+void og_hd_substep(const og_grid *g, int o, double dt, double nu, ocplx *vx, ocplx *vy, ocplx *vz,
+                   const ocplx *c1, const ocplx *c2, const ocplx *c3, const ocplx *fx,
+                   const ocplx *fy, const ocplx *fz) {
+  hd_substep_rot(g, o, dt, nu, 0.0, vx, vy, vz, c1, c2, c3, fx, fy, fz);
+}
+
+void og_roth_substep(const og_grid *g, int o, double dt, double nu, double omegaz, ocplx *vx, ocplx *vy,
+                     ocplx *vz, const ocplx *c1, const ocplx *c2, const ocplx *c3, const ocplx *fx,
+                     const ocplx *fy, const ocplx *fz) {
+  hd_substep_rot(g, o, dt, nu, omegaz, vx, vy, vz, c1, c2, c3, fx, fy, fz);
+}
+
+/* main3D.fpp:3202-3231 with include/roth_rkstep1.f90 and roth_rkstep2.f90 */
+void og_roth_step(const og_grid *g, int ord, double dt, double nu, double omegaz, ocplx *vx, ocplx *vy,
+                  ocplx *vz, const ocplx *fx, const ocplx *fy, const ocplx *fz) {
+  size_t nc = ncplx(g);
+  ocplx *C = (ocplx *)malloc(sizeof(ocplx) * nc * 3);
+  memcpy(C, vx, sizeof(ocplx) * nc);
+  memcpy(C + nc, vy, sizeof(ocplx) * nc);
+  memcpy(C + 2 * nc, vz, sizeof(ocplx) * nc);
+  for (int o = ord; o >= 1; --o)
+    hd_substep_rot(g, o, dt, nu, omegaz, vx, vy, vz, C, C + nc, C + 2 * nc, fx, fy, fz);
+  free(C);
 }
 
 /* main3D.fpp:3202-3231 with include/hd_rkstep1.f90 */
@@ -1281,14 +1319,16 @@ void og_advect3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c
  * correction of :73-145 is og_bouss_ord3_correction).  State (vx,vy,vz,th)
  * updated in place from the step-start copies (c1,c2,c3,c20); forcing
  * (fx,fy,fz,fs) may be NULL. */
-void og_bouss_substep(const og_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
-                      ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *c1, const ocplx *c2,
-                      const ocplx *c3, const ocplx *c20, const ocplx *fx, const ocplx *fy, const ocplx *fz,
-                      const ocplx *fs) {
+static void bouss_substep_rot(const og_grid *g, int o, double dt, double nu, double kappa, double xmom,
+                              double xtemp, double omegaz, ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th,
+                              const ocplx *c1, const ocplx *c2, const ocplx *c3, const ocplx *c20,
+                              const ocplx *fx, const ocplx *fy, const ocplx *fz, const ocplx *fs) {
   size_t nc = ncplx(g);
   ocplx *W = (ocplx *)malloc(sizeof(ocplx) * nc * 5);
   ocplx *C4 = W, *C5 = W + nc, *C6 = W + 2 * nc, *C7 = W + 3 * nc, *C8 = W + 4 * nc;
   og_prodre3(g, vx, vy, vz, C4, C5, C6);
+  /* include/rotbouss_rkstep2.f90:6-15: Coriolis, then gravity */
+  if (omegaz != 0.0) coriolis(g, omegaz, vx, vy, C4, C5);
   OMP_FOR
   for (size_t m = 0; m < nc; ++m) C6[m] = C6[m] + xmom * th[m]; /* gravity */
   og_nonlhd3(g, C4, C5, C6, C7, 1);
@@ -1318,6 +1358,13 @@ void og_bouss_substep(const og_grid *g, int o, double dt, double nu, double kapp
     }
   }
   free(W);
+}
+
+void og_bouss_substep(const og_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
+                      ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *c1, const ocplx *c2,
+                      const ocplx *c3, const ocplx *c20, const ocplx *fx, const ocplx *fy, const ocplx *fz,
+                      const ocplx *fs) {
+  bouss_substep_rot(g, o, dt, nu, kappa, xmom, xtemp, 0.0, vx, vy, vz, th, c1, c2, c3, c20, fx, fy, fz, fs);
 }
 
 /* include/bouss_rkstep2.f90:73-145, applied after the last substep when
@@ -1380,6 +1427,23 @@ void og_bouss_step(const og_grid *g, int ord, double dt, double nu, double kappa
     if (ord == 3 && o <= 1)
       bouss_ord3_correction(g, dt, xmom, vx, vy, vz, C, C + nc, C + 2 * nc, C + 3 * nc);
   }
+  free(C);
+}
+
+/* main3D.fpp:3202-3231 with include/rotbouss_rkstep1.f90 and
+ * rotbouss_rkstep2.f90 (no ord = 3 correction there) */
+void og_rotbouss_step(const og_grid *g, int ord, double dt, double nu, double kappa, double xmom, double xtemp,
+                      double omegaz, ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *fx,
+                      const ocplx *fy, const ocplx *fz, const ocplx *fs) {
+  size_t nc = ncplx(g);
+  ocplx *C = (ocplx *)malloc(sizeof(ocplx) * nc * 4);
+  memcpy(C, vx, sizeof(ocplx) * nc);
+  memcpy(C + nc, vy, sizeof(ocplx) * nc);
+  memcpy(C + 2 * nc, vz, sizeof(ocplx) * nc);
+  memcpy(C + 3 * nc, th, sizeof(ocplx) * nc);
+  for (int o = ord; o >= 1; --o)
+    bouss_substep_rot(g, o, dt, nu, kappa, xmom, xtemp, omegaz, vx, vy, vz, th, C, C + nc, C + 2 * nc,
+                      C + 3 * nc, fx, fy, fz, fs);
   free(C);
 }
 

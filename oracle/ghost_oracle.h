@@ -93,6 +93,16 @@ void og_hd_substep(const og_grid *g, int o, double dt, double nu,
                    const ocplx *c1, const ocplx *c2, const ocplx *c3,
                    const ocplx *fx, const ocplx *fy, const ocplx *fz);
 
+/* HD in a frame rotating about z (include/roth_rkstep{1,2}.f90): the
+ * Coriolis force enters the products before the projection */
+void og_roth_step(const og_grid *g, int ord, double dt, double nu, double omegaz,
+                  ocplx *vx, ocplx *vy, ocplx *vz,
+                  const ocplx *fx, const ocplx *fy, const ocplx *fz);
+void og_roth_substep(const og_grid *g, int o, double dt, double nu, double omegaz,
+                     ocplx *vx, ocplx *vy, ocplx *vz,
+                     const ocplx *c1, const ocplx *c2, const ocplx *c3,
+                     const ocplx *fx, const ocplx *fy, const ocplx *fz);
+
 /* MHD (pseudospec3D_mhd.f90, include/mhd_rkstep2.f90) */
 void og_vector3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c,
                 const ocplx *d, const ocplx *e, const ocplx *f,
@@ -136,6 +146,11 @@ void og_bouss_substep(const og_grid *g, int o, double dt, double nu, double kapp
 void og_bouss_step(const og_grid *g, int ord, double dt, double nu, double kappa, double xmom, double xtemp,
                    ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *fx, const ocplx *fy,
                    const ocplx *fz, const ocplx *fs);
+
+/* rotating Boussinesq (include/rotbouss_rkstep{1,2}.f90) */
+void og_rotbouss_step(const og_grid *g, int ord, double dt, double nu, double kappa, double xmom, double xtemp,
+                      double omegaz, ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *fx,
+                      const ocplx *fy, const ocplx *fz, const ocplx *fs);
 
 /* analysis/trunc3D.fpp: spectral truncation of c (grid g) onto the smaller grid t */
 void og_truncate(const og_grid *g, const ocplx *c, const og_grid *t, ocplx *out);

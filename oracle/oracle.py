@@ -91,6 +91,9 @@ def _declare(L):
     L.og_advect3.argtypes = [_G, _P, _P, _P, _P, _P]
     L.og_truncate.argtypes = [_G, _P, _G, _P]
     L.og_bouss_step.argtypes = [_G, C.c_int] + [C.c_double] * 5 + [_P] * 8
+    L.og_rotbouss_step.argtypes = [_G, C.c_int] + [C.c_double] * 6 + [_P] * 8
+    L.og_roth_step.argtypes = [_G, C.c_int] + [C.c_double] * 3 + [_P] * 6
+    L.og_roth_substep.argtypes = [_G, C.c_int] + [C.c_double] * 3 + [_P] * 9
     L.og_spline_coef.argtypes = [_G, _P, _P, C.c_int]
     L.og_spline_interp.argtypes = [_G, _P, C.c_long, _P, _P, _P, C.c_int, _P]
     L.og_tracer_stage.argtypes = [_G, _P, _P, _P, C.c_long, _P, _P, _P, _P, _P, _P, C.c_double,
@@ -312,6 +315,15 @@ class Grid:
         fp = [None] * 3 if f is None else list(map(_p, f))
         lib().og_hd_step(self._g, ord, dt, nu, *map(_p, v), *fp)
 
+    def roth_step(self, v, dt, nu, omegaz, f=None, ord=2):
+        """One RK step of HD in a frame rotating about z, in place."""
+        fp = [None] * 3 if f is None else list(map(_p, f))
+        lib().og_roth_step(self._g, ord, dt, nu, omegaz, *map(_p, v), *fp)
+
+    def roth_substep(self, v, c, o, dt, nu, omegaz, f=None):
+        fp = [None] * 3 if f is None else list(map(_p, f))
+        lib().og_roth_substep(self._g, o, dt, nu, omegaz, *map(_p, v), *map(_p, c), *fp)
+
     def hd_substep(self, v, c, o, dt, nu, f=None):
         fp = [None] * 3 if f is None else list(map(_p, f))
         lib().og_hd_substep(self._g, o, dt, nu, *map(_p, v), *map(_p, c), *fp)
@@ -333,6 +345,11 @@ class Grid:
         """One Boussinesq RK step in place on v = [vx, vy, vz] and th."""
         fp = [None] * 3 if f is None else list(map(_p, f))
         lib().og_bouss_step(self._g, ord, dt, nu, kappa, xmom, xtemp, *map(_p, v), _p(th), *fp, _p(fs))
+
+    def rotbouss_step(self, v, th, dt, nu, kappa, xmom, xtemp, omegaz, f=None, fs=None, ord=2):
+        """One rotating-Boussinesq RK step in place on v = [vx, vy, vz] and th."""
+        fp = [None] * 3 if f is None else list(map(_p, f))
+        lib().og_rotbouss_step(self._g, ord, dt, nu, kappa, xmom, xtemp, omegaz, *map(_p, v), _p(th), *fp, _p(fs))
 
     # -- Lagrangian tracers (positions in grid units, float64 arrays) --------
     def spline_coef(self, field, ghost=True):

@@ -116,5 +116,8 @@ def test_bouss_pruned_equals_full_and_ord3_refused(gpu):
             os.environ.pop("GHX_NO_PRUNE", None)
     for k in range(4):
         assert rel(outs[0][k], outs[1][k]) < 1e-14
+    # ord = 3 without rotation needs bouss_rkstep2.f90's correction: refused
+    # before any work is launched
+    s3 = BoussSolver(g, 2e-3, 4e-3, 4e-3, 1.0, 1.0, ord=3)
     with pytest.raises(GhxError):
-        BoussSolver(g, 2e-3, 4e-3, 4e-3, 1.0, 1.0, ord=3)
+        s3.step(1)

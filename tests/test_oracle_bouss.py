@@ -99,3 +99,130 @@ def test_bouss_reduces_to_hd_without_temperature():
     for c in range(3):
         assert np.array_equal(v[c], v2[c])
     assert np.all(th == 0)
+
+
+def solenoidal_basis(k):
+    """Two orthonormal real vectors perpendicular to k."""
+    kv = np.asarray(k, float)
+    a = np.cross(kv, [0.3, 0.5, 0.8])
+    a /= np.linalg.norm(a)
+    b = np.cross(kv, a)
+    return a, b / np.linalg.norm(b)
+
+
+def coriolis_operator(k, omegaz):
+    """The projected Coriolis force -P_k(2 Omega z_hat x v) on the plane
+    perpendicular to k, in the basis of solenoidal_basis."""
+    a, b = solenoidal_basis(k)
+    kv = np.asarray(k, float)
+    P = np.eye(3) - np.outer(kv, kv) / kv.dot(kv)
+    Om = np.array([0, 0, 2.0 * omegaz])
+    B = np.stack([a, b], axis=1)
+    return B.T @ (-P @ np.stack([np.cross(Om, B[:, q]) for q in range(2)], axis=1))
+
+
+@pytest.mark.parametrize("k", [(2, 1, 3), (1, -2, 2), (3, 0, 1)])
+def test_coriolis_operator_gives_inertial_waves(k):
+    """Textbook dispersion relation of inertial waves, omega = +-2 Omega kz/|k|,
+    for the operator the next tests use."""
+    omegaz = 0.8
+    ev = np.linalg.eigvals(coriolis_operator(k, omegaz))
+    want = 2 * omegaz * k[2] / np.linalg.norm(k)
+    assert np.allclose(sorted(ev.imag), sorted([-want, want]), atol=1e-14)
+    assert np.allclose(ev.real, 0, atol=1e-14)
+
+
+@pytest.mark.parametrize("ord", [1, 2, 3])
+@pytest.mark.parametrize("k", [(2, 1, 3), (1, -2, 2)])
+def test_single_mode_inertial_wave(ord, k):
+    """One Fourier mode is an exact solution in a rotating frame (the
+    nonlinear term is a gradient); the RK step must apply the Taylor
+    polynomial of nu*lap + Coriolis to its two solenoidal amplitudes."""
+    n = 16
+    g = O.Grid(n)
+    nu, omegaz, dt = 0.02, 0.9, 0.05
+    a, b = solenoidal_basis(k)
+    amp = np.array([0.3 + 0.1j, -0.2 + 0.25j])
+    idx = (k[0], k[1] % n, k[2] % n)
+    N3 = n ** 3
+    v = [g.czeros() for _ in range(3)]
+    for c in range(3):
+        v[c][idx] = (amp[0] * a[c] + amp[1] * b[c]) * N3
+    g.roth_step(v, dt, nu, omegaz, ord=ord)
+    A = dt * (-nu * np.dot(k, k) * np.eye(2) + coriolis_operator(k, omegaz))
+    R = np.eye(2)
+    term = np.eye(2)
+    for p in range(1, ord + 1):
+        term = term @ A / p
+        R = R + term
+    want = R @ amp
+    got = np.array([sum(v[c][idx] * a[c] for c in range(3)), sum(v[c][idx] * b[c] for c in range(3))]) / N3
+    assert np.max(np.abs(got - want)) < 1e-14
+    for c in range(3):
+        rest = v[c].copy()
+        rest[idx] = 0
+        assert np.max(np.abs(rest)) < 1e-10
+
+
+def test_rotation_off_is_hd_and_bouss():
+    g = O.Grid(16)
+    v, _ = g.initialv_k4exp(3.0, 1.0, 5)
+    th = g.r2c(np.cos(2 * np.pi * np.arange(16) / 16)[None, None, :] * np.ones((16, 16, 16)))
+    v1, v2, v3, v4 = ([x.copy() for x in v] for _ in range(4))
+    t3, t4 = th.copy(), th.copy()
+    for _ in range(2):
+        g.roth_step(v1, 5e-3, 1e-2, 0.0)
+        g.hd_step(v2, 5e-3, 1e-2)
+        g.rotbouss_step(v3, t3, 5e-3, 1e-2, 2e-2, 0.5, -0.5, 0.0)
+        g.bouss_step(v4, t4, 5e-3, 1e-2, 2e-2, 0.5, -0.5)
+    for c in range(3):
+        assert np.array_equal(v1[c], v2[c])
+        assert np.array_equal(v3[c], v4[c])
+    assert np.array_equal(t3, t4)
+
+
+@pytest.mark.parametrize("ord", [2, 3])
+def test_single_mode_inertia_gravity_wave(ord):
+    """Rotation and stratification together: the 3x3 linear operator of one
+    mode (two solenoidal amplitudes and th) has the textbook inertia-gravity
+    frequencies omega^2 = (N^2 k_perp^2 + f^2 kz^2)/k^2 with f = 2 Omega and
+    N^2 = xmom xtemp, and the RK step applies its Taylor polynomial."""
+    n = 16
+    g = O.Grid(n)
+    k = (2, 1, 3)
+    k2 = float(np.dot(k, k))
+    nu, kappa, omegaz, bv, dt = 0.02, 0.03, 0.7, 1.3, 0.02
+    xmom, xtemp = bv, bv
+    a, b = solenoidal_basis(k)
+    kv = np.array(k, float)
+    pz = np.array([0, 0, 1.0]) - kv * k[2] / k2
+    M = np.zeros((3, 3))
+    M[:2, :2] = coriolis_operator(k, omegaz)
+    # buoyancy -xmom th P(z_hat) (the products enter the projection of
+    # nonlhd3 with a minus sign), heat current +xtemp v_z
+    M[0, 2], M[1, 2] = -xmom * a.dot(pz), -xmom * b.dot(pz)
+    M[2, 0], M[2, 1] = xtemp * a[2], xtemp * b[2]
+    freq = np.sort(np.abs(np.linalg.eigvals(M).imag))
+    kp2 = k[0] ** 2 + k[1] ** 2
+    w = np.sqrt((bv ** 2 * kp2 + (2 * omegaz) ** 2 * k[2] ** 2) / k2)
+    assert np.allclose(freq, [0, w, w], atol=1e-13)
+    D = np.diag([-nu * k2, -nu * k2, -kappa * k2])
+    A = dt * (D + M)
+    R = np.eye(3)
+    term = np.eye(3)
+    for p in range(1, ord + 1):
+        term = term @ A / p
+        R = R + term
+    amp = np.array([0.3 + 0.1j, -0.2 + 0.25j, 0.15 - 0.3j])
+    idx = (k[0], k[1], k[2])
+    N3 = n ** 3
+    v = [g.czeros() for _ in range(3)]
+    th = g.czeros()
+    for c in range(3):
+        v[c][idx] = (amp[0] * a[c] + amp[1] * b[c]) * N3
+    th[idx] = amp[2] * N3
+    g.rotbouss_step(v, th, dt, nu, kappa, xmom, xtemp, omegaz, ord=ord)
+    want = R @ amp
+    got = np.array([sum(v[c][idx] * a[c] for c in range(3)) / N3, sum(v[c][idx] * b[c] for c in range(3)) / N3,
+                    th[idx] / N3])
+    assert np.max(np.abs(got - want)) < 1e-14
