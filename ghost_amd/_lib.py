@@ -168,6 +168,21 @@ def _declare(L):
         "ghx_heltrans": (_I, [_P] * 7 + [_DP]),
         "ghx_crosspecc": (_I, [_P] * 7 + [_D, _DP]),
         "ghx_spectr1d": (_I, [_P, _P, _I, _DP]),
+        "ghx_variance": (_I, [_P, _P, _I, _DP]),
+        "ghx_product": (_I, [_P, _P, _P, _DP]),
+        "ghx_pscheck": (_I, [_P, _P, _P, _DP]),
+        "ghx_spectrscc": (_I, [_P, _P, _D, _DP]),
+        "ghx_sctrans": (_I, [_P, _P, _P, _DP]),
+        "ghx_difucx": (_I, [_P, _P, _P, _DP]),
+        "ghx_difucz": (_I, [_P, _P, _P, _DP]),
+        "ghx_axis_bins": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "ghx_specpara": (_I, [_P] * 4 + [_I, _I, _DP, _DP]),
+        "ghx_specperp": (_I, [_P] * 4 + [_I, _I, _DP, _DP]),
+        "ghx_entpara": (_I, [_P] * 7 + [_I, _DP]),
+        "ghx_entperp": (_I, [_P] * 7 + [_I, _DP]),
+        "ghx_heltpara": (_I, [_P] * 7 + [_DP]),
+        "ghx_heltperp": (_I, [_P] * 7 + [_DP]),
+        "ghx_spec2d": (_I, [_P] * 4 + [_I, _I, _DP, _DP]),
         "ghx_hdcheck": (_I, [_P] * 7 + [_I, _I, _DP]),
         "ghx_mhdcheck": (_I, [_P] * 7 + [_I, _I, _I, _DP]),
         "ghx_prof_enable": (_I, [_I]),

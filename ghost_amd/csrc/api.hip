@@ -821,6 +821,184 @@ extern "C" int ghx_spectr1d(ghx_grid *g, const void *a, int dir, double *Ek) {
   return GHX_OK;
 }
 
+// ---- scalar diagnostics (pseudospec3D_phd.f90) ----------------------------
+
+// variance (:116-217): <th^2> (kin = 1) or <kk2 |th|^2> (kin = 0)
+extern "C" int ghx_variance(ghx_grid *g, const void *a, int kin, double *out) {
+  ARGCHK(g && a && out && (kin == 0 || kin == 1), "bad variance arguments");
+  return reduce(g, kin == 1 ? 6 : 7, a, a, a, nullptr, nullptr, nullptr, out);
+}
+
+// product (:219-284): <a b>
+extern "C" int ghx_product(ghx_grid *g, const void *a, const void *b, double *out) {
+  ARGCHK(g && a && b && out, "bad product arguments");
+  return reduce(g, 8, a, a, a, b, b, b, out);
+}
+
+// pscheck (:286-333) without its file output: {<th^2>, <|grad th|^2>, <th s>}
+extern "C" int ghx_pscheck(ghx_grid *g, const void *a, const void *b, double *out) {
+  ARGCHK(g && a && b && out, "bad pscheck arguments");
+  int rc = ghx_variance(g, a, 1, &out[0]);
+  if (!rc) rc = ghx_variance(g, a, 0, &out[1]);
+  if (!rc) rc = ghx_product(g, a, b, &out[2]);
+  return rc;
+}
+
+// spectrscc (:516-610): |th|^2 per shell int(sqrt(kk2)/Dkk + shift + .501),
+// the offset in working precision
+extern "C" int ghx_spectrscc(ghx_grid *g, const void *a, double shift, double *Ek) {
+  ARGCHK(g && a && Ek, "bad spectrscc arguments");
+  const int nb = g->nmax / 2 + 1;
+  int rc = grid_work(g, ghx::spectrum_work_doubles(g->gk, nb));
+  if (rc) return rc;
+  HIPCHK(ghx::op_spectrum(g->gk, g->stream, 3, g->Dkk, nb, C_(a), C_(a), C_(a), C_(a), C_(a), C_(a), g->work, Ek,
+                          shift + 0.501));
+  return GHX_OK;
+}
+
+// sctrans (:612-727): Re(a b*) per shell (single-precision .501 offset)
+extern "C" int ghx_sctrans(ghx_grid *g, const void *a, const void *b, double *Ek) {
+  ARGCHK(g && a && b && Ek, "bad sctrans arguments");
+  const int nb = g->nmax / 2 + 1;
+  int rc = grid_work(g, ghx::spectrum_work_doubles(g->gk, nb));
+  if (rc) return rc;
+  HIPCHK(ghx::op_spectrum(g->gk, g->stream, 4, g->Dkk, nb, C_(a), C_(a), C_(a), C_(b), C_(b), C_(b), g->work, Ek));
+  return GHX_OK;
+}
+
+// difucx (dir 1, :729-828) / difucz (dir 3, :830-940): plane means of v_i,
+// th and v_i th along x or z; out = 3 n host doubles
+static int difuc(ghx_grid *g, const void *a, const void *b, int dir, double *out) {
+  ARGCHK(g && a && b && out, "bad difuc arguments");
+  const int n = dir == 1 ? g->gk.nx : g->gk.nz;
+  int rc = grid_rtmp(g, 2);
+  if (!rc) rc = grid_work(g, 3 * (size_t)n);
+  if (rc) return rc;
+  double *r1 = g->rtmp, *r2 = g->rtmp + g->nr;
+  const void *in[2] = {a, b};
+  double *ro[2] = {r1, r2};
+  if ((rc = do_c2r(&g->plan, 2, in, ro))) return rc;
+  const double nx = g->gk.nx, ny = g->gk.ny, nz = g->gk.nz;
+  const double tmp = dir == 1 ? 1.0 / (nx * (ny * ny) * (nz * nz)) : 1.0 / ((nx * nx) * (ny * ny) * nz);
+  const double tmq = dir == 1 ? 1.0 / ((nx * nx) * (ny * ny * ny) * (nz * nz * nz))
+                              : 1.0 / ((nx * nx * nx) * (ny * ny * ny) * (nz * nz));
+  HIPCHK(ghx::op_plane_means(g->gk, g->stream, r1, r2, dir, tmp, tmq, g->work, out));
+  return GHX_OK;
+}
+extern "C" int ghx_difucx(ghx_grid *g, const void *a, const void *b, double *out) { return difuc(g, a, b, 1, out); }
+extern "C" int ghx_difucz(ghx_grid *g, const void *a, const void *b, double *out) { return difuc(g, a, b, 3, out); }
+
+// ---- axisymmetric spectra (pseudospec3D_rot.f90), preferred axis z --------
+
+// bins of the plane spectra (nz/2+1) and of the cylinder spectra
+// (nmaxperp/2+1, nmaxperp = int(max(nx Dkx, ny Dky)/Dkk), main3D.fpp:1729)
+static int nb_para(const ghx_grid *g) { return g->gk.nz / 2 + 1; }
+static int nb_perp(const ghx_grid *g) {
+  const double a = g->gk.nx * g->gk.dkx, b = g->gk.ny * g->gk.dky;
+  return (int)((a > b ? a : b) / g->Dkk) / 2 + 1;
+}
+extern "C" int ghx_axis_bins(const ghx_grid *g, int *nbpara, int *nbperp) {
+  ARGCHK(g, "null grid");
+  if (nbpara) *nbpara = nb_para(g);
+  if (nbperp) *nbperp = nb_perp(g);
+  return GHX_OK;
+}
+
+static int axispec(ghx_grid *g, int bins, int kind, const void *a, const void *b, const void *c, const void *d,
+                   const void *e, const void *f, double *out) {
+  const int nb = bins == 0 ? nb_para(g) : nb_perp(g);
+  ARGCHK(3 * (size_t)nb * sizeof(double) <= 48 * 1024, "axisymmetric spectrum: too many bins for one block");
+  int rc = grid_work(g, ghx::axispec_work_doubles(g->gk, nb));
+  if (rc) return rc;
+  if (!d) { d = a; e = b; f = c; }
+  HIPCHK(ghx::op_axispec(g->gk, g->stream, bins, kind, nb, g->Dkk, g->L[2], C_(a), C_(b), C_(c), C_(d), C_(e), C_(f),
+                         g->work, out));
+  return GHX_OK;
+}
+
+// specpara (:25-335) / specperp (:338-686) without file output: Ek (kin 1:
+// of (a,b,c); kin 0: of its curl; kin 2: skipped) and Hk (hel = 1: Re(a .
+// conj(curl a))), each 3 nb host doubles {total, xy, z}; NULL skips one.
+static int axis_ek(ghx_grid *g, int bins, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
+                   double *Hk) {
+  ARGCHK(g && a && b && c && kin >= 0 && kin <= 2, "bad specpara/specperp arguments");
+  int rc = GHX_OK;
+  const bool need_curl = kin == 0 || hel == 1;
+  if (need_curl && ((rc = grid_ctmp(g, 3)) || (rc = curl3(g, a, b, c, g->ctmp)))) return rc;
+  const cplx *t0 = g->ctmp, *t1 = g->ctmp + g->nc, *t2 = g->ctmp + 2 * g->nc;
+  if (kin == 1 && Ek && (rc = axispec(g, bins, 0, a, b, c, nullptr, nullptr, nullptr, Ek))) return rc;
+  if (kin == 0 && Ek && (rc = axispec(g, bins, 0, t0, t1, t2, nullptr, nullptr, nullptr, Ek))) return rc;
+  if (hel == 1 && Hk && (rc = axispec(g, bins, 1, a, b, c, t0, t1, t2, Hk))) return rc;
+  return GHX_OK;
+}
+extern "C" int ghx_specpara(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
+                            double *Hk) {
+  return axis_ek(g, 0, a, b, c, kin, hel, Ek, Hk);
+}
+extern "C" int ghx_specperp(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
+                            double *Hk) {
+  return axis_ek(g, 1, a, b, c, kin, hel, Ek, Hk);
+}
+
+// entpara (:689-875) / entperp (:878-1064): Re(a d* + b e* + c f*) per bin,
+// times kk2 when kin = 0; Ek: nb host doubles
+static int axis_ent(ghx_grid *g, int bins, const void *a, const void *b, const void *c, const void *d, const void *e,
+                    const void *f, int kin, double *Ek) {
+  ARGCHK(g && a && b && c && d && e && f && Ek && kin >= 0, "bad entpara/entperp arguments");
+  const int nb = bins == 0 ? nb_para(g) : nb_perp(g);
+  std::vector<double> o(3 * (size_t)nb);
+  int rc = axispec(g, bins, kin == 0 ? 2 : 1, a, b, c, d, e, f, o.data());
+  if (rc) return rc;
+  for (int q = 0; q < nb; ++q) Ek[q] = o[q];
+  return GHX_OK;
+}
+extern "C" int ghx_entpara(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                           const void *f, int kin, double *Ek) {
+  return axis_ent(g, 0, a, b, c, d, e, f, kin, Ek);
+}
+extern "C" int ghx_entperp(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                           const void *f, int kin, double *Ek) {
+  return axis_ent(g, 1, a, b, c, d, e, f, kin, Ek);
+}
+
+// heltpara (:1067-1196) / heltperp (:1199-1328): curl(a,b,c) against (d,e,f)
+static int axis_helt(ghx_grid *g, int bins, const void *a, const void *b, const void *c, const void *d,
+                     const void *e, const void *f, double *Hk) {
+  ARGCHK(g && a && b && c && d && e && f && Hk, "bad heltpara/heltperp arguments");
+  int rc = grid_ctmp(g, 3);
+  if (!rc) rc = curl3(g, a, b, c, g->ctmp);
+  if (rc) return rc;
+  return axis_ent(g, bins, g->ctmp, g->ctmp + g->nc, g->ctmp + 2 * g->nc, d, e, f, 1, Hk);
+}
+extern "C" int ghx_heltpara(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                            const void *f, double *Hk) {
+  return axis_helt(g, 0, a, b, c, d, e, f, Hk);
+}
+extern "C" int ghx_heltperp(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                            const void *f, double *Hk) {
+  return axis_helt(g, 1, a, b, c, d, e, f, Hk);
+}
+
+// spec2D (:1331-1629) without file output: Ek, Hk of nbperp * nbpara host
+// doubles, kperp fastest (GHOST's Ek(kperp, kpara)); kin and hel as specpara
+extern "C" int ghx_spec2d(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
+                          double *Hk) {
+  ARGCHK(g && a && b && c && kin >= 0 && kin <= 2, "bad spec2d arguments");
+  const int np = nb_perp(g), nzb = nb_para(g);
+  int rc = grid_work(g, (size_t)np * nzb);
+  if (rc) return rc;
+  const bool need_curl = kin == 0 || hel == 1;
+  if (need_curl && ((rc = grid_ctmp(g, 3)) || (rc = curl3(g, a, b, c, g->ctmp)))) return rc;
+  const cplx *t0 = g->ctmp, *t1 = g->ctmp + g->nc, *t2 = g->ctmp + 2 * g->nc;
+  if (kin <= 1 && Ek) {
+    const cplx *x = kin == 1 ? C_(a) : t0, *y = kin == 1 ? C_(b) : t1, *z = kin == 1 ? C_(c) : t2;
+    HIPCHK(ghx::op_spec2d(g->gk, g->stream, 0, np, nzb, g->Dkk, g->L[2], x, y, z, x, y, z, g->work, Ek));
+  }
+  if (hel == 1 && Hk)
+    HIPCHK(ghx::op_spec2d(g->gk, g->stream, 1, np, nzb, g->Dkk, g->L[2], C_(a), C_(b), C_(c), t0, t1, t2, g->work, Hk));
+  return GHX_OK;
+}
+
 // hdcheck (pseudospec3D_hd.f90:984-1105) without its file output: out =
 // {eng, ens, pot, khe, div}: <v^2>, <omega^2>, the injection rate <v.f>,
 // the kinetic helicity (hel = 1) and <(div v)^2> (chk = 1); entries that

@@ -49,6 +49,21 @@ hipError_t op_reduce(const GridK &g, hipStream_t st, int kind, const cplx *a, co
 size_t spectrum_work_doubles(const GridK &g, int nb);
 hipError_t op_spectrum(const GridK &g, hipStream_t st, int kind, double dkk, int nb, const cplx *a, const cplx *b,
                        const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *work, double *out, double off = (double)0.501f);
+// Axisymmetric spectra (pseudospec3D_rot.f90): bins 0 = planes of |kz|,
+// 1 = cylinders of k_perp; kind 0 energy, 1 Re dot, 2 kk2-weighted dot.
+// out: 3 nb host doubles (total, xy part, z part); work: axispec_work_doubles.
+size_t axispec_work_doubles(const GridK &g, int nb);
+hipError_t op_axispec(const GridK &g, hipStream_t st, int bins, int kind, int nb, double dkk, double lz,
+                      const cplx *a, const cplx *b, const cplx *c, const cplx *d, const cplx *e, const cplx *f,
+                      double *work, double *out);
+// spec2D: np * nzb host doubles (kperp fastest); work holds np * nzb doubles.
+hipError_t op_spec2d(const GridK &g, hipStream_t st, int kind, int np, int nzb, double dkk, double lz, const cplx *a,
+                     const cplx *b, const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *work,
+                     double *out);
+// plane means of difucx (dir 1) / difucz (dir 3) of two real fields: 3 n host
+// doubles; work holds 3 n doubles.
+hipError_t op_plane_means(const GridK &g, hipStream_t st, const double *r1, const double *r2, int dir, double tmp,
+                          double tmq, double *work, double *out);
 hipError_t op_spectr1d(const GridK &g, hipStream_t st, int dir, double ldir, int nb, const cplx *a, double *work,
                        double *out);
 // max over real points of sqrt(r1^2+r2^2+r3^2) (maxabs)

@@ -332,6 +332,10 @@ __global__ void k_reduce(GridK g, int kind, const cplx *a, const cplx *b, const 
       double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky), kz = wavenum(q.k, g.nz, g.dkz);
       cplx dv = c_add(c_add(c_ik(kx, a[m]), c_ik(ky, b[m])), c_ik(kz, c[m]));
       s = cabs2(dv);
+    } else if (kind >= 6) {
+      // scalars (pseudospec3D_phd.f90): 6 variance |a|^2, 7 kk2 |a|^2, 8 product Re(a d*)
+      s = kind == 8 ? cre_dot(a[m], d[m]) : cabs2(a[m]);
+      if (kind == 7) s *= kk2_at(g, q);
     } else {
       // kind 4: count of modes outside tiny <= kn2 <= kmax holding a non-zero
       // value; kind 5: the same outside kn2 <= kmax only (a scalar whose mean,
@@ -374,6 +378,8 @@ __global__ void k_spectrum(GridK g, int kind, double dkk, int nb, double off, co
       const double w = q.i == 0 ? 1.0 : 2.0;
       double s;
       if (kind == 0) s = cabs2(a[m]) + cabs2(b[m]) + cabs2(c[m]);
+      else if (kind == 3) s = cabs2(a[m]);             // one scalar (spectrscc)
+      else if (kind == 4) s = cre_dot(a[m], d[m]);     // scalar transfer (sctrans)
       else s = cre_dot(a[m], d[m]) + cre_dot(b[m], e[m]) + cre_dot(c[m], f[m]);
       if (kind == 2) s = kk2 * s;
       atomicAdd(&bins[kmn - 1], w * s * tmp);
@@ -410,6 +416,104 @@ __global__ void k_spectr1d(GridK g, int dir, double ldir, int nb, const cplx *a,
   }
   __syncthreads();
   for (int q = threadIdx.x; q < nb; q += blockDim.x) part[(size_t)blockIdx.x * nb + q] = bins[q];
+}
+
+// Axisymmetric spectra of pseudospec3D_rot.f90 about z.  bins 0: planes
+// kmn = int(|kz| Lz + 1) (specpara, entpara, heltpara); 1: cylinders
+// kmn = int(sqrt(kx^2 + ky^2)/Dkk + 1) (specperp, entperp, heltperp).
+// kind 0: |a|^2 + |b|^2 (xy) and |c|^2 (z); 1: Re(a d*) + Re(b e*) and
+// Re(c f*); 2: kind 1 times kk2.  Per block, three rows of nb bins: the total,
+// and the xy and z parts (every mode for planes, the kz = 0 plane only for
+// cylinders).
+__global__ void k_axispec(GridK g, int bins, int kind, int nb, double dkk, double lz, const cplx *a, const cplx *b,
+                          const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *part) {
+  extern __shared__ double bins3[];
+  for (int q = threadIdx.x; q < 3 * nb; q += blockDim.x) bins3[q] = 0.0;
+  __syncthreads();
+  const double tmp = g.inv_n6;
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    int kmn;
+    if (bins == 0) {
+      kmn = (int)(fabs(wavenum(q.k, g.nz, g.dkz)) * lz + 1);
+    } else {
+      const double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky);
+      kmn = (int)(sqrt(kx * kx + ky * ky) / dkk + 1);
+    }
+    if (kmn <= 0 || kmn > nb) continue;
+    const double w = (q.i == 0 ? 1.0 : 2.0) * tmp;
+    double h, v;
+    if (kind == 0) {
+      h = cabs2(a[m]) + cabs2(b[m]);
+      v = cabs2(c[m]);
+    } else {
+      h = cre_dot(a[m], d[m]) + cre_dot(b[m], e[m]);
+      v = cre_dot(c[m], f[m]);
+      if (kind == 2) {
+        const double k2 = kk2_at(g, q);
+        h *= k2;
+        v *= k2;
+      }
+    }
+    atomicAdd(&bins3[kmn - 1], w * (h + v));
+    if (bins == 0 || q.k == 0) {
+      atomicAdd(&bins3[nb + kmn - 1], w * h);
+      atomicAdd(&bins3[2 * nb + kmn - 1], w * v);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 3 * nb; q += blockDim.x) part[(size_t)blockIdx.x * 3 * nb + q] = bins3[q];
+}
+
+// spec2D (pseudospec3D_rot.f90:1331-1629): E(kperp, kpara), kperp fastest,
+// np by nzb bins, accumulated with device-wide atomics into out (zeroed by
+// the caller).  kind 0: |a|^2 + |b|^2 + |c|^2; 1: Re(a d* + b e* + c f*).
+__global__ void k_spec2d(GridK g, int kind, int np, int nzb, double dkk, double lz, const cplx *a, const cplx *b,
+                         const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *out) {
+  const double tmp = g.inv_n6;
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    const double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky);
+    const int kmn = (int)(sqrt(kx * kx + ky * ky) / dkk + 1);
+    const int kmz = (int)(fabs(wavenum(q.k, g.nz, g.dkz)) * lz + 1);
+    if (kmn <= 0 || kmn > np || kmz <= 0 || kmz > nzb) continue;
+    const double s = kind == 0 ? cabs2(a[m]) + cabs2(b[m]) + cabs2(c[m])
+                               : cre_dot(a[m], d[m]) + cre_dot(b[m], e[m]) + cre_dot(c[m], f[m]);
+    atomicAdd(&out[(size_t)(kmz - 1) * np + (kmn - 1)], (q.i == 0 ? 1.0 : 2.0) * s * tmp);
+  }
+}
+
+// difucx / difucz (pseudospec3D_phd.f90:729-940): for each x (dir 1) or z
+// (dir 3) plane s, the sums of r1, r2 and r1 r2 over the plane, scaled by
+// tmp, tmp and tmq; out = {<v>(s), <th>(s), <v th>(s)}, n entries each.
+__global__ void k_plane_means(const double *r1, const double *r2, int nx, int ny, int nz, int dir, double tmp,
+                              double tmq, double *out) {
+  const int s = blockIdx.x;
+  const int n = dir == 1 ? nx : nz;
+  double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (dir == 1) {
+    for (size_t t = threadIdx.x; t < (size_t)ny * nz; t += blockDim.x) {
+      const size_t m = t * nx + s;  // (k, j) = (t / ny, t % ny)
+      a1 += r1[m];
+      a2 += r2[m];
+      a3 += r1[m] * r2[m];
+    }
+  } else {
+    const size_t base = (size_t)s * nx * ny;
+    for (size_t t = threadIdx.x; t < (size_t)nx * ny; t += blockDim.x) {
+      a1 += r1[base + t];
+      a2 += r2[base + t];
+      a3 += r1[base + t] * r2[base + t];
+    }
+  }
+  a1 = block_sum(a1);
+  a2 = block_sum(a2);
+  a3 = block_sum(a3);
+  if (threadIdx.x == 0) {
+    out[s] = a1 * tmp;
+    out[n + s] = a2 * tmp;
+    out[2 * n + s] = a3 * tmq;
+  }
 }
 
 __global__ void k_maxnorm(size_t n, const double *r1, const double *r2, const double *r3, double *part) {
@@ -545,6 +649,50 @@ hipError_t op_spectrum(const GridK &g, hipStream_t st, int kind, double dkk, int
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   err = hipMemcpyAsync(out, res, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(st);
+}
+
+size_t axispec_work_doubles(const GridK &g, int nb) { return (size_t)spectrum_blocks(g) * 3 * nb + 3 * nb; }
+
+hipError_t op_axispec(const GridK &g, hipStream_t st, int bins, int kind, int nb, double dkk, double lz,
+                      const cplx *a, const cplx *b, const cplx *c, const cplx *d, const cplx *e, const cplx *f,
+                      double *work, double *out) {
+  unsigned nblk = spectrum_blocks(g);
+  hipLaunchKernelGGL(k_axispec, dim3(nblk), dim3(kBlock), 3 * nb * sizeof(double), st, g, bins, kind, nb, dkk, lz, a,
+                     b, c, d, e, f, work);
+  double *res = work + (size_t)nblk * 3 * nb;
+  hipLaunchKernelGGL(k_spectrum_sum, dim3((3 * nb + kBlock - 1) / kBlock), dim3(kBlock), 0, st, work, (int)nblk,
+                     3 * nb, res);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipMemcpyAsync(out, res, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, st);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(st);
+}
+
+hipError_t op_spec2d(const GridK &g, hipStream_t st, int kind, int np, int nzb, double dkk, double lz, const cplx *a,
+                     const cplx *b, const cplx *c, const cplx *d, const cplx *e, const cplx *f, double *work,
+                     double *out) {
+  const size_t n = (size_t)np * nzb;
+  hipError_t err = op_zero(st, work, n * sizeof(double));
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(k_spec2d, dim3(spectrum_blocks(g)), dim3(kBlock), 0, st, g, kind, np, nzb, dkk, lz, a, b, c, d,
+                     e, f, work);
+  err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipMemcpyAsync(out, work, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(st);
+}
+
+hipError_t op_plane_means(const GridK &g, hipStream_t st, const double *r1, const double *r2, int dir, double tmp,
+                          double tmq, double *work, double *out) {
+  const int n = dir == 1 ? g.nx : g.nz;
+  hipLaunchKernelGGL(k_plane_means, dim3(n), dim3(kBlock), 0, st, r1, r2, g.nx, g.ny, g.nz, dir, tmp, tmq, work);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipMemcpyAsync(out, work, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st);
   if (err != hipSuccess) return err;
   return hipStreamSynchronize(st);
 }

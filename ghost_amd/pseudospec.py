@@ -164,6 +164,93 @@ class Grid:
         _lib.check(_lib.load().ghx_spectr1d(self._h, self._c(a), int(dir), C.cast(out, C.POINTER(C.c_double))))
         return list(out)
 
+    # ---- scalar diagnostics (pseudospec3D_phd.f90) -------------------------
+    def _dp(self, n):
+        buf = (C.c_double * n)()
+        return buf, C.cast(buf, C.POINTER(C.c_double))
+
+    def variance(self, a, kin=1):
+        out = C.c_double()
+        _lib.check(_lib.load().ghx_variance(self._h, self._c(a), int(kin), C.byref(out)))
+        return out.value
+
+    def product(self, a, b):
+        out = C.c_double()
+        _lib.check(_lib.load().ghx_product(self._h, self._c(a), self._c(b), C.byref(out)))
+        return out.value
+
+    def pscheck(self, a, b):
+        buf, p = self._dp(3)
+        _lib.check(_lib.load().ghx_pscheck(self._h, self._c(a), self._c(b), p))
+        return list(buf)
+
+    def spectrscc(self, a, shift=0.0):
+        buf, p = self._dp(self.nbins)
+        _lib.check(_lib.load().ghx_spectrscc(self._h, self._c(a), float(shift), p))
+        return list(buf)
+
+    def sctrans(self, a, b):
+        buf, p = self._dp(self.nbins)
+        _lib.check(_lib.load().ghx_sctrans(self._h, self._c(a), self._c(b), p))
+        return list(buf)
+
+    def difuc(self, a, b, dir):
+        """difucx (dir 1) / difucz (dir 3): [<v>, <th>, <v th>] along x or z."""
+        n = self.nx if dir == 1 else self.nz
+        buf, p = self._dp(3 * n)
+        fn = _lib.load().ghx_difucx if dir == 1 else _lib.load().ghx_difucz
+        _lib.check(fn(self._h, self._c(a), self._c(b), p))
+        return [list(buf[q * n:(q + 1) * n]) for q in range(3)]
+
+    # ---- axisymmetric spectra (pseudospec3D_rot.f90) -------------------------
+    def axis_bins(self):
+        a, b = C.c_int(), C.c_int()
+        _lib.check(_lib.load().ghx_axis_bins(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def _axis(self, fn, nb, a, b, c, kin, hel):
+        E, pe = self._dp(3 * nb) if kin <= 1 else (None, None)
+        H, ph = self._dp(3 * nb) if hel == 1 else (None, None)
+        _lib.check(fn(self._h, *map(self._c, (a, b, c)), int(kin), int(hel), pe, ph))
+        split = (lambda x: [list(x[q * nb:(q + 1) * nb]) for q in range(3)])
+        return (split(E) if E is not None else None), (split(H) if H is not None else None)
+
+    def specpara(self, a, b, c, kin=1, hel=0):
+        """specpara (pseudospec3D_rot.f90:25-335): ([tot, xy, z], [tot, xy, z]) per |kz| plane."""
+        return self._axis(_lib.load().ghx_specpara, self.axis_bins()[0], a, b, c, kin, hel)
+
+    def specperp(self, a, b, c, kin=1, hel=0):
+        """specperp (:338-686): per k_perp cylinder; xy and z from the kz = 0 plane."""
+        return self._axis(_lib.load().ghx_specperp, self.axis_bins()[1], a, b, c, kin, hel)
+
+    def _axis1(self, fn, nb, args, kin=None):
+        buf, p = self._dp(nb)
+        extra = (int(kin),) if kin is not None else ()
+        _lib.check(fn(self._h, *map(self._c, args), *extra, p))
+        return list(buf)
+
+    def entpara(self, a, b, c, d, e, f, kin=1):
+        return self._axis1(_lib.load().ghx_entpara, self.axis_bins()[0], (a, b, c, d, e, f), kin)
+
+    def entperp(self, a, b, c, d, e, f, kin=1):
+        return self._axis1(_lib.load().ghx_entperp, self.axis_bins()[1], (a, b, c, d, e, f), kin)
+
+    def heltpara(self, a, b, c, d, e, f):
+        return self._axis1(_lib.load().ghx_heltpara, self.axis_bins()[0], (a, b, c, d, e, f))
+
+    def heltperp(self, a, b, c, d, e, f):
+        return self._axis1(_lib.load().ghx_heltperp, self.axis_bins()[1], (a, b, c, d, e, f))
+
+    def spec2d(self, a, b, c, kin=1, hel=0):
+        """spec2D (:1331-1629): arrays [kpara][kperp] of E and H (None where not asked)."""
+        npara, nperp = self.axis_bins()
+        n = npara * nperp
+        E, pe = self._dp(n) if kin <= 1 else (None, None)
+        H, ph = self._dp(n) if hel == 1 else (None, None)
+        _lib.check(_lib.load().ghx_spec2d(self._h, *map(self._c, (a, b, c)), int(kin), int(hel), pe, ph))
+        shape = lambda x: [list(x[q * nperp:(q + 1) * nperp]) for q in range(npara)]
+        return (shape(E) if E is not None else None), (shape(H) if H is not None else None)
+
     def hdcheck(self, a, b, c, d, e, f, hel=1, chk=1):
         """hdcheck (pseudospec3D_hd.f90:984-1105): {eng, ens, pot, khe, div}."""
         out = (C.c_double * 5)()

@@ -148,6 +148,45 @@ int ghx_crosspecc(ghx_grid *g, const void *a, const void *b, const void *c, cons
                   const void *f, double shift, double *Ck);
 /* spectr1d (hd :1418-1626): 1D spectrum of one component along k_dir, bins 1..n_dir/2+1 */
 int ghx_spectr1d(ghx_grid *g, const void *a, int dir, double *Ek);
+/* Scalar diagnostics (pseudo/pseudospec3D_phd.f90), values only.
+ * variance (:116-217): <th^2> (kin 1) or <kk2 |th|^2> (kin 0); product
+ * (:219-284): <a b>; pscheck (:286-333): out[3] = {<th^2>, <|grad th|^2>,
+ * <th s>}; spectrscc (:516-610): |th|^2 per shell, offset shift + .501 in
+ * working precision; sctrans (:612-727): Re(a b*) per shell; difucx/difucz
+ * (:729-940): out[3 n] = plane means {<v>, <th>, <v th>} along x (n = nx) or
+ * z (n = nz).  Shell arrays: nmax/2+1 host doubles. */
+int ghx_variance(ghx_grid *g, const void *a, int kin, double *out);
+int ghx_product(ghx_grid *g, const void *a, const void *b, double *out);
+int ghx_pscheck(ghx_grid *g, const void *a, const void *b, double *out);
+int ghx_spectrscc(ghx_grid *g, const void *a, double shift, double *Ek);
+int ghx_sctrans(ghx_grid *g, const void *a, const void *b, double *Ek);
+int ghx_difucx(ghx_grid *g, const void *a, const void *b, double *out);
+int ghx_difucz(ghx_grid *g, const void *a, const void *b, double *out);
+
+/* Axisymmetric spectra about z (pseudo/pseudospec3D_rot.f90), values only.
+ * Plane bins int(|kz| Lz + 1), nbpara = nz/2+1; cylinder bins
+ * int(sqrt(kx^2+ky^2)/Dkk + 1), nbperp = nmaxperp/2+1 (main3D.fpp:1729).
+ * specpara (:25-335), specperp (:338-686): Ek (kin 1: of (a,b,c), 0: of its
+ * curl, 2: skipped), Hk (hel 1: Re(a . conj(curl a))), each 3 nb doubles
+ * {total, xy part, z part}, the parts over the kz = 0 plane only for
+ * cylinders; either array may be NULL.  entpara (:689-875), entperp
+ * (:878-1064): Re(a d* + b e* + c f*) per bin (times kk2 for kin 0);
+ * heltpara (:1067-1196), heltperp (:1199-1328): curl(a,b,c) against (d,e,f);
+ * spec2D (:1331-1629): nbperp * nbpara doubles, kperp fastest. */
+int ghx_axis_bins(const ghx_grid *g, int *nbpara, int *nbperp);
+int ghx_specpara(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
+                 double *Hk);
+int ghx_specperp(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
+                 double *Hk);
+int ghx_entpara(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                const void *f, int kin, double *Ek);
+int ghx_entperp(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                const void *f, int kin, double *Ek);
+int ghx_heltpara(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                 const void *f, double *Hk);
+int ghx_heltperp(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,
+                 const void *f, double *Hk);
+int ghx_spec2d(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek, double *Hk);
 /* hdcheck (hd :984-1105) without file output: out[5] = {eng, ens, pot, khe, div}
  * (<v^2>, <omega^2>, <v.f>, kinetic helicity if hel = 1, <(div v)^2> if chk = 1) */
 int ghx_hdcheck(ghx_grid *g, const void *a, const void *b, const void *c, const void *d, const void *e,

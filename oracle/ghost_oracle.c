@@ -1648,6 +1648,162 @@ void og_spectr1d(const og_grid *g, const ocplx *a, int dir, double *Ek) {
       }
 }
 
+/* ---- scalar diagnostics (pseudospec3D_phd.f90) ------------------------- */
+
+/* variance (:116-217): <th^2> (kin = 1) or <kk2 |th|^2> (kin = 0) */
+double og_variance(const og_grid *g, const ocplx *a, int kin) {
+  const double tmp = nsix(g);
+  double acc = 0.0;
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        const size_t m = IDX(g, i, j, k);
+        const double w = i == 0 ? 1.0 : 2.0;
+        acc += kin == 1 ? w * tmp * abs2(a[m]) : w * tmp * g->kk2[m] * abs2(a[m]);
+      }
+  return acc;
+}
+
+/* product (:219-284): <a b> = sum Re(a b*) / N^6 */
+double og_product(const og_grid *g, const ocplx *a, const ocplx *b) {
+  const double tmp = nsix(g);
+  double acc = 0.0;
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        const size_t m = IDX(g, i, j, k);
+        acc += (i == 0 ? 1.0 : 2.0) * tmp * creal(a[m] * conj(b[m]));
+      }
+  return acc;
+}
+
+/* spectrscc (:516-610): |th|^2 per shell int(sqrt(kk2)/Dkk + shift + .501),
+ * the offset in working precision; sctrans (:612-727): Re(a b*) per shell
+ * with the single-precision .501 (dot != 0) */
+static void scalar_shells(const og_grid *g, const ocplx *a, const ocplx *b, double off, double *Ek) {
+  const double tmp = nsix(g);
+  const int nb = g->nmax / 2 + 1;
+  for (int q = 0; q < nb; ++q) Ek[q] = 0.0;
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        const size_t m = IDX(g, i, j, k);
+        const int kmn = (int)(sqrt(g->kk2[m]) / g->Dkk + off);
+        if (kmn > 0 && kmn <= nb) {
+          const double s = b ? creal(a[m] * conj(b[m])) : abs2(a[m]);
+          Ek[kmn - 1] += (i == 0 ? 1.0 : 2.0) * tmp * s;
+        }
+      }
+}
+void og_spectrscc(const og_grid *g, const ocplx *a, double shift, double *Ek) {
+  scalar_shells(g, a, NULL, shift + 0.501, Ek);
+}
+void og_sctrans(const og_grid *g, const ocplx *a, const ocplx *b, double *Ek) {
+  scalar_shells(g, a, b, (double)0.501f, Ek);
+}
+
+/* difucx / difucz (:729-940): plane means of v_i, th and v_i th as functions
+ * of x (dir = 1) or z (dir = 3); out = {<v>(s), <th>(s), <v th>(s)}, n each */
+void og_difuc(const og_grid *g, const ocplx *a, const ocplx *b, int dir, double *out) {
+  const size_t nr = (size_t)g->nx * g->ny * g->nz;
+  double *r1 = (double *)malloc(sizeof(double) * nr), *r2 = (double *)malloc(sizeof(double) * nr);
+  og_c2r(g, a, r1);
+  og_c2r(g, b, r2);
+  const int n = dir == 1 ? g->nx : g->nz;
+  for (int q = 0; q < 3 * n; ++q) out[q] = 0.0;
+  for (int k = 0; k < g->nz; ++k)
+    for (int j = 0; j < g->ny; ++j)
+      for (int i = 0; i < g->nx; ++i) {
+        const size_t m = ((size_t)k * g->ny + j) * g->nx + i;
+        const int s = dir == 1 ? i : k;
+        out[s] += r1[m];
+        out[n + s] += r2[m];
+        out[2 * n + s] += r1[m] * r2[m];
+      }
+  const double nx = g->nx, ny = g->ny, nz = g->nz;
+  const double tmp = dir == 1 ? 1.0 / (nx * (ny * ny) * (nz * nz)) : 1.0 / ((nx * nx) * (ny * ny) * nz);
+  const double tmq = dir == 1 ? 1.0 / ((nx * nx) * (ny * ny * ny) * (nz * nz * nz))
+                              : 1.0 / ((nx * nx * nx) * (ny * ny * ny) * (nz * nz));
+  for (int q = 0; q < n; ++q) {
+    out[q] *= tmp;
+    out[n + q] *= tmp;
+    out[2 * n + q] *= tmq;
+  }
+  free(r1);
+  free(r2);
+}
+
+/* ---- axisymmetric spectra (pseudospec3D_rot.f90), preferred axis z ----- */
+
+/* nmaxperp of main3D.fpp:1729 */
+int og_nmaxperp(const og_grid *g) {
+  const double a = g->nx * g->Dkx, b = g->ny * g->Dky;
+  return (int)((a > b ? a : b) / g->Dkk);
+}
+
+/* bins = 0: planes kmn = int(|kz| Lz + 1), 1..nz/2+1 (specpara :25-335);
+ * bins = 1: cylinders kmn = int(sqrt(kx^2 + ky^2)/Dkk + 1), 1..nmaxperp/2+1
+ * (specperp :338-686).  Per mode, with the Hermitian weight and 1/N^6:
+ * kind 0: |a|^2 + |b|^2 (xy) and |c|^2 (z); kind 1: Re(a d*) + Re(b e*) and
+ * Re(c f*); kind 2: kind 1 times kk2 (entpara's kin = 0).  tot gets xy + z;
+ * xy and z (either may be NULL) get every mode for planes and only the
+ * kz = 0 plane for cylinders, as the two files write them. */
+void og_axispec(const og_grid *g, int bins, int kind, const ocplx *a, const ocplx *b, const ocplx *c,
+                const ocplx *d, const ocplx *e, const ocplx *f, double *tot, double *xy, double *zz) {
+  const double tmp = nsix(g);
+  const int nb = bins == 0 ? g->nz / 2 + 1 : og_nmaxperp(g) / 2 + 1;
+  for (int q = 0; q < nb; ++q) {
+    tot[q] = 0.0;
+    if (xy) xy[q] = 0.0;
+    if (zz) zz[q] = 0.0;
+  }
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        const size_t m = IDX(g, i, j, k);
+        const int kmn = bins == 0 ? (int)(fabs(g->kz[k]) * g->Lz + 1)
+                                  : (int)(sqrt(g->kx[i] * g->kx[i] + g->ky[j] * g->ky[j]) / g->Dkk + 1);
+        if (kmn <= 0 || kmn > nb) continue;
+        const double w = (i == 0 ? 1.0 : 2.0) * tmp;
+        double h, v;
+        if (kind == 0) {
+          h = abs2(a[m]) + abs2(b[m]);
+          v = abs2(c[m]);
+        } else {
+          h = creal(a[m] * conj(d[m])) + creal(b[m] * conj(e[m]));
+          v = creal(c[m] * conj(f[m]));
+          if (kind == 2) { h *= g->kk2[m]; v *= g->kk2[m]; }
+        }
+        tot[kmn - 1] += w * (h + v);
+        if (bins == 0 || k == 0) {
+          if (xy) xy[kmn - 1] += w * h;
+          if (zz) zz[kmn - 1] += w * v;
+        }
+      }
+}
+
+/* spec2D (:1331-1629): E(kperp, kpara) in Fortran order (kperp fastest),
+ * nmaxperp/2+1 by nz/2+1, kind 0 or 1 as og_axispec */
+void og_spec2d(const og_grid *g, int kind, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+               const ocplx *e, const ocplx *f, double *E) {
+  const double tmp = nsix(g);
+  const int np = og_nmaxperp(g) / 2 + 1, nzb = g->nz / 2 + 1;
+  for (int q = 0; q < np * nzb; ++q) E[q] = 0.0;
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j) {
+      const int kmn = (int)(sqrt(g->kx[i] * g->kx[i] + g->ky[j] * g->ky[j]) / g->Dkk + 1);
+      if (kmn <= 0 || kmn > np) continue;
+      for (int k = 0; k < g->nz; ++k) {
+        const int kmz = (int)(fabs(g->kz[k]) * g->Lz + 1);
+        if (kmz <= 0 || kmz > nzb) continue;
+        const size_t m = IDX(g, i, j, k);
+        const double s = kind == 0 ? abs2(a[m]) + abs2(b[m]) + abs2(c[m])
+                                   : creal(a[m] * conj(d[m])) + creal(b[m] * conj(e[m])) + creal(c[m] * conj(f[m]));
+        E[(size_t)(kmz - 1) * np + (kmn - 1)] += (i == 0 ? 1.0 : 2.0) * s * tmp;
+      }
+    }
+}
+
 /* mhdcheck (pseudospec3D_mhd.f90:339-547) without its file output and with
  * crs <= 1: out = {eng, ens, cur, engk, engm, helk, helm, crh, asq, divk, divm}
  * (entries not requested by hel/crs/chk are 0) */

@@ -147,6 +147,23 @@ void og_bouss_step(const og_grid *g, int ord, double dt, double nu, double kappa
                    ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *fx, const ocplx *fy,
                    const ocplx *fz, const ocplx *fs);
 
+/* scalar diagnostics (pseudospec3D_phd.f90): variance, product, spectrscc
+ * (offset shift + .501 in working precision), sctrans, and the plane means
+ * of difucx (dir 1) / difucz (dir 3): out = {<v>, <th>, <v th>} */
+double og_variance(const og_grid *g, const ocplx *a, int kin);
+double og_product(const og_grid *g, const ocplx *a, const ocplx *b);
+void og_spectrscc(const og_grid *g, const ocplx *a, double shift, double *Ek);
+void og_sctrans(const og_grid *g, const ocplx *a, const ocplx *b, double *Ek);
+void og_difuc(const og_grid *g, const ocplx *a, const ocplx *b, int dir, double *out);
+
+/* axisymmetric spectra (pseudospec3D_rot.f90): bins 0 = planes |kz|,
+ * 1 = cylinders k_perp; kind 0 = energy, 1 = Re dot, 2 = kk2-weighted dot */
+int og_nmaxperp(const og_grid *g);
+void og_axispec(const og_grid *g, int bins, int kind, const ocplx *a, const ocplx *b, const ocplx *c,
+                const ocplx *d, const ocplx *e, const ocplx *f, double *tot, double *xy, double *zz);
+void og_spec2d(const og_grid *g, int kind, const ocplx *a, const ocplx *b, const ocplx *c, const ocplx *d,
+               const ocplx *e, const ocplx *f, double *E);
+
 /* rotating Boussinesq (include/rotbouss_rkstep{1,2}.f90) */
 void og_rotbouss_step(const og_grid *g, int ord, double dt, double nu, double kappa, double xmom, double xtemp,
                       double omegaz, ocplx *vx, ocplx *vy, ocplx *vz, ocplx *th, const ocplx *fx,

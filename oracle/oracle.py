@@ -105,6 +105,17 @@ def _declare(L):
     L.og_crosspecc.argtypes = [_G] + [_P] * 6 + [C.c_double, _P]
     L.og_spectr1d.argtypes = [_G, _P, C.c_int, _P]
     L.og_mhdcheck.argtypes = [_G] + [_P] * 6 + [C.c_int, C.c_int, C.c_int, _P]
+    L.og_variance.restype = C.c_double
+    L.og_variance.argtypes = [_G, _P, C.c_int]
+    L.og_product.restype = C.c_double
+    L.og_product.argtypes = [_G, _P, _P]
+    L.og_spectrscc.argtypes = [_G, _P, C.c_double, _P]
+    L.og_sctrans.argtypes = [_G, _P, _P, _P]
+    L.og_difuc.argtypes = [_G, _P, _P, C.c_int, _P]
+    L.og_nmaxperp.restype = C.c_int
+    L.og_nmaxperp.argtypes = [_G]
+    L.og_axispec.argtypes = [_G, C.c_int, C.c_int] + [_P] * 9
+    L.og_spec2d.argtypes = [_G, C.c_int] + [_P] * 7
     L.og_mlp_forward.argtypes = [_P, C.c_long, _P, C.c_int, _P]
     L.og_mlp_grad.restype = C.c_double
     L.og_mlp_grad.argtypes = [_P, C.c_long, _P, _P, C.c_int, _P]
@@ -267,6 +278,95 @@ class Grid:
         out = np.zeros(self.nbins)
         lib().og_spectr1d(self._g, _p(a), int(dir), _p(out))
         return out
+
+    # -- scalar diagnostics (pseudospec3D_phd.f90) ----------------------------
+    def variance(self, a, kin=1):
+        return lib().og_variance(self._g, _p(a), int(kin))
+
+    def product(self, a, b):
+        return lib().og_product(self._g, _p(a), _p(b))
+
+    def pscheck(self, a, b):
+        """pscheck (pseudospec3D_phd.f90:286-333): <th^2>, <|grad th|^2>, <th s>."""
+        return np.array([self.variance(a, 1), self.variance(a, 0), self.product(a, b)])
+
+    def spectrscc(self, a, shift=0.0):
+        out = np.zeros(self.nbins)
+        lib().og_spectrscc(self._g, _p(a), float(shift), _p(out))
+        return out
+
+    def sctrans(self, a, b):
+        out = np.zeros(self.nbins)
+        lib().og_sctrans(self._g, _p(a), _p(b), _p(out))
+        return out
+
+    def difuc(self, a, b, dir):
+        n = self.nx if dir == 1 else self.nz
+        out = np.zeros(3 * n)
+        lib().og_difuc(self._g, _p(a), _p(b), int(dir), _p(out))
+        return out.reshape(3, n)
+
+    # -- axisymmetric spectra (pseudospec3D_rot.f90) --------------------------
+    @property
+    def nbpara(self):
+        return self.nz // 2 + 1
+
+    @property
+    def nbperp(self):
+        return lib().og_nmaxperp(self._g) // 2 + 1
+
+    def axispec(self, bins, kind, a, b, c, d=None, e=None, f=None):
+        nb = self.nbpara if bins == 0 else self.nbperp
+        out = np.zeros((3, nb))
+        z = [None] * 3 if d is None else [d, e, f]
+        lib().og_axispec(self._g, int(bins), int(kind), _p(a), _p(b), _p(c), *[_p(x) if x is not None else None
+                                                                              for x in z],
+                         _p(out[0]), _p(out[1]), _p(out[2]))
+        return out
+
+    def _axis_ek(self, bins, a, b, c, kin, hel):
+        """specpara (bins 0) / specperp (bins 1): energy (kin 1), energy of
+        curl (a,b,c) (kin 0), skipped (kin 2); helicity a . curl a (hel 1)."""
+        cu = self.curl(a, b, c) if (kin == 0 or hel == 1) else None
+        E = None
+        if kin == 1:
+            E = self.axispec(bins, 0, a, b, c)
+        elif kin == 0:
+            E = self.axispec(bins, 0, *cu)
+        H = self.axispec(bins, 1, a, b, c, *cu) if hel == 1 else None
+        return E, H
+
+    def specpara(self, a, b, c, kin=1, hel=0):
+        return self._axis_ek(0, a, b, c, kin, hel)
+
+    def specperp(self, a, b, c, kin=1, hel=0):
+        return self._axis_ek(1, a, b, c, kin, hel)
+
+    def entpara(self, a, b, c, d, e, f, kin=1):
+        return self.axispec(0, 2 if kin == 0 else 1, a, b, c, d, e, f)[0]
+
+    def entperp(self, a, b, c, d, e, f, kin=1):
+        return self.axispec(1, 2 if kin == 0 else 1, a, b, c, d, e, f)[0]
+
+    def heltpara(self, a, b, c, d, e, f):
+        return self.axispec(0, 1, *self.curl(a, b, c), d, e, f)[0]
+
+    def heltperp(self, a, b, c, d, e, f):
+        return self.axispec(1, 1, *self.curl(a, b, c), d, e, f)[0]
+
+    def spec2d(self, a, b, c, kin=1, hel=0):
+        """spec2D: E(kperp, kpara) as an array [kpara][kperp] (Fortran order of Ek(kperp, kpara))."""
+        cu = self.curl(a, b, c) if (kin == 0 or hel == 1) else None
+        shape = (self.nbpara, self.nbperp)
+        E = H = None
+        if kin <= 1:
+            E = np.zeros(shape)
+            src = (a, b, c) if kin == 1 else cu
+            lib().og_spec2d(self._g, 0, *map(_p, src), None, None, None, _p(E))
+        if hel == 1:
+            H = np.zeros(shape)
+            lib().og_spec2d(self._g, 1, _p(a), _p(b), _p(c), *map(_p, cu), _p(H))
+        return E, H
 
     def hdcheck(self, a, b, c, d, e, f, hel=1, chk=1):
         """hdcheck (pseudospec3D_hd.f90:984-1105) from its parts: {eng, ens, pot, khe, div}."""

@@ -63,3 +63,52 @@ def test_hdcheck(gpu, shape, arb, L):
         scale = max(abs(x) for x in want)
         for x, y in zip(got, want):
             assert x == pytest.approx(y, rel=1e-12, abs=1e-13 * scale)
+
+
+@pytest.mark.parametrize("shape,arb,L", CASES)
+def test_scalar_diagnostics(gpu, shape, arb, L):
+    g, og = grids(shape, arb, L)
+    a, b = rand_spec(og, 71), rand_spec(og, 72)
+    A, B = to_gpu(a), to_gpu(b)
+    for kin in (0, 1):
+        assert g.variance(A, kin) == pytest.approx(og.variance(a, kin), rel=1e-13)
+    assert g.product(A, B) == pytest.approx(og.product(a, b), rel=1e-12, abs=1e-14 * og.variance(a, 1))
+    assert rel(g.pscheck(A, B), og.pscheck(a, b)) < 1e-13
+    for shift in (0.0, 1.0):
+        assert rel(g.spectrscc(A, shift), og.spectrscc(a, shift)) < 1e-13
+    assert rel(g.sctrans(A, B), og.sctrans(a, b)) < 1e-12
+    for d in (1, 3):
+        got, want = np.asarray(g.difuc(A, B, d)), og.difuc(a, b, d)
+        for q in range(3):
+            assert rel(got[q], want[q]) < 1e-12
+
+
+@pytest.mark.parametrize("shape,arb,L", CASES)
+def test_axisymmetric_spectra(gpu, shape, arb, L):
+    g, og = grids(shape, arb, L)
+    v = [rand_spec(og, s) for s in (73, 74, 75)]
+    w = [rand_spec(og, s) for s in (76, 77, 78)]
+    V, W = [to_gpu(x) for x in v], [to_gpu(x) for x in w]
+    assert g.axis_bins() == (og.nbpara, og.nbperp)
+    for name in ("specpara", "specperp"):
+        for kin, hel in ((1, 1), (0, 1), (2, 1), (1, 0)):
+            E, H = getattr(g, name)(*V, kin=kin, hel=hel)
+            Er, Hr = getattr(og, name)(*v, kin=kin, hel=hel)
+            assert (E is None) == (Er is None) and (H is None) == (Hr is None)
+            if E is not None:
+                for q in range(3):
+                    assert rel(E[q], Er[q]) < 1e-12, (name, kin, q)
+            if H is not None:
+                for q in range(3):
+                    assert rel(H[q], Hr[q]) < 1e-11, (name, kin, q)
+    for name in ("entpara", "entperp"):
+        for kin in (0, 1, 2):
+            assert rel(getattr(g, name)(*V, *W, kin=kin), getattr(og, name)(*v, *w, kin=kin)) < 1e-12
+    for name in ("heltpara", "heltperp"):
+        assert rel(getattr(g, name)(*V, *W), getattr(og, name)(*v, *w)) < 1e-12
+    for kin, hel in ((1, 1), (0, 0)):
+        E, H = g.spec2d(*V, kin=kin, hel=hel)
+        Er, Hr = og.spec2d(*v, kin=kin, hel=hel)
+        assert rel(np.asarray(E), Er) < 1e-12
+        if hel:
+            assert rel(np.asarray(H), Hr) < 1e-11

@@ -3,6 +3,7 @@ mhdcheck (pseudospec3D_hd.f90:1418-1949, pseudospec3D_mhd.f90:339-712),
 pinned by identities with the already-pinned spectrumc/energy/helicity/cross
 and by single-mode bin placement."""
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 
@@ -63,3 +64,115 @@ def test_mhdcheck_is_the_composition_of_the_diagnostics():
             og.divergence(*v), og.divergence(*a)]
     assert np.allclose(out, want, rtol=1e-14, atol=1e-300)
     assert out[9] < 1e-20 * out[3]  # solenoidal fields
+
+
+# ---- scalar diagnostics (pseudospec3D_phd.f90) ------------------------------
+
+def test_variance_product_parseval():
+    og = O.Grid(16, 16, 32, L=(1.0, 1.0, 2.0), arbsize=True)
+    rng = np.random.default_rng(3)
+    # no kx = nx/2 content: GHOST's Hermitian weights count that plane twice
+    a, b = og.r2c(rng.standard_normal(og.rshape)), og.r2c(rng.standard_normal(og.rshape))
+    a[-1], b[-1] = 0, 0
+    n3 = og.nx * og.ny * og.nz
+    ra, rb = og.c2r(a) / n3, og.c2r(b) / n3
+    assert og.variance(a, 1) == pytest.approx(np.mean(ra * ra), rel=1e-13)
+    assert og.product(a, b) == pytest.approx(np.mean(ra * rb), rel=1e-12)
+    # <|grad th|^2> from the derivatives (derivk3 is pinned by closed forms)
+    gx, gy, gz = (og.derivk3(a, d) for d in (1, 2, 3))
+    assert og.variance(a, 0) == pytest.approx(og.energy(gx, gy, gz, 1), rel=1e-13)
+    assert np.allclose(og.pscheck(a, b), [og.variance(a, 1), og.variance(a, 0), og.product(a, b)], rtol=0)
+
+
+def test_scalar_spectra():
+    og = O.Grid(32)
+    v, _ = og.initialv_k4exp(3.0, 1.0, 7)
+    a = v[0]
+    z = og.czeros()
+    S = og.spectrscc(a)
+    assert np.sum(S) == pytest.approx(og.variance(a, 1), rel=1e-13)  # dealiased: every mode in a shell
+    Ek, _ = og.spectrumc(a, z, z, kin=1)
+    assert np.allclose(S, Ek, rtol=1e-13, atol=0)
+    assert np.allclose(og.spectrscc(a, shift=1.0)[1:], S[:-1], rtol=1e-13, atol=0)
+    assert np.allclose(og.sctrans(a, a), S, rtol=1e-13, atol=0)
+    m = og.czeros()
+    m[3, 4, 0] = 1.0  # |k| = 5
+    assert np.nonzero(og.spectrscc(m))[0].tolist() == [4]
+
+
+def test_difuc_closed_forms():
+    n = 16
+    og = O.Grid(n)
+    zz, yy, xx = np.meshgrid(*(2 * np.pi * np.arange(n) / n,) * 3, indexing="ij")
+    v = og.r2c(np.cos(xx) + 0.5)
+    th = og.r2c(np.cos(xx) + np.sin(zz))
+    x = 2 * np.pi * np.arange(n) / n
+    px = og.difuc(v, th, 1)
+    assert np.allclose(px[0], np.cos(x) + 0.5, atol=1e-14)
+    assert np.allclose(px[1], np.cos(x), atol=1e-14)
+    assert np.allclose(px[2], (np.cos(x) + 0.5) * np.cos(x), atol=1e-14)
+    pz = og.difuc(v, th, 3)
+    assert np.allclose(pz[0], 0.5, atol=1e-14)
+    assert np.allclose(pz[1], np.sin(x), atol=1e-14)
+    assert np.allclose(pz[2], 0.5 + 0.5 * np.sin(x), atol=1e-14)
+
+
+# ---- axisymmetric spectra (pseudospec3D_rot.f90) ----------------------------
+
+@pytest.mark.parametrize("shape,L,arb", [((32, 32, 32), (1, 1, 1), False), ((32, 32, 16), (1.0, 1.0, 0.5), True)])
+def test_axisymmetric_sums(shape, L, arb):
+    og = O.Grid(*shape, L=L, arbsize=arb)
+    v, _ = og.initialv_k4exp(3.0, 1.0, 9)
+    z = og.czeros()
+    eng = og.energy(*v, 1)
+    Ep, Hp = og.specpara(*v, kin=1, hel=1)
+    Ec, Hc = og.specperp(*v, kin=1, hel=1)
+    for E in (Ep, Ec):
+        assert np.sum(E[0]) == pytest.approx(eng, rel=1e-13)  # dealiased: no mode is dropped
+    assert np.sum(Ep[1]) == pytest.approx(og.energy(v[0], v[1], z, 1), rel=1e-13)
+    assert np.sum(Ep[2]) == pytest.approx(og.energy(z, z, v[2], 1), rel=1e-13)
+    # cylinders: the xy and z columns hold the kz = 0 plane only
+    plane = [x.copy() for x in v]
+    for x in plane:
+        x[:, :, 1:] = 0
+    assert np.sum(Ec[1]) == pytest.approx(og.energy(plane[0], plane[1], z, 1), rel=1e-13)
+    assert np.sum(Ec[2]) == pytest.approx(og.energy(z, z, plane[2], 1), rel=1e-13)
+    hel = og.helicity(*v)
+    for H in (Hp, Hc):
+        assert np.sum(H[0]) == pytest.approx(hel, rel=1e-12, abs=1e-14 * eng)
+    # the 2D spectrum's marginals are the two reduced spectra
+    E2, H2 = og.spec2d(*v, kin=1, hel=1)
+    assert np.allclose(E2.sum(axis=1), Ep[0], rtol=1e-12, atol=1e-15 * eng)
+    assert np.allclose(E2.sum(axis=0), Ec[0], rtol=1e-12, atol=1e-15 * eng)
+    assert np.allclose(H2.sum(axis=1), Hp[0], rtol=1e-11, atol=1e-14 * eng)
+    # magnetic spectra (kin 0) are the spectra of the curl
+    w = og.curl(*v)
+    assert np.allclose(og.specpara(*v, kin=0)[0], og.specpara(*w, kin=1)[0], rtol=0, atol=0)
+
+
+def test_axisymmetric_transfers():
+    og = O.Grid(32)
+    v, _ = og.initialv_k4exp(3.0, 1.0, 11)
+    eng = og.energy(*v, 1)
+    Ep, Hp = og.specpara(*v, kin=1, hel=1)
+    Ec, Hc = og.specperp(*v, kin=1, hel=1)
+    assert np.allclose(og.entpara(*v, *v, kin=1), Ep[0], rtol=1e-13, atol=0)
+    assert np.allclose(og.entperp(*v, *v, kin=1), Ec[0], rtol=1e-13, atol=0)
+    # kk2 |v|^2 = |k x v|^2 for a solenoidal field: the curl's spectra
+    assert np.allclose(og.entpara(*v, *v, kin=0), og.specpara(*v, kin=0)[0][0], rtol=1e-11,
+                       atol=1e-14 * og.energy(*v, 0))
+    assert np.allclose(og.heltpara(*v, *v), Hp[0], rtol=1e-11, atol=1e-14 * eng)
+    assert np.allclose(og.heltperp(*v, *v), Hc[0], rtol=1e-11, atol=1e-14 * eng)
+
+
+def test_axisymmetric_single_mode_bins():
+    og = O.Grid(32, 32, 16, L=(1.0, 1.0, 0.5), arbsize=True)
+    m = og.czeros()
+    m[3, 4, 16 - 2] = 1.0  # kx = 3, ky = 4, kz index -2 (kz = -4 in a box of length 0.5)
+    z = og.czeros()
+    Ep, _ = og.specpara(m, z, z, kin=1)
+    Ec, _ = og.specperp(m, z, z, kin=1)
+    assert np.nonzero(Ep[0])[0].tolist() == [2]  # int(|kz| Lz + 1) = 3
+    assert np.nonzero(Ec[0])[0].tolist() == [5]  # int(sqrt(9 + 16)/Dkk + 1) = 6
+    assert not np.any(Ec[1]) and not np.any(Ec[2])  # kz != 0
+    assert og.nbpara == 9 and og.nbperp == 17
