@@ -175,6 +175,9 @@ def _declare(L):
         "ghx_sctrans": (_I, [_P, _P, _P, _DP]),
         "ghx_difucx": (_I, [_P, _P, _P, _DP]),
         "ghx_difucz": (_I, [_P, _P, _P, _DP]),
+        "ghx_havgcomp": (_I, [_P] * 5 + [_D, _D, _I, _DP]),
+        "ghx_tbouss": (_I, [_P] * 5 + [_D, _D, _DP, _DP, _DP]),
+        "ghx_spectpv": (_I, [_P] * 5 + [_DP]),
         "ghx_axis_bins": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "ghx_specpara": (_I, [_P] * 4 + [_I, _I, _DP, _DP]),
         "ghx_specperp": (_I, [_P] * 4 + [_I, _I, _DP, _DP]),

@@ -999,6 +999,133 @@ extern "C" int ghx_spec2d(ghx_grid *g, const void *a, const void *b, const void 
   return GHX_OK;
 }
 
+// ---- Boussinesq diagnostics (pseudospec3D_bouss.f90) ---------------------
+
+// havgcomp (:19-555): horizontal averages as functions of z, gsh = nz host
+// doubles.  itype 0 <(du_perp/dz)^2>, 1 <(dth/dz)^2>, 2 <u_z dth/dz>, 3
+// <u^2 + v^2>, 4 <w^2>, 5 <u_perp . (curl u)_perp>, 6 <omega_z th>, 7 <PV^2>
+// (PV = fo dth/dz - bv omega_z + omega . grad th - fo bv), 8 <omega . curl
+// omega>, 9 the shear of itype 0 (GHOST's first branch serves 0 and 9 and
+// returns), 10 <u_z th>.  For itype 2 GHOST reads a work array the call never
+// fills; the documented correlation with dth/dz is formed here.
+extern "C" int ghx_havgcomp(ghx_grid *g, const void *u, const void *v, const void *w, const void *s, double fo,
+                            double bv, int itype, double *gsh) {
+  ARGCHK(g && u && v && w && s && gsh && itype >= 0 && itype <= 10, "bad havgcomp arguments");
+  int rc = grid_ctmp(g, 4);
+  if (!rc) rc = grid_rtmp(g, 4);
+  if (!rc) rc = grid_work(g, (size_t)g->gk.nz);
+  if (rc) return rc;
+  const size_t nc = g->nc, nr = g->nr;
+  cplx *c1 = g->ctmp, *c2 = c1 + nc, *c3 = c2 + nc, *c4 = c3 + nc;
+  double *r1 = g->rtmp, *r2 = r1 + nr, *r3 = r2 + nr, *r4 = r3 + nr;
+  const double nx = g->gk.nx, ny = g->gk.ny, nz = g->gk.nz;
+  const double tmp = 1.0 / (nx * nx * nx * ny * ny * ny * nz * nz);
+  auto c2r = [&](int n, const void *a0, double *o0, const void *a1 = nullptr, double *o1 = nullptr,
+                 const void *a2 = nullptr, double *o2 = nullptr) {
+    const void *in[3] = {a0, a1, a2};
+    double *out[3] = {o0, o1, o2};
+    return do_c2r(&g->plan, n, in, out);
+  };
+  auto zsum = [&](int kind, double scale, double *out) {
+    HIPCHK(ghx::op_zplane(g->gk, g->stream, kind, r1, r2, r3, r4, fo, bv, scale, g->work, out));
+    return GHX_OK;
+  };
+  switch (itype) {
+    case 0:
+    case 9:
+      if ((rc = ghx_derivk3(g, u, c1, 3)) || (rc = ghx_derivk3(g, v, c2, 3)) || (rc = c2r(2, c1, r1, c2, r2)))
+        return rc;
+      return zsum(0, tmp, gsh);
+    case 1:
+      if ((rc = ghx_derivk3(g, s, c1, 3)) || (rc = c2r(1, c1, r1))) return rc;
+      return zsum(1, tmp, gsh);
+    case 2:
+      if ((rc = ghx_derivk3(g, s, c1, 3)) || (rc = c2r(2, c1, r1, w, r2))) return rc;
+      return zsum(2, tmp, gsh);
+    case 3:
+      if ((rc = c2r(2, u, r1, v, r2))) return rc;
+      return zsum(0, tmp, gsh);
+    case 4:
+      if ((rc = c2r(1, w, r1))) return rc;
+      return zsum(1, tmp, gsh);
+    case 5:
+      if ((rc = ghx_derivk3(g, v, c1, 3)) || (rc = ghx_derivk3(g, u, c2, 3))) return rc;
+      if ((rc = c2r(2, u, r1, c1, r2)) || (rc = c2r(2, v, r3, c2, r4))) return rc;
+      return zsum(3, tmp, gsh);
+    case 6:
+    case 10:
+      if (itype == 6) {
+        if ((rc = ghx_rotor3(g, u, v, c1, 3)) || (rc = c2r(2, c1, r1, s, r2))) return rc;
+      } else if ((rc = c2r(2, w, r1, s, r2))) {
+        return rc;
+      }
+      return zsum(2, tmp, gsh);
+    case 7:
+      // r1 = dth/dz, r3 = omega_z, r4 = -omega . grad th (advect3)
+      if ((rc = ghx_derivk3(g, s, c4, 3)) || (rc = curl3(g, u, v, w, c1))) return rc;
+      if ((rc = ghx_advect3(g, c1, c2, c3, s, c1))) return rc;
+      if ((rc = ghx_rotor3(g, u, v, c2, 3))) return rc;
+      if ((rc = c2r(3, c4, r1, c2, r3, c1, r4))) return rc;
+      return zsum(5, 1.0 / (nx * ny), gsh);
+    case 8: {
+      // omega . curl omega, one component at a time: r3 (r1 - r2)
+      const int n = g->gk.nz;
+      std::vector<double> part(3 * (size_t)n);
+      if ((rc = curl3(g, u, v, w, c1))) return rc;  // c1, c2, c3 = omega_x, omega_y, omega_z
+      if ((rc = ghx_derivk3(g, c3, c4, 2)) || (rc = c2r(2, c4, r1, c1, r3))) return rc;
+      if ((rc = ghx_derivk3(g, c2, c4, 3)) || (rc = c2r(1, c4, r2))) return rc;
+      if ((rc = zsum(4, 1.0, part.data()))) return rc;
+      if ((rc = ghx_derivk3(g, c1, c4, 3)) || (rc = c2r(2, c4, r1, c2, r3))) return rc;
+      if ((rc = ghx_derivk3(g, c3, c4, 1)) || (rc = c2r(1, c4, r2))) return rc;
+      if ((rc = zsum(4, 1.0, part.data() + n))) return rc;
+      if ((rc = ghx_derivk3(g, c2, c4, 1)) || (rc = c2r(2, c4, r1, c3, r3))) return rc;
+      if ((rc = ghx_derivk3(g, c1, c4, 2)) || (rc = c2r(1, c4, r2))) return rc;
+      if ((rc = zsum(4, 1.0, part.data() + 2 * n))) return rc;
+      for (int k = 0; k < n; ++k) gsh[k] = (part[k] + part[n + k] + part[2 * n + k]) * tmp;
+      return GHX_OK;
+    }
+  }
+  return GHX_OK;
+}
+
+// tbouss (:612-709) without its file output: the mean, maximum and minimum
+// over z of havgcomp's itype 0..10 (11 host doubles each)
+extern "C" int ghx_tbouss(ghx_grid *g, const void *u, const void *v, const void *w, const void *s, double fo,
+                          double bv, double *avg, double *mx, double *mn) {
+  ARGCHK(g && u && v && w && s && avg && mx && mn, "bad tbouss arguments");
+  std::vector<double> gsh((size_t)g->gk.nz);
+  for (int i = 0; i < 11; ++i) {
+    int rc = ghx_havgcomp(g, u, v, w, s, fo, bv, i, gsh.data());
+    if (rc) return rc;
+    double sum = 0.0, hi = gsh[0], lo = gsh[0];
+    for (double x : gsh) {
+      sum += x;
+      hi = x > hi ? x : hi;
+      lo = x < lo ? x : lo;
+    }
+    avg[i] = sum / (double)g->gk.nz;
+    mx[i] = hi;
+    mn[i] = lo;
+  }
+  return GHX_OK;
+}
+
+// spectpv (:712-812): |omega . grad th|^2 over planes int(|kz| Lz + 1),
+// nz/2+1 host doubles
+extern "C" int ghx_spectpv(ghx_grid *g, const void *u, const void *v, const void *w, const void *s, double *Ek) {
+  ARGCHK(g && u && v && w && s && Ek, "bad spectpv arguments");
+  int rc = grid_ctmp(g, 4);
+  if (!rc) rc = curl3(g, u, v, w, g->ctmp);
+  if (rc) return rc;
+  cplx *c = g->ctmp, *a = g->ctmp + 3 * g->nc;
+  if ((rc = ghx_advect3(g, c, c + g->nc, c + 2 * g->nc, s, a))) return rc;
+  const int nb = nb_para(g);
+  std::vector<double> o(3 * (size_t)nb);
+  if ((rc = axispec(g, 0, 3, a, a, a, nullptr, nullptr, nullptr, o.data()))) return rc;
+  for (int q = 0; q < nb; ++q) Ek[q] = o[q];
+  return GHX_OK;
+}
+
 // hdcheck (pseudospec3D_hd.f90:984-1105) without its file output: out =
 // {eng, ens, pot, khe, div}: <v^2>, <omega^2>, the injection rate <v.f>,
 // the kinetic helicity (hel = 1) and <(div v)^2> (chk = 1); entries that

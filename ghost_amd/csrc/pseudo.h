@@ -64,6 +64,10 @@ hipError_t op_spec2d(const GridK &g, hipStream_t st, int kind, int np, int nzb, 
 // doubles; work holds 3 n doubles.
 hipError_t op_plane_means(const GridK &g, hipStream_t st, const double *r1, const double *r2, int dir, double tmp,
                           double tmq, double *work, double *out);
+// havgcomp's per-z-plane sums of real fields (kinds in pseudo.hip): nz host
+// doubles, each times scale; work holds nz doubles.
+hipError_t op_zplane(const GridK &g, hipStream_t st, int kind, const double *r1, const double *r2, const double *r3,
+                     const double *r4, double fo, double bv, double scale, double *work, double *out);
 hipError_t op_spectr1d(const GridK &g, hipStream_t st, int dir, double ldir, int nb, const cplx *a, double *work,
                        double *out);
 // max over real points of sqrt(r1^2+r2^2+r3^2) (maxabs)

@@ -446,6 +446,9 @@ __global__ void k_axispec(GridK g, int bins, int kind, int nb, double dkk, doubl
     if (kind == 0) {
       h = cabs2(a[m]) + cabs2(b[m]);
       v = cabs2(c[m]);
+    } else if (kind == 3) {  // one field (spectpv)
+      h = cabs2(a[m]);
+      v = 0.0;
     } else {
       h = cre_dot(a[m], d[m]) + cre_dot(b[m], e[m]);
       v = cre_dot(c[m], f[m]);
@@ -514,6 +517,36 @@ __global__ void k_plane_means(const double *r1, const double *r2, int nx, int ny
     out[n + s] = a2 * tmp;
     out[2 * n + s] = a3 * tmq;
   }
+}
+
+// havgcomp (pseudospec3D_bouss.f90:19-555): per z plane (one block each),
+// the sum over x, y of kind 0: r1^2 + r2^2, 1: r1^2, 2: r1 r2, 3: -r1 r2 +
+// r3 r4, 4: r3 (r1 - r2), 5: PV^2 with PV = fo r1 t - bv r3 t - r4 t - fo bv
+// accumulated in GHOST's order (r1 = dth/dz, r3 = omega_z, r4 = the advect3
+// term, t = 1/(nx ny nz)); out[k] = sum times scale.
+__global__ void k_zplane(int kind, size_t plane, const double *r1, const double *r2, const double *r3,
+                         const double *r4, double fo, double bv, double t, double scale, double *out) {
+  const size_t base = (size_t)blockIdx.x * plane;
+  double acc = 0.0;
+  for (size_t q = threadIdx.x; q < plane; q += blockDim.x) {
+    const size_t m = base + q;
+    switch (kind) {
+      case 0: acc += r1[m] * r1[m] + r2[m] * r2[m]; break;
+      case 1: acc += r1[m] * r1[m]; break;
+      case 2: acc += r1[m] * r2[m]; break;
+      case 3: acc += -(r1[m] * r2[m]) + r3[m] * r4[m]; break;
+      case 4: acc += r3[m] * (r1[m] - r2[m]); break;
+      default: {
+        double pv = 0.0 + fo * r1[m] * t;
+        pv = pv - bv * r3[m] * t;
+        pv = pv + r4[m] * (-t);
+        pv = pv - fo * bv;
+        acc += pv * pv;
+      }
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = acc * scale;
 }
 
 __global__ void k_maxnorm(size_t n, const double *r1, const double *r2, const double *r3, double *part) {
@@ -693,6 +726,18 @@ hipError_t op_plane_means(const GridK &g, hipStream_t st, const double *r1, cons
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   err = hipMemcpyAsync(out, work, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(st);
+}
+
+hipError_t op_zplane(const GridK &g, hipStream_t st, int kind, const double *r1, const double *r2, const double *r3,
+                     const double *r4, double fo, double bv, double scale, double *work, double *out) {
+  const double t = 1.0 / ((double)g.nx * (double)g.ny * (double)g.nz);
+  hipLaunchKernelGGL(k_zplane, dim3(g.nz), dim3(kBlock), 0, st, kind, (size_t)g.nx * g.ny, r1, r2, r3, r4, fo, bv, t,
+                     scale, work);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = hipMemcpyAsync(out, work, sizeof(double) * g.nz, hipMemcpyDeviceToHost, st);
   if (err != hipSuccess) return err;
   return hipStreamSynchronize(st);
 }

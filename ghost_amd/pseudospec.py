@@ -202,6 +202,24 @@ class Grid:
         _lib.check(fn(self._h, self._c(a), self._c(b), p))
         return [list(buf[q * n:(q + 1) * n]) for q in range(3)]
 
+    # ---- Boussinesq diagnostics (pseudospec3D_bouss.f90) ---------------------
+    def havgcomp(self, u, v, w, s, fo, bv, itype):
+        """havgcomp (pseudospec3D_bouss.f90:19-555): horizontal average of quantity itype, per z."""
+        buf, p = self._dp(self.nz)
+        _lib.check(_lib.load().ghx_havgcomp(self._h, *map(self._c, (u, v, w, s)), float(fo), float(bv), int(itype), p))
+        return list(buf)
+
+    def tbouss(self, u, v, w, s, fo, bv):
+        """tbouss (:612-709): (avg, max, min) over z of havgcomp itype 0..10."""
+        (a, pa), (b, pb), (c, pc) = self._dp(11), self._dp(11), self._dp(11)
+        _lib.check(_lib.load().ghx_tbouss(self._h, *map(self._c, (u, v, w, s)), float(fo), float(bv), pa, pb, pc))
+        return list(a), list(b), list(c)
+
+    def spectpv(self, u, v, w, s):
+        buf, p = self._dp(self.nz // 2 + 1)
+        _lib.check(_lib.load().ghx_spectpv(self._h, *map(self._c, (u, v, w, s)), p))
+        return list(buf)
+
     # ---- axisymmetric spectra (pseudospec3D_rot.f90) -------------------------
     def axis_bins(self):
         a, b = C.c_int(), C.c_int()

@@ -163,6 +163,18 @@ int ghx_sctrans(ghx_grid *g, const void *a, const void *b, double *Ek);
 int ghx_difucx(ghx_grid *g, const void *a, const void *b, double *out);
 int ghx_difucz(ghx_grid *g, const void *a, const void *b, double *out);
 
+/* Boussinesq diagnostics (pseudo/pseudospec3D_bouss.f90), values only.
+ * havgcomp (:19-555): horizontal averages gsh[nz] of quantity itype 0..10
+ * (fo: Coriolis parameter, bv: Brunt-Vaisala frequency); itype 9 returns the
+ * shear of itype 0 as GHOST's branch does, itype 2 the documented <u_z
+ * dth/dz>.  tbouss (:612-709): avg, max, min over z of itype 0..10, 11 each.
+ * spectpv (:712-812): |omega . grad th|^2 over |kz| planes, nz/2+1 doubles. */
+int ghx_havgcomp(ghx_grid *g, const void *u, const void *v, const void *w, const void *s, double fo, double bv,
+                 int itype, double *gsh);
+int ghx_tbouss(ghx_grid *g, const void *u, const void *v, const void *w, const void *s, double fo, double bv,
+               double *avg, double *mx, double *mn);
+int ghx_spectpv(ghx_grid *g, const void *u, const void *v, const void *w, const void *s, double *Ek);
+
 /* Axisymmetric spectra about z (pseudo/pseudospec3D_rot.f90), values only.
  * Plane bins int(|kz| Lz + 1), nbpara = nz/2+1; cylinder bins
  * int(sqrt(kx^2+ky^2)/Dkk + 1), nbperp = nmaxperp/2+1 (main3D.fpp:1729).

@@ -1733,6 +1733,135 @@ void og_difuc(const og_grid *g, const ocplx *a, const ocplx *b, int dir, double 
   free(r2);
 }
 
+/* ---- Boussinesq diagnostics (pseudospec3D_bouss.f90) ------------------- */
+
+/* horizontal (x, y) sums per z plane of f(r1, r2, r3, r4) for kinds
+ * 0: r1^2 + r2^2, 1: r1^2, 2: r1 r2, 3: -r1 r2 + r3 r4, 4: r3 (r1 - r2) */
+static void zplane_sums(const og_grid *g, int kind, const double *r1, const double *r2, const double *r3,
+                        const double *r4, double *sh) {
+  const size_t pl = (size_t)g->nx * g->ny;
+  for (int k = 0; k < g->nz; ++k) {
+    double acc = 0.0;
+    for (size_t t = 0; t < pl; ++t) {
+      const size_t m = (size_t)k * pl + t;
+      switch (kind) {
+        case 0: acc += r1[m] * r1[m] + r2[m] * r2[m]; break;
+        case 1: acc += r1[m] * r1[m]; break;
+        case 2: acc += r1[m] * r2[m]; break;
+        case 3: acc += -(r1[m] * r2[m]) + r3[m] * r4[m]; break;
+        default: acc += r3[m] * (r1[m] - r2[m]); break;
+      }
+    }
+    sh[k] = acc;
+  }
+}
+
+/* havgcomp (:19-555): horizontal averages as functions of z, gsh[nz].
+ * itype 0 shear <(du_perp/dz)^2>, 1 <(dth/dz)^2>, 2 <u_z dth/dz>, 3
+ * <u^2 + v^2>, 4 <w^2>, 5 <u_perp . (curl u)_perp>, 6 <omega_z th>, 7 <PV^2>
+ * with PV = fo dth/dz - bv omega_z + omega . grad th - fo bv, 8 <omega . curl
+ * omega>, 9 the shear of itype 0 (GHOST's first branch serves 0 and 9 and
+ * returns before the Richardson number is formed), 10 <u_z th>.  For itype 2
+ * GHOST multiplies a work array that this call never fills; here the
+ * documented correlation with dth/dz is formed. */
+void og_havgcomp(const og_grid *g, const ocplx *u, const ocplx *v, const ocplx *w, const ocplx *s, double fo,
+                 double bv, int itype, double *gsh) {
+  const size_t nc = ncplx(g), nr = (size_t)g->nx * g->ny * g->nz;
+  const double nx = g->nx, ny = g->ny, nz = g->nz;
+  const double tmp = 1.0 / (nx * nx * nx * ny * ny * ny * nz * nz);
+  ocplx *c = (ocplx *)malloc(sizeof(ocplx) * nc * 4);
+  ocplx *c1 = c, *c2 = c + nc, *c3 = c + 2 * nc, *c4 = c + 3 * nc;
+  double *r = (double *)malloc(sizeof(double) * nr * 4);
+  double *r1 = r, *r2 = r + nr, *r3 = r + 2 * nr, *r4 = r + 3 * nr;
+  if (itype == 0 || itype == 9) {
+    og_derivk3(g, u, c1, 3); og_c2r(g, c1, r1);
+    og_derivk3(g, v, c1, 3); og_c2r(g, c1, r2);
+    zplane_sums(g, 0, r1, r2, NULL, NULL, gsh);
+  } else if (itype == 1 || itype == 2) {
+    og_derivk3(g, s, c1, 3); og_c2r(g, c1, r3);
+    if (itype == 1) {
+      zplane_sums(g, 1, r3, NULL, NULL, NULL, gsh);
+    } else {
+      og_c2r(g, w, r2);
+      zplane_sums(g, 2, r3, r2, NULL, NULL, gsh);
+    }
+  } else if (itype == 3) {
+    og_c2r(g, u, r1); og_c2r(g, v, r2);
+    zplane_sums(g, 0, r1, r2, NULL, NULL, gsh);
+  } else if (itype == 4) {
+    og_c2r(g, w, r1);
+    zplane_sums(g, 1, r1, NULL, NULL, NULL, gsh);
+  } else if (itype == 5) {
+    og_c2r(g, u, r1); og_derivk3(g, v, c1, 3); og_c2r(g, c1, r2);
+    og_c2r(g, v, r3); og_derivk3(g, u, c1, 3); og_c2r(g, c1, r4);
+    zplane_sums(g, 3, r1, r2, r3, r4, gsh);
+  } else if (itype == 6 || itype == 10) {
+    if (itype == 6) { og_rotor3(g, u, v, c1, 3); og_c2r(g, c1, r1); }
+    else og_c2r(g, w, r1);
+    og_c2r(g, s, r2);
+    zplane_sums(g, 2, r1, r2, NULL, NULL, gsh);
+  } else if (itype == 7) {
+    const double t = 1.0 / (nx * ny * nz);
+    og_derivk3(g, s, c1, 3); og_c2r(g, c1, r1);          /* dth/dz */
+    og_rotor3(g, u, v, c1, 3); og_c2r(g, c1, r3);        /* omega_z */
+    og_rotor3(g, v, w, c1, 1); og_rotor3(g, u, w, c2, 2); og_rotor3(g, u, v, c3, 3);
+    og_advect3(g, c1, c2, c3, s, c4); og_c2r(g, c4, r4); /* -omega . grad th */
+    for (size_t m = 0; m < nr; ++m) {
+      double pv = 0.0 + fo * r1[m] * t;
+      pv = pv - bv * r3[m] * t;
+      pv = pv + r4[m] * (-t);
+      pv = pv - fo * bv;
+      r2[m] = pv;
+    }
+    zplane_sums(g, 1, r2, NULL, NULL, NULL, gsh);
+    for (int k = 0; k < g->nz; ++k) gsh[k] *= 1.0 / (nx * ny);
+    free(c); free(r);
+    return;
+  } else if (itype == 8) {
+    double *sa = (double *)malloc(sizeof(double) * g->nz * 3);
+    /* omega_x (d omega_z/dy - d omega_y/dz) */
+    og_rotor3(g, u, v, c1, 3); og_derivk3(g, c1, c2, 2); og_c2r(g, c2, r1);
+    og_rotor3(g, u, w, c1, 2); og_derivk3(g, c1, c2, 3); og_c2r(g, c2, r2);
+    og_rotor3(g, v, w, c1, 1); og_c2r(g, c1, r3);
+    zplane_sums(g, 4, r1, r2, r3, NULL, sa);
+    /* omega_y (d omega_x/dz - d omega_z/dx) */
+    og_rotor3(g, v, w, c1, 1); og_derivk3(g, c1, c2, 3); og_c2r(g, c2, r1);
+    og_rotor3(g, u, v, c1, 3); og_derivk3(g, c1, c2, 1); og_c2r(g, c2, r2);
+    og_rotor3(g, u, w, c1, 2); og_c2r(g, c1, r3);
+    zplane_sums(g, 4, r1, r2, r3, NULL, sa + g->nz);
+    /* omega_z (d omega_y/dx - d omega_x/dy) */
+    og_rotor3(g, u, w, c1, 2); og_derivk3(g, c1, c2, 1); og_c2r(g, c2, r1);
+    og_rotor3(g, v, w, c1, 1); og_derivk3(g, c1, c2, 2); og_c2r(g, c2, r2);
+    og_rotor3(g, u, v, c1, 3); og_c2r(g, c1, r3);
+    zplane_sums(g, 4, r1, r2, r3, NULL, sa + 2 * g->nz);
+    for (int k = 0; k < g->nz; ++k) gsh[k] = sa[k] + sa[g->nz + k] + sa[2 * g->nz + k];
+    free(sa);
+  } else {
+    for (int k = 0; k < g->nz; ++k) gsh[k] = 0.0;
+  }
+  for (int k = 0; k < g->nz; ++k) gsh[k] *= tmp;
+  free(c);
+  free(r);
+}
+
+/* spectpv (:712-812): |omega . grad th|^2 over planes of |kz|, nz/2+1 bins */
+void og_spectpv(const og_grid *g, const ocplx *u, const ocplx *v, const ocplx *w, const ocplx *s, double *Ek) {
+  const size_t nc = ncplx(g);
+  ocplx *c = (ocplx *)malloc(sizeof(ocplx) * nc * 4);
+  og_rotor3(g, v, w, c, 1); og_rotor3(g, u, w, c + nc, 2); og_rotor3(g, u, v, c + 2 * nc, 3);
+  og_advect3(g, c, c + nc, c + 2 * nc, s, c + 3 * nc);
+  const double tmp = nsix(g);
+  const int nb = g->nz / 2 + 1;
+  for (int q = 0; q < nb; ++q) Ek[q] = 0.0;
+  for (int i = 0; i < g->nxh; ++i)
+    for (int j = 0; j < g->ny; ++j)
+      for (int k = 0; k < g->nz; ++k) {
+        const int kmn = (int)(fabs(g->kz[k]) * g->Lz + 1);
+        if (kmn > 0 && kmn <= nb) Ek[kmn - 1] += (i == 0 ? 1.0 : 2.0) * tmp * abs2(c[3 * nc + IDX(g, i, j, k)]);
+      }
+  free(c);
+}
+
 /* ---- axisymmetric spectra (pseudospec3D_rot.f90), preferred axis z ----- */
 
 /* nmaxperp of main3D.fpp:1729 */

@@ -156,6 +156,12 @@ void og_spectrscc(const og_grid *g, const ocplx *a, double shift, double *Ek);
 void og_sctrans(const og_grid *g, const ocplx *a, const ocplx *b, double *Ek);
 void og_difuc(const og_grid *g, const ocplx *a, const ocplx *b, int dir, double *out);
 
+/* Boussinesq diagnostics (pseudospec3D_bouss.f90): havgcomp's horizontal
+ * averages gsh[nz] for itype 0..10, and spectpv (nz/2+1 bins) */
+void og_havgcomp(const og_grid *g, const ocplx *u, const ocplx *v, const ocplx *w, const ocplx *s, double fo,
+                 double bv, int itype, double *gsh);
+void og_spectpv(const og_grid *g, const ocplx *u, const ocplx *v, const ocplx *w, const ocplx *s, double *Ek);
+
 /* axisymmetric spectra (pseudospec3D_rot.f90): bins 0 = planes |kz|,
  * 1 = cylinders k_perp; kind 0 = energy, 1 = Re dot, 2 = kk2-weighted dot */
 int og_nmaxperp(const og_grid *g);

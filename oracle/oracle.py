@@ -116,6 +116,8 @@ def _declare(L):
     L.og_nmaxperp.argtypes = [_G]
     L.og_axispec.argtypes = [_G, C.c_int, C.c_int] + [_P] * 9
     L.og_spec2d.argtypes = [_G, C.c_int] + [_P] * 7
+    L.og_havgcomp.argtypes = [_G] + [_P] * 4 + [C.c_double, C.c_double, C.c_int, _P]
+    L.og_spectpv.argtypes = [_G] + [_P] * 5
     L.og_mlp_forward.argtypes = [_P, C.c_long, _P, C.c_int, _P]
     L.og_mlp_grad.restype = C.c_double
     L.og_mlp_grad.argtypes = [_P, C.c_long, _P, _P, C.c_int, _P]
@@ -367,6 +369,23 @@ class Grid:
             H = np.zeros(shape)
             lib().og_spec2d(self._g, 1, _p(a), _p(b), _p(c), *map(_p, cu), _p(H))
         return E, H
+
+    # -- Boussinesq diagnostics (pseudospec3D_bouss.f90) ----------------------
+    def havgcomp(self, u, v, w, s, fo, bv, itype):
+        out = np.zeros(self.nz)
+        lib().og_havgcomp(self._g, _p(u), _p(v), _p(w), _p(s), float(fo), float(bv), int(itype), _p(out))
+        return out
+
+    def tbouss(self, u, v, w, s, fo, bv):
+        """tbouss (:612-709): (avg, max, min) over z of havgcomp itype 0..10."""
+        prof = [self.havgcomp(u, v, w, s, fo, bv, i) for i in range(11)]
+        return (np.array([np.sum(p) / self.nz for p in prof]), np.array([np.max(p) for p in prof]),
+                np.array([np.min(p) for p in prof]))
+
+    def spectpv(self, u, v, w, s):
+        out = np.zeros(self.nz // 2 + 1)
+        lib().og_spectpv(self._g, _p(u), _p(v), _p(w), _p(s), _p(out))
+        return out
 
     def hdcheck(self, a, b, c, d, e, f, hel=1, chk=1):
         """hdcheck (pseudospec3D_hd.f90:984-1105) from its parts: {eng, ens, pot, khe, div}."""

@@ -112,3 +112,16 @@ def test_axisymmetric_spectra(gpu, shape, arb, L):
         assert rel(np.asarray(E), Er) < 1e-12
         if hel:
             assert rel(np.asarray(H), Hr) < 1e-11
+
+
+@pytest.mark.parametrize("shape,arb,L", CASES)
+def test_boussinesq_diagnostics(gpu, shape, arb, L):
+    g, og = grids(shape, arb, L)
+    f = [rand_spec(og, s) for s in (81, 82, 83, 84)]
+    F = [to_gpu(x) for x in f]
+    fo, bv = 1.7, 0.9
+    for it in range(11):
+        assert rel(g.havgcomp(*F, fo, bv, it), og.havgcomp(*f, fo, bv, it)) < 1e-12, it
+    for got, want in zip(g.tbouss(*F, fo, bv), og.tbouss(*f, fo, bv)):
+        assert rel(got, want) < 1e-12
+    assert rel(g.spectpv(*F), og.spectpv(*f)) < 1e-12

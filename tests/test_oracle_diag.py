@@ -176,3 +176,58 @@ def test_axisymmetric_single_mode_bins():
     assert np.nonzero(Ec[0])[0].tolist() == [5]  # int(sqrt(9 + 16)/Dkk + 1) = 6
     assert not np.any(Ec[1]) and not np.any(Ec[2])  # kz != 0
     assert og.nbpara == 9 and og.nbperp == 17
+
+
+# ---- Boussinesq diagnostics (pseudospec3D_bouss.f90) ------------------------
+
+def bouss_fields(og, A=0.7, B=-0.4, C=0.3, W0=0.25, F=0.5, D=0.6, E=-0.35):
+    """u = A sin z + F cos y, v = B cos z, w = W0 + C cos x, th = D sin z + E sin y,
+    whose derivatives and horizontal averages are known in closed form."""
+    n = og.nx
+    zz, yy, xx = np.meshgrid(*(2 * np.pi * np.arange(n) / n,) * 3, indexing="ij")
+    real = (A * np.sin(zz) + F * np.cos(yy), B * np.cos(zz) + 0 * xx, W0 + C * np.cos(xx) + 0 * zz,
+            D * np.sin(zz) + E * np.sin(yy) + 0 * xx)
+    return [og.r2c(r) for r in real], (xx, yy, zz), dict(A=A, B=B, C=C, W0=W0, F=F, D=D, E=E)
+
+
+def test_havgcomp_closed_forms():
+    n = 16
+    og = O.Grid(n)
+    (u, v, w, s), (xx, yy, zz), p = bouss_fields(og)
+    A, B, C, W0, F, D, E = (p[k] for k in ("A", "B", "C", "W0", "F", "D", "E"))
+    fo, bv = 1.3, 0.8
+    z = 2 * np.pi * np.arange(n) / n
+    want = {
+        0: A ** 2 * np.cos(z) ** 2 + B ** 2 * np.sin(z) ** 2,
+        1: D ** 2 * np.cos(z) ** 2,
+        2: W0 * D * np.cos(z),
+        3: A ** 2 * np.sin(z) ** 2 + F ** 2 / 2 + B ** 2 * np.cos(z) ** 2,
+        4: np.full(n, W0 ** 2 + C ** 2 / 2),
+        5: np.full(n, A * B),
+        6: np.full(n, F * E / 2),
+        8: np.full(n, A * B),
+        10: W0 * D * np.sin(z),
+    }
+    want[9] = want[0]  # GHOST's first branch serves itype 0 and 9
+    # potential vorticity from the analytic derivatives
+    om = (B * np.sin(zz), A * np.cos(zz) + C * np.sin(xx), F * np.sin(yy))
+    gth = (0 * xx, E * np.cos(yy), D * np.cos(zz))
+    pv = fo * D * np.cos(zz) - bv * om[2] + sum(o * g_ for o, g_ in zip(om, gth)) - fo * bv
+    want[7] = np.mean(pv ** 2, axis=(1, 2))
+    for it, prof in want.items():
+        got = og.havgcomp(u, v, w, s, fo, bv, it)
+        assert np.allclose(got, prof, rtol=1e-12, atol=1e-13), it
+    avg, mx, mn = og.tbouss(u, v, w, s, fo, bv)
+    assert np.allclose(avg[[4, 5]], [W0 ** 2 + C ** 2 / 2, A * B], rtol=1e-12)
+    assert mx[0] == pytest.approx(max(A, B, key=abs) ** 2, rel=1e-12) and mn[10] == pytest.approx(-abs(W0 * D), rel=1e-12)
+
+
+def test_spectpv_closed_form():
+    og = O.Grid(16)
+    (u, v, w, s), _, p = bouss_fields(og)
+    A, C, F, D, E = (p[k] for k in ("A", "C", "F", "D", "E"))
+    S = og.spectpv(u, v, w, s)
+    # omega . grad th = E C sin x cos y (kz = 0) + (E A cos y + F D sin y) cos z (|kz| = 1)
+    assert S[0] == pytest.approx(E ** 2 * C ** 2 / 4, rel=1e-12)
+    assert S[1] == pytest.approx((E ** 2 * A ** 2 + F ** 2 * D ** 2) / 4, rel=1e-12)
+    assert np.all(np.abs(S[2:]) < 1e-25)
