@@ -181,7 +181,7 @@ def test_randu_matches_oracle(gpu):
         assert a == b and s1 == s2
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (24, 16, 12), (64, 64, 64)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (24, 16, 12), (64, 64, 64), (16, 16, 512), (8, 12, 512)])
 def test_hd_substep_fused(gpu, shape):
     g, og = grids(shape)
     v, _ = og.initialv_k4exp(2.5, 1.0, 2024)
