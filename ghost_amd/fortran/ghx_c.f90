@@ -209,6 +209,124 @@ MODULE ghx_c
       INTEGER(C_INT), VALUE :: hel, crs, chk
       REAL(C_DOUBLE) :: out(11)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_variance(g, a, kin, out) &
+        BIND(C, name="ghx_variance")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a
+      INTEGER(C_INT), VALUE :: kin
+      REAL(C_DOUBLE) :: out
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_product(g, a, b, out) &
+        BIND(C, name="ghx_product")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b
+      REAL(C_DOUBLE) :: out
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_pscheck(g, a, b, out) &
+        BIND(C, name="ghx_pscheck")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b
+      REAL(C_DOUBLE) :: out(3)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_spectrscc(g, a, shift, Ek) &
+        BIND(C, name="ghx_spectrscc")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a
+      REAL(C_DOUBLE), VALUE :: shift
+      REAL(C_DOUBLE) :: Ek(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_sctrans(g, a, b, Ek) &
+        BIND(C, name="ghx_sctrans")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b
+      REAL(C_DOUBLE) :: Ek(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_difucx(g, a, b, out) &
+        BIND(C, name="ghx_difucx")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b
+      REAL(C_DOUBLE) :: out(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_difucz(g, a, b, out) &
+        BIND(C, name="ghx_difucz")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b
+      REAL(C_DOUBLE) :: out(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_axis_bins(g, nbpara, nbperp) &
+        BIND(C, name="ghx_axis_bins")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g
+      INTEGER(C_INT) :: nbpara, nbperp
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_specpara(g, a, b, c, kin, hel, Ek, Hk) &
+        BIND(C, name="ghx_specpara")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c
+      INTEGER(C_INT), VALUE :: kin, hel
+      REAL(C_DOUBLE) :: Ek(*), Hk(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_specperp(g, a, b, c, kin, hel, Ek, Hk) &
+        BIND(C, name="ghx_specperp")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c
+      INTEGER(C_INT), VALUE :: kin, hel
+      REAL(C_DOUBLE) :: Ek(*), Hk(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_spec2d(g, a, b, c, kin, hel, Ek, Hk) &
+        BIND(C, name="ghx_spec2d")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c
+      INTEGER(C_INT), VALUE :: kin, hel
+      REAL(C_DOUBLE) :: Ek(*), Hk(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_entpara(g, a, b, c, d, e, f, kin, Ek) &
+        BIND(C, name="ghx_entpara")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c, d, e, f
+      INTEGER(C_INT), VALUE :: kin
+      REAL(C_DOUBLE) :: Ek(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_entperp(g, a, b, c, d, e, f, kin, Ek) &
+        BIND(C, name="ghx_entperp")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c, d, e, f
+      INTEGER(C_INT), VALUE :: kin
+      REAL(C_DOUBLE) :: Ek(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_heltpara(g, a, b, c, d, e, f, Hk) &
+        BIND(C, name="ghx_heltpara")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c, d, e, f
+      REAL(C_DOUBLE) :: Hk(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_heltperp(g, a, b, c, d, e, f, Hk) &
+        BIND(C, name="ghx_heltperp")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, a, b, c, d, e, f
+      REAL(C_DOUBLE) :: Hk(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_havgcomp(g, u, v, w, s, fo, bv, itype, gsh) &
+        BIND(C, name="ghx_havgcomp")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, u, v, w, s
+      REAL(C_DOUBLE), VALUE :: fo, bv
+      INTEGER(C_INT), VALUE :: itype
+      REAL(C_DOUBLE) :: gsh(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_tbouss(g, u, v, w, s, fo, bv, avg, mx, mn) &
+        BIND(C, name="ghx_tbouss")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, u, v, w, s
+      REAL(C_DOUBLE), VALUE :: fo, bv
+      REAL(C_DOUBLE) :: avg(11), mx(11), mn(11)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_spectpv(g, u, v, w, s, Ek) &
+        BIND(C, name="ghx_spectpv")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g, u, v, w, s
+      REAL(C_DOUBLE) :: Ek(*)
+    END FUNCTION
     INTEGER(C_INT) FUNCTION ghx_phaseshift(g, a, cre, cim) BIND(C, name="ghx_phaseshift")
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: g, a

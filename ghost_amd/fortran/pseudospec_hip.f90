@@ -24,7 +24,8 @@ MODULE pseudospec_hip
   INTEGER(C_SIZE_T), SAVE :: gnc = 0, gnr = 0
   ! shell width and count of the 3D spectra, wavenumber steps of the 1D ones
   INTEGER, SAVE :: gnmax = 0
-  DOUBLE PRECISION, SAVE :: gDkk = 1.0D0, gDk(3) = 1.0D0
+  DOUBLE PRECISION, SAVE :: gDkk = 1.0D0, gDk(3) = 1.0D0, gL(3) = 1.0D0
+  DOUBLE PRECISION, PARAMETER :: gpi = 3.14159265358979323846D0
 
 CONTAINS
 
@@ -46,7 +47,11 @@ CONTAINS
     gnmax = nmax
     gDkk = dk
     gDk = 1.0D0
-    IF (arbsize .EQ. 1) gDk = (/1.0D0/Lx, 1.0D0/Ly, 1.0D0/Lz/)
+    gL = 1.0D0
+    IF (arbsize .EQ. 1) THEN
+      gDk = (/1.0D0/Lx, 1.0D0/Ly, 1.0D0/Lz/)
+      gL = (/DBLE(Lx), DBLE(Ly), DBLE(Lz)/)
+    END IF
   END SUBROUTINE ghx_pseudospec_init
 
   SUBROUTINE ghx_pseudospec_finalize()
@@ -322,6 +327,337 @@ CONTAINS
       CLOSE(u)
     END IF
   END SUBROUTINE mhdcheck
+
+  ! -- pseudo/pseudospec3D_phd.f90: passive-scalar diagnostics --------------
+  SUBROUTINE variance(a, b, kin)
+    TYPE(ghx_field), INTENT(IN) :: a
+    DOUBLE PRECISION, INTENT(OUT) :: b
+    INTEGER, INTENT(IN) :: kin
+    REAL(C_DOUBLE) :: r
+    CALL ghx_check(ghx_variance(ghx_grid_h, a%p, INT(kin, C_INT), r), 'variance')
+    b = r
+  END SUBROUTINE variance
+
+  SUBROUTINE product(a, b, c)
+    TYPE(ghx_field), INTENT(IN) :: a, b
+    DOUBLE PRECISION, INTENT(OUT) :: c
+    REAL(C_DOUBLE) :: r
+    CALL ghx_check(ghx_product(ghx_grid_h, a%p, b%p, r), 'product')
+    c = r
+  END SUBROUTINE product
+
+  ! pscheck (:286-333): appends scalar.txt
+  SUBROUTINE pscheck(a, b, t, dt)
+    TYPE(ghx_field), INTENT(IN) :: a, b
+    INTEGER, INTENT(IN) :: t
+    REAL(KIND=GP), INTENT(IN) :: dt
+    REAL(C_DOUBLE) :: r(3)
+    INTEGER :: u
+    CALL ghx_check(ghx_pscheck(ghx_grid_h, a%p, b%p, r), 'pscheck')
+    OPEN(NEWUNIT=u, FILE='scalar.txt', POSITION='append')
+    WRITE(u, '(E13.6,E22.14,E22.14,E22.14)') (t - 1)*dt, r(1), r(2), r(3)
+    CLOSE(u)
+  END SUBROUTINE pscheck
+
+  ! spectrsc (:450-514): sspectrum, sNspectrum (isc > 0) or rhospectrum (isc = -1)
+  SUBROUTINE spectrsc(a, nmb, isc)
+    TYPE(ghx_field), INTENT(IN) :: a
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: isc
+    DOUBLE PRECISION :: Ek(gnmax/2 + 1)
+    CHARACTER(len=1) :: si
+    CALL ghx_check(ghx_spectrscc(ghx_grid_h, a%p, 0.0_C_DOUBLE, Ek), 'spectrsc')
+    IF (isc .GT. 0) THEN
+      WRITE(si, '(i1.1)') isc
+      CALL shell_file('s'//si//'spectrum.'//nmb//'.txt', Ek, 1.0D0)
+    ELSE IF (isc .EQ. 0) THEN
+      CALL shell_file('sspectrum.'//nmb//'.txt', Ek, 1.0D0)
+    ELSE IF (isc .EQ. -1) THEN
+      CALL shell_file('rhospectrum.'//nmb//'.txt', Ek, 1.0D0)
+    ELSE
+      WRITE(*, '(A)') 'SPECTRSC: Invalid option'
+      STOP 1
+    END IF
+  END SUBROUTINE spectrsc
+
+  ! spectrscc (:516-610): shell sums, no file
+  SUBROUTINE spectrscc(a, Ektot, shift)
+    TYPE(ghx_field), INTENT(IN) :: a
+    DOUBLE PRECISION, INTENT(OUT) :: Ektot(*)
+    REAL(KIND=GP), INTENT(IN) :: shift
+    CALL ghx_check(ghx_spectrscc(ghx_grid_h, a%p, REAL(shift, C_DOUBLE), Ektot), 'spectrscc')
+  END SUBROUTINE spectrscc
+
+  ! sctrans (:612-727): stransfer or sNtransfer (isc > 0)
+  SUBROUTINE sctrans(a, b, nmb, isc)
+    TYPE(ghx_field), INTENT(IN) :: a, b
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: isc
+    DOUBLE PRECISION :: Ek(gnmax/2 + 1)
+    CHARACTER(len=1) :: si
+    CALL ghx_check(ghx_sctrans(ghx_grid_h, a%p, b%p, Ek), 'sctrans')
+    IF (isc .GT. 0) THEN
+      WRITE(si, '(i1.1)') isc
+      CALL shell_file('s'//TRIM(si)//'transfer.'//nmb//'.txt', Ek, 1.0D0)
+    ELSE
+      CALL shell_file('stransfer.'//nmb//'.txt', Ek, 1.0D0)
+    END IF
+  END SUBROUTINE sctrans
+
+  ! difucx (:729-828) / difucz (:830-940): profilex / profilez.nmb.txt
+  SUBROUTINE difucx(a, b, nmb)
+    TYPE(ghx_field), INTENT(IN) :: a, b
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    DOUBLE PRECISION :: p(3*gnx)
+    INTEGER :: i, u
+    CALL ghx_check(ghx_difucx(ghx_grid_h, a%p, b%p, p), 'difucx')
+    OPEN(NEWUNIT=u, FILE='profilex.'//nmb//'.txt')
+    DO i = 1, gnx
+      WRITE(u, '(E23.15,E23.15,E23.15,E23.15)') REAL(2*gpi*gL(1)*(i - 1)/gnx, GP), p(i), p(gnx + i), p(2*gnx + i)
+    END DO
+    CLOSE(u)
+  END SUBROUTINE difucx
+
+  SUBROUTINE difucz(a, b, nmb)
+    TYPE(ghx_field), INTENT(IN) :: a, b
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    DOUBLE PRECISION :: p(3*gnz)
+    INTEGER :: k, u
+    CALL ghx_check(ghx_difucz(ghx_grid_h, a%p, b%p, p), 'difucz')
+    OPEN(NEWUNIT=u, FILE='profilez.'//nmb//'.txt')
+    DO k = 1, gnz
+      WRITE(u, '(E23.15,E23.15,E23.15,E23.15)') REAL(2*gpi*gL(3)*(k - 1)/gnz, GP), p(k), p(gnz + k), p(2*gnz + k)
+    END DO
+    CLOSE(u)
+  END SUBROUTINE difucz
+
+  ! -- pseudo/pseudospec3D_rot.f90: spectra about z --------------------------
+  ! specpara (:25-335): k/mspecpara (kin 1/0) and k/m/ghelipara (hel 1)
+  SUBROUTINE specpara(a, b, c, nmb, kin, hel)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: kin, hel
+    INTEGER(C_INT) :: np, nq
+    DOUBLE PRECISION, ALLOCATABLE :: E(:), H(:)
+    CALL ghx_check(ghx_axis_bins(ghx_grid_h, np, nq), 'specpara')
+    ALLOCATE(E(3*np), H(3*np))
+    CALL ghx_check(ghx_specpara(ghx_grid_h, a%p, b%p, c%p, INT(kin, C_INT), INT(hel, C_INT), E, H), 'specpara')
+    IF (kin .EQ. 1) CALL axis_file('kspecpara.'//nmb//'.txt', E, np, 0, 0.5D0)
+    IF (kin .EQ. 0) CALL axis_file('mspecpara.'//nmb//'.txt', E, np, 0, 0.5D0)
+    IF (hel .EQ. 1) CALL axis_file(hel_name(kin)//'helipara.'//nmb//'.txt', H, np, 0, 1.0D0)
+    DEALLOCATE(E, H)
+  END SUBROUTINE specpara
+
+  ! specperp (:338-686): k/mspecperp and k/m/gheliperp
+  SUBROUTINE specperp(a, b, c, nmb, kin, hel)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: kin, hel
+    INTEGER(C_INT) :: np, nq
+    DOUBLE PRECISION, ALLOCATABLE :: E(:), H(:)
+    CALL ghx_check(ghx_axis_bins(ghx_grid_h, np, nq), 'specperp')
+    ALLOCATE(E(3*nq), H(3*nq))
+    CALL ghx_check(ghx_specperp(ghx_grid_h, a%p, b%p, c%p, INT(kin, C_INT), INT(hel, C_INT), E, H), 'specperp')
+    IF (kin .EQ. 1) CALL axis_file('kspecperp.'//nmb//'.txt', E, nq, 1, 0.5D0)
+    IF (kin .EQ. 0) CALL axis_file('mspecperp.'//nmb//'.txt', E, nq, 1, 0.5D0)
+    IF (hel .EQ. 1) CALL axis_file(hel_name(kin)//'heliperp.'//nmb//'.txt', H, nq, 1, 1.0D0)
+    DEALLOCATE(E, H)
+  END SUBROUTINE specperp
+
+  ! entpara (:689-875) / entperp (:878-1064): m/k/jtranpara, m/k/jtranperp
+  SUBROUTINE entpara(a, b, c, d, e, f, nmb, kin)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c, d, e, f
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: kin
+    INTEGER(C_INT) :: np, nq
+    DOUBLE PRECISION, ALLOCATABLE :: T(:)
+    CALL ghx_check(ghx_axis_bins(ghx_grid_h, np, nq), 'entpara')
+    ALLOCATE(T(np))
+    CALL ghx_check(ghx_entpara(ghx_grid_h, a%p, b%p, c%p, d%p, e%p, f%p, INT(kin, C_INT), T), 'entpara')
+    CALL axis_file(tran_name(kin)//'tranpara.'//nmb//'.txt', T, np, 0, 1.0D0, 1)
+    DEALLOCATE(T)
+  END SUBROUTINE entpara
+
+  SUBROUTINE entperp(a, b, c, d, e, f, nmb, kin)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c, d, e, f
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: kin
+    INTEGER(C_INT) :: np, nq
+    DOUBLE PRECISION, ALLOCATABLE :: T(:)
+    CALL ghx_check(ghx_axis_bins(ghx_grid_h, np, nq), 'entperp')
+    ALLOCATE(T(nq))
+    CALL ghx_check(ghx_entperp(ghx_grid_h, a%p, b%p, c%p, d%p, e%p, f%p, INT(kin, C_INT), T), 'entperp')
+    CALL axis_file(tran_name(kin)//'tranperp.'//nmb//'.txt', T, nq, 1, 1.0D0, 1)
+    DEALLOCATE(T)
+  END SUBROUTINE entperp
+
+  ! heltpara (:1067-1196) / heltperp (:1199-1328): hm/hktranpara, hm/hktranperp
+  SUBROUTINE heltpara(a, b, c, d, e, f, nmb, kin)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c, d, e, f
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: kin
+    INTEGER(C_INT) :: np, nq
+    DOUBLE PRECISION, ALLOCATABLE :: T(:)
+    CALL ghx_check(ghx_axis_bins(ghx_grid_h, np, nq), 'heltpara')
+    ALLOCATE(T(np))
+    CALL ghx_check(ghx_heltpara(ghx_grid_h, a%p, b%p, c%p, d%p, e%p, f%p, T), 'heltpara')
+    IF (kin .EQ. 0) THEN
+      CALL axis_file('hmtranpara.'//nmb//'.txt', T, np, 0, 1.0D0, 1)
+    ELSE
+      CALL axis_file('hktranpara.'//nmb//'.txt', T, np, 0, 1.0D0, 1)
+    END IF
+    DEALLOCATE(T)
+  END SUBROUTINE heltpara
+
+  SUBROUTINE heltperp(a, b, c, d, e, f, nmb, kin)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c, d, e, f
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    INTEGER, INTENT(IN) :: kin
+    INTEGER(C_INT) :: np, nq
+    DOUBLE PRECISION, ALLOCATABLE :: T(:)
+    CALL ghx_check(ghx_axis_bins(ghx_grid_h, np, nq), 'heltperp')
+    ALLOCATE(T(nq))
+    CALL ghx_check(ghx_heltperp(ghx_grid_h, a%p, b%p, c%p, d%p, e%p, f%p, T), 'heltperp')
+    IF (kin .EQ. 0) THEN
+      CALL axis_file('hmtranperp.'//nmb//'.txt', T, nq, 1, 1.0D0, 1)
+    ELSE
+      CALL axis_file('hktranperp.'//nmb//'.txt', T, nq, 1, 1.0D0, 1)
+    END IF
+    DEALLOCATE(T)
+  END SUBROUTINE heltperp
+
+  ! spec2D (:1331-1629): unformatted dir/k,mspec2D.nmb.out and k,m,gheli2D,
+  ! one record of REAL(GP) E(kperp, kpara)
+  SUBROUTINE spec2D(a, b, c, nmb, dir, kin, hel)
+    TYPE(ghx_field), INTENT(IN) :: a, b, c
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    CHARACTER(len=100), INTENT(IN) :: dir
+    INTEGER, INTENT(IN) :: kin, hel
+    INTEGER(C_INT) :: np, nq
+    DOUBLE PRECISION, ALLOCATABLE :: E(:), H(:)
+    INTEGER :: u
+    CALL ghx_check(ghx_axis_bins(ghx_grid_h, np, nq), 'spec2D')
+    ALLOCATE(E(np*nq), H(np*nq))
+    CALL ghx_check(ghx_spec2d(ghx_grid_h, a%p, b%p, c%p, INT(kin, C_INT), INT(hel, C_INT), E, H), 'spec2D')
+    IF (kin .LE. 1) THEN
+      IF (kin .EQ. 1) THEN
+        OPEN(NEWUNIT=u, FILE=TRIM(dir)//'/'//'kspec2D.'//nmb//'.out', FORM='unformatted')
+      ELSE
+        OPEN(NEWUNIT=u, FILE=TRIM(dir)//'/'//'mspec2D.'//nmb//'.out', FORM='unformatted')
+      END IF
+      WRITE(u) REAL(E, GP)
+      CLOSE(u)
+    END IF
+    IF (hel .EQ. 1) THEN
+      OPEN(NEWUNIT=u, FILE=TRIM(dir)//'/'//hel_name(kin)//'heli2D.'//nmb//'.out', FORM='unformatted')
+      WRITE(u) REAL(H, GP)
+      CLOSE(u)
+    END IF
+    DEALLOCATE(E, H)
+  END SUBROUTINE spec2D
+
+  ! -- pseudo/pseudospec3D_bouss.f90: stratified-flow diagnostics -------------
+  SUBROUTINE havgcomp(gsh, u, v, w, s, fo, bv, itype)
+    DOUBLE PRECISION, INTENT(INOUT) :: gsh(*)
+    TYPE(ghx_field), INTENT(IN) :: u, v, w, s
+    REAL(KIND=GP), INTENT(IN) :: fo, bv
+    INTEGER, INTENT(IN) :: itype
+    CALL ghx_check(ghx_havgcomp(ghx_grid_h, u%p, v%p, w%p, s%p, REAL(fo, C_DOUBLE), REAL(bv, C_DOUBLE), &
+                                INT(itype, C_INT), gsh), 'havgcomp')
+  END SUBROUTINE havgcomp
+
+  ! havgwrite (:557-610): spref.nmb.txt, z and the profile
+  SUBROUTINE havgwrite(itype, spref, nmb, u, v, w, s, fo, bv)
+    INTEGER, INTENT(IN) :: itype
+    CHARACTER(len=*), INTENT(IN) :: spref, nmb
+    TYPE(ghx_field), INTENT(IN) :: u, v, w, s
+    REAL(KIND=GP), INTENT(IN) :: fo, bv
+    DOUBLE PRECISION :: gsh(gnz)
+    INTEGER :: k, un
+    CALL havgcomp(gsh, u, v, w, s, fo, bv, itype)
+    OPEN(NEWUNIT=un, FILE=TRIM(spref)//'.'//TRIM(nmb)//'.txt')
+    DO k = 1, gnz
+      WRITE(un, '(E23.15,E23.15)') REAL(2*gpi*gL(3)*(k - 1)/gnz, GP), gsh(k)
+    END DO
+    CLOSE(un)
+  END SUBROUTINE havgwrite
+
+  ! tbouss (:612-709): appends tboussavg, tboussmax, tboussmin.txt
+  SUBROUTINE tbouss(u, v, w, s, t, dt, fo, bv)
+    TYPE(ghx_field), INTENT(IN) :: u, v, w, s
+    INTEGER, INTENT(IN) :: t
+    REAL(KIND=GP), INTENT(IN) :: dt, fo, bv
+    REAL(C_DOUBLE) :: avg(11), mx(11), mn(11)
+    INTEGER :: un
+    CALL ghx_check(ghx_tbouss(ghx_grid_h, u%p, v%p, w%p, s%p, REAL(fo, C_DOUBLE), REAL(bv, C_DOUBLE), avg, mx, mn), &
+                   'tbouss')
+    OPEN(NEWUNIT=un, FILE='tboussavg.txt', POSITION='append')
+    WRITE(un, '(E13.6,1x,11(E26.18,1x))') (t - 1)*dt, avg
+    CLOSE(un)
+    OPEN(NEWUNIT=un, FILE='tboussmax.txt', POSITION='append')
+    WRITE(un, '(E13.6,1x,11(E26.18,1x))') (t - 1)*dt, mx
+    CLOSE(un)
+    OPEN(NEWUNIT=un, FILE='tboussmin.txt', POSITION='append')
+    WRITE(un, '(E13.6,1x,11(E26.18,1x))') (t - 1)*dt, mn
+    CLOSE(un)
+  END SUBROUTINE tbouss
+
+  ! spectpv (:712-812): pvspectrum.nmb.txt, kz and the PV spectrum
+  SUBROUTINE spectpv(u, v, w, s, nmb)
+    TYPE(ghx_field), INTENT(IN) :: u, v, w, s
+    CHARACTER(len=*), INTENT(IN) :: nmb
+    DOUBLE PRECISION :: Ek(gnz/2 + 1)
+    INTEGER :: k, un
+    CALL ghx_check(ghx_spectpv(ghx_grid_h, u%p, v%p, w%p, s%p, Ek), 'spectpv')
+    OPEN(NEWUNIT=un, FILE='pvspectrum.'//nmb//'.txt')
+    DO k = 1, gnz/2 + 1
+      WRITE(un, FMT='(E13.6,E23.15)') REAL(gDk(3)*(k - 1), GP), Ek(k)
+    END DO
+    CLOSE(un)
+  END SUBROUTINE spectpv
+
+  ! file-name prefixes of the helicity (k, m, g by kin) and transfer (m, k, j) files
+  CHARACTER(len=1) FUNCTION hel_name(kin)
+    INTEGER, INTENT(IN) :: kin
+    hel_name = 'g'
+    IF (kin .EQ. 1) hel_name = 'k'
+    IF (kin .EQ. 0) hel_name = 'm'
+  END FUNCTION hel_name
+  CHARACTER(len=1) FUNCTION tran_name(kin)
+    INTEGER, INTENT(IN) :: kin
+    tran_name = 'j'
+    IF (kin .EQ. 1) tran_name = 'k'
+    IF (kin .EQ. 0) tran_name = 'm'
+  END FUNCTION tran_name
+
+  ! plane (bins 0: kz = Dkz (q-1), values times Lz) or cylinder (bins 1:
+  ! kperp = Dkk (q-1), values over Dkk) files; S holds 3 n values {total, xy,
+  ! z} written as four columns, or ncol = 1: n values as two columns
+  SUBROUTINE axis_file(fname, S, n, bins, scale, ncol)
+    CHARACTER(len=*), INTENT(IN) :: fname
+    DOUBLE PRECISION, INTENT(IN) :: S(*), scale
+    INTEGER(C_INT), INTENT(IN) :: n
+    INTEGER, INTENT(IN) :: bins
+    INTEGER, INTENT(IN), OPTIONAL :: ncol
+    DOUBLE PRECISION :: k, f
+    INTEGER :: q, u
+    OPEN(NEWUNIT=u, FILE=fname)
+    DO q = 1, n
+      IF (bins .EQ. 0) THEN
+        k = gDk(3)*(q - 1)
+        f = scale*gL(3)
+      ELSE
+        k = gDkk*(q - 1)
+        f = scale/gDkk
+      END IF
+      IF (PRESENT(ncol)) THEN
+        WRITE(u, FMT='(E13.6,E23.15)') REAL(k, GP), f*S(q)
+      ELSE
+        WRITE(u, FMT='(E13.6,E23.15,E23.15,E23.15)') REAL(k, GP), f*S(q), f*S(n + q), f*S(2*n + q)
+      END IF
+    END DO
+    CLOSE(u)
+  END SUBROUTINE axis_file
 
   ! shells 1..nmax/2+1 as GHOST writes them: Dkk*i, scale*S(i)/Dkk
   SUBROUTINE shell_file(fname, S, scale)

@@ -6,8 +6,9 @@
 ! 2. Device-resident fields through pseudospec_hip: TG enstrophy, and an RK2
 !    loop written as GHOST's main3D (C = v; DO o = ord,1,-1: hd_rkstep2).
 ! 3. The resident solver with tracers attached.
-! 2b. GHOST's diagnostic text files (spectrum, entrans, heltrans, spectr1d,
-!    hdcheck) from the resident fields, written to the working directory.
+! 2b. GHOST's diagnostic files (spectrum, entrans, heltrans, spectr1d, hdcheck,
+!    the spectra about z, the scalar and the Boussinesq diagnostics) from the
+!    resident fields, written to the working directory (spec2D to OUTDIR).
 ! Writes OUTDIR/fortran_hd32.bin (v after the RK loop, 3 x C(nz,ny,nx/2+1))
 ! and OUTDIR/fortran_tracers32.bin (initial and final positions) for the
 ! comparison with the CPU oracle; prints one line per check and stops with a
@@ -19,7 +20,9 @@ PROGRAM test_fftp_hip
   USE ghx_c
   USE pseudospec_hip, ONLY: ghx_pseudospec_init, ghx_pseudospec_finalize, ghx_field, field_alloc, field_free, &
                             field_upload, field_download, ghx_grid_h, d_rotor3 => rotor3, d_energy => energy, &
-                            hd_rkstep2, spectrum, entrans, heltrans, spectr1d, hdcheck
+                            hd_rkstep2, spectrum, entrans, heltrans, spectr1d, hdcheck, specpara, specperp, &
+                            entpara, entperp, heltpara, spec2D, pscheck, spectrsc, sctrans, difucz, tbouss, &
+                            havgwrite, spectpv
   USE iovar, ONLY: IOPLAN
   IMPLICIT NONE
   INTEGER, PARAMETER :: n = 32, nh = n/2 + 1, nsteps = 20, npart = 1000, psteps = 5
@@ -139,6 +142,20 @@ PROGRAM test_fftp_hip
   CALL heltrans(w(1), w(2), w(3), cst(1), cst(2), cst(3), '0001', 1)
   CALL spectr1d(w(1), '0001', 'k', 1, 3)
   CALL hdcheck(w(1), w(2), w(3), cst(1), cst(2), cst(3), nsteps + 1, dt, 1, 1)
+  CALL specpara(w(1), w(2), w(3), '0001', 1, 1)
+  CALL specperp(w(1), w(2), w(3), '0001', 1, 1)
+  CALL entpara(w(1), w(2), w(3), cst(1), cst(2), cst(3), '0001', 1)
+  CALL entperp(w(1), w(2), w(3), cst(1), cst(2), cst(3), '0001', 0)
+  CALL heltpara(w(1), w(2), w(3), cst(1), cst(2), cst(3), '0001', 1)
+  iodir = outdir
+  CALL spec2D(w(1), w(2), w(3), '0001', iodir, 1, 0)
+  CALL pscheck(w(1), cst(1), nsteps + 1, dt)
+  CALL spectrsc(w(1), '0001', 0)
+  CALL sctrans(w(1), cst(1), '0001', 2)
+  CALL difucz(w(1), w(2), '0001')
+  CALL tbouss(w(1), w(2), w(3), cst(1), nsteps + 1, dt, 1.3_GP, 0.8_GP)
+  CALL havgwrite(7, 'pvprof', '0001', w(1), w(2), w(3), cst(1), 1.3_GP, 0.8_GP)
+  CALL spectpv(w(1), w(2), w(3), cst(1), '0001')
   WRITE(*, '(A)') 'ok   diagnostic files written'
 
   ! GHOST's binary output of v (main3D.fpp:2675-2710) from the resident

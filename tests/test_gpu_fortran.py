@@ -62,6 +62,40 @@ def test_fortran_host_side(gpu, tmp_path):
     assert np.allclose(bal[1:], want[:3], rtol=1e-13, atol=1e-13 * max(abs(want[:3])))
     assert table("helicity.txt")[1] == pytest.approx(want[3], rel=1e-13, abs=1e-13 * want[0])
     assert table("divergence.txt")[1] == pytest.approx(want[4], abs=1e-13 * want[0])
+    # the spectra about z: planes kz = (q-1) with values times Lz, cylinders
+    # kperp = q-1 over Dkk (1 here); energy columns carry 1/2
+    Ep, Hp = og.specpara(*v, kin=1, hel=1)
+    Ec, Hc = og.specperp(*v, kin=1, hel=1)
+    for name, cols in (("kspecpara", 0.5 * Ep), ("khelipara", Hp), ("kspecperp", 0.5 * Ec), ("kheliperp", Hc)):
+        t = table(name + ".0001.txt")
+        assert np.array_equal(t[:, 0], np.arange(cols.shape[1], dtype=float)), name
+        for q in range(3):
+            assert close(t[:, 1 + q], cols[q]), (name, q)
+    for name, col in (("ktranpara", og.entpara(*v, *w, kin=1)), ("mtranperp", og.entperp(*v, *w, kin=0)),
+                      ("hktranpara", og.heltpara(*v, *w))):
+        t = table(name + ".0001.txt")
+        assert close(t[:, 1], col), name
+    # spec2D: one unformatted record (4-byte markers) of E(kperp, kpara)
+    raw = (tmp_path / "kspec2D.0001.out").read_bytes()
+    E2, _ = og.spec2d(*v, kin=1, hel=0)
+    assert len(raw) == 8 + 8 * E2.size
+    assert close(np.frombuffer(raw[4:-4], dtype=np.float64), E2.ravel())
+    # scalar diagnostics of v_x with f_x as the source
+    sc = table("scalar.txt")
+    assert close(sc[1:], og.pscheck(v[0], w[0]))
+    assert close(table("sspectrum.0001.txt")[:, 1], og.spectrscc(v[0]))
+    assert close(table("s2transfer.0001.txt")[:, 1], og.sctrans(v[0], w[0]))
+    pz = table("profilez.0001.txt")
+    assert np.allclose(pz[:, 0], 2 * np.pi * np.arange(n) / n, rtol=1e-14)
+    want_pz = og.difuc(v[0], v[1], 3)
+    for q in range(3):
+        assert close(pz[:, 1 + q], want_pz[q])
+    # Boussinesq diagnostics of (v, f_x as the temperature), fo = 1.3, bv = 0.8
+    avg, mx, mn = og.tbouss(*v, w[0], 1.3, 0.8)
+    for name, col in (("tboussavg", avg), ("tboussmax", mx), ("tboussmin", mn)):
+        assert close(table(name + ".txt")[1:], col), name
+    assert close(table("pvprof.0001.txt")[:, 1], og.havgcomp(*v, w[0], 1.3, 0.8, 7))
+    assert close(table("pvspectrum.0001.txt")[:, 1], og.spectpv(*v, w[0]))
     # GHOST's binary output of that v: the resident path (vx, vy, vz) and the
     # host-array path through io_write (hx) write the same bytes; the file is
     # c2r(v/N^3) in Fortran order
