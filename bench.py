@@ -19,7 +19,10 @@ Lz = pi, nu = eta = 3e-4, dt = 2e-4, velocity and vector potential from the
 k^4 exp IC at unit energy ratio, mechanical forcing in 2 <= |k| <= 3).
 --workload les measures the learned-closure LES: the 512^3 DNS above (without
 tracers) trains a 10-64-64-6 closure on its own filtered states (kc = 42,
-128^3), and a 2000-step 128^3 LES is judged against the filtered DNS spectrum.
+128^3) with a per-shell spectral part fitted from the network's residual
+subgrid transfer and calibrated on the last 1000 spin-up steps, and a
+1000-step 128^3 LES (dt = 4 DNS steps) is judged against the filtered DNS
+spectrum over the following 4000 DNS steps.
 --workload ensemble measures the batched ensemble: 96 unforced HD runs, 24
 each at 32^3, 48^3, 64^3 and 96^3, k^4 exp(-2 (k/k0)^2) ICs with k0 drawn
 uniformly in [2, 4], nu set so that k_max eta = 1.5 at the start, RK2.
@@ -312,18 +315,25 @@ def run_ensemble(args, torch):
 
 # learned-closure LES: the 512^3 DNS of the HD workload (no tracers) filtered
 # at kc = 42 onto a 128^3 LES that runs 2000 steps of twice the DNS step
-LES_N, LES_KC, LES_SPIN, LES_WINDOW = 128, 42, 4000, 4000
+LES_N, LES_KC, LES_SPIN, LES_WINDOW, LES_RATIO = 128, 42, 4000, 4000, 4
 
 
 def run_les(args, torch):
     from ghost_amd.les import Experiment
     N = args.n
     ex = Experiment(nfine=N, nles=LES_N, kc=LES_KC, nu=NU, dt=DT, spin=LES_SPIN,
-                    snaps=[LES_SPIN - 1000, LES_SPIN - 500, LES_SPIN], window=LES_WINDOW, ratio=2, samples=8,
-                    f0=F0, fstep=FSTEP, k0=K0, u0=U0, ic_seed=IC_SEED, f_seed=F_SEED, log=log)
+                    snaps=[LES_SPIN - 1000, LES_SPIN - 500, LES_SPIN], window=LES_WINDOW, ratio=LES_RATIO,
+                    samples=8, f0=F0, fstep=FSTEP, k0=K0, u0=U0, ic_seed=IC_SEED, f_seed=F_SEED,
+                    cal_steps=list(range(LES_SPIN - 1000, LES_SPIN + 1, 100)), log=log)
     ex.run_dns()
     ex.fit(steps=3000, batch=65536, lr=1e-3)
-    with_c = ex.run_les(closure=True)
+    # the spectral part: the network's residual shell transfer on the training
+    # states as a per-shell viscosity (signed, down to -nu), then calibrated
+    # a posteriori on the last 1000 spin-up steps (before the window)
+    nu_fit = ex.train.spectral_viscosity(ex.mlp, floor=-NU)
+    nuk = ex.calibrate(nu_fit, closure=True, iters=5)
+    with_c = ex.run_les(closure=True, nuk=nuk, label="closure (network + spectral part)")
+    net_only = ex.run_les(closure=True, label="network alone")
     without = ex.run_les(closure=False)
     # the closure's inference kernel alone on the LES grid, for its roofline
     import math as _m
@@ -358,12 +368,16 @@ def run_les(args, torch):
         "config": {"workload": f"HD3D {N}^3 FP64 DNS (nu={NU}, dt={DT}, forcing f0={F0} in 2<=|k|<=3, "
                                f"phase shift every {FSTEP} steps) spun up {LES_SPIN} steps; sharp filter kc={LES_KC} "
                                f"and truncation onto {LES_N}^3; 10-64-64-6 GELU closure trained on the filtered states "
-                               f"at steps {LES_SPIN - 1000}, {LES_SPIN - 500}, {LES_SPIN}; LES {with_c['steps']} steps "
-                               f"of dt={2 * DT} against the DNS's next {LES_WINDOW} steps",
+                               f"at steps {LES_SPIN - 1000}, {LES_SPIN - 500}, {LES_SPIN}, with a per-shell spectral "
+                               f"part fitted from its residual subgrid transfer and calibrated on spin-up steps "
+                               f"{LES_SPIN - 1000}-{LES_SPIN}; LES {with_c['steps']} steps of dt={LES_RATIO * DT} "
+                               f"against the DNS's next {LES_WINDOW} steps",
                    "grid_dns": N, "grid_les": LES_N, "kc": LES_KC},
         "within_5pct": bool(within),
         "fallback": None if within else "spectrum error above 5%: the plain DNS is the result (speedup 1)",
-        "les_closure": with_c, "les_no_closure": without,
+        "les_closure": with_c, "les_network_alone": net_only, "les_no_closure": without,
+        "spectral_part": {"nu_fit_k10_20_30_40": [float(nu_fit[k - 1]) for k in (10, 20, 30, 40)],
+                          "nu_calibrated_k10_20_30_40": [float(nuk[k - 1]) for k in (10, 20, 30, 40)]},
         "dns_window_seconds": round(ex.t_dns, 3), "dns_spinup_seconds": round(ex.t_spin, 3),
         "training": {"points": ex.n_train, "steps": len(ex.losses), "seconds": round(ex.t_train, 3),
                      "loss_first": float(ex.losses[0]), "loss_last": float(ex.losses[-1]),

@@ -164,6 +164,7 @@ def _declare(L):
         "ghx_velocity_gradients": (_I, [_P, _PP, _PP]),
         "ghx_sgs_targets": (_I, [_P, _P, _PP, _D, _PP, _PP]),
         "ghx_hd_set_sgs": (_I, [_P, _P, _D]),
+        "ghx_hd_set_sgs_spectral": (_I, [_P, _I, _DP]),
         "ghx_entrans": (_I, [_P] * 7 + [_I, _DP]),
         "ghx_heltrans": (_I, [_P] * 7 + [_DP]),
         "ghx_crosspecc": (_I, [_P] * 7 + [_D, _DP]),

@@ -1554,6 +1554,8 @@ struct ghx_hd {
   double sgs_delta = 0.0;
   void *sgs_mem = nullptr;  // grad 9 + tau 6 real, tau 6 + force 3 spectral
   uint8_t *sgs_fline = nullptr;
+  double *sgs_nuk = nullptr;  // spectral viscosity per shell of the closure (device), sgs_nb shells
+  int sgs_nb = 0;
   double omegaz = 0.0;  // rotation rate about z (include/roth_rkstep2.f90)
 };
 
@@ -1595,6 +1597,7 @@ extern "C" int ghx_hd_destroy(ghx_hd *s) {
   (void)hipFree(s->fline);
   if (s->sgs_mem) (void)hipFree(s->sgs_mem);
   if (s->sgs_fline) (void)hipFree(s->sgs_fline);
+  if (s->sgs_nuk) (void)hipFree(s->sgs_nuk);
   delete s;
   return GHX_OK;
 }
@@ -1720,6 +1723,14 @@ static int sgs_force(ghx_hd *s, const cplx *const *vin, const cplx **fs) {
   double *gr = static_cast<double *>(s->sgs_mem);
   double *tr = gr + 9 * nr;
   cplx *tc = reinterpret_cast<cplx *>(tr + 6 * nr), *fc = tc + 6 * nc;
+  const cplx *f0[3] = {s->f[0], s->f[1], s->f[2]};
+  cplx *out[3] = {fc, fc + nc, fc + 2 * nc};
+  if (!s->sgs) {  // the spectral viscosity alone
+    HIPCHK(ghx::op_sgs_force(g->gk, g->stream, nullptr, s->forced ? f0 : nullptr, out, vin, s->sgs_nuk, s->sgs_nb,
+                             g->Dkk));
+    for (int k = 0; k < 3; ++k) fs[k] = out[k];
+    return GHX_OK;
+  }
   double *grad9[9], *tau6[6];
   for (int k = 0; k < 9; ++k) grad9[k] = gr + k * nr;
   for (int k = 0; k < 6; ++k) tau6[k] = tr + k * nr;
@@ -1737,12 +1748,13 @@ static int sgs_force(ghx_hd *s, const cplx *const *vin, const cplx **fs) {
   if ((rc = do_r2c(&g->plan, 6, ct, tcv))) return rc;
   const cplx *tau[6];
   for (int k = 0; k < 6; ++k) tau[k] = tc + k * nc;
-  const cplx *f0[3] = {s->f[0], s->f[1], s->f[2]};
-  cplx *out[3] = {fc, fc + nc, fc + 2 * nc};
-  HIPCHK(ghx::op_sgs_force(g->gk, g->stream, tau, s->forced ? f0 : nullptr, out));
+  HIPCHK(ghx::op_sgs_force(g->gk, g->stream, tau, s->forced ? f0 : nullptr, out, s->sgs_nuk ? vin : nullptr,
+                           s->sgs_nuk, s->sgs_nb, g->Dkk));
   for (int k = 0; k < 3; ++k) fs[k] = out[k];
   return GHX_OK;
 }
+
+static int sgs_buffers(ghx_hd *s);
 
 // Attach the closure (m = NULL detaches): before every substep the forcing
 // becomes f + P(-div tau) with tau = m(grad u, delta).
@@ -1753,6 +1765,31 @@ extern "C" int ghx_hd_set_sgs(ghx_hd *s, ghx_mlp *m, double delta) {
     return GHX_OK;
   }
   ARGCHK(delta > 0, "filter width must be positive");
+  int rc = sgs_buffers(s);
+  if (rc) return rc;
+  s->sgs = m;
+  s->sgs_delta = delta;
+  return GHX_OK;
+}
+
+// Spectral part of the closure: an extra viscosity nuk[q] on the shells
+// q + 1 = int(sqrt(kk2)/Dkk + .501) = 1..nb, -nuk kk2 v added to the force of
+// every substep (with or without a network attached); nb = 0 removes it.
+extern "C" int ghx_hd_set_sgs_spectral(ghx_hd *s, int nb, const double *nuk) {
+  ARGCHK(s && nb >= 0 && (nb == 0 || nuk), "bad hd_set_sgs_spectral arguments");
+  if (s->sgs_nuk) HIPCHK(hipFree(s->sgs_nuk));
+  s->sgs_nuk = nullptr;
+  s->sgs_nb = 0;
+  if (nb == 0) return GHX_OK;
+  int rc = sgs_buffers(s);
+  if (rc) return rc;
+  HIPCHK(hipMalloc(&s->sgs_nuk, nb * sizeof(double)));
+  HIPCHK(hipMemcpy(s->sgs_nuk, nuk, nb * sizeof(double), hipMemcpyHostToDevice));
+  s->sgs_nb = nb;
+  return GHX_OK;
+}
+
+static int sgs_buffers(ghx_hd *s) {
   ghx_grid *g = s->g;
   if (!s->sgs_mem) {
     HIPCHK(hipMalloc(&s->sgs_mem, 15 * g->nr * sizeof(double) + 9 * g->nc * sizeof(cplx)));
@@ -1760,8 +1797,6 @@ extern "C" int ghx_hd_set_sgs(ghx_hd *s, ghx_mlp *m, double delta) {
     HIPCHK(hipMalloc(&s->sgs_fline, nl));
     HIPCHK(hipMemset(s->sgs_fline, 1, nl));  // the subgrid force reaches every line of the dealias set
   }
-  s->sgs = m;
-  s->sgs_delta = delta;
   return GHX_OK;
 }
 
@@ -1788,7 +1823,7 @@ static int hd_step_once(ghx_hd *s) {
     const cplx *const *fo = f;
     const uint8_t *fl = s->fline;
     const cplx *fs[3];
-    if (s->sgs) {  // the forcing of this substep is f + P(-div tau(vin))
+    if (s->sgs || s->sgs_nuk) {  // the forcing of this substep is f + P(-div tau(vin)) - nu(k) kk2 vin
       int rc = sgs_force(s, vin, fs);
       if (rc) return rc;
       fo = fs;

@@ -19,11 +19,13 @@ hipError_t op_normalize_apply(const GridK &g, hipStream_t st, cplx *x, cplx *y, 
 hipError_t op_sharp(const GridK &g, hipStream_t st, const cplx *a, cplx *b, double kc2);
 // trunc3D's spectral truncation of c (grid g) onto the smaller grid t
 // learned LES: subgrid stress products on the fine grid and the projected
-// subgrid force (f0 may be null)
+// subgrid force (f0 and tau may be null; v and nuk, when given, add the
+// spectral viscosity -nuk[shell - 1] kk2 v for shells 1..nb of width dkk)
 hipError_t op_sgs_products(const GridK &g, hipStream_t st, const double *const *u, const double *const *ub, double s2,
                            double *const *out);
 hipError_t op_sgs_force(const GridK &g, hipStream_t st, const cplx *const *tau, const cplx *const *f0,
-                        cplx *const *out);
+                        cplx *const *out, const cplx *const *v = nullptr, const double *nuk = nullptr, int nb = 0,
+                        double dkk = 1.0);
 hipError_t op_truncate(const GridK &g, const GridK &t, hipStream_t st, const cplx *c, cplx *out);
 hipError_t op_zero(hipStream_t st, void *p, size_t bytes);
 // real-space cross product r7=(r2 r6-r5 r3)tmp ... in place (prodre3's loop)

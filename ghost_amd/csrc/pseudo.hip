@@ -187,15 +187,23 @@ __global__ void k_sgs_products(size_t n, CR3 u, CR3 ub, double s2, R6 out) {
   }
 }
 
-// Subgrid force of the LES: out = f0 + P(-div tau) on the kept modes
-// (tiny <= kn2 <= kmax), f0 elsewhere (zero without f0).  tau: six spectral
-// fields xx, yy, zz, xy, xz, yz; P removes the component along k, as the
-// projections of nonlhd3 do (the pressure of the subgrid force).
+// Subgrid force of the LES: out = f0 + P(-div tau) - nu(k) kk2 v on the kept
+// modes (tiny <= kn2 <= kmax), f0 elsewhere (zero without f0).  tau: six
+// spectral fields xx, yy, zz, xy, xz, yz (null: no stress term); P removes
+// the component along k, as the projections of nonlhd3 do (the pressure of
+// the subgrid force).  nuk (may be null): a viscosity per shell
+// int(sqrt(kk2)/Dkk + .501) = 1..nb acting on the velocity v.
 struct C6 {
   const cplx *p[6];
 };
-__global__ void k_sgs_force(GridK g, C6 t, const cplx *f0x, const cplx *f0y, const cplx *f0z, cplx *ox, cplx *oy,
-                            cplx *oz) {
+struct SpecVisc {
+  const cplx *v[3];
+  const double *nuk;
+  int nb;
+  double dkk;
+};
+__global__ void k_sgs_force(GridK g, C6 t, SpecVisc sv, const cplx *f0x, const cplx *f0y, const cplx *f0z, cplx *ox,
+                            cplx *oy, cplx *oz) {
   GRID_STRIDE(m, ncplx(g)) {
     Idx q = decompose(m, g);
     const cplx z = c_make(0.0, 0.0);
@@ -204,15 +212,27 @@ __global__ void k_sgs_force(GridK g, C6 t, const cplx *f0x, const cplx *f0y, con
     if (kn2 <= g.kmax && kn2 >= g.tiny) {
       const double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky), kz = wavenum(q.k, g.nz, g.dkz);
       const double kk2 = g.anis ? kk2_of(g, kx, ky, kz) : kn2;
-      const cplx txx = t.p[0][m], tyy = t.p[1][m], tzz = t.p[2][m], txy = t.p[3][m], txz = t.p[4][m], tyz = t.p[5][m];
-      // -i k_j tau_ij
-      const cplx sx = c_make(-(-kx * txx.y - ky * txy.y - kz * txz.y), -(kx * txx.x + ky * txy.x + kz * txz.x));
-      const cplx sy = c_make(-(-kx * txy.y - ky * tyy.y - kz * tyz.y), -(kx * txy.x + ky * tyy.x + kz * tyz.x));
-      const cplx sz = c_make(-(-kx * txz.y - ky * tyz.y - kz * tzz.y), -(kx * txz.x + ky * tyz.x + kz * tzz.x));
-      const cplx dot = c_add(c_add(c_scale(sx, kx), c_scale(sy, ky)), c_scale(sz, kz));
-      fx = c_add(fx, c_sub(sx, c_scale(dot, kx / kk2)));
-      fy = c_add(fy, c_sub(sy, c_scale(dot, ky / kk2)));
-      fz = c_add(fz, c_sub(sz, c_scale(dot, kz / kk2)));
+      if (t.p[0]) {
+        const cplx txx = t.p[0][m], tyy = t.p[1][m], tzz = t.p[2][m], txy = t.p[3][m], txz = t.p[4][m],
+                   tyz = t.p[5][m];
+        // -i k_j tau_ij
+        const cplx sx = c_make(-(-kx * txx.y - ky * txy.y - kz * txz.y), -(kx * txx.x + ky * txy.x + kz * txz.x));
+        const cplx sy = c_make(-(-kx * txy.y - ky * tyy.y - kz * tyz.y), -(kx * txy.x + ky * tyy.x + kz * tyz.x));
+        const cplx sz = c_make(-(-kx * txz.y - ky * tyz.y - kz * tzz.y), -(kx * txz.x + ky * tyz.x + kz * tzz.x));
+        const cplx dot = c_add(c_add(c_scale(sx, kx), c_scale(sy, ky)), c_scale(sz, kz));
+        fx = c_add(fx, c_sub(sx, c_scale(dot, kx / kk2)));
+        fy = c_add(fy, c_sub(sy, c_scale(dot, ky / kk2)));
+        fz = c_add(fz, c_sub(sz, c_scale(dot, kz / kk2)));
+      }
+      if (sv.nuk) {
+        const int kmn = (int)(sqrt(kk2) / sv.dkk + 0.501);
+        if (kmn >= 1 && kmn <= sv.nb) {
+          const double a = -sv.nuk[kmn - 1] * kk2;
+          fx = c_add(fx, c_scale(sv.v[0][m], a));
+          fy = c_add(fy, c_scale(sv.v[1][m], a));
+          fz = c_add(fz, c_scale(sv.v[2][m], a));
+        }
+      }
     }
     ox[m] = fx;
     oy[m] = fy;
@@ -625,11 +645,16 @@ hipError_t op_sgs_products(const GridK &g, hipStream_t st, const double *const *
   LAUNCH(k_sgs_products, nreal(g), st, nreal(g), a, b, s2, o);
 }
 hipError_t op_sgs_force(const GridK &g, hipStream_t st, const cplx *const *tau, const cplx *const *f0,
-                        cplx *const *out) {
+                        cplx *const *out, const cplx *const *v, const double *nuk, int nb, double dkk) {
   C6 t;
-  for (int c = 0; c < 6; ++c) t.p[c] = tau[c];
-  LAUNCH(k_sgs_force, ncplx(g), st, g, t, f0 ? f0[0] : nullptr, f0 ? f0[1] : nullptr, f0 ? f0[2] : nullptr, out[0],
-         out[1], out[2]);
+  for (int c = 0; c < 6; ++c) t.p[c] = tau ? tau[c] : nullptr;
+  SpecVisc sv;
+  for (int c = 0; c < 3; ++c) sv.v[c] = v ? v[c] : nullptr;
+  sv.nuk = v ? nuk : nullptr;
+  sv.nb = nb;
+  sv.dkk = dkk;
+  LAUNCH(k_sgs_force, ncplx(g), st, g, t, sv, f0 ? f0[0] : nullptr, f0 ? f0[1] : nullptr, f0 ? f0[2] : nullptr,
+         out[0], out[1], out[2]);
 }
 hipError_t op_sharp(const GridK &g, hipStream_t st, const cplx *a, cplx *b, double kc2) {
   LAUNCH(k_sharp, ncplx(g), st, g, a, b, kc2);

@@ -94,12 +94,60 @@ class TrainingSet:
     def __init__(self, les: Grid, kc):
         self.les, self.kc, self.delta = les, kc, math.pi / kc
         self.snaps = []  # (grad9, tau6) per state
+        self.vels = []  # the filtered states themselves (LES grid, spectral)
 
     def add(self, fine: Grid, v):
         tau, vs = sgs_targets(fine, self.les, v, self.kc)
         grad = velocity_gradients(self.les, vs)
         self.snaps.append((grad, tau))
+        self.vels.append(vs)
         return vs
+
+    def force(self, tau):
+        """P(-div tau) on the LES grid, in the solver's spectral units, from
+        the six physical stress components (xx, yy, zz, xy, xz, yz)."""
+        g = self.les
+        th = [g.r2c(t) for t in tau]
+        rows = ((0, 3, 4), (3, 1, 5), (4, 5, 2))  # tau_xj, tau_yj, tau_zj
+        d = []
+        for r in rows:
+            acc = g.czeros()
+            for dirn, c in enumerate(r, start=1):
+                acc += g.derivk3(th[c], g.czeros(), dirn)
+            d.append(acc)
+        return [g.nonlhd3(*d, g.czeros(), dirn) for dirn in (1, 2, 3)]  # -P(div tau)
+
+    def transfers(self, mlp=None):
+        """Shell sums over the training states of the energy the exact subgrid
+        force and the network's force put into each shell (entrans of the
+        filtered velocity with each force), and of kk2 |v|^2."""
+        g = self.les
+        te = tm = dk = 0.0
+        for (grad, tau), vs in zip(self.snaps, self.vels):
+            te = te + np.array(g.entrans(*vs, *self.force(tau), kin=1))
+            if mlp is not None:
+                p = [torch.empty_like(x) for x in tau]
+                mlp.stress(grad, self.delta, p)
+                tm = tm + np.array(g.entrans(*vs, *self.force(p), kin=1))
+            dk = dk + np.array(g.entrans(*vs, *vs, kin=0))
+        return te, tm, dk
+
+    def spectral_viscosity(self, mlp=None, floor=0.0):
+        """Spectral part of the closure, fitted on the training states: the
+        viscosity per shell nu(k) = (T_net(k) - T_exact(k)) / sum kk2 |v|^2
+        that makes the network's shell transfer (zero without one) equal the
+        exact subgrid transfer; clipped below at floor, and carried from the
+        last shell with data to the shells past kc."""
+        te, tm, dk = self.transfers(mlp)
+        diff = (tm - te) if mlp is not None else -te
+        nu = np.zeros_like(dk)
+        has = dk > 0
+        nu[has] = diff[has] / dk[has]
+        nu = np.maximum(nu, floor)
+        last = int(np.nonzero(has)[0][-1]) if np.any(has) else -1
+        if 0 <= last < len(nu) - 1:
+            nu[last + 1:] = nu[last]
+        return nu
 
     def arrays(self, mlp: MLP):
         """Fit the network's normalisation on all states (the pooled moments of
@@ -158,10 +206,10 @@ class Experiment:
     spectra (and of the spectra at the end of the window)."""
 
     def __init__(self, nfine, nles, kc, nu, dt, spin, snaps, window, ratio=2, f0=0.4, fstep=20, k0=3.0, u0=1.0,
-                 ic_seed=1000, f_seed=2000, samples=8, log=print):
+                 ic_seed=1000, f_seed=2000, samples=8, cal_steps=(), log=print):
         self.__dict__.update(dict(nfine=nfine, nles=nles, kc=kc, nu=nu, dt=dt, spin=spin, snaps=set(snaps),
                                   window=window, ratio=ratio, f0=f0, fstep=fstep, k0=k0, u0=u0, ic_seed=ic_seed,
-                                  f_seed=f_seed, samples=samples, log=log))
+                                  f_seed=f_seed, samples=samples, cal_steps=sorted(cal_steps), log=log))
         assert window % (ratio * samples) == 0, "window must hold whole LES steps per sample"
 
     def run_dns(self):
@@ -189,12 +237,24 @@ class Experiment:
             st["timef"] += 1
 
         self.train = TrainingSet(self.small, self.kc)
+        self.cal_ref = []
+        cal = set(self.cal_steps)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for it in range(1, self.spin + 1):
             step()
             if it in self.snaps:
                 self.train.add(self.fine, s.velocity())
+            if it in cal:
+                # the spin-up's filtered spectra, and the state and forcing at the
+                # first of these steps, for the a-posteriori calibration
+                _, vf = sgs_targets(self.fine, self.small, s.velocity(), self.kc)
+                if it == self.cal_steps[0]:
+                    self.cal_v = vf
+                    self.cal_f = filter_forcing(self.fine, self.small, s.forcing(), self.kc)
+                    self.cal_phase, self.cal_timef = len(self.phases), st["timef"]
+                else:
+                    self.cal_ref.append(spectrum(self.small, vf))
         torch.cuda.synchronize()
         self.t_spin = time.perf_counter() - t0
         _, self.v_start = sgs_targets(self.fine, self.small, s.velocity(), self.kc)
@@ -236,22 +296,42 @@ class Experiment:
         self.log(f"closure trained on {self.n_train} points, {steps} Adam steps in {self.t_train:.2f} s: "
                  f"loss {self.losses[0]:.4f} -> {self.losses[-1]:.4f}; dissipation-matching scale {self.scale:.3f}")
 
-    def run_les(self, closure=True):
+    def fit_spectral(self, with_network=True):
+        """The spectral part of the closure on the same training states (the
+        network's residual transfer, or the whole subgrid transfer without a
+        network)."""
+        nu = self.train.spectral_viscosity(self.mlp if with_network else None)
+        if with_network:
+            self.nuk = nu
+        else:
+            self.nuk_alone = nu
+        self.log(f"spectral part ({'residual of the network' if with_network else 'alone'}): nu(k) at k = 10, 20, "
+                 f"30, 40: {nu[9]:.2e}, {nu[19]:.2e}, {nu[29]:.2e}, {nu[39]:.2e}")
+        return nu
+
+    def _les(self, v0, f0, phase0, timef0, dns_steps, every_dns, closure, nuk, ratio):
+        """LES from state v0 (forcing f0, the DNS's phase schedule from index
+        phase0 with timef0) over dns_steps DNS steps: spectra of the sharp-
+        filtered velocity every every_dns DNS steps and the GPU seconds of the
+        steps themselves."""
         from .solvers import HDSolver
         import time
-        L = HDSolver(self.small, dt=self.dt * self.ratio, nu=self.nu, ord=2)
-        L.set_velocity(self.v_start)
-        L.set_forcing(self.f_start)
+        assert dns_steps % ratio == 0 and every_dns % ratio == 0, "samples must fall on whole LES steps"
+        L = HDSolver(self.small, dt=self.dt * ratio, nu=self.nu, ord=2)
+        L.set_velocity(v0)
+        L.set_forcing(f0)
         if closure:
             L.set_sgs(self.mlp, math.pi / self.kc)
-        st = {"timef": self.timef0, "pi": self.phase0}
-        every = self.window // self.samples // self.ratio
+        if nuk is not None:
+            L.set_sgs_spectral(nuk)
+        st = {"timef": timef0, "pi": phase0}
+        every = every_dns // ratio
         spectra = []
         t_sample = 0.0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for it in range(1, self.window // self.ratio + 1):
-            for _ in range(self.ratio):  # the DNS's phase shifts that fall in this step
+        for it in range(1, dns_steps // ratio + 1):
+            for _ in range(ratio):  # the DNS's phase shifts that fall in this step
                 if st["timef"] == self.fstep:
                     st["timef"] = 0
                     ph = self.phases[st["pi"]]
@@ -268,16 +348,55 @@ class Experiment:
                 spectra.append(spectrum(self.small, vel))
                 t_sample += time.perf_counter() - ta
         torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0 - t_sample
+        return spectra, time.perf_counter() - t0 - t_sample
+
+    def calibrate(self, nuk, closure=True, iters=4, gamma=1.0, ratio=None):
+        """A-posteriori fit of the spectral part on the spin-up, before the
+        window: LES from the filtered state at the first calibration step over
+        the rest of the spin-up, against the filtered DNS spectra sampled
+        there; each pass scales the effective viscosity nu(k) + nu_mol by
+        (E_les/E_dns)^gamma on the shells 1..kc (shells past kc take the value
+        at kc)."""
+        ratio = ratio or self.ratio
+        c0 = self.cal_steps[0]
+        span = self.cal_steps[-1] - c0
+        every = self.cal_steps[1] - c0
+        ref = np.mean(self.cal_ref, axis=0)
+        nu = np.array(nuk, dtype=np.float64)
+        kc = int(self.kc)
+        for it in range(iters):
+            spectra, _ = self._les(self.cal_v, self.cal_f, self.cal_phase, self.cal_timef, span, every, closure, nu,
+                                   ratio)
+            if not all(np.all(np.isfinite(e)) for e in spectra):
+                self.log(f"calibration pass {it + 1}: the LES blew up; stopping with the previous nu")
+                break
+            e = np.mean(spectra, axis=0)
+            err = spectrum_error(e, ref, kc)
+            r = np.ones_like(nu)
+            r[:kc] = e[:kc] / ref[:kc]
+            r[kc:] = r[kc - 1]
+            # the effective viscosity nu + nu_mol (positive) is what is scaled
+            nu = (nu + self.nu) * np.power(r, gamma) - self.nu
+            self.log(f"calibration pass {it + 1}: spectrum error on the spin-up {err:.4f}; E_les/E_dns at k = 30, 40, "
+                     f"{kc}: {r[29]:.3f}, {r[39]:.3f}, {r[kc - 1]:.3f}")
+        return nu
+
+    def run_les(self, closure=True, nuk=None, ratio=None, label=None):
+        """closure: attach the network; nuk: the spectral part (or None);
+        ratio: LES steps per DNS step (default the experiment's)."""
+        ratio = ratio or self.ratio
+        assert self.window % (ratio * self.samples) == 0, "window must hold whole LES steps per sample"
+        spectra, elapsed = self._les(self.v_start, self.f_start, self.phase0, self.timef0, self.window,
+                                     self.window // self.samples, closure, nuk, ratio)
         ok = all(np.all(np.isfinite(e)) for e in spectra)
         mean_err = spectrum_error(np.mean(spectra, axis=0), np.mean(self.ref, axis=0), self.kc) if ok else math.inf
         end_err = spectrum_error(spectra[-1], self.ref[-1], self.kc) if ok else math.inf
-        name = "closure" if closure else "no closure"
-        self.log(f"LES {self.nles}^3 {name}: {self.window // self.ratio} steps {elapsed:.2f} s, spectrum error "
+        name = label or ("closure" if closure else "no closure")
+        self.log(f"LES {self.nles}^3 {name}: {self.window // ratio} steps {elapsed:.2f} s, spectrum error "
                  f"(time average) {mean_err:.4f}, at the end {end_err:.4f}")
         if ok:
             ea, er = np.mean(spectra, axis=0), np.mean(self.ref, axis=0)
             shells = [k for k in (1, 2, 3, 5, 8, 12, 16, 20, 25, 30, 35, 40, int(self.kc)) if k <= self.kc]
             self.log("  E_les/E_dns at k = " + ", ".join(f"{k}: {ea[k - 1] / er[k - 1]:.3f}" for k in shells))
-        return {"seconds": elapsed, "steps": self.window // self.ratio, "spectrum_error_mean": mean_err,
+        return {"seconds": elapsed, "steps": self.window // ratio, "spectrum_error_mean": mean_err,
                 "spectrum_error_end": end_err, "finite": bool(ok)}

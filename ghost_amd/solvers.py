@@ -89,6 +89,14 @@ class HDSolver(_Resident):
         _lib.check(_lib.load().ghx_hd_set_sgs(self._h, mlp._h if mlp is not None else None, float(delta)))
         self._sgs = mlp  # keep the network alive while attached
 
+    def set_sgs_spectral(self, nuk):
+        """Spectral part of the closure: -nuk[q] kk2 v on shell q + 1 added to
+        the force of every substep (None or empty removes it)."""
+        import numpy as np
+        a = np.ascontiguousarray(nuk if nuk is not None else [], dtype=np.float64)
+        p = a.ctypes.data_as(C.POINTER(C.c_double)) if a.size else None
+        _lib.check(_lib.load().ghx_hd_set_sgs_spectral(self._h, int(a.size), p))
+
 
 class MHDSolver(_Resident):
     """MHD3D state: velocity v, vector potential a, forcings f (mechanical) and m (electromotive)."""

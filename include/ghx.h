@@ -445,6 +445,10 @@ int ghx_sgs_targets(ghx_grid *g, ghx_grid *t, const void *const *v, double kc, d
  * the forcing becomes f + P(-div tau), tau = m(grad u, delta), P the
  * projection of nonlhd3 (pseudospec3D_hd.f90:399-509). */
 int ghx_hd_set_sgs(ghx_hd *s, ghx_mlp *m, double delta);
+/* Spectral part of the closure (no counterpart in GHOST): -nuk[q] kk2 v on
+ * the shells q + 1 = int(sqrt(kk2)/Dkk + .501) = 1..nb is added to the force
+ * of every substep, with or without a network; nb = 0 removes it. */
+int ghx_hd_set_sgs_spectral(ghx_hd *s, int nb, const double *nuk);
 
 /* ---------------------------------------------------------------------- */
 /* Pass profiler (replaces the GTStart/GTStop handles around the fftp      */

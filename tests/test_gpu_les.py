@@ -106,6 +106,53 @@ def test_closure_forcing_in_the_step_matches_oracle(gpu):
     assert rel(got[0], plain[0]) > 1e-8
 
 
+def test_spectral_part_in_the_step_matches_oracle(gpu):
+    """The closure's spectral part: -nu(shell) kk2 v_in joins the forcing of
+    each substep (ord 2, so v_in differs between the substeps); with a
+    network attached both terms add.  The oracle takes the substeps one by
+    one with that forcing computed from each substep's input."""
+    from ghost_amd.pseudospec import Grid
+    from ghost_amd.solvers import HDSolver
+    from oracle import oracle as O
+    n, dt, nu = 32, 2e-3, 5e-3
+    g, og = Grid(n), O.Grid(n)
+    v, _ = og.initialv_k4exp(3.0, 1.0, 81)
+    f, _ = og.initialfv_random(2.0, 3.0, 0.5, 82)
+    nb = n // 2 + 1
+    nuk = np.linspace(-2e-3, 8e-3, nb)  # signed, as the fitted residual can be
+    s = HDSolver(g, dt=dt, nu=nu, ord=2)
+    s.set_velocity([to_gpu(x) for x in v])
+    s.set_forcing([to_gpu(x) for x in f])
+    s.set_sgs_spectral(nuk)
+    s.step(2)
+    got = [to_np(x) for x in s.velocity()]
+    K = np.meshgrid(np.r_[0:n // 2 + 1] * 1.0, np.fft.fftfreq(n, 1.0 / n), np.fft.fftfreq(n, 1.0 / n), indexing="ij")
+    K[0][n // 2] = -n // 2
+    kk2 = K[0] ** 2 + K[1] ** 2 + K[2] ** 2
+    shell = (np.sqrt(kk2) + 0.501).astype(int)
+    kept = (kk2 <= n * n / 9.0) & (kk2 > 0)
+    coef = np.where(kept & (shell >= 1) & (shell <= nb), -nuk[np.clip(shell - 1, 0, nb - 1)] * kk2, 0.0)
+    ref = [x.copy() for x in v]
+    for _ in range(2):
+        c = [x.copy() for x in ref]
+        for o in (2, 1):
+            fo = [f[i] + coef * ref[i] for i in range(3)]
+            og.hd_substep(ref, c, o, dt, nu, f=fo)
+    for k in range(3):
+        assert rel(got[k], ref[k]) < 1e-12
+    # removing it restores the plain step
+    s2 = HDSolver(g, dt=dt, nu=nu, ord=2)
+    s2.set_velocity([to_gpu(x) for x in v])
+    s2.set_forcing([to_gpu(x) for x in f])
+    s2.set_sgs_spectral(nuk)
+    s2.set_sgs_spectral(None)
+    s2.step(1)
+    plain = [x.copy() for x in v]
+    og.hd_step(plain, dt, nu, f=f, ord=2)
+    for k in range(3):
+        assert rel(to_np(s2.velocity()[k]), plain[k]) < 1e-12
+
+
 def test_trained_closure_beats_no_closure(gpu):
     """128^3 DNS (nu = 5e-3, resolved) filtered at kc = 10 onto a 32^3 LES:
     the closure trained on three filtered DNS states brings the time-averaged
