@@ -56,7 +56,7 @@ def edge_positions(shape):
 
 
 @pytest.mark.parametrize("zcompat", [True, False])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 48, 32), (64, 64, 64), (16, 12, 512)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 48, 32), (64, 64, 64), (32, 32, 512)])
 def test_euler_to_lag(gpu, shape, zcompat):
     """GPart_EulerToLag of a random real field: spline control points + Interp3D."""
     from ghost_amd.lagpart import GPart
