@@ -78,6 +78,16 @@ GHX_FFTCFGLOW(128, 4, 4, 2, 4, 4, 4)
 GHX_FFTCFGLOW(256, 4, 4, 4, 4, 4, 4)
 GHX_FFTCFGLOW(512, 4, 5, 2, 4, 4, 4, 4)
 #undef GHX_FFTCFGLOW
+
+// Wide schedules (16 points per lane, half the lanes per line) for passes
+// whose blocks should hold more lines with fewer lanes.
+template <int N> struct FFTCfgWide : FFTCfg<N> {};
+template <> struct FFTCfgWide<512> {
+  static constexpr int E = 16;
+  static constexpr int NS = 3;
+  static constexpr int T = 512 / 16;
+  GHX_DEV static constexpr int radix(int) { return 8; }
+};
 #undef GHX_FFTCFG
 
 // ---------------------------------------------------------------------------

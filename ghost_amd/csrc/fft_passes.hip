@@ -641,15 +641,15 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
 // an LDS tile with L-point (L*16-byte) chunks along kx.  Only kx < kxlim is
 // touched (kxlim < nx/2+1 when the fields are dealiased).
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, typename Cfg = FFTCfg<N>>
 __global__ __launch_bounds__(512) void k_yinv(FieldSet in_, FieldSet out_, int nf, GridK g, int nxp,
                                              const cplx *__restrict__ twt, int kxlim, int prune) {
-  using F = LineFFT<N, +1>;
+  using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  TwCache<N, Cfg> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
   const FieldView in = batch_view(in_, blockIdx.z);
   const FieldView out = batch_view(out_, blockIdx.z);
@@ -672,13 +672,20 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in_, FieldSet out_, int n
       v[m] = need ? x : c_make(0.0, 0.0);
     }
   };
-  cplx nxt[E];
-  load(0, nxt);
+  // E <= 8: the next field's line is loaded while this one is transformed;
+  // wider schedules have no registers for it and rely on a second block per CU
+  constexpr bool prefetch = E <= 8;
+  cplx nxt[prefetch ? E : 1];
+  if constexpr (prefetch) load(0, nxt);
   for (int f = 0; f < nf; ++f) {
     cplx v[E];
+    if constexpr (prefetch) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = nxt[m];
-    if (f + 1 < nf) load(f + 1, nxt);  // the next field's line in flight during this one
+      for (int m = 0; m < E; ++m) v[m] = nxt[m];
+      if (f + 1 < nf) load(f + 1, nxt);  // the next field's line in flight during this one
+    } else {
+      load(f, v);
+    }
     F::run(v, row, t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
@@ -1533,6 +1540,17 @@ hipError_t yinv(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
+    if constexpr (FFTCfgWide<n_>::E != FFTCfg<n_>::E) if (tile_want("GHX_YWIDE", 1) == 1) {  \
+      /* 16 points per lane: 8-line tiles from 256-lane blocks, two blocks per CU */       \
+      constexpr int TW = FFTCfgWide<n_>::T;                                                  \
+      const int L = 8;                                                                       \
+      size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                           \
+      dim3 grid((g.nxh + L - 1) / L, g.nz, c.batch);                                         \
+      allow_lds(k_yinv<n_, FFTCfgWide<n_>>, lds);                                            \
+      hipLaunchKernelGGL((k_yinv<n_, FFTCfgWide<n_>>), grid, dim3(L * TW), lds, c.stream, in, out, nf, g, nxp, \
+                         c.tw_y, kxlim, prune);                                              \
+      break;                                                                                 \
+    }                                                                                        \
     int L = pick_L(T, lds_line_stride(n_), g.nxh, tile_pick("GHX_LY", c.tile_y, 8));        \
     size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
     dim3 grid((g.nxh + L - 1) / L, g.nz, c.batch);                                          \

@@ -10,7 +10,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 8, 8), (6, 4, 10), (12, 12, 12), (16, 8, 32), (32, 32, 32), (48, 48, 48), (64, 64, 64),
-          (96, 96, 96), (128, 128, 128), (256, 64, 128), (20, 24, 6)]
+          (96, 96, 96), (128, 128, 128), (256, 64, 128), (20, 24, 6), (16, 512, 8), (24, 512, 16)]
 
 
 def rel(a, b):
@@ -181,7 +181,8 @@ def test_randu_matches_oracle(gpu):
         assert a == b and s1 == s2
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (24, 16, 12), (64, 64, 64), (16, 16, 512), (8, 12, 512)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (24, 16, 12), (64, 64, 64), (16, 16, 512), (8, 12, 512),
+                                   (16, 512, 16)])
 def test_hd_substep_fused(gpu, shape):
     g, og = grids(shape)
     v, _ = og.initialv_k4exp(2.5, 1.0, 2024)
