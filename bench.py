@@ -102,7 +102,7 @@ def kernel_name(slot, n):
     """The kernel a profiler slot times at size n (the 512-point y inverse
     runs the 16-points-per-lane schedule)."""
     if slot in ("ypass_inv", "ypass_inv_tracer") and n == 512:
-        return "ghx::k_yinv<512, ghx::FFTCfgWide<512>>"
+        return "ghx::k_yinv<512, FFTCfgWide<512> >"
     return KERNEL_NAME.get(slot, slot).replace("<N", f"<{n}")
 
 
