@@ -95,6 +95,9 @@ def _declare(L):
         "ghx_initialv_k4exp": (_I, [_P, _D, _D, C.POINTER(C.c_int32), _P, _P, _P]),
         "ghx_hd_rkstep2": (_I, [_P, _I, _D, _D, _PP, _PP, _PP, _PP]),
         "ghx_roth_rkstep2": (_I, [_P, _I, _D, _D, _D, _PP, _PP, _PP, _PP]),
+        "ghx_mhd_rkstep2": (_I, [_P, _I, _D, _D, _D, _PP, _PP, _PP, _PP, _PP]),
+        "ghx_bouss_rkstep2": (_I, [_P, _I, _D, _D, _D, _D, _D, _PP, _PP, _PP, _PP]),
+        "ghx_rotbouss_rkstep2": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _PP, _PP, _PP, _PP]),
         "ghx_hd_create": (_I, [_PP, _P, _I, _D, _D]),
         "ghx_hd_destroy": (_I, [_P]),
         "ghx_hd_fields": (_I, [_P, _PP, _PP]),

@@ -2347,6 +2347,90 @@ static int bouss_substep(ghx_bouss *s, int o, const cplx *const *vin, cplx *cons
   return GHX_OK;
 }
 
+// Forcing-line flags for the stateless substeps below: the lines (kx, ky)
+// where any of the nf fields is non-zero, OR-ed, in g->work (2 nl bytes).
+static int stateless_flags(ghx_grid *g, const cplx *const *fa, int nf, uint8_t **flags) {
+  const size_t nl = (size_t)g->gk.nxh * g->gk.ny;
+  int rc = grid_work(g, (2 * nl + 7) / 8 + 1);
+  if (rc) return rc;
+  uint8_t *fl = (uint8_t *)g->work, *tmp = fl + nl;
+  for (int b = 0; b < nf; b += 3) {
+    const cplx *x = fa[b], *y = fa[b + 1 < nf ? b + 1 : b], *z = fa[b + 2 < nf ? b + 2 : b];
+    if ((rc = line_flags(g, x, y, z, b == 0 ? fl : tmp))) return rc;
+    if (b > 0) {
+      hipLaunchKernelGGL(k_or_flags, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, g->stream, fl, tmp, nl);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  *flags = fl;
+  return GHX_OK;
+}
+
+// include/mhd_rkstep2.f90:4-65 for order index o on caller fields: vin, C
+// and vout hold (v_x, v_y, v_z, a_x, a_y, a_z); f (mechanical) and m
+// (electromotive) forcing, 3 fields each, both given or both NULL.  vout may
+// equal vin or C.
+extern "C" int ghx_mhd_rkstep2(ghx_grid *g, int o, double dt, double nu, double mu, const void *const *vin,
+                               const void *const *C, const void *const *f, const void *const *m, void *const *vout) {
+  ARGCHK(g && o >= 1 && vin && C && vout && (f == nullptr) == (m == nullptr), "bad mhd_rkstep2 arguments");
+  int rc = plan_scratch(&g->plan, 24);
+  if (rc) return rc;
+  ghx_mhd h;
+  h.g = g; h.ord = o; h.dt = dt; h.nu = nu; h.mu = mu;
+  h.q = g->plan.q;
+  const cplx *vi[6];
+  cplx *vo[6];
+  for (int k = 0; k < 6; ++k) { h.s[k] = M_(C[k]); vi[k] = C_(vin[k]); vo[k] = M_(vout[k]); }
+  if (f) {
+    for (int k = 0; k < 3; ++k) { h.f[k] = M_(f[k]); h.m[k] = M_(m[k]); }
+    const cplx *fa[6] = {h.f[0], h.f[1], h.f[2], h.m[0], h.m[1], h.m[2]};
+    if ((rc = stateless_flags(g, fa, 6, &h.fline))) return rc;
+    h.forced = true;
+  }
+  rc = mhd_substep(&h, o, vi, vo, 0);
+  h.fline = nullptr;  // g->work, not owned
+  return rc;
+}
+
+// include/bouss_rkstep2.f90:1-71 (rotbouss_rkstep2.f90 with rot = 1: the
+// Coriolis force of rotation omegaz about z) for order index o on caller
+// fields: vin, C and vout hold (v_x, v_y, v_z, th); f holds (f_x, f_y, f_z,
+// f_s) or is NULL.  The ord = 3 correction of bouss_rkstep2.f90:73-145 is not
+// applied here.
+static int bouss_rkstep2_impl(ghx_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
+                              bool rot, double omegaz, const void *const *vin, const void *const *C,
+                              const void *const *f, void *const *vout) {
+  ARGCHK(g && o >= 1 && vin && C && vout, "bad bouss_rkstep2 arguments");
+  int rc = plan_scratch(&g->plan, 18);
+  if (rc) return rc;
+  ghx_bouss h;
+  h.g = g; h.ord = o; h.dt = dt; h.nu = nu; h.kappa = kappa; h.xmom = xmom; h.xtemp = xtemp;
+  h.rot = rot; h.omegaz = omegaz;
+  h.q = g->plan.q;
+  const cplx *vi[4];
+  cplx *vo[4];
+  for (int k = 0; k < 4; ++k) { h.s[k] = M_(C[k]); vi[k] = C_(vin[k]); vo[k] = M_(vout[k]); }
+  if (f) {
+    for (int k = 0; k < 4; ++k) h.f[k] = M_(f[k]);
+    const cplx *fa[4] = {h.f[0], h.f[1], h.f[2], h.f[3]};
+    if ((rc = stateless_flags(g, fa, 4, &h.fline))) return rc;
+    h.forced = true;
+  }
+  rc = bouss_substep(&h, o, vi, vo, 0);
+  h.fline = nullptr;
+  return rc;
+}
+extern "C" int ghx_bouss_rkstep2(ghx_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
+                                 const void *const *vin, const void *const *C, const void *const *f,
+                                 void *const *vout) {
+  return bouss_rkstep2_impl(g, o, dt, nu, kappa, xmom, xtemp, false, 0.0, vin, C, f, vout);
+}
+extern "C" int ghx_rotbouss_rkstep2(ghx_grid *g, int o, double dt, double nu, double kappa, double xmom,
+                                    double xtemp, double omegaz, const void *const *vin, const void *const *C,
+                                    const void *const *f, void *const *vout) {
+  return bouss_rkstep2_impl(g, o, dt, nu, kappa, xmom, xtemp, true, omegaz, vin, C, f, vout);
+}
+
 extern "C" int ghx_bouss_step(ghx_bouss *s, int nsteps) {
   ARGCHK(s && nsteps >= 0, "bad bouss_step arguments");
   if (s->ord == 3 && !s->rot)

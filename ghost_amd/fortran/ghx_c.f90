@@ -369,6 +369,33 @@ MODULE ghx_c
     END FUNCTION
 
     ! RK substep and resident solvers (include/hd_rkstep2.f90, main3D.fpp:3202-3267)
+    INTEGER(C_INT) FUNCTION ghx_mhd_rkstep2(g, o, dt, nu, mu, v_in, c, f, m, v_out) &
+        BIND(C, name="ghx_mhd_rkstep2")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g
+      INTEGER(C_INT), VALUE :: o
+      REAL(C_DOUBLE), VALUE :: dt, nu, mu
+      TYPE(C_PTR) :: v_in(6), c(6), v_out(6)
+      TYPE(C_PTR), VALUE :: f, m
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_bouss_rkstep2(g, o, dt, nu, kappa, xmom, xtemp, v_in, c, f, v_out) &
+        BIND(C, name="ghx_bouss_rkstep2")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g
+      INTEGER(C_INT), VALUE :: o
+      REAL(C_DOUBLE), VALUE :: dt, nu, kappa, xmom, xtemp
+      TYPE(C_PTR) :: v_in(4), c(4), v_out(4)
+      TYPE(C_PTR), VALUE :: f
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_rotbouss_rkstep2(g, o, dt, nu, kappa, xmom, xtemp, omegaz, v_in, c, f, v_out) &
+        BIND(C, name="ghx_rotbouss_rkstep2")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g
+      INTEGER(C_INT), VALUE :: o
+      REAL(C_DOUBLE), VALUE :: dt, nu, kappa, xmom, xtemp, omegaz
+      TYPE(C_PTR) :: v_in(4), c(4), v_out(4)
+      TYPE(C_PTR), VALUE :: f
+    END FUNCTION
     INTEGER(C_INT) FUNCTION ghx_roth_rkstep2(g, o, dt, nu, omegaz, v_in, c, f, v_out) &
         BIND(C, name="ghx_roth_rkstep2")
       IMPORT :: C_INT, C_PTR, C_DOUBLE

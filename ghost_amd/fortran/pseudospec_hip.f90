@@ -767,4 +767,93 @@ CONTAINS
     END IF
   END SUBROUTINE roth_rkstep2
 
+  ! -- include/mhd_rkstep2.f90: the fused MHD substep for stage o -------------
+  ! v(1:3), a(1:3) <- C + dt (diffusion + nonlinear terms + forcing) / o;
+  ! c(1:6) holds (v, a) at the step start; f and m (mechanical and
+  ! electromotive forcing) are both present or both absent.
+  SUBROUTINE mhd_rkstep2(o, dt, nu, mu, v, a, c, f, m)
+    INTEGER, INTENT(IN) :: o
+    REAL(KIND=GP), INTENT(IN) :: dt, nu, mu
+    TYPE(ghx_field), INTENT(INOUT) :: v(3), a(3)
+    TYPE(ghx_field), INTENT(IN) :: c(6)
+    TYPE(ghx_field), INTENT(IN), OPTIONAL :: f(3), m(3)
+    TYPE(C_PTR), TARGET :: pv(6), pc(6), pf(3), pm(3)
+    INTEGER :: k
+    DO k = 1, 3
+      pv(k) = v(k)%p
+      pv(3 + k) = a(k)%p
+    END DO
+    DO k = 1, 6
+      pc(k) = c(k)%p
+    END DO
+    IF (PRESENT(f) .AND. PRESENT(m)) THEN
+      DO k = 1, 3
+        pf(k) = f(k)%p
+        pm(k) = m(k)%p
+      END DO
+      CALL ghx_check(ghx_mhd_rkstep2(ghx_grid_h, INT(o, C_INT), REAL(dt, C_DOUBLE), REAL(nu, C_DOUBLE), &
+                                     REAL(mu, C_DOUBLE), pv, pc, C_LOC(pf), C_LOC(pm), pv), 'mhd_rkstep2')
+    ELSE
+      CALL ghx_check(ghx_mhd_rkstep2(ghx_grid_h, INT(o, C_INT), REAL(dt, C_DOUBLE), REAL(nu, C_DOUBLE), &
+                                     REAL(mu, C_DOUBLE), pv, pc, C_NULL_PTR, C_NULL_PTR, pv), 'mhd_rkstep2')
+    END IF
+  END SUBROUTINE mhd_rkstep2
+
+  ! -- include/bouss_rkstep2.f90 (:1-71) and rotbouss_rkstep2.f90 --------------
+  ! v(1:3), th <- C + dt (...) / o; c(1:4) holds (v, th) at the step start;
+  ! f(1:3), fs the forcings (both present or both absent).
+  SUBROUTINE bouss_rkstep2(o, dt, nu, kappa, xmom, xtemp, v, th, c, f, fs)
+    INTEGER, INTENT(IN) :: o
+    REAL(KIND=GP), INTENT(IN) :: dt, nu, kappa, xmom, xtemp
+    TYPE(ghx_field), INTENT(INOUT) :: v(3), th
+    TYPE(ghx_field), INTENT(IN) :: c(4)
+    TYPE(ghx_field), INTENT(IN), OPTIONAL :: f(3), fs
+    CALL bouss_body(o, dt, nu, kappa, xmom, xtemp, .FALSE., 0.0_GP, v, th, c, f, fs)
+  END SUBROUTINE bouss_rkstep2
+
+  SUBROUTINE rotbouss_rkstep2(o, dt, nu, kappa, xmom, xtemp, omegaz, v, th, c, f, fs)
+    INTEGER, INTENT(IN) :: o
+    REAL(KIND=GP), INTENT(IN) :: dt, nu, kappa, xmom, xtemp, omegaz
+    TYPE(ghx_field), INTENT(INOUT) :: v(3), th
+    TYPE(ghx_field), INTENT(IN) :: c(4)
+    TYPE(ghx_field), INTENT(IN), OPTIONAL :: f(3), fs
+    CALL bouss_body(o, dt, nu, kappa, xmom, xtemp, .TRUE., omegaz, v, th, c, f, fs)
+  END SUBROUTINE rotbouss_rkstep2
+
+  SUBROUTINE bouss_body(o, dt, nu, kappa, xmom, xtemp, rot, omegaz, v, th, c, f, fs)
+    INTEGER, INTENT(IN) :: o
+    REAL(KIND=GP), INTENT(IN) :: dt, nu, kappa, xmom, xtemp, omegaz
+    LOGICAL, INTENT(IN) :: rot
+    TYPE(ghx_field), INTENT(INOUT) :: v(3), th
+    TYPE(ghx_field), INTENT(IN) :: c(4)
+    TYPE(ghx_field), INTENT(IN), OPTIONAL :: f(3), fs
+    TYPE(C_PTR), TARGET :: pv(4), pc(4), pf(4)
+    TYPE(C_PTR) :: fp
+    INTEGER :: k
+    DO k = 1, 3
+      pv(k) = v(k)%p
+    END DO
+    pv(4) = th%p
+    DO k = 1, 4
+      pc(k) = c(k)%p
+    END DO
+    fp = C_NULL_PTR
+    IF (PRESENT(f) .AND. PRESENT(fs)) THEN
+      DO k = 1, 3
+        pf(k) = f(k)%p
+      END DO
+      pf(4) = fs%p
+      fp = C_LOC(pf)
+    END IF
+    IF (rot) THEN
+      CALL ghx_check(ghx_rotbouss_rkstep2(ghx_grid_h, INT(o, C_INT), REAL(dt, C_DOUBLE), REAL(nu, C_DOUBLE), &
+                                          REAL(kappa, C_DOUBLE), REAL(xmom, C_DOUBLE), REAL(xtemp, C_DOUBLE), &
+                                          REAL(omegaz, C_DOUBLE), pv, pc, fp, pv), 'rotbouss_rkstep2')
+    ELSE
+      CALL ghx_check(ghx_bouss_rkstep2(ghx_grid_h, INT(o, C_INT), REAL(dt, C_DOUBLE), REAL(nu, C_DOUBLE), &
+                                       REAL(kappa, C_DOUBLE), REAL(xmom, C_DOUBLE), REAL(xtemp, C_DOUBLE), &
+                                       pv, pc, fp, pv), 'bouss_rkstep2')
+    END IF
+  END SUBROUTINE bouss_body
+
 END MODULE pseudospec_hip

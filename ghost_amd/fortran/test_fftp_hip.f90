@@ -5,6 +5,7 @@
 !    Taylor-Green closed forms through the host-array pseudospec drop-ins.
 ! 2. Device-resident fields through pseudospec_hip: TG enstrophy, and an RK2
 !    loop written as GHOST's main3D (C = v; DO o = ord,1,-1: hd_rkstep2).
+! 2c. GHOST's MHD loop through mhd_rkstep2 on resident fields.
 ! 3. The resident solver with tracers attached.
 ! 2b. GHOST's diagnostic files (spectrum, entrans, heltrans, spectr1d, hdcheck,
 !    the spectra about z, the scalar and the Boussinesq diagnostics) from the
@@ -22,7 +23,7 @@ PROGRAM test_fftp_hip
                             field_upload, field_download, ghx_grid_h, d_rotor3 => rotor3, d_energy => energy, &
                             hd_rkstep2, spectrum, entrans, heltrans, spectr1d, hdcheck, specpara, specperp, &
                             entpara, entperp, heltpara, spec2D, pscheck, spectrsc, sctrans, difucz, tbouss, &
-                            havgwrite, spectpv
+                            havgwrite, spectpv, mhd_rkstep2
   USE iovar, ONLY: IOPLAN
   IMPLICIT NONE
   INTEGER, PARAMETER :: n = 32, nh = n/2 + 1, nsteps = 20, npart = 1000, psteps = 5
@@ -32,7 +33,7 @@ PROGRAM test_fftp_hip
   REAL(KIND=GP), ALLOCATABLE, TARGET :: ux(:,:,:), uy(:,:,:), uz(:,:,:), r(:,:,:)
   COMPLEX(KIND=GP), ALLOCATABLE, TARGET :: c1(:,:,:), c2(:,:,:), c3(:,:,:), c4(:,:,:)
   REAL(KIND=GP), ALLOCATABLE, TARGET :: pos0(:,:), pos1(:,:)
-  TYPE(ghx_field) :: v(3), cst(3), w(3)
+  TYPE(ghx_field) :: v(3), cst(3), w(3), c6(6)
   TYPE(C_PTR) :: hd, hv(3), hf(3), tr, pp(3)
   CHARACTER(LEN=512) :: outdir
   CHARACTER(LEN=100) :: iodir
@@ -157,6 +158,37 @@ PROGRAM test_fftp_hip
   CALL havgwrite(7, 'pvprof', '0001', w(1), w(2), w(3), cst(1), 1.3_GP, 0.8_GP)
   CALL spectpv(w(1), w(2), w(3), cst(1), '0001')
   WRITE(*, '(A)') 'ok   diagnostic files written'
+
+  ! GHOST's MHD loop (include/mhd_rkstep1/2.f90) on resident fields: v = the
+  ! k^4 exp field, a = the random field above, 3 unforced RK2 steps
+  DO q = 1, 6
+    CALL field_alloc(c6(q), .TRUE.)
+  END DO
+  DO q = 1, 3
+    DO o = 2, 1, -1
+      IF (o .EQ. 2) THEN
+        DO k = 1, 3
+          CALL ghx_check(ghx_memcpy_d2d(c6(k)%p, w(k)%p, INT(n, C_SIZE_T)**2*nh*16), 'copy C = v')
+          CALL ghx_check(ghx_memcpy_d2d(c6(3 + k)%p, cst(k)%p, INT(n, C_SIZE_T)**2*nh*16), 'copy C = a')
+        END DO
+      END IF
+      CALL mhd_rkstep2(o, dt, nu, nu, w, cst, c6)
+    END DO
+  END DO
+  OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/fortran_mhd32.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+  DO k = 1, 3
+    CALL field_download(w(k), C_LOC(c4))
+    WRITE(u) c4
+  END DO
+  DO k = 1, 3
+    CALL field_download(cst(k), C_LOC(c4))
+    WRITE(u) c4
+  END DO
+  CLOSE(u)
+  DO q = 1, 6
+    CALL field_free(c6(q))
+  END DO
+  WRITE(*, '(A)') 'ok   MHD loop written (fortran_mhd32.bin)'
 
   ! GHOST's binary output of v (main3D.fpp:2675-2710) from the resident
   ! fields, and the same file written from host arrays through GHOST's own

@@ -359,6 +359,39 @@ class Grid:
             _lib.check(L_.ghx_roth_rkstep2(self._h, int(o), float(dt), float(nu), float(omegaz), pv, pc, pf, po))
 
 
+    def _ptrs(self, fields):
+        if fields is None:
+            return None, None
+        for t in fields:
+            _dptr(t, torch.complex128, self.cshape)
+        return _lib.ptr_array([t.data_ptr() for t in fields])
+
+    def mhd_rkstep2(self, o, dt, nu, mu, v_in, C_, f, m, v_out):
+        """include/mhd_rkstep2.f90 for one value of o: fields (v_x, v_y, v_z,
+        a_x, a_y, a_z); f and m the forcings (3 each, or both None)."""
+        pv, _a = self._ptrs(v_in)
+        pc, _b = self._ptrs(C_)
+        pf, _c = self._ptrs(f)
+        pm, _d = self._ptrs(m)
+        po, _e = self._ptrs(v_out)
+        _lib.check(_lib.load().ghx_mhd_rkstep2(self._h, int(o), float(dt), float(nu), float(mu), pv, pc, pf, pm, po))
+
+    def bouss_rkstep2(self, o, dt, nu, kappa, xmom, xtemp, v_in, C_, f, v_out, omegaz=None):
+        """include/bouss_rkstep2.f90 (rotbouss_rkstep2.f90 with omegaz) for one
+        value of o: fields (v_x, v_y, v_z, th); f the 4 forcings or None."""
+        pv, _a = self._ptrs(v_in)
+        pc, _b = self._ptrs(C_)
+        pf, _c = self._ptrs(f)
+        po, _e = self._ptrs(v_out)
+        L_ = _lib.load()
+        if omegaz is None:
+            _lib.check(L_.ghx_bouss_rkstep2(self._h, int(o), float(dt), float(nu), float(kappa), float(xmom),
+                                            float(xtemp), pv, pc, pf, po))
+        else:
+            _lib.check(L_.ghx_rotbouss_rkstep2(self._h, int(o), float(dt), float(nu), float(kappa), float(xmom),
+                                               float(xtemp), float(omegaz), pv, pc, pf, po))
+
+
 def randu(seed):
     """randu (pseudospec3D_mod.fpp:130-148); returns (value, new seed)."""
     s = C.c_int32(seed)

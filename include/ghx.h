@@ -267,6 +267,22 @@ int ghx_hd_step(ghx_hd *s, int nsteps);
 /* the ROTH solver (include/roth_rkstep{1,2}.f90): rotation rate omegaz about z */
 int ghx_hd_set_rotation(ghx_hd *s, double omegaz);
 
+/* include/mhd_rkstep2.f90:4-65 for order index o on caller fields (the fused
+ * substep of the MHD solver): vin, C, vout hold (v_x, v_y, v_z, a_x, a_y,
+ * a_z); f (mechanical) and m (electromotive) forcing, 3 fields each, both
+ * given or both NULL; vout may equal vin or C. */
+int ghx_mhd_rkstep2(ghx_grid *g, int o, double dt, double nu, double mu, const void *const *vin, const void *const *C,
+                    const void *const *f, const void *const *m, void *const *vout);
+/* include/bouss_rkstep2.f90:1-71 (without the ord = 3 correction of :73-145)
+ * and include/rotbouss_rkstep2.f90 (Coriolis force of rotation omegaz about
+ * z) for order index o on caller fields: vin, C, vout hold (v_x, v_y, v_z,
+ * th); f holds (f_x, f_y, f_z, f_s) or is NULL; vout may equal vin or C. */
+int ghx_bouss_rkstep2(ghx_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
+                      const void *const *vin, const void *const *C, const void *const *f, void *const *vout);
+int ghx_rotbouss_rkstep2(ghx_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
+                         double omegaz, const void *const *vin, const void *const *C, const void *const *f,
+                         void *const *vout);
+
 typedef struct ghx_mhd ghx_mhd;
 /* MHD solver state: v (3), a (3, vector potential), f (3), m (3). */
 int ghx_mhd_create(ghx_mhd **s, ghx_grid *g, int ord, double dt, double nu, double mu);

@@ -96,6 +96,13 @@ def test_fortran_host_side(gpu, tmp_path):
         assert close(table(name + ".txt")[1:], col), name
     assert close(table("pvprof.0001.txt")[:, 1], og.havgcomp(*v, w[0], 1.3, 0.8, 7))
     assert close(table("pvspectrum.0001.txt")[:, 1], og.spectpv(*v, w[0]))
+    # the MHD loop: (v, a) = (the k^4 exp field, the random field), 3 RK2 steps
+    got_m = np.fromfile(tmp_path / "fortran_mhd32.bin", dtype=np.complex128).reshape(6, nh, n, n)
+    mv, ma = [x.copy() for x in v], [x.copy() for x in w]
+    for _ in range(3):
+        og.mhd_step(mv, ma, 1e-3, 1e-2, 1e-2, ord=2)
+    for k, want in enumerate(mv + ma):
+        assert np.linalg.norm(got_m[k] - want) <= 1e-12 * np.linalg.norm(want), k
     # GHOST's binary output of that v: the resident path (vx, vy, vz) and the
     # host-array path through io_write (hx) write the same bytes; the file is
     # c2r(v/N^3) in Fortran order
