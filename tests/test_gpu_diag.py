@@ -125,3 +125,20 @@ def test_boussinesq_diagnostics(gpu, shape, arb, L):
     for got, want in zip(g.tbouss(*F, fo, bv), og.tbouss(*f, fo, bv)):
         assert rel(got, want) < 1e-12
     assert rel(g.spectpv(*F), og.spectpv(*f)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 512])
+def test_spectrumc_bins_match_exact_lattice_counts(gpu, n):
+    """A field with |a|^2 = N^6 on every mode: each shell of the GPU spectrum is
+    the Hermitian-weighted mode count, equal to the exact rational count (the
+    binning of pseudospec3D_hd.f90:1190-1415 at the 512^3 workload size)."""
+    import torch
+    from ghost_amd.pseudospec import Grid
+    from test_oracle_diag import exact_shell_weights
+    g = Grid(n, n, n)
+    a = torch.full(g.cshape, float(n) ** 3, dtype=torch.complex128, device="cuda")
+    z = torch.zeros(g.cshape, dtype=torch.complex128, device="cuda")
+    Ek, _ = g.spectrumc(a, z, z, kin=1, hel=0)
+    del a, z
+    want = exact_shell_weights(n, n, n, g.nbins)
+    assert np.array_equal(np.array(Ek), want.astype(np.float64))

@@ -231,3 +231,59 @@ def test_spectpv_closed_form():
     assert S[0] == pytest.approx(E ** 2 * C ** 2 / 4, rel=1e-12)
     assert S[1] == pytest.approx((E ** 2 * A ** 2 + F ** 2 * D ** 2) / 4, rel=1e-12)
     assert np.all(np.abs(S[2:]) < 1e-25)
+
+
+def exact_shell_weights(nx, ny, nz, nb):
+    """Hermitian-weighted mode count per shell of spectrumc on the 2 pi cube
+    (Dkk = 1), computed in exact integer and rational arithmetic.
+
+    A mode with integer |k|^2 = n lies in shell s = int(sqrt(n) + 0.501) with
+    GHOST's offset the single-precision literal .501 (pseudospec3D_hd.f90:1190-
+    1415).  With r = isqrt(n), s is r + 1 exactly when n >= (r + 1 - off)^2 as
+    rationals, and r otherwise, so no floating-point rounding enters."""
+    from fractions import Fraction
+    from math import isqrt
+    off = Fraction(float(np.float32(0.501)))
+    ky = np.fft.fftfreq(ny, 1.0 / ny).astype(np.int64)
+    kz = np.fft.fftfreq(nz, 1.0 / nz).astype(np.int64)
+    yz = np.bincount((ky[:, None] ** 2 + kz[None, :] ** 2).ravel())
+    cnt = np.zeros((nx // 2) ** 2 + yz.size, dtype=np.int64)
+    for kx in range(nx // 2 + 1):
+        cnt[kx * kx:kx * kx + yz.size] += (1 if kx == 0 else 2) * yz
+    out = np.zeros(nb, dtype=np.int64)
+    for n in np.nonzero(cnt)[0].tolist():
+        r = isqrt(n)
+        s = r + 1 if Fraction(n) >= (r + 1 - off) ** 2 else r
+        if 1 <= s <= nb:
+            out[s - 1] += cnt[n]
+    return out
+
+
+def test_exact_shell_weights_small_cases():
+    # shell 1 holds |k| = 1 and |k| = sqrt 2 (1.414 + .501 < 2); shell 2
+    # starts at |k| = sqrt 3.  kx = 0: 4 + 4 modes, weight 1; kx = 1: the
+    # axis mode and (1, +-1, 0), (1, 0, +-1), weight 2.
+    w = exact_shell_weights(8, 8, 8, 5)
+    assert w[0] == 1 * (4 + 4) + 2 * (1 + 4)
+    # shell 2: |k|^2 in {3, 4, 5, 6} (sqrt 6 + .501 = 2.95); shell 3 from 7
+    # (2.646 + .501 = 3.147).  Count |k|^2 = 3..6 by brute force.
+    k = np.fft.fftfreq(8, 1 / 8).astype(int)
+    n2 = k[:5, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2
+    wt = np.where(np.arange(5) == 0, 1, 2)[:, None, None]
+    assert w[1] == int(np.sum(wt * ((n2 >= 3) & (n2 <= 6))))
+    # every mode of the half space with 0 < |k| < 4.499 lands in shells 1..4
+    assert w[:4].sum() == int(np.sum(wt * ((n2 > 0) & (np.sqrt(n2) + 0.501 < 5))))
+
+
+@pytest.mark.parametrize("n", [64, 512])
+def test_spectrumc_bins_match_exact_lattice_counts(n):
+    """spectrumc of a field with |a|^2 = N^6 on every mode returns, per shell,
+    the Hermitian-weighted number of modes in it (exact in double), which must
+    equal the exact rational count, at 64^3 and at the 512^3 workload size."""
+    og = O.Grid(n)
+    a = np.full(og.cshape, float(n) ** 3, dtype=np.complex128)
+    z = np.zeros(og.cshape, dtype=np.complex128)
+    Ek, _ = og.spectrumc(a, z, z, kin=1)
+    del a
+    want = exact_shell_weights(n, n, n, og.nbins)
+    assert np.array_equal(Ek, want.astype(np.float64))
