@@ -53,8 +53,10 @@ static int rows_per_block(int T) {
   return r < 1 ? 1 : r;
 }
 
+// Tile widths are powers of two (the tile store indexes lines with a mask).
 static int pick_L(int T, int rowslots, int ny, int want) {
-  int L = want;
+  int L = 1;
+  while (L * 2 <= want) L *= 2;
   while (L > 1 && ((size_t)L * rowslots * sizeof(cplx) > max_lds() || L * T > 512)) L >>= 1;
   if (L > ny) {
     L = 1;
@@ -125,16 +127,23 @@ GHX_DEV void xcd_tile(int &bx, int &by) {
 // Tile helpers (block = L lines x T lanes; line lt = threadIdx.x / T)
 // ---------------------------------------------------------------------------
 
-// Store this block's L lines (each NP points, kept in LDS rows) to Q with the
-// line index (ky) fastest: dst[(p*nxh + i)*ny + j0 + l].
-// Only lines whose bit is set in lmask and points p < pmax are stored; the
-// loader zero-fills the rest without touching memory (dealias pruning).
-GHX_DEV void tile_store_q(cplx *lds, int stride, int L, int NP, cplx *dst, size_t z_or_p_stride,
-                          size_t base, int j0, int ny, unsigned lmask = ~0u, int pmax = 1 << 30) {
-  for (int idx = threadIdx.x; idx < L * NP; idx += blockDim.x) {
-    int l = idx % L, p = idx / L;
-    if (j0 + l < ny && ((lmask >> l) & 1u) && p < pmax)
-      dst[base + (size_t)p * z_or_p_stride + j0 + l] = lds[l * stride + lds_pad(p)];
+// Store the L lines of a block (kept in LDS rows) with the line index
+// fastest: point p of line l goes to dst[p * pstride + l] when keep(l) (the
+// dealias pruning and the tile's edge; the loader zero-fills the rest without
+// touching memory).  The block is exactly L x T lanes with L a power of two
+// and E * T points per row, so every lane moves E points: line
+// l = idx & (L - 1), point idx >> log2 L.  The loop stays rolled (two points
+// per trip): a fully unrolled one holds all E points and costs the y and z
+// kernels a wave per SIMD or spills, for no gain.
+template <int E, typename Keep>
+GHX_DEV void tile_store_rows(const cplx *lds, int stride, int L, cplx *dst, size_t pstride, Keep keep) {
+  const int lsh = 31 - __builtin_clz((unsigned)L);
+#pragma unroll 2
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * blockDim.x;
+    const int l = idx & (L - 1);
+    const cplx x = lds[l * stride + lds_pad(idx >> lsh)];
+    if (keep(l)) dst[(size_t)(idx >> lsh) * pstride + l] = x;
   }
 }
 // Block = L lines x T lanes, E = N / T points per thread.  The E loads of a
@@ -180,7 +189,9 @@ GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
 #pragma unroll
   for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
   __syncthreads();
-  tile_store_q(lds, stride, L, N, dst, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny, lmask);
+  const int ny = g.ny;
+  tile_store_rows<E>(lds, stride, L, dst + (size_t)i * ny + j0, (size_t)g.nxh * ny,
+                     [=](int l) { return j0 + l < ny && ((lmask >> l) & 1u); });
   __syncthreads();
 }
 
@@ -690,11 +701,8 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in_, FieldSet out_, int n
 #pragma unroll
     for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
     __syncthreads();
-    cplx *dst = out.p[f] + (size_t)z * g.ny * nxp;
-    for (int idx = threadIdx.x; idx < L * N; idx += blockDim.x) {
-      const int l = idx % L, p = idx / L;
-      if (kx0 + l < kxlim) dst[(size_t)p * nxp + kx0 + l] = lds[l * stride + lds_pad(p)];
-    }
+    tile_store_rows<E>(lds, stride, L, out.p[f] + (size_t)z * g.ny * nxp + kx0, (size_t)nxp,
+                       [=](int l) { return kx0 + l < kxlim; });
     __syncthreads();
   }
 }
