@@ -134,8 +134,10 @@ GHX_DEV void xcd_tile(int &bx, int &by) {
 // and E * T points per row, so every lane moves E points: line
 // l = idx & (L - 1), point idx >> log2 L.  The loop stays rolled (two points
 // per trip): a fully unrolled one holds all E points and costs the y and z
-// kernels a wave per SIMD or spills, for no gain.
-template <int E, typename Keep>
+// kernels a wave per SIMD or spills, for no gain.  NT selects streaming
+// stores: the y inverse (128-byte chunks) gains 10-14% with them, the z
+// inverse (64-byte chunks at L = 4) loses 5-11%.
+template <int E, bool NT, typename Keep>
 GHX_DEV void tile_store_rows(const cplx *lds, int stride, int L, cplx *dst, size_t pstride, Keep keep) {
   const int lsh = 31 - __builtin_clz((unsigned)L);
 #pragma unroll 2
@@ -143,7 +145,7 @@ GHX_DEV void tile_store_rows(const cplx *lds, int stride, int L, cplx *dst, size
     const int idx = threadIdx.x + q * blockDim.x;
     const int l = idx & (L - 1);
     const cplx x = lds[l * stride + lds_pad(idx >> lsh)];
-    if (keep(l)) dst[(size_t)(idx >> lsh) * pstride + l] = x;
+    if (keep(l)) c_put<NT>(dst + (size_t)(idx >> lsh) * pstride + l, x);
   }
 }
 // Block = L lines x T lanes, E = N / T points per thread.  The E loads of a
@@ -190,7 +192,7 @@ GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
   for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
   __syncthreads();
   const int ny = g.ny;
-  tile_store_rows<E>(lds, stride, L, dst + (size_t)i * ny + j0, (size_t)g.nxh * ny,
+  tile_store_rows<E, false>(lds, stride, L, dst + (size_t)i * ny + j0, (size_t)g.nxh * ny,
                      [=](int l) { return j0 + l < ny && ((lmask >> l) & 1u); });
   __syncthreads();
 }
@@ -468,13 +470,13 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in_, GridK g, const
       cplx n0, n1, n2;
       if (rk.om2 != 0.0) coriolis(P[0][m], P[1][m], rk.v[0][s], rk.v[1][s], rk.om2);
       project_hd(P[0][m], P[1][m], P[2][m], kx, ky, kz, kk2, i, j, k, n0, n1, n2);
-      rk.out[0][s] = rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
-      rk.out[1][s] = rk_update(rk.c[1][s], rk.v[1][s], n1, forced ? rk.f[1][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
-      rk.out[2][s] = rk_update(rk.c[2][s], rk.v[2][s], n2, forced ? rk.f[2][s] : z, kk2, rk.nu, rk.dt, rk.rmp);
+      c_put<true>(&rk.out[0][s], rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp));
+      c_put<true>(&rk.out[1][s], rk_update(rk.c[1][s], rk.v[1][s], n1, forced ? rk.f[1][s] : z, kk2, rk.nu, rk.dt, rk.rmp));
+      c_put<true>(&rk.out[2][s], rk_update(rk.c[2][s], rk.v[2][s], n2, forced ? rk.f[2][s] : z, kk2, rk.nu, rk.dt, rk.rmp));
     } else if (!prune) {
-      rk.out[0][s] = z;
-      rk.out[1][s] = z;
-      rk.out[2][s] = z;
+      c_put<true>(&rk.out[0][s], z);
+      c_put<true>(&rk.out[1][s], z);
+      c_put<true>(&rk.out[2][s], z);
     }
     // keep the loads of one mode from being hoisted over the others (register pressure)
     __builtin_amdgcn_sched_barrier(0);
@@ -701,7 +703,7 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in_, FieldSet out_, int n
 #pragma unroll
     for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
     __syncthreads();
-    tile_store_rows<E>(lds, stride, L, out.p[f] + (size_t)z * g.ny * nxp + kx0, (size_t)nxp,
+    tile_store_rows<E, true>(lds, stride, L, out.p[f] + (size_t)z * g.ny * nxp + kx0, (size_t)nxp,
                        [=](int l) { return kx0 + l < kxlim; });
     __syncthreads();
   }
@@ -762,7 +764,7 @@ __global__ __launch_bounds__(512) void k_yfwd(FieldSet in_, FieldSet out_, int n
       for (int m = 0; m < E; ++m) {
         const int j = t + T * m;
         // dealiased output: only the ky of the dealias disc are used downstream
-        if (!prune || kn2_of(g, kx, j, 0) <= g.kmax) dst[j] = v[m];
+        if (!prune || kn2_of(g, kx, j, 0) <= g.kmax) c_put<true>(dst + j, v[m]);
       }
     }
     __syncthreads();
@@ -1147,9 +1149,9 @@ __global__ __launch_bounds__(256) void k_xinv3i2(FieldSet in, double *out, GridK
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int n = t + T * m;
-    dst[3 * n + 0] = z0[m].x;
-    dst[3 * n + 1] = z0[m].y;
-    dst[3 * n + 2] = z1[m].x;
+    d_put<true>(dst + 3 * n + 0, z0[m].x);
+    d_put<true>(dst + 3 * n + 1, z0[m].y);
+    d_put<true>(dst + 3 * n + 2, z1[m].x);
   }
 }
 

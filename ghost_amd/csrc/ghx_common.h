@@ -9,6 +9,21 @@
 typedef double2 cplx;
 
 GHX_DEV cplx c_make(double x, double y) { cplx r; r.x = x; r.y = y; return r; }
+// Streaming store of one complex value (the nontemporal hint on a vector
+// store): for outputs of many GB that no cache holds until they are read back.
+typedef double ghx_d2 __attribute__((ext_vector_type(2)));
+GHX_DEV void c_store_nt(cplx *p, cplx v) {
+  ghx_d2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<ghx_d2 *>(p));
+}
+template <bool NT> GHX_DEV void c_put(cplx *p, cplx v) {
+  if constexpr (NT) c_store_nt(p, v);
+  else *p = v;
+}
+template <bool NT> GHX_DEV void d_put(double *p, double v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
 GHX_DEV cplx c_add(cplx a, cplx b) { return c_make(a.x + b.x, a.y + b.y); }
 GHX_DEV cplx c_sub(cplx a, cplx b) { return c_make(a.x - b.x, a.y - b.y); }
 GHX_DEV cplx c_mul(cplx a, cplx b) { return c_make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
