@@ -23,6 +23,10 @@ tracers) trains a 10-64-64-6 closure on its own filtered states (kc = 42,
 subgrid transfer and calibrated on the last 1000 spin-up steps, and a
 1000-step 128^3 LES (dt = 4 DNS steps) is judged against the filtered DNS
 spectrum over the following 4000 DNS steps.
+--workload slab times one GPU's share of an 8-way slab decomposition of the
+512^3 transform at its real shapes: the y and x passes on 64 z-planes (a
+512x512x64 grid) and the z pass on 33 of the 257 kx columns (a 64x512x512
+grid), 20 r2c and 20 c2r each.  The all-to-all between them is not built.
 --workload ensemble measures the batched ensemble: 96 unforced HD runs, 24
 each at 32^3, 48^3, 64^3 and 96^3, k^4 exp(-2 (k/k0)^2) ICs with k0 drawn
 uniformly in [2, 4], nu set so that k_max eta = 1.5 at the start, RK2.
@@ -425,6 +429,49 @@ def cpu_baseline(n, threads, npart):
                       f"steps/s = 1/(2 x (substep + stage))"}
 
 
+SLAB_SHAPES = ((512, 512, 64), (64, 512, 512))
+
+
+def run_slab(args, torch):
+    from ghost_amd.pseudospec import Grid
+    res = {}
+    for shape in SLAB_SHAPES:
+        g = Grid(*shape)
+        r = torch.randn(g.rshape, dtype=torch.float64, device="cuda")
+        c = g.r2c(r)
+        out_r = torch.empty_like(r)
+        out_c = torch.empty_like(c)
+        s_r, s_c = r.numel() * 8.0, c.numel() * 16.0
+        times = {}
+        for name, fn in (("r2c", lambda: g.r2c(r, out_c)), ("c2r", lambda: g.c2r(c, out_r))):
+            for _ in range(args.warmup):
+                fn()
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(args.steps):
+                fn()
+            b.record()
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b) / args.steps
+            times[name] = {"ms": round(ms, 4), "GBps_on_Sr_plus_Sc": round((s_r + s_c) / ms / 1e6, 1)}
+        key = "x".join(map(str, shape))
+        res[key] = times
+        log(f"slab {key}: {times}")
+        del g, r, c, out_r, out_c
+        torch.cuda.empty_cache()
+    return {
+        "metric": "per-GPU slab share of the 512^3 FP64 transform (8-way decomposition), ms per transform",
+        "value": round(sum(t["ms"] for v in res.values() for t in v.values()) / 2, 4), "unit": "ms (r2c + c2r pair, both shapes, averaged)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (standard normal real fields)",
+        "config": {"workload": "512x512x64 (y and x passes on one GPU's 64 z-planes) and 64x512x512 (z pass on "
+                               "33 kx columns) FP64 r2c and c2r; the all-to-all is not built",
+                   "shapes": [list(s) for s in SLAB_SHAPES]},
+        "shapes": res,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -434,7 +481,7 @@ def main():
     ap.add_argument("--tracers", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=("hd", "mhd", "ensemble", "les"), default="hd")
+    ap.add_argument("--workload", choices=("hd", "mhd", "ensemble", "les", "slab"), default="hd")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel for the rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE scale factors) and exit")
@@ -476,6 +523,9 @@ def main():
         return
     if args.workload == "les":
         print(json.dumps(run_les(args, torch)), flush=True)
+        return
+    if args.workload == "slab":
+        print(json.dumps(run_slab(args, torch)), flush=True)
         return
 
     g = Grid(n)
