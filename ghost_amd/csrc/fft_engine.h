@@ -76,6 +76,11 @@ GHX_FFTCFGLOW(32, 4, 3, 2, 4, 4)
 GHX_FFTCFGLOW(64, 4, 3, 4, 4, 4)
 GHX_FFTCFGLOW(128, 4, 4, 2, 4, 4, 4)
 GHX_FFTCFGLOW(256, 4, 4, 4, 4, 4, 4)
+// 3 * 2^k: 6 points per lane (one radix-3 stage, then radix 2), in place of
+// the default 24 points per lane that the six live fields would spill
+GHX_FFTCFGLOW(24, 6, 4, 3, 2, 2, 2)
+GHX_FFTCFGLOW(48, 6, 5, 3, 2, 2, 2, 2)
+GHX_FFTCFGLOW(96, 6, 6, 3, 2, 2, 2, 2, 2)
 GHX_FFTCFGLOW(512, 4, 5, 2, 4, 4, 4, 4)
 #undef GHX_FFTCFGLOW
 
