@@ -42,13 +42,13 @@ GHX_FFTCFG(10, 10, 2, 5, 2)
 GHX_FFTCFG(12, 12, 2, 3, 4)
 GHX_FFTCFG(16, 8, 2, 2, 8)
 GHX_FFTCFG(20, 20, 2, 5, 4)
-GHX_FFTCFG(24, 24, 2, 3, 8)
+GHX_FFTCFG(24, 6, 4, 3, 2, 2, 2)
 GHX_FFTCFG(32, 8, 2, 4, 8)
 GHX_FFTCFG(40, 40, 2, 5, 8)
-GHX_FFTCFG(48, 24, 3, 3, 2, 8)
+GHX_FFTCFG(48, 6, 5, 3, 2, 2, 2, 2)
 GHX_FFTCFG(64, 8, 2, 8, 8)
 GHX_FFTCFG(80, 40, 3, 5, 2, 8)
-GHX_FFTCFG(96, 24, 3, 3, 4, 8)
+GHX_FFTCFG(96, 6, 6, 3, 2, 2, 2, 2, 2)
 GHX_FFTCFG(128, 8, 3, 2, 8, 8)
 GHX_FFTCFG(160, 40, 3, 5, 4, 8)
 GHX_FFTCFG(192, 24, 3, 3, 8, 8)
@@ -76,11 +76,6 @@ GHX_FFTCFGLOW(32, 4, 3, 2, 4, 4)
 GHX_FFTCFGLOW(64, 4, 3, 4, 4, 4)
 GHX_FFTCFGLOW(128, 4, 4, 2, 4, 4, 4)
 GHX_FFTCFGLOW(256, 4, 4, 4, 4, 4, 4)
-// 3 * 2^k: 6 points per lane (one radix-3 stage, then radix 2), in place of
-// the default 24 points per lane that the six live fields would spill
-GHX_FFTCFGLOW(24, 6, 4, 3, 2, 2, 2)
-GHX_FFTCFGLOW(48, 6, 5, 3, 2, 2, 2, 2)
-GHX_FFTCFGLOW(96, 6, 6, 3, 2, 2, 2, 2, 2)
 GHX_FFTCFGLOW(512, 4, 5, 2, 4, 4, 4, 4)
 #undef GHX_FFTCFGLOW
 
