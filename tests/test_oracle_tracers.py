@@ -159,3 +159,75 @@ def test_tracers_in_steady_shear_flow(zcompat):
     assert np.max(np.abs(np.mod(pos[0] - start[0] - disp + n / 2, n) - n / 2)) < 1e-4 * np.max(np.abs(disp))
     assert np.max(np.abs(pos[1] - start[1])) < 1e-12
     assert np.max(np.abs(pos[2] - start[2])) < 1e-12
+
+
+# The stage coupling of main3D.fpp:3229-3262 with GPart_StepRKK
+# (gparts_mod.f90:2046-2145): after RK substep o the fluid holds the substep's
+# output v_o, and the tracers move to x = x0 + dt (1/o) u_o(x) with u_o the
+# spline interpolant of c2r(v_o / N^3) at their current positions x and x0
+# their positions at the start of the step.  For RK2 (o = 2, then 1) and a
+# linear mode that the nonlinear term leaves alone, v_2 = (1 + z/2) v and
+# v_1 = (1 + z + z^2/2) v with z = -nu |k|^2 dt, so the paths are closed forms.
+# A tracer that saw the step's input field, the midpoint field in both stages,
+# advanced from x* instead of x0, or was sampled at x0 in the second stage
+# would land elsewhere by far more than the tolerance.
+
+def test_tracer_coupling_decaying_shear_at_nodes():
+    """u = A sin(z) x: an exact Navier-Stokes solution (its Lamb vector is a
+    gradient).  With the tracers on z nodes the spline interpolant is exact, so
+    x1 = x0 + dt c A (1 + z + z^2/2) sin(z_node + 1 cell) (GHOST's z stencil)."""
+    n, A, nu, dt = 32, 0.8, 2.0, 0.05
+    g = O.Grid(n)
+    zc = np.arange(n) * 2 * np.pi / n
+    ux = np.broadcast_to((A * np.sin(zc))[:, None, None], g.rshape).copy()
+    v = [g.r2c(ux), g.czeros(), g.czeros()]
+    rng = np.random.default_rng(12)
+    npart = 40
+    zn = rng.integers(0, n - 2, npart).astype(float)  # nodes below the clamped top cell
+    pos = [rng.uniform(0, n, npart), rng.uniform(0, n, npart), zn]
+    start = [p.copy() for p in pos]
+    g.hd_step_tracers(v, pos, dt, nu, ord=2, zcompat=True)
+    lam = -nu * dt
+    c = n / (2 * np.pi)
+    s = np.sin(2 * np.pi * (zn + 1.0) / n)
+    want = np.mod(start[0] + dt * c * A * (1 + lam + lam * lam / 2) * s, n)
+    assert np.max(np.abs(pos[0] - want)) < 1e-11
+    assert np.max(np.abs(pos[1] - start[1])) < 1e-12 and np.max(np.abs(pos[2] - start[2])) < 1e-12
+    # the alternatives differ from GHOST's coupling by far more than the tolerance
+    for factor in (1.0, 1 + lam / 2):  # input field in stage 2, midpoint field in stage 2
+        alt = np.mod(start[0] + dt * c * A * factor * s, n)
+        assert np.max(np.abs(alt - want)) > 1e-4
+    alt = np.mod(start[0] + dt * c * A * ((1 + lam / 2) / 2 + (1 + lam + lam * lam / 2)) * s, n)  # from x*
+    assert np.max(np.abs(alt - want)) > 1e-2
+
+
+def test_tracer_coupling_decaying_taylor_green_2d():
+    """u = A (sin x cos y, -cos x sin y, 0), an exact Navier-Stokes solution
+    that decays at |k|^2 = 2.  Stage 1: x* = x0 + (dt/2) c s(x0) (1 + z/2);
+    stage 2: x1 = x0 + dt c s(x*) (1 + z + z^2/2), with s the exact periodic
+    spline interpolant of the sampled field (numpy FFT solve, closed-form
+    B-spline sum).  Sampling stage 2 at x0 moves the tracers by O(dt^2)."""
+    n, A, nu, dt = 32, 1.0, 1.0, 0.05
+    g = O.Grid(n)
+    zc, yc, xc = np.meshgrid(*(np.arange(n) * 2 * np.pi / n,) * 3, indexing="ij")
+    ur = [A * np.sin(xc) * np.cos(yc), -A * np.cos(xc) * np.sin(yc), np.zeros(g.rshape)]
+    v = [g.r2c(u) for u in ur]
+    coef = [spline_coef_fft(u) for u in ur[:2]]
+    rng = np.random.default_rng(13)
+    npart = 12
+    pos = [rng.uniform(0, n, npart), rng.uniform(0, n, npart), rng.uniform(0, n - 1.5, npart)]
+    x0 = [p.copy() for p in pos]
+    g.hd_step_tracers(v, pos, dt, nu, ord=2, zcompat=True)
+    lam = -2.0 * nu * dt
+    c = n / (2 * np.pi)
+    f2, f1 = 1 + lam / 2, 1 + lam + lam * lam / 2
+    s0 = [interp_bspline(cf, x0, zshift=1.0) for cf in coef]
+    xs = [np.mod(x0[q] + 0.5 * dt * c * f2 * s0[q], n) for q in range(2)] + [x0[2].copy()]
+    s1 = [interp_bspline(cf, xs, zshift=1.0) for cf in coef]
+    for q in range(2):
+        want = np.mod(x0[q] + dt * c * f1 * s1[q], n)
+        err = np.abs(np.mod(pos[q] - want + n / 2, n) - n / 2)
+        assert np.max(err) < 1e-11, q
+        at_x0 = np.mod(x0[q] + dt * c * f1 * s0[q], n)  # stage 2 sampled at x0
+        assert np.max(np.abs(np.mod(at_x0 - want + n / 2, n) - n / 2)) > 1e-4
+    assert np.max(np.abs(pos[2] - x0[2])) < 1e-12
