@@ -47,23 +47,122 @@ static size_t max_lds() {
   return g_max_lds;
 }
 
-// lines per block for a transposed pass: chunk L along ky, bounded by LDS and 1024 threads
-static int rows_per_block(int T) {
-  int r = 256 / T;
-  return r < 1 ? 1 : r;
-}
+// ---------------------------------------------------------------------------
+// Launch shapes.  The z and y passes run tiles of L lines x T lanes
+// (__launch_bounds__(kTileBound)); the x passes run blocks of whole rows
+// (__launch_bounds__(kRowBound)).  Both shapes are computed here, for the
+// launchers and for ghx_pass_shape, and every launch is checked against them
+// before it is issued (launch_check).
+// ---------------------------------------------------------------------------
+constexpr unsigned kTileBound = 512;
+constexpr unsigned kRowBound = 256;
+constexpr int kMaxTile = 32;  // the tile passes keep one bit per line in a 32-bit mask
 
-// Tile widths are powers of two (the tile store indexes lines with a mask).
-static int pick_L(int T, int rowslots, int ny, int want) {
+struct PassShape {
+  int lines;       // tile width L, or rows per block
+  unsigned block;  // lanes per block
+  size_t lds;      // dynamic LDS bytes
+};
+
+// Tile width: a power of two no wider than `want`, kMaxTile and the lines
+// there are, with L * T <= kTileBound and L rows of `rowslots` points in LDS.
+static int pick_L(int T, int rowslots, int nlines, int want, size_t ldsmax) {
   int L = 1;
-  while (L * 2 <= want) L *= 2;
-  while (L > 1 && ((size_t)L * rowslots * sizeof(cplx) > max_lds() || L * T > 512)) L >>= 1;
-  if (L > ny) {
+  while (L * 2 <= want && L * 2 <= kMaxTile) L *= 2;
+  while (L > 1 && ((size_t)L * rowslots * sizeof(cplx) > ldsmax || L * T > (int)kTileBound)) L >>= 1;
+  if (L > nlines) {
     L = 1;
-    while (L * 2 <= ny) L *= 2;
+    while (L * 2 <= nlines) L *= 2;
   }
   return L < 1 ? 1 : L;
 }
+static PassShape tile_shape(int T, int rowslots, int nlines, int want, size_t ldsmax) {
+  const int L = pick_L(T, rowslots, nlines, want, ldsmax);
+  return PassShape{L, (unsigned)(L * T), (size_t)L * rowslots * sizeof(cplx)};
+}
+// The wide y inverse: 8-line tiles (the kernel stores 128-byte chunks); the
+// launcher takes it only where their LDS fits, else the tile pass.
+static PassShape wide_shape(int TW, int rowslots, int nlines) {
+  (void)nlines;
+  return PassShape{8, (unsigned)(8 * TW), (size_t)8 * rowslots * sizeof(cplx)};
+}
+// Rows per block of an x pass: kRowBound lanes, fewer rows when their LDS
+// lines would not fit (the kernels read the row count from blockDim.x / T).
+static PassShape row_shape(int T, int rowslots, size_t ldsmax) {
+  int r = (int)kRowBound / T;
+  if (r < 1) r = 1;
+  while (r > 1 && (size_t)r * rowslots * sizeof(cplx) > ldsmax) r >>= 1;
+  return PassShape{r, (unsigned)(r * T), (size_t)r * rowslots * sizeof(cplx)};
+}
+
+// Refuses, before it is issued, a launch that its kernel was not built for:
+// more lanes than its __launch_bounds__ or than the code object's maximum
+// work-group size, a block that is not whole lines of T lanes, a tile wider
+// than the line masks, or more LDS than the device gives one block.
+static hipError_t launch_check(const void *kern, const char *name, const PassShape &s, unsigned bound, int T,
+                               bool tile) {
+  const char *why = nullptr;
+  if (s.block < 1 || s.block > bound) why = "more lanes than the kernel's launch bounds";
+  else if (T < 1 || s.block % (unsigned)T) why = "a block that is not whole lines";
+  else if (tile && (s.lines < 1 || s.lines > kMaxTile || (s.lines & (s.lines - 1)))) why = "a tile width that is not a power of two up to 32";
+  else if (s.lds > max_lds()) why = "more LDS than one block may hold";
+  if (!why) {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, kern) == hipSuccess && a.maxThreadsPerBlock > 0 &&
+        s.block > (unsigned)a.maxThreadsPerBlock)
+      why = "more lanes than the code object allows";
+  }
+  if (!why) return hipSuccess;
+  fprintf(stderr, "ghx: refused launch of %s: %s (block %u, %d lines, T %d, LDS %zu bytes, bound %u)\n", name, why,
+          s.block, s.lines, T, s.lds, bound);
+  return hipErrorInvalidConfiguration;
+}
+#define GHX_LAUNCH_CHECK(kern, s, bound, T, tile)                                              \
+  do {                                                                                         \
+    const hipError_t ce_ = launch_check((const void *)(kern), #kern, (s), (bound), (T), (tile)); \
+    if (ce_ != hipSuccess) return ce_;                                                         \
+  } while (0)
+
+#if GHX_PART == 1
+// The launch shape a pass of line length n takes (kind 0: z and y tile
+// passes; 1: the Boussinesq z forward, whose LDS rows hold n more points; 2:
+// the wide y inverse; 3: two-for-one x passes, n = nx; 4: half-length x
+// passes, n = nx/2), for `nlines` lines along the tiled axis, a requested
+// tile width `want` (as the GHX_LZ/GHX_LY knobs and the autotuner give it)
+// and a device LDS limit; then the check the launchers apply.  Host-only.
+extern "C" int ghx_pass_shape(int kind, int n, int nlines, int want, long long ldsmax, int *lines, int *block,
+                              long long *lds, int *lanes) {
+  int T = 0, rows = 0;
+  switch (n) {
+#define GHX_CASE(n_)                                                                   \
+  case n_:                                                                             \
+    if (kind == 0 || kind == 1) T = FFTCfg<n_>::T, rows = lds_line_stride(n_) + (kind == 1 ? n_ : 0); \
+    if (kind == 2 && FFTCfgWide<n_>::E != FFTCfg<n_>::E) T = FFTCfgWide<n_>::T, rows = lds_line_stride(n_); \
+    if (kind == 3 && (n_ == 32 || n_ == 64 || n_ == 128 || n_ == 256 || n_ == 512))    \
+      T = FFTCfg<n_>::T, rows = lds_line_stride(n_);                                   \
+    if (kind == 4) T = FFTCfgLow<n_>::T, rows = lds_line_stride(n_);                 \
+    break;
+    GHX_SIZES(GHX_CASE)
+#undef GHX_CASE
+  default:
+    break;
+  }
+  if (T <= 0 || nlines < 1 || ldsmax <= 0) return -1;
+  PassShape s;
+  unsigned bound = kTileBound;
+  if (kind == 0 || kind == 1) s = tile_shape(T, rows, nlines, want, (size_t)ldsmax);
+  else if (kind == 2) s = wide_shape(T, rows, nlines);
+  else s = row_shape(T, rows, (size_t)ldsmax), bound = kRowBound;
+  if (lines) *lines = s.lines;
+  if (block) *block = (int)s.block;
+  if (lds) *lds = (long long)s.lds;
+  if (lanes) *lanes = T;
+  const bool tile = kind <= 2;
+  if (s.block < 1 || s.block > bound || s.block % (unsigned)T || s.lds > (size_t)ldsmax) return -2;
+  if (tile && (s.lines > kMaxTile || (s.lines & (s.lines - 1)))) return -2;
+  return 0;
+}
+#endif
 
 // Tile widths (lines per block along ky) for the z and x passes; overridable
 // with GHX_LZ / GHX_LX for tuning sweeps.
@@ -81,8 +180,8 @@ static int tile_pick(const char *env, int ctx, int dflt) {
 }
 
 // Row lengths with a two-for-one x kernel (a full-length schedule of E = 8
-// points per lane and at most 64 lanes); GHX_XC2=2 selects the half-length
-// kernels instead, for comparisons.
+// points per lane and at most 64 lanes; ghx_pass_shape lists the same sizes);
+// GHX_XC2=2 selects the half-length kernels instead, for comparisons.
 #define GHX_X2_SIZES(X) X(32) X(64) X(128) X(256) X(512)
 static bool use_x2(int nx) {
   if (tile_want("GHX_XC2", 1) != 1) return false;
@@ -198,7 +297,7 @@ GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
 }
 
 template <int N, int MODE>
-__global__ __launch_bounds__(512) void k_zinv(FieldSet in_, FieldSet out_, int nf, GridK g, const cplx *__restrict__ twt,
+__global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_, int nf, GridK g, const cplx *__restrict__ twt,
                                              int prune, const double *__restrict__ aux) {
   using F = LineFFT<N, +1>;
   constexpr int T = F::T, E = F::E;
@@ -364,7 +463,7 @@ GHX_DEV void zfwd_load_diff(cplx (&v)[E], cplx *lds, cplx *row, int stride, int 
 }
 
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twt) {
+__global__ __launch_bounds__(kTileBound) void k_zfwd(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twt) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
@@ -416,7 +515,7 @@ GHX_DEV cplx rk_update(cplx cc, cplx vv, cplx nl, cplx ff, double kk2, double nu
 }
 
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in_, GridK g, const cplx *__restrict__ twt, RKParams rk,
+__global__ __launch_bounds__(kTileBound) void k_zfwd_rk_hd(FieldSet in_, GridK g, const cplx *__restrict__ twt, RKParams rk,
                                                    int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
@@ -489,7 +588,7 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_hd(FieldSet in_, GridK g, const
 // advection, then the updates of v (nu) and th (kappa).  The k = 0 mode keeps
 // the step-start mean of th; v is zero there and outside the dealias set.
 template <int N>
-__global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, const cplx *__restrict__ twt,
+__global__ __launch_bounds__(kTileBound) void k_zfwd_rk_bouss(FieldSet in, GridK g, const cplx *__restrict__ twt,
                                                       BoussRKParams rk, int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
@@ -577,7 +676,7 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_bouss(FieldSet in, GridK g, con
 // (PART 1).  The two parts are separate launches so that each keeps its
 // fields in registers.
 template <int N, int PART>
-__global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const cplx *__restrict__ twt, MHDRKParams rk,
+__global__ __launch_bounds__(kTileBound) void k_zfwd_rk_mhd(FieldSet in, GridK g, const cplx *__restrict__ twt, MHDRKParams rk,
                                                     int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
@@ -655,7 +754,7 @@ __global__ __launch_bounds__(512) void k_zfwd_rk_mhd(FieldSet in, GridK g, const
 // touched (kxlim < nx/2+1 when the fields are dealiased).
 // ---------------------------------------------------------------------------
 template <int N, typename Cfg = FFTCfg<N>>
-__global__ __launch_bounds__(512) void k_yinv(FieldSet in_, FieldSet out_, int nf, GridK g, int nxp,
+__global__ __launch_bounds__(kTileBound) void k_yinv(FieldSet in_, FieldSet out_, int nf, GridK g, int nxp,
                                              const cplx *__restrict__ twt, int kxlim, int prune) {
   using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T, E = F::E;
@@ -710,7 +809,7 @@ __global__ __launch_bounds__(512) void k_yinv(FieldSet in_, FieldSet out_, int n
 }
 
 template <int N>
-__global__ __launch_bounds__(512) void k_yfwd(FieldSet in_, FieldSet out_, int nf, GridK g, int nxp,
+__global__ __launch_bounds__(kTileBound) void k_yfwd(FieldSet in_, FieldSet out_, int nf, GridK g, int nxp,
                                              const cplx *__restrict__ twt, int kxlim, int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
@@ -875,7 +974,7 @@ struct RealSet {
 };
 
 template <int N>
-__global__ __launch_bounds__(256) void k_xinv(FieldSet in, RealSet out, int nf, GridK g, int nxp,
+__global__ __launch_bounds__(kRowBound) void k_xinv(FieldSet in, RealSet out, int nf, GridK g, int nxp,
                                              const cplx *__restrict__ twh, const cplx *__restrict__ twx,
                                              int kxlim) {
   using Cfg = FFTCfgLow<N>;
@@ -914,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_xinv(FieldSet in, RealSet out, int nf, 
 // layout the tracer gather reads (one 4-point stencil row of all three
 // components is 96 contiguous bytes).
 template <int N>
-__global__ __launch_bounds__(256) void k_xinv3i(FieldSet in, double *out, GridK g, int nxp,
+__global__ __launch_bounds__(kRowBound) void k_xinv3i(FieldSet in, double *out, GridK g, int nxp,
                                                const cplx *__restrict__ twh, const cplx *__restrict__ twx,
                                                int kxlim) {
   using Cfg = FFTCfgLow<N>;
@@ -954,7 +1053,7 @@ __global__ __launch_bounds__(256) void k_xinv3i(FieldSet in, double *out, GridK 
 }
 
 template <int N>
-__global__ __launch_bounds__(256) void k_xfwd(RealSet in, FieldSet out, int nf, GridK g, int nxp,
+__global__ __launch_bounds__(kRowBound) void k_xfwd(RealSet in, FieldSet out, int nf, GridK g, int nxp,
                                              const cplx *__restrict__ twh, const cplx *__restrict__ twx) {
   using Cfg = FFTCfgLow<N>;
   constexpr int T = Cfg::T, E = Cfg::E;
@@ -1083,7 +1182,7 @@ GHX_DEV void xpair_real(cplx (&z)[E], const double *sa, const double *sb, int t,
 // X^-1 of nf fields, two rows per transform (plain c2r; the last field of an
 // odd count goes alone).
 template <int NX>
-__global__ __launch_bounds__(256) void k_xinv2(FieldSet in, RealSet out, int nf, GridK g, int nxp,
+__global__ __launch_bounds__(kRowBound) void k_xinv2(FieldSet in, RealSet out, int nf, GridK g, int nxp,
                                               const cplx *__restrict__ twt, int kxlim) {
   using Cfg = FFTCfg<NX>;
   using FI = LineFFT<NX, +1>;
@@ -1123,7 +1222,7 @@ __global__ __launch_bounds__(256) void k_xinv2(FieldSet in, RealSet out, int nf,
 // X^-1 of three fields written interleaved (the tracer layout of k_xinv3i),
 // pairs (0, 1) and (2, alone).
 template <int NX>
-__global__ __launch_bounds__(256) void k_xinv3i2(FieldSet in, double *out, GridK g, int nxp,
+__global__ __launch_bounds__(kRowBound) void k_xinv3i2(FieldSet in, double *out, GridK g, int nxp,
                                                 const cplx *__restrict__ twt, int kxlim) {
   using Cfg = FFTCfg<NX>;
   using FI = LineFFT<NX, +1>;
@@ -1157,7 +1256,7 @@ __global__ __launch_bounds__(256) void k_xinv3i2(FieldSet in, double *out, GridK
 
 // X of nf real rows, two rows per transform.
 template <int NX>
-__global__ __launch_bounds__(256) void k_xfwd2(RealSet in, FieldSet out, int nf, GridK g, int nxp,
+__global__ __launch_bounds__(kRowBound) void k_xfwd2(RealSet in, FieldSet out, int nf, GridK g, int nxp,
                                               const cplx *__restrict__ twt) {
   using Cfg = FFTCfg<NX>;
   using FF = LineFFT<NX, -1>;
@@ -1194,7 +1293,7 @@ __global__ __launch_bounds__(256) void k_xfwd2(RealSet in, FieldSet out, int nf,
 // Fused X^-1 . ((A_x B_x + A_y B_y) + A_z B_z) tmp . X (advect3) with pairs
 // (A_x, A_y), (A_z, B_x), (B_y, B_z); fields as k_xdot.
 template <int NX>
-__global__ __launch_bounds__(256) void k_xdot2(FieldSet in, FieldSet out, GridK g, int nxp,
+__global__ __launch_bounds__(kRowBound) void k_xdot2(FieldSet in, FieldSet out, GridK g, int nxp,
                                               const cplx *__restrict__ twt, int kxlim) {
   using Cfg = FFTCfg<NX>;
   using FI = LineFFT<NX, +1>;
@@ -1234,7 +1333,7 @@ __global__ __launch_bounds__(256) void k_xdot2(FieldSet in, FieldSet out, GridK 
 // fields, products and operand order as k_xcross.  One row per line of
 // T <= 64 lanes.
 template <int NX>
-__global__ __launch_bounds__(256) void k_xcross2(FieldSet in_, FieldSet out_, GridK g, int nxp,
+__global__ __launch_bounds__(kRowBound) void k_xcross2(FieldSet in_, FieldSet out_, GridK g, int nxp,
                                                 const cplx *__restrict__ twt, int kxlim) {
   using Cfg = FFTCfg<NX>;
   using FI = LineFFT<NX, +1>;
@@ -1287,7 +1386,7 @@ __global__ __launch_bounds__(256) void k_xcross2(FieldSet in_, FieldSet out_, Gr
 // read and written (dealiased inputs, dealiased use of the outputs).  Each
 // row's next input is loaded before the current one is transformed.
 template <int N, int W, int SB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_xcross(
+__global__ __launch_bounds__(kRowBound) __attribute__((amdgpu_waves_per_eu(W))) void k_xcross(
     FieldSet in_, FieldSet out_, GridK g, int nxp, const cplx *__restrict__ twh, const cplx *__restrict__ twx,
     int kxlim) {
   using Cfg = FFTCfgLow<N>;
@@ -1367,7 +1466,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 // 23-115): in.p[0..2] = A, in.p[3..5] = B (grad of the scalar), out.p[0] =
 // FFT[((A_x B_x + A_y B_y) + A_z B_z) tmp] with tmp = -1/N^6.
 template <int N>
-__global__ __launch_bounds__(256) void k_xdot(FieldSet in, FieldSet out, GridK g, int nxp,
+__global__ __launch_bounds__(kRowBound) void k_xdot(FieldSet in, FieldSet out, GridK g, int nxp,
                                              const cplx *__restrict__ twh, const cplx *__restrict__ twx, int kxlim) {
   using Cfg = FFTCfgLow<N>;
   constexpr int T = Cfg::T, E = Cfg::E;
@@ -1433,27 +1532,27 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_pick("GHX_LZ", c.tile_z, mode == 0 ? 8 : 4));              \
-    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    const PassShape s = tile_shape(T, lds_line_stride(n_), g.ny, tile_pick("GHX_LZ", c.tile_z, mode == 0 ? 8 : 4), max_lds());              \
+    const int L = s.lines; const size_t lds = s.lds;                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);                                          \
     if (mode == 0) {                                                                        \
       allow_lds(k_zinv<n_, 0>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 0>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+      GHX_LAUNCH_CHECK((k_zinv<n_, 0>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zinv<n_, 0>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     } else if (mode == 1) {                                                                 \
       allow_lds(k_zinv<n_, 1>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 1>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+      GHX_LAUNCH_CHECK((k_zinv<n_, 1>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zinv<n_, 1>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     } else if (mode == 2) {                                                                 \
       allow_lds(k_zinv<n_, 2>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 2>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+      GHX_LAUNCH_CHECK((k_zinv<n_, 2>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zinv<n_, 2>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     } else if (mode == 4) {                                                                 \
       allow_lds(k_zinv<n_, 4>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 4>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+      GHX_LAUNCH_CHECK((k_zinv<n_, 4>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zinv<n_, 4>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     } else if (mode == 5) {                                                                 \
       allow_lds(k_zinv<n_, 5>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 5>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+      GHX_LAUNCH_CHECK((k_zinv<n_, 5>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zinv<n_, 5>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     } else {                                                                                \
       allow_lds(k_zinv<n_, 3>, lds);                                                        \
-      hipLaunchKernelGGL((k_zinv<n_, 3>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+      GHX_LAUNCH_CHECK((k_zinv<n_, 3>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zinv<n_, 3>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     }                                                                                       \
     break;                                                                                  \
   }
@@ -1474,11 +1573,11 @@ hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 8));                                        \
-    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    const PassShape s = tile_shape(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 8), max_lds());                                        \
+    const int L = s.lines; const size_t lds = s.lds;                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd<n_>, lds);                                                             \
-    hipLaunchKernelGGL((k_zfwd<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
+    GHX_LAUNCH_CHECK((k_zfwd<n_>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zfwd<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -1498,11 +1597,11 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, i
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_pick("GHX_LZ", c.tile_z, 4));         \
-    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    const PassShape s = tile_shape(T, lds_line_stride(n_), g.ny, tile_pick("GHX_LZ", c.tile_z, 4), max_lds());         \
+    const int L = s.lines; const size_t lds = s.lds;                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);                                          \
     allow_lds(k_zfwd_rk_hd<n_>, lds);                                                       \
-    hipLaunchKernelGGL((k_zfwd_rk_hd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
+    GHX_LAUNCH_CHECK((k_zfwd_rk_hd<n_>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zfwd_rk_hd<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -1522,13 +1621,13 @@ hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &r
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4));                                        \
-    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    const PassShape s = tile_shape(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4), max_lds());                                        \
+    const int L = s.lines; const size_t lds = s.lds;                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd_rk_mhd<n_, 0>, lds);                                                   \
     allow_lds(k_zfwd_rk_mhd<n_, 1>, lds);                                                   \
-    hipLaunchKernelGGL((k_zfwd_rk_mhd<n_, 0>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
-    hipLaunchKernelGGL((k_zfwd_rk_mhd<n_, 1>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
+    GHX_LAUNCH_CHECK((k_zfwd_rk_mhd<n_, 0>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zfwd_rk_mhd<n_, 0>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
+    GHX_LAUNCH_CHECK((k_zfwd_rk_mhd<n_, 1>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zfwd_rk_mhd<n_, 1>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -1550,22 +1649,25 @@ hipError_t yinv(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    if constexpr (FFTCfgWide<n_>::E != FFTCfg<n_>::E) if (tile_want("GHX_YWIDE", 1) == 1) {  \
+    if constexpr (FFTCfgWide<n_>::E != FFTCfg<n_>::E)                                        \
+    if (tile_want("GHX_YWIDE", 1) == 1 &&                                                    \
+        wide_shape(FFTCfgWide<n_>::T, lds_line_stride(n_), g.nxh).lds <= max_lds()) {         \
       /* 16 points per lane: 8-line tiles from 256-lane blocks, two blocks per CU */       \
       constexpr int TW = FFTCfgWide<n_>::T;                                                  \
-      const int L = 8;                                                                       \
-      size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                           \
+      const PassShape s = wide_shape(TW, lds_line_stride(n_), g.nxh);                        \
+      const int L = s.lines; const size_t lds = s.lds;                                       \
       dim3 grid((g.nxh + L - 1) / L, g.nz, c.batch);                                         \
+      GHX_LAUNCH_CHECK((k_yinv<n_, FFTCfgWide<n_>>), s, kTileBound, TW, true);               \
       allow_lds(k_yinv<n_, FFTCfgWide<n_>>, lds);                                            \
       hipLaunchKernelGGL((k_yinv<n_, FFTCfgWide<n_>>), grid, dim3(L * TW), lds, c.stream, in, out, nf, g, nxp, \
                          c.tw_y, kxlim, prune);                                              \
       break;                                                                                 \
     }                                                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.nxh, tile_pick("GHX_LY", c.tile_y, 8));        \
-    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    const PassShape s = tile_shape(T, lds_line_stride(n_), g.nxh, tile_pick("GHX_LY", c.tile_y, 8), max_lds());        \
+    const int L = s.lines; const size_t lds = s.lds;                            \
     dim3 grid((g.nxh + L - 1) / L, g.nz, c.batch);                                          \
     allow_lds(k_yinv<n_>, lds);                                                             \
-    hipLaunchKernelGGL((k_yinv<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune); \
+    GHX_LAUNCH_CHECK((k_yinv<n_>), s, kTileBound, T, true); hipLaunchKernelGGL((k_yinv<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -1584,11 +1686,11 @@ hipError_t yfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_), g.nxh, tile_pick("GHX_LY", c.tile_y, 4));        \
-    size_t lds = (size_t)L * lds_line_stride(n_) * sizeof(cplx);                            \
+    const PassShape s = tile_shape(T, lds_line_stride(n_), g.nxh, tile_pick("GHX_LY", c.tile_y, 4), max_lds());        \
+    const int L = s.lines; const size_t lds = s.lds;                            \
     dim3 grid((g.nxh + L - 1) / L, g.nz, c.batch);                                          \
     allow_lds(k_yfwd<n_>, lds);                                                             \
-    hipLaunchKernelGGL((k_yfwd<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune); \
+    GHX_LAUNCH_CHECK((k_yfwd<n_>), s, kTileBound, T, true); hipLaunchKernelGGL((k_yfwd<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -1615,10 +1717,10 @@ hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out,
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
     const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                      \
-    hipLaunchKernelGGL((k_xinv2<n_>), gr, bl, lds, c.stream, in, rs, nf, g, nxp, c.tw_x, kxlim);                                                                 \
+    GHX_LAUNCH_CHECK((k_xinv2<n_>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xinv2<n_>), gr, bl, lds, c.stream, in, rs, nf, g, nxp, c.tw_x, kxlim);                                                                 \
     return hipGetLastError();                                                               \
   }
       GHX_X2_SIZES(GHX_CASE)
@@ -1631,9 +1733,9 @@ hipError_t xinv(const FFTCtx &c, int nf, const FieldSet &in, double *const *out,
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfgLow<n_>::T;                                                     \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
-    hipLaunchKernelGGL((k_xinv<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, rs, nf, g, nxp, c.tw_xh, c.tw_x, kxlim); \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
+    GHX_LAUNCH_CHECK((k_xinv<n_>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xinv<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, rs, nf, g, nxp, c.tw_xh, c.tw_x, kxlim); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -1655,10 +1757,10 @@ hipError_t xinv3i(const FFTCtx &c, const FieldSet &in, double *out, int prune) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
     const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                      \
-    hipLaunchKernelGGL((k_xinv3i2<n_>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_x, kxlim);                                                                 \
+    GHX_LAUNCH_CHECK((k_xinv3i2<n_>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xinv3i2<n_>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_x, kxlim);                                                                 \
     return hipGetLastError();                                                               \
   }
       GHX_X2_SIZES(GHX_CASE)
@@ -1671,9 +1773,9 @@ hipError_t xinv3i(const FFTCtx &c, const FieldSet &in, double *out, int prune) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfgLow<n_>::T;                                                     \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
-    hipLaunchKernelGGL((k_xinv3i<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
+    GHX_LAUNCH_CHECK((k_xinv3i<n_>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xinv3i<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -1696,10 +1798,10 @@ hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
     const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                      \
-    hipLaunchKernelGGL((k_xfwd2<n_>), gr, bl, lds, c.stream, rs, out, nf, g, nxp, c.tw_x);                                                                 \
+    GHX_LAUNCH_CHECK((k_xfwd2<n_>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xfwd2<n_>), gr, bl, lds, c.stream, rs, out, nf, g, nxp, c.tw_x);                                                                 \
     return hipGetLastError();                                                               \
   }
       GHX_X2_SIZES(GHX_CASE)
@@ -1712,9 +1814,9 @@ hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfgLow<n_>::T;                                                     \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
-    hipLaunchKernelGGL((k_xfwd<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, rs, out, nf, g, nxp, c.tw_xh, c.tw_x); \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
+    GHX_LAUNCH_CHECK((k_xfwd<n_>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xfwd<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, rs, out, nf, g, nxp, c.tw_xh, c.tw_x); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -1739,10 +1841,10 @@ hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet 
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
     const dim3 gr((unsigned)((nrows + rpb - 1) / rpb), c.batch), bl(rpb * T);                      \
-    hipLaunchKernelGGL((k_xcross2<n_>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_x, kxlim);                                                                 \
+    GHX_LAUNCH_CHECK((k_xcross2<n_>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xcross2<n_>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_x, kxlim);                                                                 \
     return hipGetLastError();                                                               \
   }
       GHX_X2_SIZES(GHX_CASE)
@@ -1755,14 +1857,14 @@ hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet 
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfgLow<n_>::T;                                                     \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
     const dim3 gr((unsigned)((nrows + rpb - 1) / rpb), c.batch), bl(rpb * T);               \
     switch (tile_want("GHX_XV", 1)) {                                                       \
-    case 4: hipLaunchKernelGGL((k_xcross<n_, 2, 1>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
+    case 4: GHX_LAUNCH_CHECK((k_xcross<n_, 2, 1>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xcross<n_, 2, 1>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
     /* 3 * 2^k rows: one wave per SIMD leaves the registers to hold the six */ \
     /* fields without scratch (7.5% faster on the ensemble's 48^3 and 96^3) */ \
-    default: hipLaunchKernelGGL((k_xcross<n_, (n_ % 3 == 0) ? 1 : 2, 0>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
+    default: GHX_LAUNCH_CHECK((k_xcross<n_, (n_ % 3 == 0) ? 1 : 2, 0>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xcross<n_, (n_ % 3 == 0) ? 1 : 2, 0>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); break; \
     }                                                                                       \
     break;                                                                                  \
   }
@@ -1790,10 +1892,10 @@ hipError_t xdot(const FFTCtx &c, const FieldSet &in, const FieldSet &out, int pr
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
     const dim3 gr((unsigned)((nrows + rpb - 1) / rpb)), bl(rpb * T);                      \
-    hipLaunchKernelGGL((k_xdot2<n_>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_x, kxlim);                                                                 \
+    GHX_LAUNCH_CHECK((k_xdot2<n_>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xdot2<n_>), gr, bl, lds, c.stream, in, out, g, nxp, c.tw_x, kxlim);                                                                 \
     return hipGetLastError();                                                               \
   }
       GHX_X2_SIZES(GHX_CASE)
@@ -1806,9 +1908,9 @@ hipError_t xdot(const FFTCtx &c, const FieldSet &in, const FieldSet &out, int pr
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfgLow<n_>::T;                                                     \
-    const int rpb = rows_per_block(T);                                                      \
-    size_t lds = (size_t)rpb * lds_line_stride(n_) * sizeof(cplx);                          \
-    hipLaunchKernelGGL((k_xdot<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); \
+    const PassShape s = row_shape(T, lds_line_stride(n_), max_lds());                                                      \
+    const int rpb = s.lines; const size_t lds = s.lds;                          \
+    GHX_LAUNCH_CHECK((k_xdot<n_>), s, kRowBound, T, false); hipLaunchKernelGGL((k_xdot<n_>), dim3((unsigned)((nrows + rpb - 1) / rpb)), dim3(rpb * T), lds, c.stream, in, out, g, nxp, c.tw_xh, c.tw_x, kxlim); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
@@ -1825,11 +1927,11 @@ hipError_t zfwd_rk_bouss(const FFTCtx &c, const FieldSet &in, const BoussRKParam
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
-    int L = pick_L(T, lds_line_stride(n_) + n_, g.ny, tile_want("GHX_LZ", 4));              \
-    size_t lds = (size_t)L * (lds_line_stride(n_) + n_) * sizeof(cplx);                     \
+    const PassShape s = tile_shape(T, lds_line_stride(n_) + n_, g.ny, tile_want("GHX_LZ", 4), max_lds());              \
+    const int L = s.lines; const size_t lds = s.lds;                     \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd_rk_bouss<n_>, lds);                                                    \
-    hipLaunchKernelGGL((k_zfwd_rk_bouss<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
+    GHX_LAUNCH_CHECK((k_zfwd_rk_bouss<n_>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zfwd_rk_bouss<n_>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
