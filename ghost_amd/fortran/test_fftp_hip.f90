@@ -7,6 +7,8 @@
 !    loop written as GHOST's main3D (C = v; DO o = ord,1,-1: hd_rkstep2).
 ! 2c. GHOST's MHD loop through mhd_rkstep2 on resident fields.
 ! 3. The resident solver with tracers attached.
+! 4. fftp3d_* on host arrays with 512-point x and z lines: OUTDIR/
+!    fortran_fftp_<nx>x<ny>x<nz>.bin holds the input and the r2c output.
 ! 2b. GHOST's diagnostic files (spectrum, entrans, heltrans, spectr1d, hdcheck,
 !    the spectra about z, the scalar and the Boussinesq diagnostics) from the
 !    resident fields, written to the working directory (spec2D to OUTDIR).
@@ -253,9 +255,44 @@ PROGRAM test_fftp_hip
   CALL fftp3d_destroy_plan(plancr)
   CALL ghx_pseudospec_finalize()
   WRITE(*, '(A,ES10.3,A)') 'ok   ffttime accumulated: ', ffttime, ' s'
+
+  ! -- 4. 512-point lines through GHOST's fftp calls ---------------------------
+  CALL fftp_shape(512, 8, 16)
+  CALL fftp_shape(16, 8, 512)
   WRITE(*, '(A)') 'ALL OK'
 
 CONTAINS
+
+  SUBROUTINE fftp_shape(mx, my, mz)
+    ! a fixed aperiodic real field, r2c, c2r round trip, both arrays to a file
+    INTEGER, INTENT(IN) :: mx, my, mz
+    TYPE(FFTPLAN) :: prc, pcr
+    REAL(KIND=GP), ALLOCATABLE, TARGET :: a(:,:,:), b(:,:,:)
+    COMPLEX(KIND=GP), ALLOCATABLE, TARGET :: s(:,:,:)
+    CHARACTER(LEN=64) :: fname
+    INTEGER :: ii, jj, kk, uu
+    CALL fftp3d_create_plan(prc, (/mx, my, mz/), FFTW_REAL_TO_COMPLEX, FFTW_ESTIMATE)
+    CALL fftp3d_create_plan(pcr, (/mx, my, mz/), FFTW_COMPLEX_TO_REAL, FFTW_ESTIMATE)
+    ALLOCATE(a(mx, my, mz), b(mx, my, mz), s(mz, my, mx/2 + 1))
+    DO kk = 1, mz
+      DO jj = 1, my
+        DO ii = 1, mx
+          a(ii, jj, kk) = SIN(0.37_GP*ii + 1.3_GP*jj*jj) + COS(0.011_GP*kk*ii) - 0.5_GP*SIN(0.7_GP*kk*kk + jj)
+        END DO
+      END DO
+    END DO
+    CALL fftp3d_real_to_complex(prc, a, s, 0)
+    CALL fftp3d_complex_to_real(pcr, s, b, 0)
+    WRITE(fname, '(A,I0,A,I0,A,I0,A)') 'fortran_fftp_', mx, 'x', my, 'x', mz, '.bin'
+    CALL report('fftp round trip '//TRIM(fname), MAXVAL(ABS(b/(REAL(mx, GP)*my*mz) - a)), 1.0E-13_GP)
+    OPEN(NEWUNIT=uu, FILE=TRIM(outdir)//'/'//TRIM(fname), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(uu) a
+    WRITE(uu) s
+    CLOSE(uu)
+    DEALLOCATE(a, b, s)
+    CALL fftp3d_destroy_plan(prc)
+    CALL fftp3d_destroy_plan(pcr)
+  END SUBROUTINE fftp_shape
 
   SUBROUTINE report(what, err, tol)
     CHARACTER(LEN=*), INTENT(IN) :: what

@@ -24,8 +24,10 @@ def to_np(t):
     return t.detach().cpu().numpy()
 
 
-SIZES = [32, 48, 32, 64, 48, 32, 64]
-NUS = [4e-3, 3e-3, 6e-3, 2e-3, 5e-3, 8e-3, 3e-3]
+# all four member sizes of the 96-member workload, 96^3 twice so that the
+# batched 96-point kernels see a member index above zero
+SIZES = [32, 48, 32, 64, 96, 48, 32, 64, 96]
+NUS = [4e-3, 3e-3, 6e-3, 2e-3, 1.5e-3, 5e-3, 8e-3, 3e-3, 2.5e-3]
 
 
 def members():
@@ -52,7 +54,7 @@ def test_ensemble_members_match_oracle(gpu):
     from ghost_amd.solvers import Ensemble
     init = members()
     e = make(Ensemble, init)
-    assert sorted(e.groups()) == [(32, 3), (48, 2), (64, 2)]
+    assert sorted(e.groups()) == [(32, 3), (48, 2), (64, 2), (96, 2)]
     e.step(6)
     for m, (og, v0) in enumerate(init):
         ref = [x.copy() for x in v0]
@@ -77,7 +79,7 @@ def test_ensemble_graph_replay_equals_eager(gpu):
         junk = torch.randn(1 << 20, device="cuda").sum()  # unrelated eager work between replays
         torch.cuda.synchronize()
         del junk
-        for m in (0, 1, 3):
+        for m in (0, 1, 3, 4, 8):
             a, b = eg.velocity(m), ee.velocity(m)
             for k in range(3):
                 assert torch.equal(a[k], b[k]), (rep, m, k)
@@ -89,7 +91,7 @@ def test_ensemble_autotune_keeps_states_and_results(gpu):
     init = members()
     ea = make(Ensemble, init)
     tiles = ea.autotune(steps=2)
-    assert len(tiles) == 3 and all(tz > 0 and ty > 0 for tz, ty in tiles)
+    assert len(tiles) == 4 and all(tz > 0 and ty > 0 for tz, ty in tiles)
     eb = make(Ensemble, init)
     ea.step(3)
     eb.step(3)

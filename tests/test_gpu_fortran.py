@@ -112,6 +112,16 @@ def test_fortran_host_side(gpu, tmp_path):
         r = np.fromfile(tmp_path / f"{name}.0001.out").reshape((n, n, n), order="F").transpose(2, 1, 0)
         want = og.c2r(got[k] * (1.0 / n ** 3))
         assert np.linalg.norm(r - want) <= 1e-13 * max(np.linalg.norm(want), 1e-300) + 1e-16
+    # fftp3d_real_to_complex on host arrays with 512-point x and z lines
+    for shape in ((512, 8, 16), (16, 8, 512)):
+        mx, my, mz = shape
+        raw = np.fromfile(tmp_path / f"fortran_fftp_{mx}x{my}x{mz}.bin", dtype=np.float64)
+        nr = mx * my * mz
+        a = raw[:nr].reshape(mz, my, mx)  # Fortran a(mx,my,mz) read in C order
+        s = raw[nr:].view(np.complex128).reshape(mx // 2 + 1, my, mz)  # s(mz,my,mx/2+1)
+        ogs = O.Grid(mx, my, mz)
+        want = ogs.r2c(a)
+        assert np.linalg.norm(s - want) <= 1e-13 * np.linalg.norm(want), shape
     # resident solver with 1000 tracers attached, 5 steps, GHOST's z stencil
     raw = np.fromfile(tmp_path / "fortran_tracers32.bin", dtype=np.float64)
     p0 = raw[:3000].reshape(3, 1000)  # Fortran pos(npart, 3): column c is axis c

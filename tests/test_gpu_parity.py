@@ -10,7 +10,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 8, 8), (6, 4, 10), (12, 12, 12), (16, 8, 32), (32, 32, 32), (48, 48, 48), (64, 64, 64),
-          (96, 96, 96), (128, 128, 128), (256, 64, 128), (20, 24, 6), (16, 512, 8), (24, 512, 16)]
+          (96, 96, 96), (128, 128, 128), (256, 64, 128), (20, 24, 6), (16, 512, 8), (24, 512, 16),
+          # 512-point x and z lines (the drop-in's k_xfwd2/k_xinv2<512>, k_zfwd/k_zinv<512>)
+          (512, 8, 16), (16, 8, 512), (512, 16, 512),
+          # every other compiled schedule: 192, 384, 768 and 1024 points on each axis
+          (384, 192, 8), (16, 384, 192), (768, 8, 16), (8, 8, 384), (16, 8, 1024), (8, 1024, 8), (2048, 8, 8)]
+
+# the two configurations' own grids (one transform each way, oracle on all threads)
+FULL_SHAPES = [(512, 512, 512), (512, 512, 256)]
 
 
 def rel(a, b):
@@ -59,6 +66,25 @@ def test_r2c_c2r_match_oracle(gpu, shape):
     assert rel(to_np(back), og.c2r(c_ref)) < 1e-13
     assert rel(to_np(back) / (shape[0] * shape[1] * shape[2]), r) < 1e-13
     assert bool((cin == keep).all()), "c2r must preserve its input"
+
+
+@pytest.mark.parametrize("shape", FULL_SHAPES)
+def test_r2c_c2r_full_size_match_oracle(gpu, shape):
+    """The plain transforms at the workload sizes (1 GB per field at 512^3)."""
+    import torch
+    from oracle import oracle as O
+    O.set_threads(16)
+    g, og = grids(shape)
+    r = rand_real(og, 3)
+    c_ref = og.r2c(r)
+    c = g.r2c(to_gpu(r))
+    assert rel(to_np(c), c_ref) < 1e-13
+    del c
+    back = g.c2r(to_gpu(c_ref))
+    del c_ref
+    torch.cuda.empty_cache()
+    n3 = shape[0] * shape[1] * shape[2]
+    assert rel(to_np(back) / n3, r) < 1e-13
 
 
 def test_fftp_module_interface(gpu):
@@ -182,7 +208,8 @@ def test_randu_matches_oracle(gpu):
 
 
 @pytest.mark.parametrize("shape", [(32, 32, 32), (24, 16, 12), (64, 64, 64), (16, 16, 512), (8, 12, 512),
-                                   (16, 512, 16)])
+                                   (16, 512, 16), (48, 48, 48), (96, 96, 96), (96, 32, 96), (96, 48, 24),
+                                   (192, 16, 16), (16, 384, 32)])
 def test_hd_substep_fused(gpu, shape):
     g, og = grids(shape)
     v, _ = og.initialv_k4exp(2.5, 1.0, 2024)
@@ -347,3 +374,29 @@ def test_mhd_512x512x256_step_matches_oracle(gpu):
     for k in range(3):
         assert rel(gv[k], v0[k]) < 1e-12
         assert rel(ga[k], a0[k]) < 1e-12
+
+
+def test_mhd_pruned_equals_full_path(gpu):
+    """The MHD chain with and without dealias pruning on a dealiased state
+    (velocity and vector potential from the k^4 exp fill), 4 forced steps."""
+    import os
+    from ghost_amd.solvers import MHDSolver
+    g, og = grids((48, 48, 48))
+    v0, _ = og.initialv_k4exp(3.0, 1.0, 515)
+    a0, _ = og.initialv_k4exp(3.0, 0.5, 516)
+    f0, _ = og.initialfv_random(2.0, 3.0, 0.3, 517)
+    outs = []
+    for env in (None, "1"):
+        if env:
+            os.environ["GHX_NO_PRUNE"] = env
+        try:
+            s = MHDSolver(g, dt=2e-3, nu=4e-3, mu=4e-3, ord=2)
+            s.set_state([to_gpu(x) for x in v0], [to_gpu(x) for x in a0])
+            s.set_forcing([to_gpu(x) for x in f0], None)
+            s.step(4)
+            gv, ga = s.state()
+            outs.append([to_np(x) for x in gv + ga])
+        finally:
+            os.environ.pop("GHX_NO_PRUNE", None)
+    for k in range(6):
+        assert rel(outs[0][k], outs[1][k]) < 1e-14, k
