@@ -1,12 +1,13 @@
 // tracers.hip -- Lagrangian tracer kernels on gfx950.
 //
-// One lane per particle.  The stencil and weights follow
+// The stencil and weights follow
 // GPSplineInt_PartUpdate3D / GPSplineInt_Interp3D (lagpart/gpsplineint_mod.fpp:
-// 227-480, 560-667) and are evaluated in GHOST's operand order without fused
-// multiply-adds, so that the interpolant differs from GHOST's only through the
-// control points.  The control points come from the spectral prefilter of the
+// 227-480, 560-667); weights and terms are evaluated in GHOST's operand order
+// without fused multiply-adds.  k_spline_interp also adds the 64 terms in
+// GHOST's order; the tracer stage adds them in a tree (see there), which
+// changes the interpolant by rounding only.  The control points come from the spectral prefilter of the
 // c2r (one division per mode in place of GHOST's three cyclic tridiagonal
-// sweeps), so the kernel only gathers: 4x4x4 points per component.
+// sweeps), so the kernels only gather: 4x4x4 points per component.
 #include "tracers.h"
 
 namespace ghx {
@@ -120,23 +121,68 @@ GHX_DEV void gather3(const double *__restrict__ c, const Stencil &s, int nx, siz
 // GPart_MakePeriodicP (gparts_mod.f90:2270-2318): modulo(p + 2 n, n).
 GHX_DEV double wrap(double p, double n) { return fmod(add(p, mul(2.0, n)), n); }
 
-__global__ __launch_bounds__(256) void k_tracer_stage(TracerStage p) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= p.np) return;
-  const size_t nxy = (size_t)p.nx * p.ny;
+// One stage of the tracer RK step, 16 lanes per particle.  The 4 x 4 stencil
+// rows of the three interleaved components are 16 rows of 12 doubles (96
+// contiguous bytes unless the row wraps in x), 192 doubles in all: lane r of
+// the particle's group loads doubles r, r + 16, ..., r + 176, so neighbouring
+// lanes read neighbouring doubles and each cache line of a row is requested
+// once per load instruction (one lane per particle asked for each of the 192
+// doubles separately).  Double d of row w is component (d mod 3) of stencil
+// point d / 3 on row (y index w mod 4, z index w / 4); since 16 = 1 mod 3,
+// the component of load m on lane r is (r + m) mod 3.  Each lane sums its
+// weighted terms per component, and the group adds its 16 partial sums with
+// a butterfly; the interpolant is GHOST's 64-term sum, added in that order
+// within a lane and pairwise across lanes.
+constexpr int kStageLanes = 16;
+constexpr int kStageBlock = 256;
+
+GHX_DEV int sel4(int i, const int v[4]) { return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3])); }
+GHX_DEV double sel4(int i, const double v[4]) { return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3])); }
+
+__global__ __launch_bounds__(kStageBlock) void k_tracer_stage(TracerStage p) {
+  const int r = threadIdx.x % kStageLanes;
+  const long long q = (long long)blockIdx.x * (kStageBlock / kStageLanes) + threadIdx.x / kStageLanes;
+  if (q >= p.np) return;  // whole groups only: the butterfly stays inside a group
+  const size_t nx = (size_t)p.nx, ny = (size_t)p.ny;
   Stencil s;
   make_stencil(s, p.pos[0][q], p.pos[1][q], p.pos[2][q], p.nx, p.ny, p.nz, p.zcompat);
+  double v[12];
+#pragma unroll
+  for (int m = 0; m < 12; ++m) {
+    const int sl = r + kStageLanes * m;  // 0..191
+    const int w = sl / 12, d = sl - 12 * (sl / 12);
+    const int pt = d / 3, comp = d - 3 * (d / 3);
+    const size_t row = (size_t)sel4(w >> 2, s.iz) * ny + (size_t)sel4(w & 3, s.iy);
+    v[m] = p.coef3[(row * nx + (size_t)sel4(pt, s.ix)) * 3 + comp];
+  }
+  double b[3] = {0.0, 0.0, 0.0};  // b[t]: component (r + t) mod 3
+#pragma unroll
+  for (int m = 0; m < 12; ++m) {
+    const int sl = r + kStageLanes * m;
+    const int w = sl / 12, d = sl - 12 * (sl / 12);
+    const int pt = d / 3;
+    b[m % 3] = add(b[m % 3], mul(mul(mul(v[m], sel4(pt, s.wx)), sel4(w & 3, s.wy)), sel4(w >> 2, s.wz)));
+  }
+  const int r3 = r % 3;
   double u[3];
-  gather3(p.coef3, s, p.nx, nxy, u);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int t = (c - r3 + 3) % 3;
+    double a = t == 0 ? b[0] : (t == 1 ? b[1] : b[2]);
+#pragma unroll
+    for (int o = kStageLanes / 2; o >= 1; o >>= 1) a = add(a, __shfl_xor(a, o, kStageLanes));
+    u[c] = a;
+  }
+  if (r != 0) return;
   // GPart_StepRKK (gparts_mod.f90:2046-2145): x = x0 + dtfact u, then x and
   // y wrapped there and again in GPart_EndStageRKK (:1911-2043), z once.
   double x = add(p.x0[0][q], mul(p.dtf[0], u[0]));
   double y = add(p.x0[1][q], mul(p.dtf[1], u[1]));
   double z = add(p.x0[2][q], mul(p.dtf[2], u[2]));
-  const double nx = (double)p.nx, ny = (double)p.ny, nz = (double)p.nz;
-  x = wrap(wrap(x, nx), nx);
-  y = wrap(wrap(y, ny), ny);
-  z = wrap(z, nz);
+  const double fnx = (double)p.nx, fny = (double)p.ny, fnz = (double)p.nz;
+  x = wrap(wrap(x, fnx), fnx);
+  y = wrap(wrap(y, fny), fny);
+  z = wrap(z, fnz);
   p.pos[0][q] = x;
   p.pos[1][q] = y;
   p.pos[2][q] = z;
@@ -171,7 +217,8 @@ inline unsigned blocks_for(long long np) { return (unsigned)((np + 255) / 256); 
 
 hipError_t tracer_stage(hipStream_t s, const TracerStage &p) {
   if (p.np <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tracer_stage, dim3(blocks_for(p.np)), dim3(256), 0, s, p);
+  constexpr long long per_block = kStageBlock / kStageLanes;
+  hipLaunchKernelGGL(k_tracer_stage, dim3((unsigned)((p.np + per_block - 1) / per_block)), dim3(kStageBlock), 0, s, p);
   return hipGetLastError();
 }
 
