@@ -195,6 +195,9 @@ def _declare(L):
         "ghx_prof_enable": (_I, [_I]),
         "ghx_prof_reset": (_I, []),
         "ghx_prof_read": (_I, [_I, C.POINTER(C.c_char_p), C.POINTER(_I), _DP, _DP]),
+        "ghx_stream_probe": (_I, [_I, _P, _P, C.c_size_t, _I, _P]),
+        "ghx_pass_shape": (_I, [_I, _I, _I, _I, C.c_longlong, C.POINTER(_I), C.POINTER(_I),
+                                C.POINTER(C.c_longlong), C.POINTER(_I)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

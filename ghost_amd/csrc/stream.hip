@@ -1,0 +1,75 @@
+// stream.hip -- the achievable-bandwidth yardstick: plain copy, read and write
+// kernels over large device buffers, timed by the benchmark beside the FFT
+// passes.  Each lane moves 16-byte vectors in a grid-stride loop, four in
+// flight per trip; `nt` selects streaming (nontemporal) stores.
+#include <hip/hip_runtime.h>
+
+#include "../../include/ghx.h"
+
+namespace {
+
+typedef double v2 __attribute__((ext_vector_type(2)));
+constexpr int kBlock = 256;
+constexpr int kUnroll = 4;
+
+template <bool NT> __device__ void put(v2 *p, v2 v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <bool NT>
+__global__ __launch_bounds__(kBlock) void k_copy(v2 *__restrict__ dst, const v2 *__restrict__ src, size_t n) {
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + (kUnroll - 1) * stride < n; i += kUnroll * stride) {
+    v2 a[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) a[u] = src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) put<NT>(dst + i + u * stride, a[u]);
+  }
+  for (; i < n; i += stride) put<NT>(dst + i, src[i]);
+}
+
+// Sums what it reads; the sum is stored only if it equals a value no input
+// here produces, so the loads cannot be dropped and nothing is written.
+__global__ __launch_bounds__(kBlock) void k_read(const v2 *__restrict__ src, size_t n, double *sink) {
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  v2 acc = {0.0, 0.0};
+  for (; i + (kUnroll - 1) * stride < n; i += kUnroll * stride) {
+    v2 a[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) a[u] = src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) acc += a[u];
+  }
+  for (; i < n; i += stride) acc += src[i];
+  if (acc.x + acc.y == -1.2345678e300) *sink = acc.x;
+}
+
+template <bool NT>
+__global__ __launch_bounds__(kBlock) void k_write(v2 *__restrict__ dst, size_t n, double val) {
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  const v2 v = {val, val};
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) put<NT>(dst + i, v);
+}
+
+}  // namespace
+
+extern "C" int ghx_stream_probe(int kind, void *dst, const void *src, size_t bytes, int blocks, void *stream) {
+  if (bytes < 16 || bytes % 16 || blocks < 1 || !dst || (kind != 2 && kind != 4 && !src)) return GHX_ERR_ARG;
+  if (kind < 0 || kind > 4) return GHX_ERR_ARG;
+  const size_t n = bytes / 16;
+  hipStream_t s = (hipStream_t)stream;
+  v2 *d = (v2 *)dst;
+  const v2 *a = (const v2 *)src;
+  switch (kind) {
+    case 0: hipLaunchKernelGGL(k_copy<false>, dim3(blocks), dim3(kBlock), 0, s, d, a, n); break;
+    case 3: hipLaunchKernelGGL(k_copy<true>, dim3(blocks), dim3(kBlock), 0, s, d, a, n); break;
+    case 1: hipLaunchKernelGGL(k_read, dim3(blocks), dim3(kBlock), 0, s, a, n, (double *)dst); break;
+    case 2: hipLaunchKernelGGL(k_write<false>, dim3(blocks), dim3(kBlock), 0, s, d, n, 1.0); break;
+    case 4: hipLaunchKernelGGL(k_write<true>, dim3(blocks), dim3(kBlock), 0, s, d, n, 1.0); break;
+  }
+  return hipGetLastError() == hipSuccess ? GHX_OK : GHX_ERR_HIP;
+}

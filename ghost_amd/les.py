@@ -196,46 +196,66 @@ class TrainingSet:
 
 
 class Experiment:
-    """The a-posteriori comparison: a forced DNS on the fine grid (GHOST's
-    rand=1 forcing, phase shift every fstep steps) spins up, leaves filtered
-    training states behind, then runs a window while its filtered spectrum is
-    sampled; LES runs on the coarse grid over the same window from the
-    filtered state, with the trained closure and without it, with the same
-    forcing and phase schedule and ratio times the DNS time step.  Errors are
-    the mean over shells 1..kc of the relative error of the time-averaged
-    spectra (and of the spectra at the end of the window)."""
+    """The a-posteriori comparison.  Two forced DNS on the fine grid (GHOST's
+    rand=1 forcing, phase shift every fstep steps) with different seeds: the
+    training DNS spins up and leaves the filtered training states and the
+    calibration spectra behind; the scored DNS spins up, then runs a window
+    while its filtered spectrum is sampled.  LES run on the coarse grid over
+    the scored window from its filtered state, with the trained closure and
+    without it, with the scored DNS's forcing and phase schedule and ratio
+    times the DNS time step.  Errors are the mean over shells 1..kc of the
+    relative error of the time-averaged spectra (and of the spectra at the end
+    of the window) against the scored DNS, which neither the training nor the
+    calibration sees.  The run-time check of the fallback compares the LES
+    with the training DNS's filtered spectrum instead (the scored DNS is not
+    known to an LES that replaces it)."""
 
     def __init__(self, nfine, nles, kc, nu, dt, spin, snaps, window, ratio=2, f0=0.4, fstep=20, k0=3.0, u0=1.0,
-                 ic_seed=1000, f_seed=2000, samples=8, cal_steps=(), log=print):
+                 ic_seed=1000, f_seed=2000, samples=8, cal_steps=(), train_seeds=None, log=print):
+        if train_seeds is None:
+            train_seeds = (ic_seed + 1, f_seed + 1)
         self.__dict__.update(dict(nfine=nfine, nles=nles, kc=kc, nu=nu, dt=dt, spin=spin, snaps=set(snaps),
                                   window=window, ratio=ratio, f0=f0, fstep=fstep, k0=k0, u0=u0, ic_seed=ic_seed,
-                                  f_seed=f_seed, samples=samples, cal_steps=sorted(cal_steps), log=log))
+                                  f_seed=f_seed, samples=samples, cal_steps=sorted(cal_steps),
+                                  train_seeds=tuple(train_seeds), log=log))
         assert window % (ratio * samples) == 0, "window must hold whole LES steps per sample"
 
-    def run_dns(self):
+    def _dns(self, ic_seed, f_seed):
+        """A forced DNS on the fine grid and its step function; the phases it
+        draws are appended to the returned list."""
         from .solvers import HDSolver
         from .pseudospec import randu
-        import time
-        self.fine, self.small = Grid(self.nfine), Grid(self.nles)
         s = HDSolver(self.fine, dt=self.dt, nu=self.nu, ord=2)
-        v, _ = self.fine.initialv_k4exp(self.k0, self.u0, self.ic_seed)
-        f, fseed = self.fine.initialfv_random(2.0, 3.0, self.f0, self.f_seed)
+        v, _ = self.fine.initialv_k4exp(self.k0, self.u0, ic_seed)
+        f, fseed = self.fine.initialfv_random(2.0, 3.0, self.f0, f_seed)
         s.set_velocity(v)
         s.set_forcing(f)
         del v, f
         st = {"timef": self.fstep, "seed": fseed}
-        self.phases = []
+        phases = []
 
         def step():
             if st["timef"] == self.fstep:
                 st["timef"] = 0
                 r, st["seed"] = randu(st["seed"])
                 ph = 2 * math.pi * r
-                self.phases.append(ph)
+                phases.append(ph)
                 s.phaseshift_forcing(complex(math.cos(ph), math.sin(ph)))
             s.step(1)
             st["timef"] += 1
 
+        return s, step, phases, st
+
+    def run_dns(self):
+        """The training DNS, then the scored DNS."""
+        self.run_training_dns()
+        self.run_scored_dns()
+
+    def run_training_dns(self):
+        import time
+        self.fine, self.small = Grid(self.nfine), Grid(self.nles)
+        s, step, phases, st = self._dns(*self.train_seeds)
+        self.phases_train = phases
         self.train = TrainingSet(self.small, self.kc)
         self.cal_ref = []
         cal = set(self.cal_steps)
@@ -246,22 +266,55 @@ class Experiment:
             if it in self.snaps:
                 self.train.add(self.fine, s.velocity())
             if it in cal:
-                # the spin-up's filtered spectra, and the state and forcing at the
-                # first of these steps, for the a-posteriori calibration
+                # the filtered spectra, and the state and forcing at the first of
+                # these steps, for the a-posteriori calibration
                 _, vf = sgs_targets(self.fine, self.small, s.velocity(), self.kc)
                 if it == self.cal_steps[0]:
                     self.cal_v = vf
                     self.cal_f = filter_forcing(self.fine, self.small, s.forcing(), self.kc)
-                    self.cal_phase, self.cal_timef = len(self.phases), st["timef"]
+                    self.cal_phase, self.cal_timef = len(phases), st["timef"]
                 else:
                     self.cal_ref.append(spectrum(self.small, vf))
+        torch.cuda.synchronize()
+        self.t_spin_train = time.perf_counter() - t0
+        # the reference an LES can be checked against at run time: the training
+        # DNS's filtered spectrum, time-averaged over the calibration steps
+        self.ref_runtime = np.mean(self.cal_ref, axis=0) if self.cal_ref else None
+        self.log(f"training DNS {self.nfine}^3 (seeds {self.train_seeds}): {self.spin} steps "
+                 f"{self.t_spin_train:.1f} s, {len(self.train.snaps)} training states")
+        del s
+        torch.cuda.empty_cache()
+
+    def run_scored_dns(self):
+        import time
+        if not hasattr(self, "fine"):
+            self.fine, self.small = Grid(self.nfine), Grid(self.nles)
+        s, step, phases, st = self._dns(self.ic_seed, self.f_seed)
+        self.phases = phases
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(self.spin):
+            step()
         torch.cuda.synchronize()
         self.t_spin = time.perf_counter() - t0
         _, self.v_start = sgs_targets(self.fine, self.small, s.velocity(), self.kc)
         self.f_start = filter_forcing(self.fine, self.small, s.forcing(), self.kc)
-        self.phase0, self.timef0 = len(self.phases), st["timef"]
+        self.phase0, self.timef0 = len(phases), st["timef"]
+        # the fine state at the window's start, for the DNS fallback
+        self.ckpt = ([x.clone() for x in s.velocity()], [x.clone() for x in s.forcing()], st["timef"], st["seed"],
+                     len(phases))
+        self.ref, self.t_dns = self._dns_window(s, step)
+        self.log(f"scored DNS {self.nfine}^3 (seeds {(self.ic_seed, self.f_seed)}): spin-up {self.spin} steps "
+                 f"{self.t_spin:.1f} s, window {self.window} steps {self.t_dns:.2f} s")
+        del s
+        torch.cuda.empty_cache()
+
+    def _dns_window(self, s, step):
+        """The window: filtered spectra every window/samples steps, and the GPU
+        seconds of the DNS steps themselves."""
+        import time
         every = self.window // self.samples
-        self.ref = []
+        ref = []
         t_sample = 0.0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -271,15 +324,36 @@ class Experiment:
                 torch.cuda.synchronize()
                 ta = time.perf_counter()
                 _, vf = sgs_targets(self.fine, self.small, s.velocity(), self.kc)
-                self.ref.append(spectrum(self.small, vf))
+                ref.append(spectrum(self.small, vf))
                 torch.cuda.synchronize()
                 t_sample += time.perf_counter() - ta
         torch.cuda.synchronize()
-        self.t_dns = time.perf_counter() - t0 - t_sample  # GPU-seconds of the DNS over the window
-        self.log(f"DNS {self.nfine}^3: spin-up {self.spin} steps {self.t_spin:.1f} s, window {self.window} steps "
-                 f"{self.t_dns:.2f} s")
+        return ref, time.perf_counter() - t0 - t_sample
+
+    def fallback_dns(self):
+        """The plain DNS over the window from the checkpoint at its start, as
+        the fallback runs it: returns (filtered spectra, GPU seconds)."""
+        from .solvers import HDSolver
+        from .pseudospec import randu
+        v, f, timef, seed, nph = self.ckpt
+        s = HDSolver(self.fine, dt=self.dt, nu=self.nu, ord=2)
+        s.set_velocity(v)
+        s.set_forcing(f)
+        st = {"timef": timef, "seed": seed}
+
+        def step():
+            if st["timef"] == self.fstep:
+                st["timef"] = 0
+                r, st["seed"] = randu(st["seed"])
+                ph = 2 * math.pi * r
+                s.phaseshift_forcing(complex(math.cos(ph), math.sin(ph)))
+            s.step(1)
+            st["timef"] += 1
+
+        out = self._dns_window(s, step)
         del s
         torch.cuda.empty_cache()
+        return out
 
     def fit(self, steps=3000, batch=65536, lr=1e-3, seed=7, match=True):
         import time
@@ -309,7 +383,7 @@ class Experiment:
                  f"30, 40: {nu[9]:.2e}, {nu[19]:.2e}, {nu[29]:.2e}, {nu[39]:.2e}")
         return nu
 
-    def _les(self, v0, f0, phase0, timef0, dns_steps, every_dns, closure, nuk, ratio):
+    def _les(self, v0, f0, phases, phase0, timef0, dns_steps, every_dns, closure, nuk, ratio):
         """LES from state v0 (forcing f0, the DNS's phase schedule from index
         phase0 with timef0) over dns_steps DNS steps: spectra of the sharp-
         filtered velocity every every_dns DNS steps and the GPU seconds of the
@@ -334,7 +408,7 @@ class Experiment:
             for _ in range(ratio):  # the DNS's phase shifts that fall in this step
                 if st["timef"] == self.fstep:
                     st["timef"] = 0
-                    ph = self.phases[st["pi"]]
+                    ph = phases[st["pi"]]
                     st["pi"] += 1
                     L.phaseshift_forcing(complex(math.cos(ph), math.sin(ph)))
                 st["timef"] += 1
@@ -351,10 +425,9 @@ class Experiment:
         return spectra, time.perf_counter() - t0 - t_sample
 
     def calibrate(self, nuk, closure=True, iters=4, gamma=1.0, ratio=None):
-        """A-posteriori fit of the spectral part on the spin-up, before the
-        window: LES from the filtered state at the first calibration step over
-        the rest of the spin-up, against the filtered DNS spectra sampled
-        there; each pass scales the effective viscosity nu(k) + nu_mol by
+        """A-posteriori fit of the spectral part on the training DNS: LES from
+        its filtered state at the first calibration step over the rest of its
+        spin-up, against its filtered spectra sampled there; each pass scales the effective viscosity nu(k) + nu_mol by
         (E_les/E_dns)^gamma on the shells 1..kc (shells past kc take the value
         at kc)."""
         ratio = ratio or self.ratio
@@ -365,8 +438,8 @@ class Experiment:
         nu = np.array(nuk, dtype=np.float64)
         kc = int(self.kc)
         for it in range(iters):
-            spectra, _ = self._les(self.cal_v, self.cal_f, self.cal_phase, self.cal_timef, span, every, closure, nu,
-                                   ratio)
+            spectra, _ = self._les(self.cal_v, self.cal_f, self.phases_train, self.cal_phase, self.cal_timef, span,
+                                   every, closure, nu, ratio)
             if not all(np.all(np.isfinite(e)) for e in spectra):
                 self.log(f"calibration pass {it + 1}: the LES blew up; stopping with the previous nu")
                 break
@@ -381,12 +454,17 @@ class Experiment:
                      f"{kc}: {r[29]:.3f}, {r[39]:.3f}, {r[kc - 1]:.3f}")
         return nu
 
-    def run_les(self, closure=True, nuk=None, ratio=None, label=None):
+    def run_les(self, closure=True, nuk=None, ratio=None, label=None, fallback_tol=None):
         """closure: attach the network; nuk: the spectral part (or None);
-        ratio: LES steps per DNS step (default the experiment's)."""
+        ratio: LES steps per DNS step (default the experiment's).  With
+        fallback_tol, the LES's time-averaged spectrum is checked against the
+        training DNS's (the reference known at run time); above the tolerance
+        the plain DNS runs the window from its checkpoint and its result is
+        the run's result."""
+        import time
         ratio = ratio or self.ratio
         assert self.window % (ratio * self.samples) == 0, "window must hold whole LES steps per sample"
-        spectra, elapsed = self._les(self.v_start, self.f_start, self.phase0, self.timef0, self.window,
+        spectra, elapsed = self._les(self.v_start, self.f_start, self.phases, self.phase0, self.timef0, self.window,
                                      self.window // self.samples, closure, nuk, ratio)
         ok = all(np.all(np.isfinite(e)) for e in spectra)
         mean_err = spectrum_error(np.mean(spectra, axis=0), np.mean(self.ref, axis=0), self.kc) if ok else math.inf
@@ -398,5 +476,22 @@ class Experiment:
             ea, er = np.mean(spectra, axis=0), np.mean(self.ref, axis=0)
             shells = [k for k in (1, 2, 3, 5, 8, 12, 16, 20, 25, 30, 35, 40, int(self.kc)) if k <= self.kc]
             self.log("  E_les/E_dns at k = " + ", ".join(f"{k}: {ea[k - 1] / er[k - 1]:.3f}" for k in shells))
-        return {"seconds": elapsed, "steps": self.window // ratio, "spectrum_error_mean": mean_err,
-                "spectrum_error_end": end_err, "finite": bool(ok)}
+        out = {"seconds": elapsed, "steps": self.window // ratio, "spectrum_error_mean": mean_err,
+               "spectrum_error_end": end_err, "finite": bool(ok)}
+        if fallback_tol is not None:
+            check = (spectrum_error(np.mean(spectra, axis=0), self.ref_runtime, self.kc)
+                     if ok and self.ref_runtime is not None else math.inf)
+            out["runtime_check_error"] = check
+            out["fallback"] = bool(not (check <= fallback_tol))
+            if out["fallback"]:
+                self.log(f"  run-time check {check:.4f} > {fallback_tol}: falling back to the {self.nfine}^3 DNS")
+                t0 = time.perf_counter()
+                dspec, dsec = self.fallback_dns()
+                out["fallback_dns_seconds"] = dsec
+                out["fallback_wall_seconds"] = time.perf_counter() - t0
+                out["result_error_mean"] = spectrum_error(np.mean(dspec, axis=0), np.mean(self.ref, axis=0), self.kc)
+                out["result_seconds"] = elapsed + dsec
+            else:
+                out["result_error_mean"] = mean_err
+                out["result_seconds"] = elapsed
+        return out

@@ -476,6 +476,25 @@ int ghx_prof_enable(int on);
 int ghx_prof_reset(void);
 int ghx_prof_read(int slot, const char **name, int *count, double *total_ms, double *total_bytes);
 
+/* Bandwidth yardstick (no counterpart in GHOST): plain kernels over `bytes`
+ * (a multiple of 16) on `blocks` blocks of 256 lanes, asynchronous on the
+ * stream.  kind 0: copy src -> dst; 1: read src (dst: an 8-byte device sink that
+ * the kernel writes only for a sum no input produces); 2: write dst;
+ * 3: copy with streaming stores; 4: write with streaming stores. */
+int ghx_stream_probe(int kind, void *dst, const void *src, size_t bytes, int blocks, void *stream);
+
+/* Launch shape of an FFT pass (no counterpart in GHOST; host-only, no GPU
+ * needed): kind 0 z/y tile passes, 1 Boussinesq z forward, 2 wide y inverse,
+ * 3 two-for-one x passes (n = nx), 4 half-length x passes (n = nx/2); for a
+ * line length n, nlines lines along the tiled axis, a requested tile width
+ * and an LDS limit in bytes.  Returns 0 when the shape passes the check every
+ * launcher applies before it launches (lanes within the kernel's launch
+ * bounds, whole lines, tile widths a power of two up to 32, LDS within the
+ * limit), -2 when it would be refused, -1 for a size or kind without a
+ * kernel. */
+int ghx_pass_shape(int kind, int n, int nlines, int want, long long ldsmax, int *lines, int *block,
+                   long long *lds, int *lanes);
+
 #ifdef __cplusplus
 }
 #endif
