@@ -86,6 +86,7 @@ def _declare(L):
         "ghx_maxabs": (_I, [_P, _P, _P, _P, _I, _DP]),
         "ghx_divergence": (_I, [_P, _P, _P, _P, _DP]),
         "ghx_spectrumc": (_I, [_P, _P, _P, _P, _I, _I, _DP, _DP]),
+        "ghx_spectrumc_noalloc": (_I, [_P, _P, _P, _P, _DP]),
         "ghx_phaseshift": (_I, [_P, _P, _D, _D]),
         "ghx_normalize": (_I, [_P, _P, _P, _P, _D, _I]),
         "ghx_sharp": (_I, [_P, _P, _P, _D]),

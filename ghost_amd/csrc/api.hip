@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -760,6 +761,56 @@ extern "C" int ghx_spectrumc(ghx_grid *g, const void *a, const void *b, const vo
   if (hel == 1 && Hk)
     HIPCHK(ghx::op_spectrum(g->gk, g->stream, 1, g->Dkk, nb, C_(a), C_(b), C_(c), t, t + g->nc, t + 2 * g->nc,
                             g->work, Hk));
+  return GHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The energy spectrum without device allocation, for callers that cannot call
+// hipMalloc (code under stream capture, fixed-memory serving loops).  The
+// per-block shell partials and their totals live in `scratch`, a file-scope
+// device array of kNoAllocDoubles doubles that every GPU the library runs on
+// holds until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): each device's
+// address is looked up once with hipGetSymbolAddress, with that device
+// current, and only that address is used.  Every call on one GPU shares its
+// array, so calls that overlap on one GPU (two host threads, two streams
+// without ordering) race; each call writes every partial it then sums, so
+// nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+static constexpr size_t kNoAllocDoubles = (size_t)1 << 19;  // 4 MiB
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+static constexpr int kNoAllocDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kNoAllocDevices];
+
+static int scratch_address(double **out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices)
+    return set_err(GHX_ERR_ARG, "spectrumc_noalloc: device %d beyond the %d the scratch table holds", dev,
+                   kNoAllocDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_err(GHX_ERR_HIP, "spectrumc_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GHX_OK;
+}
+
+extern "C" int ghx_spectrumc_noalloc(ghx_grid *g, const void *a, const void *b, const void *c, double *Ek) {
+  ARGCHK(g && a && b && c && Ek, "bad spectrumc_noalloc arguments");
+  const int nb = g->nmax / 2 + 1;
+  const size_t need = ghx::spectrum_work_doubles(g->gk, nb);
+  if (need > kNoAllocDoubles)
+    return set_err(GHX_ERR_SIZE, "spectrumc_noalloc: %zu doubles of shell partials exceed the %zu-double scratch",
+                   need, kNoAllocDoubles);
+  double *w = nullptr;
+  int rc = scratch_address(&w);
+  if (rc) return rc;
+  HIPCHK(ghx::op_spectrum(g->gk, g->stream, 0, g->Dkk, nb, C_(a), C_(b), C_(c), nullptr, nullptr, nullptr, w, Ek));
   return GHX_OK;
 }
 

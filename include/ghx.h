@@ -135,6 +135,13 @@ int ghx_divergence(ghx_grid *g, const void *a, const void *b, const void *c, dou
 /* spectrumc (hd :1190-1415): Ek, Hk host arrays of nmax/2+1 doubles (either may be NULL) */
 int ghx_spectrumc(ghx_grid *g, const void *a, const void *b, const void *c, int kin, int hel, double *Ek,
                   double *Hk);
+/* No counterpart in GHOST: for callers that cannot allocate device memory
+ * (code under stream capture, fixed-memory loops).  The energy spectrum of
+ * (a, b, c) as ghx_spectrumc with kin = 1, hel = 0 (Ek: nmax/2+1 host
+ * doubles), computed without any device allocation: its shell partials live
+ * in a fixed 4 MiB device array per GPU (grids up to about 1020^3; larger ones
+ * are refused with GHX_ERR_SIZE).  Calls that overlap on one GPU race. */
+int ghx_spectrumc_noalloc(ghx_grid *g, const void *a, const void *b, const void *c, double *Ek);
 /* Transfer and 1D spectra (reduced shell totals; GHOST's files hold them
  * divided by Dkk, or 0.5/Dk_dir for spectr1d).  Arrays of nmax/2+1 host doubles. */
 /* entrans (hd :1629-1820): Re(a d* + b e* + c f*)/N^6 per shell, times kk2 when kin = 0 */
