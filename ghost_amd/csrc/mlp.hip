@@ -167,49 +167,138 @@ struct Ptr6 {
   double *p[6];
 };
 
-__global__ __launch_bounds__(BLOCK) void k_mlp_fields(Ptr9 g, double delta, const float *__restrict__ P, Norm nm,
-                                                      Ptr6 tau, long long n) {
-  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
-  FwdW W;
-  load_fwd(W, P, l);
-  const long long wave = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6), nw = (long long)gridDim.x * WAVES;
-  const long long tiles = (n + 15) / 16;
-  // the features of lane (point c, q): fields 8q .. 8q+7 (lanes q = 0, 1)
-  auto fetch = [&](long long t, double (&raw)[8]) {
-    const long long p = t * 16 + c;
-    const long long pp = p < n ? p : 0;
+// One wave takes 64 points per trip, one point per lane for the memory side:
+// lane p loads point p's nine gradients (nine 512-byte coalesced loads; the
+// next trip's loads are issued before this trip's work), normalises them and
+// writes them as bf16 to its LDS feature row.  The four 16-point MFMA tiles
+// then read their B operands from those rows and their weights (bf16, in
+// the A-operand order of load_fwd) and biases from the block's LDS copy, and
+// leave the outputs in LDS rows from which lane p takes point p's six values
+// for six coalesced 512-byte stores.  Weights in LDS rather than registers
+// leave room for several waves per SIMD.
+constexpr int IW = 4, IBLOCK = 64 * IW;
+struct InferW {
+  bf8 w1[4][64], w2[4][2][64], w3[2][64];
+  f4 b1[4][64], b2[4][64], b3[64];
+};
+struct InferLDS {
+  InferW W;
+  bf8 feat[IW][64][2];  // features 0..15 of each point (10 used) as bf16
+  f4 out[IW][64][2];    // outputs 0..7 of each point (6 used)
+};
+
+DEV void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(IBLOCK) void k_mlp_fields(Ptr9 g, double delta, const float *__restrict__ P, Norm nm,
+                                                       Ptr6 tau, long long n) {
+  __shared__ InferLDS S;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, c = lane & 15, q = lane >> 4;
+  {
+    // the block's weights in load_fwd's operand order; wave w fills its share
+    const int cc = lane & 15, qq = lane >> 4;
+    auto row8 = [&](int off, int row, int ld, int kind, int s) {
+      bf8 o;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int f = 8 * q + r;
-      raw[r] = f < 9 ? g.p[f][pp] : 0.0;
-    }
-  };
-  double cur[8], nxt[8];
-  if (wave < tiles) fetch(wave, cur);
-  for (long long t = wave; t < tiles; t += nw) {
-    const long long p = t * 16 + c;
-    const bool ok = p < n;
-    if (t + nw < tiles) fetch(t + nw, nxt);  // next tile's loads in flight
-    bf8 xb;
+      for (int r = 0; r < 8; ++r) {
+        const int k = kind == 0 ? 8 * qq + r : uslot(s, qq, r);
+        const bool ok = kind == 0 ? k < NI : (kind == 2 ? row < NO : true);
+        o[r] = (__bf16)(ok ? P[off + row * ld + k] : 0.0f);
+      }
+      return o;
+    };
+    auto bias4 = [&](int off, int base, int lim) {
+      f4 o;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int f = 8 * q + r;
-      float v = 0.0f;
-      if (f < 9) v = (float)(cur[r] - (double)nm.mx[f]) / nm.sx[f];
-      else if (f == 9) v = (float)(delta - (double)nm.mx[9]) / nm.sx[9];
-      xb[r] = (__bf16)v;
-    }
-#pragma unroll
-    for (int r = 0; r < 8; ++r) cur[r] = nxt[r];
-    Acts a;
-    forward(W, xb, a);
-    if (ok) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int j = 4 * q + r;
-        if (j < NO) tau.p[j][p] = (double)a.y[r] * (double)nm.sy[j] + (double)nm.my[j];
+      for (int r = 0; r < 4; ++r) o[r] = base + r < lim ? P[off + base + r] : 0.0f;
+      return o;
+    };
+    if (w == 0) {
+      for (int b = 0; b < 4; ++b) S.W.w1[b][lane] = row8(OW1, 16 * b + cc, NI, 0, 0);
+      for (int s2 = 0; s2 < 2; ++s2) S.W.w3[s2][lane] = row8(OW3, cc, NH, 2, s2);
+      S.W.b3[lane] = bias4(OB3, 4 * qq, NO);
+    } else if (w == 1 || w == 2) {
+      for (int b = 0; b < 4; ++b) S.W.w2[b][w - 1][lane] = row8(OW2, 16 * b + cc, NH, 1, w - 1);
+    } else {
+      for (int b = 0; b < 4; ++b) {
+        S.W.b1[b][lane] = bias4(OB1, 16 * b + 4 * qq, NH);
+        S.W.b2[b][lane] = bias4(OB2, 16 * b + 4 * qq, NH);
       }
     }
+  }
+  __syncthreads();
+  const InferW &W = S.W;
+  const long long groups = (n + 63) / 64;
+  const long long first = (long long)blockIdx.x * IW + w, stride = (long long)gridDim.x * IW;
+  const float dn = (float)(delta - (double)nm.mx[9]) / nm.sx[9];
+  auto fetch = [&](long long gi, double (&raw)[9]) {
+    const long long p = gi * 64 + lane;
+    const long long pp = p < n ? p : n - 1;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) raw[f] = g.p[f][pp];
+  };
+  double cur[9], nxt[9];
+  if (first < groups) fetch(first, cur);
+  for (long long gi = first; gi < groups; gi += stride) {
+    if (gi + stride < groups) fetch(gi + stride, nxt);
+    bf8 f0, f1;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) f0[f] = (__bf16)((float)(cur[f] - (double)nm.mx[f]) / nm.sx[f]);
+    f1[0] = (__bf16)((float)(cur[8] - (double)nm.mx[8]) / nm.sx[8]);
+    f1[1] = (__bf16)dn;
+#pragma unroll
+    for (int f = 2; f < 8; ++f) f1[f] = (__bf16)0.0f;
+    S.feat[w][lane][0] = f0;
+    S.feat[w][lane][1] = f1;
+    wave_sync();
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+      // an index the compiler cannot prove invariant keeps the weight reads
+      // in the loop (hoisted, they would hold ~90 registers per lane)
+      int l = lane;
+      asm volatile("" : "+v"(l));
+      bf8 xb;
+      if (q < 2) {
+        xb = S.feat[w][16 * t + c][q];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) xb[r] = (__bf16)0.0f;
+      }
+      f4 z[4], h[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) z[b] = mma32(W.w1[b][l], xb, W.b1[b][l]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) h[b][r] = gelu(z[b][r]);
+      bf8 h0 = pack2(h[0], h[1]), h1 = pack2(h[2], h[3]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) z[b] = mma32(W.w2[b][1][l], h1, mma32(W.w2[b][0][l], h0, W.b2[b][l]));
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) h[b][r] = gelu(z[b][r]);
+      h0 = pack2(h[0], h[1]);
+      h1 = pack2(h[2], h[3]);
+      const f4 y = mma32(W.w3[1][l], h1, mma32(W.w3[0][l], h0, W.b3[l]));
+      if (q < 2) S.out[w][16 * t + c][q] = y;
+    }
+    wave_sync();
+    const f4 o0 = S.out[w][lane][0], o1 = S.out[w][lane][1];
+    const long long p = gi * 64 + lane;
+    if (p < n) {
+#pragma unroll
+      for (int j = 0; j < NO; ++j) {
+        const float v = j < 4 ? o0[j] : o1[j - 4];
+        tau.p[j][p] = (double)v * (double)nm.sy[j] + (double)nm.my[j];
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 9; ++f) cur[f] = nxt[f];
+    wave_sync();  // this trip's LDS reads are done before the next trip's writes
   }
 }
 
@@ -539,6 +628,12 @@ static unsigned infer_blocks(long long n) {
   const long long tiles = (n + 15) / 16, b = (tiles + 2 * WAVES - 1) / (2 * WAVES);
   return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
 }
+// k_mlp_fields: one 64-point group per wave and trip, two trips per wave at
+// least (the weight copy into LDS is amortised), at most 8 blocks per CU's worth
+static unsigned field_blocks(long long n) {
+  const long long groups = (n + 63) / 64, b = (groups + 2 * IW - 1) / (2 * IW);
+  return (unsigned)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
 
 static int check_ptr(const void *p, const char *what) {
   return p ? GHX_OK : fail(GHX_ERR_ARG, std::string("mlp: null ") + what);
@@ -738,7 +833,7 @@ extern "C" int ghx_mlp_stress(ghx_mlp *m, long long n, const double *const *grad
   for (int i = 0; i < 9; ++i) if (!grad9[i]) return fail(GHX_ERR_ARG, "mlp_stress: null gradient field");
   for (int j = 0; j < 6; ++j) if (!tau6[j]) return fail(GHX_ERR_ARG, "mlp_stress: null stress field");
   if (n == 0) return GHX_OK;
-  hipLaunchKernelGGL(k_mlp_fields, dim3(infer_blocks(n)), dim3(BLOCK), 0, m->stream, ptr9(grad9), delta, m->P,
+  hipLaunchKernelGGL(k_mlp_fields, dim3(field_blocks(n)), dim3(IBLOCK), 0, m->stream, ptr9(grad9), delta, m->P,
                      m->norm, ptr6(tau6), n);
   MHIP(hipGetLastError());
   return GHX_OK;

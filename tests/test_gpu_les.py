@@ -167,3 +167,67 @@ def test_trained_closure_beats_no_closure(gpu):
     assert with_c["finite"] and without["finite"]
     assert ex.losses[-1] < 0.9 * ex.losses[0]
     assert with_c["spectrum_error_mean"] < 0.8 * without["spectrum_error_mean"]
+
+
+CLOSURE_FIXTURE = "golden/les_closure_128_kc42.npz"
+
+
+def test_trained_closure_substep_at_128_kc42_matches_oracle(gpu):
+    """The LES of the learned-closure workload at its size: 128^3, filter
+    width pi/42, the network trained on the filtered 512^3 DNS and the
+    calibrated spectral part both attached (tests/golden/les_closure_128_kc42.npz,
+    written by bench.py --workload les --save-closure).  One ord-1 step with
+    the workload's nu and LES time step against the oracle's step whose
+    forcing is f + P(-div tau) - nu(shell) kk2 v, tau taken from the network
+    on the GPU's velocity gradients.  Relative L2 error 1e-12."""
+    import os
+    from ghost_amd import les
+    from ghost_amd.closure import MLP
+    from ghost_amd.pseudospec import Grid
+    from ghost_amd.solvers import HDSolver
+    from oracle import oracle as O
+    fx = np.load(os.path.join(os.path.dirname(__file__), CLOSURE_FIXTURE))
+    n, kc = int(fx["nles"]), int(fx["kc"])
+    delta, nu, dt, nuk = float(fx["delta"]), float(fx["nu"]), float(fx["dt"]), np.asarray(fx["nuk"])
+    assert (n, kc) == (128, 42) and abs(delta - math.pi / 42) < 1e-15
+    g, og = Grid(n), O.Grid(n)
+    v, _ = og.initialv_k4exp(3.0, 1.0, 91)
+    f, _ = og.initialfv_random(2.0, 3.0, 0.4, 92)
+    m = MLP(0)
+    m.set_params(fx["params"])
+    m.set_norm(fx["norm"])
+    grad = les.velocity_gradients(g, [to_gpu(x) for x in v])
+    tau6 = [g.rzeros() for _ in range(6)]
+    m.stress(grad, delta, tau6)
+    tau = [to_np(t) for t in tau6]
+    assert all(np.all(np.isfinite(t)) for t in tau) and max(np.abs(t).max() for t in tau) > 0
+    s = HDSolver(g, dt=dt, nu=nu, ord=1)
+    s.set_velocity([to_gpu(x) for x in v])
+    s.set_forcing([to_gpu(x) for x in f])
+    s.set_sgs(m, delta)
+    s.set_sgs_spectral(nuk)
+    s.step(1)
+    got = [to_np(x) for x in s.velocity()]
+    T = [og.r2c(t) for t in tau]
+    Tm = [[T[0], T[3], T[4]], [T[3], T[1], T[5]], [T[4], T[5], T[2]]]
+    K = np.meshgrid(np.r_[0:n // 2 + 1] * 1.0, np.fft.fftfreq(n, 1.0 / n), np.fft.fftfreq(n, 1.0 / n), indexing="ij")
+    K[0][n // 2] = -n // 2  # GHOST's Nyquist kx carries a negative sign
+    kk2 = K[0] ** 2 + K[1] ** 2 + K[2] ** 2
+    kept = (kk2 <= n * n / 9.0) & (kk2 > 0)
+    nb = nuk.size
+    shell = (np.sqrt(kk2) + 0.501).astype(int)
+    coef = np.where(kept & (shell >= 1) & (shell <= nb), -nuk[np.clip(shell - 1, 0, nb - 1)] * kk2, 0.0)
+    sgs = [-1j * (K[0] * Tm[i][0] + K[1] * Tm[i][1] + K[2] * Tm[i][2]) for i in range(3)]
+    dot = K[0] * sgs[0] + K[1] * sgs[1] + K[2] * sgs[2]
+    ftot = []
+    for i in range(3):
+        proj = np.where(kept, sgs[i] - K[i] * dot / np.where(kk2 > 0, kk2, 1.0), 0.0)
+        ftot.append(f[i] + proj + coef * v[i])
+    ref = [x.copy() for x in v]
+    og.hd_step(ref, dt, nu, f=ftot, ord=1)
+    for k in range(3):
+        assert rel(got[k], ref[k]) < 1e-12
+    # both parts of the closure act: the step differs from the plain one
+    plain = [x.copy() for x in v]
+    og.hd_step(plain, dt, nu, f=f, ord=1)
+    assert rel(got[0], plain[0]) > 1e-9
