@@ -1693,26 +1693,30 @@ extern "C" int ghx_hd_touch(ghx_hd *s) {
 // learned LES (no counterpart in GHOST)
 // ---------------------------------------------------------------------------
 // Physical velocity gradients d u_i / d x_j (i major) of a spectral state:
-// derivk3 scaled by 1/(nx ny nz), then c2r of the nine fields.
-static int velocity_gradients(ghx_grid *g, const void *const *v, cplx *tmp9, double *const *grad9) {
-  const double s = 1.0 / ((double)g->gk.nx * (double)g->gk.ny * (double)g->gk.nz);
-  const void *src[9];
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      cplx *d = tmp9 + (size_t)(3 * i + j) * g->nc;
-      HIPCHK(ghx::op_derivk3(g->gk, g->stream, C_(v[i]), d, j + 1));
-      HIPCHK(ghx::op_scale(g->gk, g->stream, d, d, s));
-      src[3 * i + j] = d;
-    }
-  return do_c2r(&g->plan, 9, src, grad9);
+// derivk3 scaled by 1/(nx ny nz), then c2r of the nine fields.  One fused
+// Z^-1 takes v to the nine scaled derivatives (the same operations as derivk3
+// then the scale), then Y^-1 and X^-1 of nine fields.
+static int velocity_gradients(ghx_grid *g, const void *const *v, double *const *grad9) {
+  int rc = plan_scratch(&g->plan, 18);
+  if (rc) return rc;
+  ghx_plan *p = &g->plan;
+  const size_t slot = ghx::slot_points(p->ctx.gk);
+  FieldSet in, q, pp;
+  for (int k = 0; k < 3; ++k) in.p[k] = const_cast<cplx *>(C_(v[k]));
+  for (int f = 0; f < 9; ++f) {
+    q.p[f] = p->q + f * slot;
+    pp.p[f] = p->q + (9 + f) * slot;
+  }
+  HIPCHK(ghx::zinv(p->ctx, 6, 9, in, q, 0, nullptr));
+  HIPCHK(ghx::yinv(p->ctx, 9, q, pp, 0));
+  HIPCHK(ghx::xinv(p->ctx, 9, pp, grad9, 0));
+  return GHX_OK;
 }
 
 extern "C" int ghx_velocity_gradients(ghx_grid *g, const void *const *v, double *const *grad9) {
   ARGCHK(g && v && grad9 && v[0] && v[1] && v[2], "bad velocity_gradients arguments");
   for (int k = 0; k < 9; ++k) ARGCHK(grad9[k], "null gradient field");
-  int rc = grid_ctmp(g, 9);
-  if (rc) return rc;
-  return velocity_gradients(g, v, g->ctmp, grad9);
+  return velocity_gradients(g, v, grad9);
 }
 
 // Training pairs of the closure from a fine-grid state v (spectral, GHOST's
@@ -1785,10 +1789,9 @@ static int sgs_force(ghx_hd *s, const cplx *const *vin, const cplx **fs) {
   double *grad9[9], *tau6[6];
   for (int k = 0; k < 9; ++k) grad9[k] = gr + k * nr;
   for (int k = 0; k < 6; ++k) tau6[k] = tr + k * nr;
-  int rc = grid_ctmp(g, 9);
-  if (rc) return rc;
+  int rc;
   const void *v[3] = {vin[0], vin[1], vin[2]};
-  if ((rc = velocity_gradients(g, v, g->ctmp, grad9))) return rc;
+  if ((rc = velocity_gradients(g, v, grad9))) return rc;
   if ((rc = ghx_mlp_set_stream(s->sgs, g->stream))) return rc;
   const double *cg[9];
   for (int k = 0; k < 9; ++k) cg[k] = grad9[k];

@@ -281,10 +281,10 @@ GHX_DEV unsigned tile_lines_active(const GridK &g, int i, int j0, int L) {
 // ---------------------------------------------------------------------------
 // Z^-1: S -> Q.  Block = (kx index i, ky chunk j0..j0+L-1).
 // ---------------------------------------------------------------------------
-template <int N, int E>
-GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N> &tw,
+template <int N, int E, typename Cfg = FFTCfg<N>>
+GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N, Cfg> &tw,
                        cplx *dst, const GridK &g, int i, int j0, unsigned lmask) {
-  using F = LineFFT<N, +1>;
+  using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T;
   F::run(v, row, t, tw);
 #pragma unroll
@@ -296,15 +296,15 @@ GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
   __syncthreads();
 }
 
-template <int N, int MODE>
+template <int N, int MODE, typename Cfg = FFTCfg<N>>
 __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_, int nf, GridK g, const cplx *__restrict__ twt,
                                              int prune, const double *__restrict__ aux) {
-  using F = LineFFT<N, +1>;
+  using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  TwCache<N, Cfg> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
   const FieldView in = batch_view(in_, blockIdx.z);
   const FieldView out = batch_view(out_, blockIdx.z);
@@ -340,8 +340,30 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
         v[m] = load(in.p[f], t + T * m);
         if constexpr (MODE == 3) v[m] = c_scale(v[m], sxy * aux[g.nxh + g.ny + t + T * m]);
       }
-      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0, lmask);
+      zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0, lmask);
     }
+  } else if constexpr (MODE == 6) {
+    // velocity gradients: v -> d v_i / d x_j for i, j = 1..3 (out.p[3 i + j],
+    // i major), derivk3 (pseudospec3D_hd.f90:22-88) scaled by 1/(nx ny nz)
+    const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
+    const double sc = 1.0 / ((double)g.nx * (double)g.ny * (double)g.nz);
+    cplx u[3][E];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int m = 0; m < E; ++m) u[c][m] = load(in.p[c], t + T * m);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int dir = 0; dir < 3; ++dir) {
+        cplx v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const double kd = dir == 0 ? kx : (dir == 1 ? ky : wavenum(t + T * m, g.nz, g.dkz));
+          v[m] = c_scale(c_ik(kd, u[c][m]), sc);
+        }
+        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[3 * c + dir], g, i, j0, lmask);
+      }
   } else {
     // MODE 1: v -> (v, curl v).  MODE 2: (v, A) -> (v, curl v, b = curl A, curl b).
     // MODE 4: (a, b, c, d) -> (a, b, c, grad d).  MODE 5: (v, th) -> (v, curl v, grad th).
@@ -366,14 +388,14 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
       cplx v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = u[c][m];
-      zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[c], g, i, j0, lmask);
+      zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[c], g, i, j0, lmask);
     }
     if constexpr (MODE != 4) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         cplx v[E];
         curl(u, c, v);
-        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[3 + c], g, i, j0, lmask);
+        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[3 + c], g, i, j0, lmask);
       }
     }
     if constexpr (MODE == 4 || MODE == 5) {
@@ -390,7 +412,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
           const double kd = dir == 0 ? kx : (dir == 1 ? ky : wavenum(t + T * m, g.nz, g.dkz));
           v[m] = c_ik(kd, d[m]);
         }
-        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[base + dir], g, i, j0, lmask);
+        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[base + dir], g, i, j0, lmask);
       }
     }
     if constexpr (MODE == 2) {
@@ -403,7 +425,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
       for (int c = 0; c < 3; ++c) {
         cplx v[E];
         curl(a, c, v);
-        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[6 + c], g, i, j0, lmask);
+        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[6 + c], g, i, j0, lmask);
       }
       // j = curl b with the two components of b = curl A it needs recomputed
       // per mode (the same operations as rotor3 of the stored b)
@@ -420,7 +442,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
           else if (c == 1) v[m] = c_sub(c_ik(kz, b0), c_ik(kx, b2));
           else v[m] = c_sub(c_ik(kx, b1), c_ik(ky, b0));
         }
-        zinv_emit<N, E>(v, lds, row, stride, L, t, tw, out.p[9 + c], g, i, j0, lmask);
+        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[9 + c], g, i, j0, lmask);
       }
     }
   }
@@ -429,10 +451,10 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
 // ---------------------------------------------------------------------------
 // Z forward: Q -> S.  Block = (kx index i, ky chunk).  Transposed load.
 // ---------------------------------------------------------------------------
-template <int N, int E>
-GHX_DEV void zfwd_load(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N> &tw,
+template <int N, int E, typename Cfg = FFTCfg<N>>
+GHX_DEV void zfwd_load(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N, Cfg> &tw,
                        const cplx *src, const GridK &g, int i, int j0, unsigned lmask = ~0u) {
-  using F = LineFFT<N, -1>;
+  using F = LineFFT<N, -1, Cfg>;
   constexpr int T = F::T;
   tile_load_q<E>(lds, stride, L, src, (size_t)g.nxh * g.ny, (size_t)i * g.ny, j0, g.ny, lmask);
   __syncthreads();
@@ -514,15 +536,15 @@ GHX_DEV cplx rk_update(cplx cc, cplx vv, cplx nl, cplx ff, double kk2, double nu
   return c_make(cc.x + dt * sx * rmp, cc.y + dt * sy * rmp);
 }
 
-template <int N>
+template <int N, typename Cfg = FFTCfg<N>>
 __global__ __launch_bounds__(kTileBound) void k_zfwd_rk_hd(FieldSet in_, GridK g, const cplx *__restrict__ twt, RKParams rk,
                                                    int prune) {
-  using F = LineFFT<N, -1>;
+  using F = LineFFT<N, -1, Cfg>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  TwCache<N, Cfg> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
   const FieldView in = batch_view(in_, blockIdx.z);
   if (blockIdx.z && rk.bstride) {
@@ -550,7 +572,7 @@ __global__ __launch_bounds__(kTileBound) void k_zfwd_rk_hd(FieldSet in_, GridK g
   cplx P[3][E];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    zfwd_load<N, E>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
+    zfwd_load<N, E, Cfg>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
     __syncthreads();
   }
   if (!ok) return;
@@ -572,7 +594,9 @@ __global__ __launch_bounds__(kTileBound) void k_zfwd_rk_hd(FieldSet in_, GridK g
       c_put<true>(&rk.out[0][s], rk_update(rk.c[0][s], rk.v[0][s], n0, forced ? rk.f[0][s] : z, kk2, rk.nu, rk.dt, rk.rmp));
       c_put<true>(&rk.out[1][s], rk_update(rk.c[1][s], rk.v[1][s], n1, forced ? rk.f[1][s] : z, kk2, rk.nu, rk.dt, rk.rmp));
       c_put<true>(&rk.out[2][s], rk_update(rk.c[2][s], rk.v[2][s], n2, forced ? rk.f[2][s] : z, kk2, rk.nu, rk.dt, rk.rmp));
-    } else if (!prune) {
+    } else if (!prune || kn2 < g.tiny) {
+      // outside the kept modes; the k = 0 mode is written in pruned passes
+      // too, since the next substep's inverse reads it from this output
       c_put<true>(&rk.out[0][s], z);
       c_put<true>(&rk.out[1][s], z);
       c_put<true>(&rk.out[2][s], z);
@@ -737,7 +761,7 @@ __global__ __launch_bounds__(kTileBound) void k_zfwd_rk_mhd(FieldSet in, GridK g
       upd(0, n0);
       upd(1, n1);
       upd(2, n2);
-    } else if (!prune) {
+    } else if (!prune || kn2 < g.tiny) {  // the k = 0 mode: as in k_zfwd_rk_hd
       rk.out[c0][s] = z;
       rk.out[c0 + 1][s] = z;
       rk.out[c0 + 2][s] = z;
@@ -838,7 +862,8 @@ __global__ __launch_bounds__(kTileBound) void k_yfwd(FieldSet in_, FieldSet out_
       const int idx = threadIdx.x + q * blockDim.x;
       const int l = idx % L, p = idx / L;
       const bool inside = kx0 + l < kxlim;
-      const cplx x = src[(size_t)p * nxp + kx0 + (inside ? l : 0)];
+      // a 32-bit offset from the block's uniform base (ny * nxp < 2^31)
+      const cplx x = src[p * nxp + kx0 + (inside ? l : 0)];
       w[q] = inside ? x : c_make(0.0, 0.0);
     }
   };
@@ -1524,10 +1549,31 @@ int active_kx(const GridK &g) {
 #endif
 
 #if GHX_PART == 2
+// 512-point z lines on the 16-points-per-lane schedule: 8-line tiles (128-byte
+// chunks of Q) from 256-lane blocks, for the passes that keep one field live
+template <int MODE>
+static hipError_t launch_zinv_wide(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune,
+                                   const double *aux) {
+  const GridK &g = c.gk;
+  using Cfg = FFTCfgWide<512>;
+  constexpr int T = Cfg::T;
+  const PassShape s = wide_shape(T, lds_line_stride(512), g.ny);
+  const int L = s.lines;
+  dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);
+  allow_lds(k_zinv<512, MODE, Cfg>, s.lds);
+  GHX_LAUNCH_CHECK((k_zinv<512, MODE, Cfg>), s, kTileBound, T, true);
+  hipLaunchKernelGGL((k_zinv<512, MODE, Cfg>), grid, dim3(L * T), s.lds, c.stream, in, out, nf, g, c.tw_z, prune, aux);
+  return hipGetLastError();
+}
+
 hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out, int prune,
                 const double *aux) {
   if (mode == 3 && !aux) return hipErrorInvalidValue;
   const GridK &g = c.gk;
+  if (g.nz == 512 && tile_want("GHX_ZWIDE", 2) == 1 && wide_shape(FFTCfgWide<512>::T, lds_line_stride(512), g.ny).lds <= max_lds()) {
+    if (mode == 0) return launch_zinv_wide<0>(c, nf, in, out, prune, aux);
+    if (mode == 3) return launch_zinv_wide<3>(c, nf, in, out, prune, aux);
+  }
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
@@ -1550,6 +1596,9 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
     } else if (mode == 5) {                                                                 \
       allow_lds(k_zinv<n_, 5>, lds);                                                        \
       GHX_LAUNCH_CHECK((k_zinv<n_, 5>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zinv<n_, 5>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
+    } else if (mode == 6) {                                                                 \
+      allow_lds(k_zinv<n_, 6>, lds);                                                        \
+      GHX_LAUNCH_CHECK((k_zinv<n_, 6>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zinv<n_, 6>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
     } else {                                                                                \
       allow_lds(k_zinv<n_, 3>, lds);                                                        \
       GHX_LAUNCH_CHECK((k_zinv<n_, 3>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zinv<n_, 3>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune, aux); \
@@ -1593,6 +1642,21 @@ hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #if GHX_PART == 4
 hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, int prune) {
   const GridK &g = c.gk;
+  if (g.nz == 512 && tile_want("GHX_ZLOW", 1) == 1) {
+    // 512-point lines on the four-points-per-lane schedule (128 lanes per
+    // line, four lines per 512-lane block): 124 registers per lane against
+    // 224, two blocks per CU, so the update's loads of v and C overlap more
+    // (0.71 -> 0.68 ms at 512^3); GHX_ZLOW=2 selects the default schedule
+    using Cfg = FFTCfgLow<512>;
+    constexpr int T = Cfg::T;
+    const PassShape s = tile_shape(T, lds_line_stride(512), g.ny, tile_pick("GHX_LZ", c.tile_z, 4), max_lds());
+    const int L = s.lines;
+    dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);
+    allow_lds(k_zfwd_rk_hd<512, Cfg>, s.lds);
+    GHX_LAUNCH_CHECK((k_zfwd_rk_hd<512, Cfg>), s, kTileBound, T, true);
+    hipLaunchKernelGGL((k_zfwd_rk_hd<512, Cfg>), grid, dim3(L * T), s.lds, c.stream, in, g, c.tw_z, rk, prune);
+    return hipGetLastError();
+  }
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \

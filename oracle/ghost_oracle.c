@@ -563,20 +563,27 @@ void og_vector3(const og_grid *g, const ocplx *a, const ocplx *b, const ocplx *c
 
 static inline double abs2(ocplx z) { return creal(z) * creal(z) + cimag(z) * cimag(z); }
 
-/* Hermitian-weighted sum over modes: weight 1 on the kx=0 plane, 2 elsewhere */
+/* Hermitian-weighted sum over modes: weight 1 on the kx=0 plane, 2 elsewhere.
+ * Each kx plane is summed by one thread, then the planes in kx order: the
+ * result does not depend on the thread count or on the order in which
+ * threads finish (an OpenMP reduction clause combines in no fixed order). */
 #define HSUM(g, expr, out)                                                            \
   do {                                                                                \
-    double acc_ = 0.0;                                                                \
-    _Pragma("omp parallel for reduction(+:acc_) schedule(static)")                    \
+    double *part_ = (double *)malloc(sizeof(double) * (size_t)(g)->nxh);              \
+    _Pragma("omp parallel for schedule(static)")                                      \
     for (int i = 0; i < (g)->nxh; ++i) {                                              \
-      double w_ = i == 0 ? 1.0 : 2.0;                                                 \
+      double w_ = i == 0 ? 1.0 : 2.0, a_ = 0.0;                                       \
       for (int j = 0; j < (g)->ny; ++j)                                               \
         for (int k = 0; k < (g)->nz; ++k) {                                           \
           size_t m = IDX(g, i, j, k);                                                 \
           (void)m;                                                                    \
-          acc_ += w_ * (expr);                                                        \
+          a_ += w_ * (expr);                                                          \
         }                                                                             \
+      part_[i] = a_;                                                                  \
     }                                                                                 \
+    double acc_ = 0.0;                                                                \
+    for (int i = 0; i < (g)->nxh; ++i) acc_ += part_[i];                              \
+    free(part_);                                                                      \
     out = acc_;                                                                       \
   } while (0)
 

@@ -400,3 +400,56 @@ def test_mhd_pruned_equals_full_path(gpu):
             os.environ.pop("GHX_NO_PRUNE", None)
     for k in range(6):
         assert rel(outs[0][k], outs[1][k]) < 1e-14, k
+
+
+def _dirty_device_memory(nbytes, value=1.0):
+    """Allocate nbytes through the C ABI, fill them with `value`, free them:
+    the next allocation of that size usually gets these bytes back, so a
+    buffer that a pruned pass forgets to write holds `value`, not zero."""
+    import ctypes as C
+    from ghost_amd import _lib
+    lib = _lib.load()
+    p = C.c_void_p()
+    _lib.check(lib.ghx_malloc(C.byref(p), nbytes))
+    host = np.full(nbytes // 8, value)
+    _lib.check(lib.ghx_memcpy_h2d(p, host.ctypes.data_as(C.c_void_p), nbytes))
+    _lib.check(lib.ghx_free(p))
+
+
+@pytest.mark.parametrize("solver", ["hd", "mhd"])
+def test_pruned_steps_write_the_mean_mode(gpu, solver):
+    """The pruned RK updates skip the modes outside the dealias set but must
+    still write the k = 0 mode (GHOST's update zeroes kn2 < tiny): the next
+    substep's inverse transform reads it from the intermediate state.  The
+    solver is created on device memory left holding 1.0 everywhere, and its
+    pruned steps must match the oracle at 1e-12."""
+    from ghost_amd.solvers import HDSolver, MHDSolver
+    n = 48
+    g, og = grids((n, n, n))
+    nc = (n // 2 + 1) * n * n
+    slot = n * n * ((n // 2 + 1 + 7) & ~7)
+    v, _ = og.initialv_k4exp(3.0, 1.0, 17)
+    if solver == "hd":
+        _dirty_device_memory((9 * nc + 12 * slot) * 16)
+        s = HDSolver(g, dt=2e-3, nu=5e-3, ord=2)
+        s.set_velocity([to_gpu(x) for x in v])
+        s.set_forcing(None)
+        s.step(3)
+        got = [to_np(x) for x in s.velocity()]
+        ref = [x.copy() for x in v]
+        for _ in range(3):
+            og.hd_step(ref, 2e-3, 5e-3, ord=2)
+    else:
+        a, _ = og.initialv_k4exp(3.0, 0.5, 18)
+        _dirty_device_memory((18 * nc + 24 * slot) * 16)
+        s = MHDSolver(g, 2e-3, 5e-3, 5e-3, ord=2)
+        s.set_state([to_gpu(x) for x in v], [to_gpu(x) for x in a])
+        s.set_forcing(None, None)
+        s.step(3)
+        gv, ga = s.state()
+        got = [to_np(x) for x in list(gv) + list(ga)]
+        ref = [x.copy() for x in v] + [x.copy() for x in a]
+        for _ in range(3):
+            og.mhd_step(ref[:3], ref[3:], 2e-3, 5e-3, 5e-3, ord=2)
+    for k in range(len(ref)):
+        assert rel(got[k], ref[k]) < 1e-12, k
