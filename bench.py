@@ -106,12 +106,15 @@ def time_steps(one_step, steps, warmup, lib, torch):
 
 
 def kernel_name(slot, n):
-    """The kernel a profiler slot times at size n (the 512-point y inverse
-    runs the 16-points-per-lane schedule)."""
+    """The kernel a profiler slot times at size n (at 512 points the y inverse
+    and the spline z inverse run the 16-points-per-lane schedule, the HD z
+    forward update the four-points-per-lane one)."""
     if slot in ("ypass_inv", "ypass_inv_tracer") and n == 512:
         return "ghx::k_yinv<512, FFTCfgWide<512> >"
     if slot == "zfwd_rk_hd" and n == 512:
         return "ghx::k_zfwd_rk_hd<512, FFTCfgLow<512> >"
+    if slot == "zinv_spline" and n == 512:
+        return "ghx::k_zinv<512, 3, FFTCfgWide<512> >"
     return KERNEL_NAME.get(slot, slot).replace("<N", f"<{n}")
 
 
@@ -119,7 +122,7 @@ def kernel_name(slot, n):
 SAME_KERNEL = {"ypass_inv_tracer": "ypass_inv"}
 KERNEL_NAME = {"ypass_inv": "ghx::k_yinv<N>", "xcross": "ghx::k_xcross2<N>",
                "zinv_curl": "ghx::k_zinv<N, 1, FFTCfg<N> >", "zfwd_rk_hd": "ghx::k_zfwd_rk_hd<N, FFTCfg<N> >",
-               "ypass_fwd": "ghx::k_yfwd<N>", "xinv_tracer": "ghx::k_xinv3i2<N>",
+               "ypass_fwd": "ghx::k_yfwd<N, FFTCfg<N> >", "xinv_tracer": "ghx::k_xinv3i2<N>",
                "tracer_stage": "ghx::k_tracer_stage", "zinv_spline": "ghx::k_zinv<N, 3, FFTCfg<N> >"}
 
 
@@ -320,8 +323,9 @@ def run_ensemble(args, torch):
         e.set_velocity(i, v)
     mem = [m[:3] for m in mem]  # drop the IC copies
     torch.cuda.empty_cache()
-    tiles = e.autotune(steps=3)
-    log(f"ensemble of {len(mem)} members, groups {e.groups()}, tiles (z, y) {tiles}")
+    e.autotune(steps=3)
+    scheds = e.schedules()
+    log(f"ensemble of {len(mem)} members, groups {e.groups()}, (schedule, z tile, y tile) {scheds}")
 
     def one_step():
         e.step(1)
@@ -347,8 +351,10 @@ def run_ensemble(args, torch):
         "data": "synthetic (seeded k^4 exp ICs, k0 uniform in [2, 4])",
         "config": {"workload": "96 unforced HD3D runs, 24 each at 32^3, 48^3, 64^3, 96^3, RK2, "
                                f"dt={ENS_DT}, nu from k_max eta = 1.5, one launch per pass per size, "
-                               "captured HIP graph, autotuned tile widths",
-                   "members": len(mem), "sizes": list(ENS_SIZES), "tiles": tiles},
+                               "captured HIP graph, autotuned radix schedules and tile widths",
+                   "members": len(mem), "sizes": list(ENS_SIZES),
+                   "schedules": [{"size": n, "sched": sc, "tile_z": tz, "tile_y": ty}
+                                 for (n, _), (sc, tz, ty) in zip(e.groups(), scheds)]},
         "member_steps_per_s": round(steps_per_s * len(mem), 1),
         "step_algorithmic_GB": round(b_step / 1e9, 3),
         "step_effective_GBps": round(b_step * steps_per_s / 1e9, 1),

@@ -99,6 +99,7 @@ def _declare(L):
         "ghx_mhd_rkstep2": (_I, [_P, _I, _D, _D, _D, _PP, _PP, _PP, _PP, _PP]),
         "ghx_bouss_rkstep2": (_I, [_P, _I, _D, _D, _D, _D, _D, _PP, _PP, _PP, _PP]),
         "ghx_rotbouss_rkstep2": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _PP, _PP, _PP, _PP]),
+        "ghx_bouss_ord3": (_I, [_P, _D, _D, _PP, _PP]),
         "ghx_hd_create": (_I, [_PP, _P, _I, _D, _D]),
         "ghx_hd_destroy": (_I, [_P]),
         "ghx_hd_fields": (_I, [_P, _PP, _PP]),
@@ -131,6 +132,8 @@ def _declare(L):
         "ghx_ensemble_set_graph": (_I, [_P, _I]),
         "ghx_ensemble_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I)]),
         "ghx_ensemble_groups": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+        "ghx_ensemble_schedules": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+        "ghx_ensemble_set_schedule": (_I, [_P, _I, _I, _I, _I]),
         "ghx_ensemble_sync": (_I, [_P]),
         "ghx_tracers_create": (_I, [_PP, _P, C.c_longlong, _I]),
         "ghx_tracers_destroy": (_I, [_P]),

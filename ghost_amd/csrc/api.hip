@@ -2276,6 +2276,7 @@ struct ghx_bouss {
   // rotation omegaz about z, and no ord = 3 correction
   bool rot = false;
   double omegaz = 0.0;
+  cplx *corr = nullptr;  // the ord = 3 correction term, 3 fields
 };
 
 extern "C" int ghx_bouss_create(ghx_bouss **out, ghx_grid *g, int ord, double dt, double nu, double kappa,
@@ -2305,6 +2306,7 @@ extern "C" int ghx_bouss_create(ghx_bouss **out, ghx_grid *g, int ord, double dt
 
 extern "C" int ghx_bouss_destroy(ghx_bouss *s) {
   if (!s) return GHX_OK;
+  (void)hipFree(s->corr);
   (void)hipFree(s->mem);
   (void)hipFree(s->fline);
   delete s;
@@ -2401,6 +2403,43 @@ static int bouss_substep(ghx_bouss *s, int o, const cplx *const *vin, cplx *cons
   return GHX_OK;
 }
 
+// The correction term of include/bouss_rkstep2.f90:73-145 from the step-start
+// state c = (u, th): x = N(L(u) + N(u)) with N(w) = nonlhd3 of prodre3(w) plus
+// the buoyancy xmom th, L the Laplacian (no viscosity, as written there) and
+// the sum masked to the kept modes.  The caller adds 2 x dt^3/24 to the
+// velocity after the last substep.  x may not alias c.
+static int bouss_ord3_term(ghx_grid *g, double xmom, const cplx *const *c, cplx *const *x) {
+  int rc = fused_cross(g, 1, c[0], c[1], c[2], nullptr, nullptr, nullptr, x[0], x[1], x[2]);
+  if (rc) return rc;
+  const cplx *w[3] = {x[0], x[1], x[2]};
+  HIPCHK(ghx::op_bouss3_nonl(g->gk, g->stream, 0, w, c[3], xmom, c, x));
+  if ((rc = fused_cross(g, 1, x[0], x[1], x[2], nullptr, nullptr, nullptr, x[0], x[1], x[2]))) return rc;
+  HIPCHK(ghx::op_bouss3_nonl(g->gk, g->stream, 1, w, c[3], xmom, nullptr, x));
+  return GHX_OK;
+}
+
+// include/bouss_rkstep2.f90:73-145 on caller fields, for the last substep
+// (o = 1) of an ord = 3 step: c holds (v_x, v_y, v_z, th) at the step start,
+// v (3 fields) the velocity after the substep, corrected in place.  c is not
+// changed (GHOST leaves L(u) + N(u) in C1-C3, which nothing reads before the
+// next step's copy).
+extern "C" int ghx_bouss_ord3(ghx_grid *g, double dt, double xmom, const void *const *c, void *const *v) {
+  ARGCHK(g && c && v, "bad bouss_ord3 arguments");
+  const size_t nc = g->nc;
+  int rc = grid_work(g, 6 * nc);
+  if (rc) return rc;
+  cplx *w = (cplx *)g->work;
+  cplx *x[3] = {w, w + nc, w + 2 * nc};
+  const cplx *ci[4] = {C_(c[0]), C_(c[1]), C_(c[2]), C_(c[3])};
+  for (int k = 0; k < 3; ++k)
+    for (int m = 0; m < 4; ++m) ARGCHK(v[k] != c[m], "bouss_ord3: v may not alias c");
+  if ((rc = bouss_ord3_term(g, xmom, ci, x))) return rc;
+  cplx *vo[3] = {M_(v[0]), M_(v[1]), M_(v[2])};
+  const cplx *xi[3] = {x[0], x[1], x[2]};
+  HIPCHK(ghx::op_bouss3_add(g->gk, g->stream, vo, xi, dt * dt * dt / 24.0));
+  return GHX_OK;
+}
+
 // Forcing-line flags for the stateless substeps below: the lines (kx, ky)
 // where any of the nf fields is non-zero, OR-ed, in g->work (2 nl bytes).
 static int stateless_flags(ghx_grid *g, const cplx *const *fa, int nf, uint8_t **flags) {
@@ -2487,9 +2526,13 @@ extern "C" int ghx_rotbouss_rkstep2(ghx_grid *g, int o, double dt, double nu, do
 
 extern "C" int ghx_bouss_step(ghx_bouss *s, int nsteps) {
   ARGCHK(s && nsteps >= 0, "bad bouss_step arguments");
-  if (s->ord == 3 && !s->rot)
-    return set_err(GHX_ERR_ARG, "Boussinesq solver without rotation: ord = 3 needs the correction of "
-                                "bouss_rkstep2.f90:73-145, which is not built (ord 1 and 2 are)");
+  // include/bouss_rkstep2.f90:73-145: an ord = 3 step without rotation adds
+  // 2 N(L(u) + N(u)) dt^3/24 after its last substep.  The term depends on the
+  // step-start state only, so it is formed before the substeps overwrite it.
+  // It is not masked, so the state is never clean afterwards.
+  const bool ord3 = s->ord == 3 && !s->rot;
+  if (ord3 && !s->corr) HIPCHK(hipMalloc(&s->corr, 3 * s->g->nc * sizeof(cplx)));
+  cplx *corr[3] = {s->corr, s->corr + (ord3 ? s->g->nc : 0), s->corr + (ord3 ? 2 * s->g->nc : 0)};
   const cplx *v0[4], *v1[4];
   for (int k = 0; k < 4; ++k) { v0[k] = s->s[k]; v1[k] = s->s1[k]; }
   for (int n = 0; n < nsteps; ++n) {
@@ -2508,15 +2551,25 @@ extern "C" int ghx_bouss_step(ghx_bouss *s, int nsteps) {
     for (int o = s->ord; o >= 1; --o) {
       const bool first = (o == s->ord), last = (o == 1);
       cplx *const *out = last ? s->s : s->s1;
+      if (ord3 && first) {
+        int rc = bouss_ord3_term(s->g, s->xmom, v0, corr);
+        if (rc) return rc;
+      }
       int rc = bouss_substep(s, o, first ? v0 : v1, out, prune);
       if (rc) return rc;
+      if (ord3 && last) {
+        const cplx *ci[3] = {corr[0], corr[1], corr[2]};
+        HIPCHK(ghx::op_bouss3_add(s->g->gk, s->g->stream, out, ci, s->dt * s->dt * s->dt / 24.0));
+      }
       if (s->tr) {  // main3D.fpp:3233-3267, velocity after the substep
         const cplx *vo[3] = {out[0], out[1], out[2]};
-        rc = tracers_stage(s->tr, s->q, vo, s->dt, 1.0 / (double)o, getenv("GHX_NO_PRUNE") ? 0 : 1);
+        // the corrected velocity is not confined to the kept modes
+        const int tprune = (ord3 && last) || getenv("GHX_NO_PRUNE") ? 0 : 1;
+        rc = tracers_stage(s->tr, s->q, vo, s->dt, 1.0 / (double)o, tprune);
         if (rc) return rc;
       }
     }
-    s->clean = true;
+    s->clean = !ord3;
   }
   return GHX_OK;
 }
@@ -2536,6 +2589,7 @@ struct EnsGroup {
   double *nu = nullptr;   // per-member viscosity (device)
   size_t nc = 0, slot = 0;
   int tz = 0, ty = 0;     // tile widths of the z and y passes (0: built-in)
+  int sched = 0;          // radix schedules of the z and y passes (FFTCtx::sched)
   bool clean = false;
   cplx *v() const { return mem; }
   cplx *v1() const { return mem + members.size() * 3 * nc; }
@@ -2551,6 +2605,11 @@ struct ghx_ensemble {
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the captured step forks one branch per size group: a stream and a join
+  // event per group (default priority), and the fork event
+  std::vector<hipStream_t> gstream;
+  std::vector<hipEvent_t> gjoin;
+  hipEvent_t fork = nullptr;
 };
 
 static void ens_drop_graph(ghx_ensemble *e) {
@@ -2563,6 +2622,9 @@ static void ens_drop_graph(ghx_ensemble *e) {
 extern "C" int ghx_ensemble_destroy(ghx_ensemble *e) {
   if (!e) return GHX_OK;
   ens_drop_graph(e);
+  for (hipStream_t st : e->gstream) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : e->gjoin) (void)hipEventDestroy(ev);
+  if (e->fork) (void)hipEventDestroy(e->fork);
   for (EnsGroup &G : e->groups) {
     if (G.mem) (void)hipFree(G.mem);
     if (G.nu) (void)hipFree(G.nu);
@@ -2652,12 +2714,14 @@ extern "C" int ghx_ensemble_set_graph(ghx_ensemble *e, int enabled) {
 
 // One substep o of every member of G (include/hd_rkstep2.f90, unforced), one
 // launch per pass for the whole group.
-static int ens_substep(ghx_ensemble *e, EnsGroup &G, int o, const cplx *vin, cplx *out, int prune) {
+static int ens_substep(ghx_ensemble *e, EnsGroup &G, int o, const cplx *vin, cplx *out, int prune,
+                       hipStream_t st) {
   FFTCtx ctx = G.g->plan.ctx;
-  ctx.stream = e->stream;
+  ctx.stream = st;
   ctx.batch = (int)G.members.size();
   ctx.tile_z = G.tz;
   ctx.tile_y = G.ty;
+  ctx.sched = G.sched;
   const size_t nc = G.nc, slot = G.slot, sstride = 12 * slot;
   Chain ch = chain_slots(G.q(), slot);
   ch.q6.bstride = ch.p6.bstride = ch.p3.bstride = ch.q3.bstride = sstride;
@@ -2689,12 +2753,60 @@ static int ens_substep(ghx_ensemble *e, EnsGroup &G, int o, const cplx *vin, cpl
   return GHX_OK;
 }
 
-static int ens_step_group(ghx_ensemble *e, EnsGroup &G, int prune) {
+static int ens_step_group(ghx_ensemble *e, EnsGroup &G, int prune, hipStream_t st = nullptr) {
   for (int o = e->ord; o >= 1; --o) {
     const bool first = (o == e->ord), last = (o == 1);
-    int rc = ens_substep(e, G, o, first ? G.v() : G.v1(), last ? G.v() : G.v1(), prune);
+    int rc = ens_substep(e, G, o, first ? G.v() : G.v1(), last ? G.v() : G.v1(), prune, st ? st : e->stream);
     if (rc) return rc;
   }
+  return GHX_OK;
+}
+
+// The captured step: the size groups are independent, so each is a branch of
+// the graph (forked from and joined back to the ensemble's stream), and the
+// small sizes' passes overlap the large ones' instead of queueing behind
+// them.  GHX_ENS_BRANCH=0 captures them one after another.
+static int ens_capture(ghx_ensemble *e) {
+  const size_t ng = e->groups.size();
+  const char *ev = getenv("GHX_ENS_BRANCH");
+  const bool branch = ng > 1 && !(ev && atoi(ev) == 0);
+  if (branch && e->gstream.size() < ng) {
+    if (!e->fork) HIPCHK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+    while (e->gstream.size() < ng) {
+      hipStream_t st = nullptr;
+      hipEvent_t j = nullptr;
+      HIPCHK(hipStreamCreate(&st));  // default priority
+      e->gstream.push_back(st);
+      HIPCHK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+      e->gjoin.push_back(j);
+    }
+  }
+  HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  int rc = GHX_OK;
+  hipError_t he = hipSuccess;
+  if (branch) {
+    he = hipEventRecord(e->fork, e->stream);
+    for (size_t k = 0; k < ng && he == hipSuccess; ++k) {
+      he = hipStreamWaitEvent(e->gstream[k], e->fork, 0);
+      if (he == hipSuccess && !rc) rc = ens_step_group(e, e->groups[k], 1, e->gstream[k]);
+      // every forked branch is joined, also after a failure, so that the
+      // capture can end
+      if (he == hipSuccess) he = hipEventRecord(e->gjoin[k], e->gstream[k]);
+      if (he == hipSuccess) he = hipStreamWaitEvent(e->stream, e->gjoin[k], 0);
+    }
+  } else {
+    for (EnsGroup &G : e->groups)
+      if (!rc) rc = ens_step_group(e, G, 1);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t hend = hipStreamEndCapture(e->stream, &graph);
+  if (rc || he != hipSuccess || hend != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    HIPCHK(he != hipSuccess ? he : hend);
+  }
+  e->graph = graph;
+  HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
   return GHX_OK;
 }
 
@@ -2729,17 +2841,9 @@ extern "C" int ghx_ensemble_step(ghx_ensemble *e, int nsteps) {
       if (!e->exec) {
         const bool prof_was = g_prof.on;
         g_prof.on = false;  // no profiler events inside the graph
-        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        int rc = GHX_OK;
-        for (EnsGroup &G : e->groups)
-          if (!rc) rc = ens_step_group(e, G, 1);
-        hipGraph_t graph = nullptr;
-        hipError_t he = hipStreamEndCapture(e->stream, &graph);
+        const int rc = ens_capture(e);
         g_prof.on = prof_was;
-        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
-        HIPCHK(he);
-        e->graph = graph;
-        HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+        if (rc) return rc;
       }
       HIPCHK(hipGraphLaunch(e->exec, e->stream));
     } else {
@@ -2762,6 +2866,10 @@ extern "C" int ghx_ensemble_sync(ghx_ensemble *e) {
 // Tile widths per grid size chosen by timing: for every group, each candidate
 // (z tile, y tile) runs `steps` steps of that group alone, from a saved copy
 // of its state that is put back afterwards; the fastest is kept.
+// The radix schedules a size can choose between: the z and y passes each on
+// the default schedule or on four points per lane where that differs.
+static int ens_schedules(int n) { return (n == 32 || n == 64 || n == 128 || n == 256) ? 4 : 1; }
+
 extern "C" int ghx_ensemble_autotune(ghx_ensemble *e, int steps, int *tz_out, int *ty_out) {
   ARGCHK(e && steps > 0, "bad ensemble_autotune arguments");
   static const int cand[][2] = {{4, 8}, {2, 4}, {4, 4}, {8, 8}, {8, 4}, {2, 8}, {4, 16}};
@@ -2779,24 +2887,27 @@ extern "C" int ghx_ensemble_autotune(ghx_ensemble *e, int steps, int *tz_out, in
     if (he == hipSuccess) he = hipMemcpyAsync(save, G.v(), bytes, hipMemcpyDeviceToDevice, e->stream);
     if (he != hipSuccess) { rc = set_err(GHX_ERR_HIP, "autotune: %s", hipGetErrorString(he)); break; }
     double best = 1e30;
-    int bz = 0, by = 0;
-    for (const auto &c : cand) {
-      G.tz = c[0];
-      G.ty = c[1];
-      (void)ens_step_group(e, G, G.clean ? 1 : 0);  // warm up this shape
-      (void)hipEventRecord(a, e->stream);
-      for (int s = 0; s < steps && !rc; ++s) rc = ens_step_group(e, G, G.clean ? 1 : 0);
-      (void)hipEventRecord(b, e->stream);
-      float ms = 0.f;
-      if ((he = hipEventSynchronize(b)) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
-        rc = set_err(GHX_ERR_HIP, "autotune: %s", hipGetErrorString(he));
-        break;
+    int bz = 0, by = 0, bs = 0;
+    for (int sc = 0; sc < ens_schedules(G.n) && !rc; ++sc)
+      for (const auto &c : cand) {
+        G.tz = c[0];
+        G.ty = c[1];
+        G.sched = sc;
+        (void)ens_step_group(e, G, G.clean ? 1 : 0);  // warm up this shape
+        (void)hipEventRecord(a, e->stream);
+        for (int s = 0; s < steps && !rc; ++s) rc = ens_step_group(e, G, G.clean ? 1 : 0);
+        (void)hipEventRecord(b, e->stream);
+        float ms = 0.f;
+        if ((he = hipEventSynchronize(b)) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+          rc = set_err(GHX_ERR_HIP, "autotune: %s", hipGetErrorString(he));
+          break;
+        }
+        if (ms < best) { best = ms; bz = c[0]; by = c[1]; bs = sc; }
+        (void)hipMemcpyAsync(G.v(), save, bytes, hipMemcpyDeviceToDevice, e->stream);
       }
-      if (ms < best) { best = ms; bz = c[0]; by = c[1]; }
-      (void)hipMemcpyAsync(G.v(), save, bytes, hipMemcpyDeviceToDevice, e->stream);
-    }
     G.tz = bz;
     G.ty = by;
+    G.sched = bs;
     if (tz_out) tz_out[gi] = bz;
     if (ty_out) ty_out[gi] = by;
     (void)hipStreamSynchronize(e->stream);
@@ -2805,6 +2916,27 @@ extern "C" int ghx_ensemble_autotune(ghx_ensemble *e, int steps, int *tz_out, in
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   return rc;
+}
+
+extern "C" int ghx_ensemble_schedules(ghx_ensemble *e, int *sched, int *tz, int *ty) {
+  ARGCHK(e, "null ensemble");
+  for (size_t k = 0; k < e->groups.size(); ++k) {
+    if (sched) sched[k] = e->groups[k].sched;
+    if (tz) tz[k] = e->groups[k].tz;
+    if (ty) ty[k] = e->groups[k].ty;
+  }
+  return GHX_OK;
+}
+
+extern "C" int ghx_ensemble_set_schedule(ghx_ensemble *e, int group, int sched, int tz, int ty) {
+  ARGCHK(e && group >= 0 && group < (int)e->groups.size() && sched >= 0 && sched <= 3 && tz >= 0 && ty >= 0,
+         "bad ensemble_set_schedule arguments");
+  ens_drop_graph(e);
+  EnsGroup &G = e->groups[group];
+  G.sched = sched < ens_schedules(G.n) ? sched : 0;
+  G.tz = tz;
+  G.ty = ty;
+  return GHX_OK;
 }
 
 extern "C" int ghx_ensemble_groups(ghx_ensemble *e, int *ngroups, int *sizes, int *counts) {

@@ -26,6 +26,10 @@ struct FFTCtx {
   int batch = 1;
   // tile widths (lines per block) of the z and y passes; 0 = built-in choice
   int tile_z = 0, tile_y = 0;
+  // radix schedules of the fused-substep passes: bit 0 the z passes (curl
+  // inverse, forward update), bit 1 the y passes, on four points per lane
+  // (FFTCfgLow) where that differs from the default schedule
+  int sched = 0;
 };
 
 constexpr int kMaxFields = 12;

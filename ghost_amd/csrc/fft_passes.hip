@@ -832,15 +832,15 @@ __global__ __launch_bounds__(kTileBound) void k_yinv(FieldSet in_, FieldSet out_
   }
 }
 
-template <int N>
+template <int N, typename Cfg = FFTCfg<N>>
 __global__ __launch_bounds__(kTileBound) void k_yfwd(FieldSet in_, FieldSet out_, int nf, GridK g, int nxp,
                                              const cplx *__restrict__ twt, int kxlim, int prune) {
-  using F = LineFFT<N, -1>;
+  using F = LineFFT<N, -1, Cfg>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  TwCache<N, Cfg> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
   const FieldView in = batch_view(in_, blockIdx.z);
   const FieldView out = batch_view(out_, blockIdx.z);
@@ -1550,7 +1550,7 @@ int active_kx(const GridK &g) {
 
 #if GHX_PART == 2
 // 512-point z lines on the 16-points-per-lane schedule: 8-line tiles (128-byte
-// chunks of Q) from 256-lane blocks, for the passes that keep one field live
+// chunks of Q) from 256-lane blocks, for a pass that keeps one field live
 template <int MODE>
 static hipError_t launch_zinv_wide(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune,
                                    const double *aux) {
@@ -1566,14 +1566,40 @@ static hipError_t launch_zinv_wide(const FFTCtx &c, int nf, const FieldSet &in, 
   return hipGetLastError();
 }
 
+// the curl Z^-1 on the four-points-per-lane schedule (FFTCtx::sched bit 0)
+template <int N>
+static hipError_t launch_zinv_low(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune,
+                                  const double *aux) {
+  const GridK &g = c.gk;
+  using Cfg = FFTCfgLow<N>;
+  constexpr int T = Cfg::T;
+  const PassShape s = tile_shape(T, lds_line_stride(N), g.ny, tile_pick("GHX_LZ", c.tile_z, 4), max_lds());
+  const int L = s.lines;
+  dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);
+  allow_lds(k_zinv<N, 1, Cfg>, s.lds);
+  GHX_LAUNCH_CHECK((k_zinv<N, 1, Cfg>), s, kTileBound, T, true);
+  hipLaunchKernelGGL((k_zinv<N, 1, Cfg>), grid, dim3(L * T), s.lds, c.stream, in, out, nf, g, c.tw_z, prune, aux);
+  return hipGetLastError();
+}
+
 hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const FieldSet &out, int prune,
                 const double *aux) {
   if (mode == 3 && !aux) return hipErrorInvalidValue;
   const GridK &g = c.gk;
-  if (g.nz == 512 && tile_want("GHX_ZWIDE", 2) == 1 && wide_shape(FFTCfgWide<512>::T, lds_line_stride(512), g.ny).lds <= max_lds()) {
-    if (mode == 0) return launch_zinv_wide<0>(c, nf, in, out, prune, aux);
-    if (mode == 3) return launch_zinv_wide<3>(c, nf, in, out, prune, aux);
+  if (mode == 1 && (c.sched & 1)) {
+    switch (g.nz) {
+    case 32: return launch_zinv_low<32>(c, nf, in, out, prune, aux);
+    case 64: return launch_zinv_low<64>(c, nf, in, out, prune, aux);
+    case 128: return launch_zinv_low<128>(c, nf, in, out, prune, aux);
+    case 256: return launch_zinv_low<256>(c, nf, in, out, prune, aux);
+    default: break;  // the same schedule either way
+    }
   }
+  // the spline prefilter pass at 512 points runs the wide schedule (GHX_ZWIDE=0
+  // selects the default one); the plain and curl passes are slower on it
+  if (mode == 3 && g.nz == 512 && tile_want("GHX_ZWIDE", 1) != 0 &&
+      wide_shape(FFTCfgWide<512>::T, lds_line_stride(512), g.ny).lds <= max_lds())
+    return launch_zinv_wide<3>(c, nf, in, out, prune, aux);
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
@@ -1642,20 +1668,31 @@ hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
 #if GHX_PART == 4
 hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, int prune) {
   const GridK &g = c.gk;
-  if (g.nz == 512 && tile_want("GHX_ZLOW", 1) == 1) {
-    // 512-point lines on the four-points-per-lane schedule (128 lanes per
-    // line, four lines per 512-lane block): 124 registers per lane against
-    // 224, two blocks per CU, so the update's loads of v and C overlap more
-    // (0.71 -> 0.68 ms at 512^3); GHX_ZLOW=2 selects the default schedule
-    using Cfg = FFTCfgLow<512>;
-    constexpr int T = Cfg::T;
-    const PassShape s = tile_shape(T, lds_line_stride(512), g.ny, tile_pick("GHX_LZ", c.tile_z, 4), max_lds());
-    const int L = s.lines;
-    dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);
-    allow_lds(k_zfwd_rk_hd<512, Cfg>, s.lds);
-    GHX_LAUNCH_CHECK((k_zfwd_rk_hd<512, Cfg>), s, kTileBound, T, true);
-    hipLaunchKernelGGL((k_zfwd_rk_hd<512, Cfg>), grid, dim3(L * T), s.lds, c.stream, in, g, c.tw_z, rk, prune);
-    return hipGetLastError();
+  // 512-point lines take the four-points-per-lane schedule by default (128
+  // lanes per line, four lines per 512-lane block): 124 registers per lane
+  // against 224, two blocks per CU, so the update's loads of v and C overlap
+  // more (0.71 -> 0.70 ms at 512^3; GHX_ZLOW=2 selects the default schedule).
+  // Other sizes take it on request (FFTCtx::sched bit 0).
+  const bool low = (c.sched & 1) || (g.nz == 512 && tile_want("GHX_ZLOW", 1) == 1);
+  if (low) {
+    switch (g.nz) {
+#define GHX_LOW(n_)                                                                                     \
+  case n_: {                                                                                            \
+    using Cfg = FFTCfgLow<n_>;                                                                          \
+    constexpr int T = Cfg::T;                                                                           \
+    const PassShape s = tile_shape(T, lds_line_stride(n_), g.ny, tile_pick("GHX_LZ", c.tile_z, 4), max_lds()); \
+    const int L = s.lines;                                                                              \
+    dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);                                                      \
+    allow_lds(k_zfwd_rk_hd<n_, Cfg>, s.lds);                                                            \
+    GHX_LAUNCH_CHECK((k_zfwd_rk_hd<n_, Cfg>), s, kTileBound, T, true);                                  \
+    hipLaunchKernelGGL((k_zfwd_rk_hd<n_, Cfg>), grid, dim3(L * T), s.lds, c.stream, in, g, c.tw_z, rk, prune); \
+    return hipGetLastError();                                                                           \
+  }
+      GHX_LOW(32) GHX_LOW(64) GHX_LOW(128) GHX_LOW(256) GHX_LOW(512)
+#undef GHX_LOW
+    default:
+      break;  // the same schedule either way
+    }
   }
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
@@ -1705,10 +1742,46 @@ hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &r
 #endif
 
 #if GHX_PART == 1
+// the y passes on the four-points-per-lane schedule (FFTCtx::sched bit 1)
+template <int N>
+static hipError_t launch_y_low(const FFTCtx &c, bool inv, int nf, const FieldSet &in, const FieldSet &out, int kxlim,
+                               int prune) {
+  const GridK &g = c.gk;
+  using Cfg = FFTCfgLow<N>;
+  constexpr int T = Cfg::T;
+  const int nxp = p_row(g);
+  const PassShape s = tile_shape(T, lds_line_stride(N), g.nxh, tile_pick("GHX_LY", c.tile_y, inv ? 8 : 4), max_lds());
+  const int L = s.lines;
+  dim3 grid((g.nxh + L - 1) / L, g.nz, c.batch);
+  if (inv) {
+    allow_lds(k_yinv<N, Cfg>, s.lds);
+    GHX_LAUNCH_CHECK((k_yinv<N, Cfg>), s, kTileBound, T, true);
+    hipLaunchKernelGGL((k_yinv<N, Cfg>), grid, dim3(L * T), s.lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune);
+  } else {
+    allow_lds(k_yfwd<N, Cfg>, s.lds);
+    GHX_LAUNCH_CHECK((k_yfwd<N, Cfg>), s, kTileBound, T, true);
+    hipLaunchKernelGGL((k_yfwd<N, Cfg>), grid, dim3(L * T), s.lds, c.stream, in, out, nf, g, nxp, c.tw_y, kxlim, prune);
+  }
+  return hipGetLastError();
+}
+static bool y_low(const FFTCtx &c, bool inv, int nf, const FieldSet &in, const FieldSet &out, int kxlim, int prune,
+                  hipError_t *err) {
+  if (!(c.sched & 2)) return false;
+  switch (c.gk.ny) {
+  case 32: *err = launch_y_low<32>(c, inv, nf, in, out, kxlim, prune); return true;
+  case 64: *err = launch_y_low<64>(c, inv, nf, in, out, kxlim, prune); return true;
+  case 128: *err = launch_y_low<128>(c, inv, nf, in, out, kxlim, prune); return true;
+  case 256: *err = launch_y_low<256>(c, inv, nf, in, out, kxlim, prune); return true;
+  default: return false;  // the same schedule either way, or the wide 512-point pass
+  }
+}
+
 hipError_t yinv(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune) {
   const GridK &g = c.gk;
   const int kxlim = prune ? active_kx(g) : g.nxh;
   const int nxp = p_row(g);
+  hipError_t lerr;
+  if (y_low(c, true, nf, in, out, kxlim, prune, &lerr)) return lerr;
   switch (g.ny) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
@@ -1746,6 +1819,8 @@ hipError_t yfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
   const GridK &g = c.gk;
   const int kxlim = prune ? active_kx(g) : g.nxh;
   const int nxp = p_row(g);
+  hipError_t lerr;
+  if (y_low(c, false, nf, in, out, kxlim, prune, &lerr)) return lerr;
   switch (g.ny) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \

@@ -35,6 +35,13 @@ hipError_t op_cross_real(const GridK &g, hipStream_t st, double *r1, double *r2,
 // v = C + dt*(nu*lap + N + f)*rmp where kept, else 0; lap given (already -kk2 v).
 hipError_t op_rk_update(const GridK &g, hipStream_t st, cplx *v, const cplx *c, const cplx *lap,
                         const cplx *nl, const cplx *f, double nu, double dt, double rmp);
+// ord = 3 correction of include/bouss_rkstep2.f90:73-145, pointwise steps.
+// n = nonlhd3 of (w0, w1, w2 + xmom th); mode 0: out = lap(c) + n on kept
+// modes, else 0; mode 1: out = n (c unused, may be null).
+hipError_t op_bouss3_nonl(const GridK &g, hipStream_t st, int mode, const cplx *const *w, const cplx *th,
+                          double xmom, const cplx *const *c, cplx *const *out);
+// v = v + 2 x rmp on every mode (three fields)
+hipError_t op_bouss3_add(const GridK &g, hipStream_t st, cplx *const *v, const cplx *const *x, double rmp);
 // TG velocity in real space (examples/initialv.f90_tg:37-56)
 hipError_t op_tg_real(const GridK &g, hipStream_t st, double *r1, double *r2, int kdn, int kup);
 

@@ -292,6 +292,56 @@ __global__ void k_rk_update(GridK g, cplx *v, const cplx *c, const cplx *lap, co
   }
 }
 
+// The pointwise steps of the ord = 3 correction, include/bouss_rkstep2.f90:
+// 73-145.  n = nonlhd3 of (w4, w5, w6 + xmom th) in the three directions (the
+// gravity loops :78-85 and :115-122 followed by nonlhd3 :87-89, :124-126).
+// mode 0 (:90-111): out_d = -kk2 c_d + n_d on kept modes, else 0 (laplak3
+// then the masked sum); mode 1: out_d = n_d.
+__global__ void k_bouss3_nonl(GridK g, int mode, const cplx *w4, const cplx *w5, const cplx *w6, const cplx *th,
+                              double xmom, const cplx *c1, const cplx *c2, const cplx *c3, cplx *o1, cplx *o2,
+                              cplx *o3) {
+  GRID_STRIDE(m, ncplx(g)) {
+    Idx q = decompose(m, g);
+    double kx = wavenum(q.i, g.nx, g.dkx), ky = wavenum(q.j, g.ny, g.dky), kz = wavenum(q.k, g.nz, g.dkz);
+    const cplx t = th[m], c6 = w6[m];
+    const cplx a = w4[m], b = w5[m], c = c_make(c6.x + xmom * t.x, c6.y + xmom * t.y);
+    cplx n1 = c_make(-a.x, -a.y), n2 = c_make(-b.x, -b.y), n3 = c_make(-c.x, -c.y);
+    if (q.i != 0 || q.j != 0 || q.k != 0) {
+      const double kk2 = kk2_at(g, q);
+      const cplx dot = c_add(c_add(c_scale(a, kx), c_scale(b, ky)), c_scale(c, kz));
+      if (q.i != 0) n1 = c_add(n1, c_make(kx * dot.x / kk2, kx * dot.y / kk2));
+      if (q.j != 0) n2 = c_add(n2, c_make(ky * dot.x / kk2, ky * dot.y / kk2));
+      if (q.k != 0) n3 = c_add(n3, c_make(kz * dot.x / kk2, kz * dot.y / kk2));
+    }
+    if (mode == 0) {
+      const double kn2 = kn2_of(g, q.i, q.j, q.k);
+      if (kn2 <= g.kmax && kn2 >= g.tiny) {
+        const double kk2 = kk2_at(g, q);
+        const cplx u1 = c1[m], u2 = c2[m], u3 = c3[m];
+        n1 = c_add(c_make(-kk2 * u1.x, -kk2 * u1.y), n1);
+        n2 = c_add(c_make(-kk2 * u2.x, -kk2 * u2.y), n2);
+        n3 = c_add(c_make(-kk2 * u3.x, -kk2 * u3.y), n3);
+      } else {
+        n1 = n2 = n3 = c_make(0.0, 0.0);
+      }
+    }
+    o1[m] = n1;
+    o2[m] = n2;
+    o3[m] = n3;
+  }
+}
+
+// include/bouss_rkstep2.f90:130-143: v_d = v_d + 2 x_d rmp on every mode
+__global__ void k_bouss3_add(size_t n, cplx *v1, cplx *v2, cplx *v3, const cplx *x1, const cplx *x2,
+                             const cplx *x3, double rmp) {
+  GRID_STRIDE(m, n) {
+    cplx a = x1[m], b = x2[m], c = x3[m], p = v1[m], r = v2[m], s = v3[m];
+    v1[m] = c_make(p.x + 2.0 * a.x * rmp, p.y + 2.0 * a.y * rmp);
+    v2[m] = c_make(r.x + 2.0 * b.x * rmp, r.y + 2.0 * b.y * rmp);
+    v3[m] = c_make(s.x + 2.0 * c.x * rmp, s.y + 2.0 * c.y * rmp);
+  }
+}
+
 // examples/initialv.f90_tg:37-56 (real space, x fastest)
 __global__ void k_tg_real(GridK g, double *r1, double *r2, int kdn, int kup) {
   const double pi = 3.14159265358979323846;
@@ -670,6 +720,14 @@ hipError_t op_cross_real(const GridK &g, hipStream_t st, double *r1, double *r2,
 hipError_t op_rk_update(const GridK &g, hipStream_t st, cplx *v, const cplx *c, const cplx *lap, const cplx *nl,
                         const cplx *f, double nu, double dt, double rmp) {
   LAUNCH(k_rk_update, ncplx(g), st, g, v, c, lap, nl, f, nu, dt, rmp);
+}
+hipError_t op_bouss3_nonl(const GridK &g, hipStream_t st, int mode, const cplx *const *w, const cplx *th,
+                          double xmom, const cplx *const *c, cplx *const *out) {
+  const cplx *c1 = c ? c[0] : nullptr, *c2 = c ? c[1] : nullptr, *c3 = c ? c[2] : nullptr;
+  LAUNCH(k_bouss3_nonl, ncplx(g), st, g, mode, w[0], w[1], w[2], th, xmom, c1, c2, c3, out[0], out[1], out[2]);
+}
+hipError_t op_bouss3_add(const GridK &g, hipStream_t st, cplx *const *v, const cplx *const *x, double rmp) {
+  LAUNCH(k_bouss3_add, ncplx(g), st, ncplx(g), v[0], v[1], v[2], x[0], x[1], x[2], rmp);
 }
 hipError_t op_tg_real(const GridK &g, hipStream_t st, double *r1, double *r2, int kdn, int kup) {
   LAUNCH(k_tg_real, nreal(g), st, g, r1, r2, kdn, kup);

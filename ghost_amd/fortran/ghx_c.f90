@@ -396,6 +396,12 @@ MODULE ghx_c
       TYPE(C_PTR) :: v_in(4), c(4), v_out(4)
       TYPE(C_PTR), VALUE :: f
     END FUNCTION
+    INTEGER(C_INT) FUNCTION ghx_bouss_ord3(g, dt, xmom, c, v) BIND(C, name="ghx_bouss_ord3")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g
+      REAL(C_DOUBLE), VALUE :: dt, xmom
+      TYPE(C_PTR) :: c(4), v(3)
+    END FUNCTION
     INTEGER(C_INT) FUNCTION ghx_roth_rkstep2(g, o, dt, nu, omegaz, v_in, c, f, v_out) &
         BIND(C, name="ghx_roth_rkstep2")
       IMPORT :: C_INT, C_PTR, C_DOUBLE

@@ -799,16 +799,33 @@ CONTAINS
     END IF
   END SUBROUTINE mhd_rkstep2
 
-  ! -- include/bouss_rkstep2.f90 (:1-71) and rotbouss_rkstep2.f90 --------------
+  ! -- include/bouss_rkstep2.f90 and rotbouss_rkstep2.f90 ----------------------
   ! v(1:3), th <- C + dt (...) / o; c(1:4) holds (v, th) at the step start;
-  ! f(1:3), fs the forcings (both present or both absent).
-  SUBROUTINE bouss_rkstep2(o, dt, nu, kappa, xmom, xtemp, v, th, c, f, fs)
+  ! f(1:3), fs the forcings (both present or both absent).  ord is GHOST's
+  ! order (main3D's variable): with ord = 3 and o = 1 the correction of
+  ! :73-145 follows (c is left unchanged, where GHOST overwrites C1-C3).
+  SUBROUTINE bouss_rkstep2(o, dt, nu, kappa, xmom, xtemp, v, th, c, f, fs, ord)
     INTEGER, INTENT(IN) :: o
     REAL(KIND=GP), INTENT(IN) :: dt, nu, kappa, xmom, xtemp
     TYPE(ghx_field), INTENT(INOUT) :: v(3), th
     TYPE(ghx_field), INTENT(IN) :: c(4)
     TYPE(ghx_field), INTENT(IN), OPTIONAL :: f(3), fs
+    INTEGER, INTENT(IN), OPTIONAL :: ord
+    TYPE(C_PTR), TARGET :: pv(3), pc(4)
+    INTEGER :: k
     CALL bouss_body(o, dt, nu, kappa, xmom, xtemp, .FALSE., 0.0_GP, v, th, c, f, fs)
+    IF (PRESENT(ord)) THEN
+      IF (ord .EQ. 3 .AND. o .LE. 1) THEN
+        DO k = 1, 3
+          pv(k) = v(k)%p
+        END DO
+        DO k = 1, 4
+          pc(k) = c(k)%p
+        END DO
+        CALL ghx_check(ghx_bouss_ord3(ghx_grid_h, REAL(dt, C_DOUBLE), REAL(xmom, C_DOUBLE), pc, pv), &
+                       'bouss_ord3')
+      END IF
+    END IF
   END SUBROUTINE bouss_rkstep2
 
   SUBROUTINE rotbouss_rkstep2(o, dt, nu, kappa, xmom, xtemp, omegaz, v, th, c, f, fs)

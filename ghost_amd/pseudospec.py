@@ -376,9 +376,11 @@ class Grid:
         po, _e = self._ptrs(v_out)
         _lib.check(_lib.load().ghx_mhd_rkstep2(self._h, int(o), float(dt), float(nu), float(mu), pv, pc, pf, pm, po))
 
-    def bouss_rkstep2(self, o, dt, nu, kappa, xmom, xtemp, v_in, C_, f, v_out, omegaz=None):
+    def bouss_rkstep2(self, o, dt, nu, kappa, xmom, xtemp, v_in, C_, f, v_out, omegaz=None, ord=None):
         """include/bouss_rkstep2.f90 (rotbouss_rkstep2.f90 with omegaz) for one
-        value of o: fields (v_x, v_y, v_z, th); f the 4 forcings or None."""
+        value of o: fields (v_x, v_y, v_z, th); f the 4 forcings or None.
+        With ord = 3, o = 1 and no rotation, the correction of :73-145 follows
+        (v_out's velocity may then not alias C_)."""
         pv, _a = self._ptrs(v_in)
         pc, _b = self._ptrs(C_)
         pf, _c = self._ptrs(f)
@@ -387,6 +389,9 @@ class Grid:
         if omegaz is None:
             _lib.check(L_.ghx_bouss_rkstep2(self._h, int(o), float(dt), float(nu), float(kappa), float(xmom),
                                             float(xtemp), pv, pc, pf, po))
+            if ord == 3 and o <= 1:
+                pu, _g = self._ptrs(list(v_out)[:3])
+                _lib.check(L_.ghx_bouss_ord3(self._h, float(dt), float(xmom), pc, pu))
         else:
             _lib.check(L_.ghx_rotbouss_rkstep2(self._h, int(o), float(dt), float(nu), float(kappa), float(xmom),
                                                float(xtemp), float(omegaz), pv, pc, pf, po))

@@ -286,7 +286,20 @@ class Ensemble:
         return [(sizes[k], counts[k]) for k in range(ng.value)]
 
     def autotune(self, steps=3):
+        """Time every radix schedule and tile-width pair per size group; returns
+        the chosen (z, y) tile widths per group (schedules() has all three)."""
         ng = len(self.groups())
         tz, ty = (C.c_int * ng)(), (C.c_int * ng)()
         _lib.check(_lib.load().ghx_ensemble_autotune(self._h, int(steps), tz, ty))
         return [(tz[k], ty[k]) for k in range(ng)]
+
+    def schedules(self):
+        """Per size group: (schedule bits, z tile width, y tile width); bit 0
+        puts the z passes, bit 1 the y passes on four points per lane."""
+        ng = len(self.groups())
+        sc, tz, ty = (C.c_int * ng)(), (C.c_int * ng)(), (C.c_int * ng)()
+        _lib.check(_lib.load().ghx_ensemble_schedules(self._h, sc, tz, ty))
+        return [(sc[k], tz[k], ty[k]) for k in range(ng)]
+
+    def set_schedule(self, group, sched, tz=0, ty=0):
+        _lib.check(_lib.load().ghx_ensemble_set_schedule(self._h, int(group), int(sched), int(tz), int(ty)))

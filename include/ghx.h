@@ -280,15 +280,22 @@ int ghx_hd_set_rotation(ghx_hd *s, double omegaz);
  * given or both NULL; vout may equal vin or C. */
 int ghx_mhd_rkstep2(ghx_grid *g, int o, double dt, double nu, double mu, const void *const *vin, const void *const *C,
                     const void *const *f, const void *const *m, void *const *vout);
-/* include/bouss_rkstep2.f90:1-71 (without the ord = 3 correction of :73-145)
- * and include/rotbouss_rkstep2.f90 (Coriolis force of rotation omegaz about
- * z) for order index o on caller fields: vin, C, vout hold (v_x, v_y, v_z,
- * th); f holds (f_x, f_y, f_z, f_s) or is NULL; vout may equal vin or C. */
+/* include/bouss_rkstep2.f90:1-71 (the ord = 3 correction of :73-145 is
+ * ghx_bouss_ord3) and include/rotbouss_rkstep2.f90 (Coriolis force of
+ * rotation omegaz about z) for order index o on caller fields: vin, C, vout
+ * hold (v_x, v_y, v_z, th); f holds (f_x, f_y, f_z, f_s) or is NULL; vout may
+ * equal vin or C. */
 int ghx_bouss_rkstep2(ghx_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
                       const void *const *vin, const void *const *C, const void *const *f, void *const *vout);
 int ghx_rotbouss_rkstep2(ghx_grid *g, int o, double dt, double nu, double kappa, double xmom, double xtemp,
                          double omegaz, const void *const *vin, const void *const *C, const void *const *f,
                          void *const *vout);
+/* include/bouss_rkstep2.f90:73-145, the ord = 3 correction that GHOST applies
+ * after ghx_bouss_rkstep2 with o = 1: v (v_x, v_y, v_z) += 2 N(L(u) + N(u))
+ * dt^3/24, from C (v_x, v_y, v_z, th at the step start), with N the projected
+ * nonlinear term and buoyancy, L the Laplacian.  C is left unchanged (GHOST
+ * overwrites C1-C3); v may not alias C. */
+int ghx_bouss_ord3(ghx_grid *g, double dt, double xmom, const void *const *C, void *const *v);
 
 typedef struct ghx_mhd ghx_mhd;
 /* MHD solver state: v (3), a (3, vector potential), f (3), m (3). */
@@ -305,8 +312,9 @@ typedef struct ghx_bouss ghx_bouss;
  * loop of main3D.fpp:3202-3231): state v (3) and th, forcing f (3) and fs.
  * kappa: diffusivity; xmom, xtemp: the buoyancy and heat-current factors
  * already multiplied by bvfreq (main3D.fpp:973-974).  ord 1 to 3; ord = 3
- * without rotation needs the correction of bouss_rkstep2.f90:73-145, which
- * is not built (ghx_bouss_step returns GHX_ERR_ARG).  With xmom = xtemp = 0
+ * without rotation adds the correction of bouss_rkstep2.f90:73-145 after the
+ * last substep (the state then keeps modes outside the dealias set, as
+ * GHOST's does, and the next step runs unpruned).  With xmom = xtemp = 0
  * th is a passive scalar: the PHD solver (include/phd_rkstep{1,2}.f90). */
 int ghx_bouss_create(ghx_bouss **s, ghx_grid *g, int ord, double dt, double nu, double kappa, double xmom,
                      double xtemp);
@@ -342,9 +350,18 @@ int ghx_ensemble_touch(ghx_ensemble *e);
 int ghx_ensemble_step(ghx_ensemble *e, int nsteps);
 /* 0: launch every pass eagerly each step; 1 (default): replay a captured graph */
 int ghx_ensemble_set_graph(ghx_ensemble *e, int enabled);
-/* choose the z and y tile widths of each size group by timing `steps` steps
- * per candidate (member states are restored); tz/ty receive one value per group */
+/* choose the radix schedules and the z and y tile widths of each size group
+ * by timing `steps` steps per candidate (member states are restored): the z
+ * and y passes each on the default schedule or on four points per lane where
+ * that differs (sizes 32..256), times seven tile-width pairs; tz/ty receive
+ * one value per group, ghx_ensemble_schedules all three */
 int ghx_ensemble_autotune(ghx_ensemble *e, int steps, int *tz, int *ty);
+/* per group: schedule bits (1: z passes, 2: y passes on four points per
+ * lane), z and y tile widths (0: built-in); any pointer may be NULL */
+int ghx_ensemble_schedules(ghx_ensemble *e, int *sched, int *tz, int *ty);
+/* set one group's schedule bits and tile widths (a choice the autotuner made
+ * earlier, or a test's) */
+int ghx_ensemble_set_schedule(ghx_ensemble *e, int group, int sched, int tz, int ty);
 /* size groups: ngroups, and per group its size and member count */
 int ghx_ensemble_groups(ghx_ensemble *e, int *ngroups, int *sizes, int *counts);
 int ghx_ensemble_sync(ghx_ensemble *e);

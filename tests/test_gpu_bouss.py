@@ -96,8 +96,7 @@ def test_bouss_anisotropic_not_dealiased_start(gpu):
     assert rel(to_np(gth), rth) < 1e-12
 
 
-def test_bouss_pruned_equals_full_and_ord3_refused(gpu):
-    from ghost_amd import GhxError
+def test_bouss_pruned_equals_full(gpu):
     from ghost_amd.solvers import BoussSolver
     g, og = grids((48, 48, 48))
     v0, th0 = bouss_ic(og, 31)
@@ -116,8 +115,31 @@ def test_bouss_pruned_equals_full_and_ord3_refused(gpu):
             os.environ.pop("GHX_NO_PRUNE", None)
     for k in range(4):
         assert rel(outs[0][k], outs[1][k]) < 1e-14
-    # ord = 3 without rotation needs bouss_rkstep2.f90's correction: refused
-    # before any work is launched
-    s3 = BoussSolver(g, 2e-3, 4e-3, 4e-3, 1.0, 1.0, ord=3)
-    with pytest.raises(GhxError):
-        s3.step(1)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 32, 16)])
+def test_bouss_ord3_steps_match_oracle(gpu, shape):
+    """ord = 3 adds bouss_rkstep2.f90:73-145's correction after the last
+    substep.  A mean temperature makes its buoyancy land in the k = 0 mode of
+    v_z; the correction is not masked, so the modes outside the dealias set
+    are non-zero after each step and the next one runs unpruned."""
+    from ghost_amd.solvers import BoussSolver
+    g, og = grids(shape)
+    v0, th0 = bouss_ic(og, 41)
+    f, _ = og.initialfv_random(2.0, 3.0, 0.3, 42)
+    fs, _ = og.initialfv_random(1.0, 2.0, 0.2, 43)
+    dt, nu, kappa, xmom, xtemp = 3e-3, 4e-3, 5e-3, 2.0, 2.0
+    s = BoussSolver(g, dt, nu, kappa, xmom, xtemp, ord=3)
+    s.set_state([to_gpu(x) for x in v0], to_gpu(th0))
+    s.set_forcing([to_gpu(x) for x in f], to_gpu(fs[0]))
+    s.step(5)
+    gv, gth = s.state()
+    rv, rth = [x.copy() for x in v0], th0.copy()
+    for _ in range(5):
+        og.bouss_step(rv, rth, dt, nu, kappa, xmom, xtemp, f=f, fs=fs[0], ord=3)
+    for k in range(3):
+        assert rel(to_np(gv[k]), rv[k]) < 1e-12, k
+    assert rel(to_np(gth), rth) < 1e-12
+    # the correction is there: its k = 0 buoyancy term on v_z is non-zero
+    assert abs(rv[2][0, 0, 0]) > 0
+    assert abs(to_np(gv[2])[0, 0, 0] - rv[2][0, 0, 0]) <= 1e-12 * abs(rv[2][0, 0, 0])

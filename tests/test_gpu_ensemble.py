@@ -65,11 +65,14 @@ def test_ensemble_members_match_oracle(gpu):
             assert rel(to_np(got[k]), ref[k]) < 1e-12, (m, k)
 
 
-def test_ensemble_graph_replay_equals_eager(gpu):
+@pytest.mark.parametrize("branch", ["1", "0"])
+def test_ensemble_graph_replay_equals_eager(gpu, branch, monkeypatch):
     """Several replays with the program's own work in between, each checked
-    against eager launches of the same step."""
+    against eager launches of the same step; the graph captures the size
+    groups as parallel branches (default) or one after another."""
     import torch
     from ghost_amd.solvers import Ensemble
+    monkeypatch.setenv("GHX_ENS_BRANCH", branch)
     init = members()
     eg = make(Ensemble, init, graph=True)
     ee = make(Ensemble, init, graph=False)
@@ -86,16 +89,52 @@ def test_ensemble_graph_replay_equals_eager(gpu):
 
 
 def test_ensemble_autotune_keeps_states_and_results(gpu):
+    """Tile widths do not change the arithmetic (bit-identical steps); the
+    radix schedules the autotuner may also pick do, so a group on another
+    schedule is held to the oracle instead."""
     import torch
     from ghost_amd.solvers import Ensemble
     init = members()
     ea = make(Ensemble, init)
     tiles = ea.autotune(steps=2)
     assert len(tiles) == 4 and all(tz > 0 and ty > 0 for tz, ty in tiles)
+    sched = ea.schedules()
+    assert [(tz, ty) for _, tz, ty in sched] == tiles
+    by_size = {n: sc for (n, _), (sc, _, _) in zip(ea.groups(), sched)}
+    assert by_size[48] == 0 and by_size[96] == 0  # one schedule at these sizes
     eb = make(Ensemble, init)
     ea.step(3)
     eb.step(3)
-    for m in range(len(SIZES)):
+    for m, (og, v0) in enumerate(init):
         a, b = ea.velocity(m), eb.velocity(m)
+        if by_size[SIZES[m]] == 0:
+            for k in range(3):
+                assert torch.equal(a[k], b[k]), (m, k)
+        else:
+            ref = [x.copy() for x in v0]
+            for _ in range(3):
+                og.hd_step(ref, 2e-3, NUS[m], ord=2)
+            for k in range(3):
+                assert rel(to_np(a[k]), ref[k]) < 1e-12, (m, k)
+
+
+@pytest.mark.parametrize("sched", [1, 2, 3])
+def test_ensemble_radix_schedules_match_oracle(gpu, sched):
+    """The z passes (bit 0) and the y passes (bit 1) on four points per lane
+    at 32^3 and 64^3 (radix 2, 4, 4 and 4, 4, 4 instead of 4, 8 and 8, 8)."""
+    from ghost_amd.solvers import Ensemble
+    init = members()
+    e = make(Ensemble, init)
+    for gi, (n, _) in enumerate(e.groups()):
+        e.set_schedule(gi, sched if n in (32, 64) else 0)
+    assert {sc for (n, _), (sc, _, _) in zip(e.groups(), e.schedules()) if n in (32, 64)} == {sched}
+    e.step(4)
+    for m, (og, v0) in enumerate(init):
+        if SIZES[m] not in (32, 64):
+            continue
+        ref = [x.copy() for x in v0]
+        for _ in range(4):
+            og.hd_step(ref, 2e-3, NUS[m], ord=2)
+        got = e.velocity(m)
         for k in range(3):
-            assert torch.equal(a[k], b[k]), (m, k)
+            assert rel(to_np(got[k]), ref[k]) < 1e-12, (m, k)

@@ -10,12 +10,12 @@ from test_gpu_bouss import bouss_ic, grids, rel, to_gpu, to_np
 pytestmark = pytest.mark.gpu
 
 
-def loop(g, fn, state, ord, *args, forcing=None, **kw):
-    """GHOST's loop on device fields: C = v; for o = ord..1: v = rkstep2(o)."""
+def loop(g, fn, state, nord, *args, forcing=None, **kw):
+    """GHOST's loop on device fields: C = v; for o = nord..1: v = rkstep2(o)."""
     import torch
     v = [to_gpu(x) for x in state]
     c = [x.clone() for x in v]
-    for o in range(ord, 0, -1):
+    for o in range(nord, 0, -1):
         out = [torch.empty_like(x) for x in v]
         fn(o, *args, v, c, *(forcing or ()), out, **kw)
         v = out
@@ -38,15 +38,18 @@ def test_mhd_rkstep2_loop(gpu, ord, forced):
         assert rel(got[k], want) < 1e-12, k
 
 
-@pytest.mark.parametrize("ord,omegaz", [(2, None), (1, None), (2, 2.5), (3, 1.5)])
+@pytest.mark.parametrize("ord,omegaz", [(2, None), (1, None), (3, None), (2, 2.5), (3, 1.5)])
 def test_bouss_rkstep2_loop(gpu, ord, omegaz):
+    """ord = 3 without rotation includes the correction of
+    bouss_rkstep2.f90:73-145 (ghx_bouss_ord3 after o = 1)."""
     g, og = grids((32, 32, 32))
     v0, th0 = bouss_ic(og, 95)
     f, _ = og.initialfv_random(2.0, 3.0, 0.3, 96)
     fs, _ = og.initialfv_random(1.0, 2.0, 0.2, 97)
     dt, nu, kappa, xmom, xtemp = 3e-3, 4e-3, 5e-3, 2.0, 2.0
     forcing = ([to_gpu(x) for x in f] + [to_gpu(fs[0])],)
-    got = loop(g, g.bouss_rkstep2, v0 + [th0], ord, dt, nu, kappa, xmom, xtemp, forcing=forcing, omegaz=omegaz)
+    got = loop(g, g.bouss_rkstep2, v0 + [th0], ord, dt, nu, kappa, xmom, xtemp, forcing=forcing, omegaz=omegaz,
+               ord=ord)
     rv, rth = [x.copy() for x in v0], th0.copy()
     if omegaz is None:
         og.bouss_step(rv, rth, dt, nu, kappa, xmom, xtemp, f=f, fs=fs[0], ord=ord)
