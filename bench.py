@@ -129,8 +129,9 @@ KERNEL_NAME = {"ypass_inv": "ghx::k_yinv<N>", "xcross": "ghx::k_xcross2<N>",
 def stream_bandwidth(torch, gib=2, reps=5):
     """Achievable HBM bandwidth from the library's own copy, read and write
     kernels (ghx_stream_probe) over `gib` GiB buffers: bytes moved (read +
-    written) over the mean time of `reps` launches, best over a few grid sizes
-    and over plain and streaming stores.  Returns {name: GB/s}."""
+    written) over the mean time of `reps` launches, best over a few grid sizes,
+    over plain and streaming stores and over grid-stride and one-chunk-per-block
+    forms.  Returns {name: GB/s}."""
     from ghost_amd import _lib
     lib = _lib.load()
     nbytes = gib << 30
@@ -140,13 +141,15 @@ def stream_bandwidth(torch, gib=2, reps=5):
     b.fill_(0.0)
     sink = torch.zeros(1, dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
-    kinds = {"copy": (0, 2), "copy_nt": (3, 2), "read": (1, 1), "write": (2, 1), "write_nt": (4, 1)}
+    kinds = {"copy": (0, 2), "copy_nt": (3, 2), "read": (1, 1), "write": (2, 1), "write_nt": (4, 1),
+             "copy_chunk": (5, 2), "copy_chunk_nt": (6, 2), "read_chunk": (7, 1), "write_chunk": (8, 1),
+             "write_chunk_nt": (9, 1)}
     out = {}
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for name, (kind, moved) in kinds.items():
         best = 0.0
-        for blocks in (1024, 2048, 4096, 8192, 16384):
-            dst = sink.data_ptr() if kind == 1 else b.data_ptr()
+        for blocks in (1024, 2048, 4096, 8192, 16384, 32768):
+            dst = sink.data_ptr() if kind in (1, 7) else b.data_ptr()
             src = a.data_ptr()
             _lib.check(lib.ghx_stream_probe(kind, dst, src, nbytes, blocks, stream))
             torch.cuda.synchronize()
@@ -682,7 +685,7 @@ def main():
     del v, s
     torch.cuda.empty_cache()
     bw = stream_bandwidth(torch)
-    copy_gbs = max(bw["copy"], bw["copy_nt"])
+    copy_gbs = max(bw["copy"], bw["copy_nt"], bw["copy_chunk"], bw["copy_chunk_nt"])
     log(f"yardstick kernels: {bw}; dominant kernel {dom} at {achieved:.1f} GB/s")
     rates = transform_rates(torch, n)
     log(f"plain transforms at {n}^3: {rates}")

@@ -1696,7 +1696,9 @@ extern "C" int ghx_hd_touch(ghx_hd *s) {
 // derivk3 scaled by 1/(nx ny nz), then c2r of the nine fields.  One fused
 // Z^-1 takes v to the nine scaled derivatives (the same operations as derivk3
 // then the scale), then Y^-1 and X^-1 of nine fields.
-static int velocity_gradients(ghx_grid *g, const void *const *v, double *const *grad9) {
+// prune: v is dealiased (zero outside the kept modes), so the passes skip
+// what is zero (the result is the same)
+static int velocity_gradients(ghx_grid *g, const void *const *v, double *const *grad9, int prune = 0) {
   int rc = plan_scratch(&g->plan, 18);
   if (rc) return rc;
   ghx_plan *p = &g->plan;
@@ -1707,9 +1709,9 @@ static int velocity_gradients(ghx_grid *g, const void *const *v, double *const *
     q.p[f] = p->q + f * slot;
     pp.p[f] = p->q + (9 + f) * slot;
   }
-  HIPCHK(ghx::zinv(p->ctx, 6, 9, in, q, 0, nullptr));
-  HIPCHK(ghx::yinv(p->ctx, 9, q, pp, 0));
-  HIPCHK(ghx::xinv(p->ctx, 9, pp, grad9, 0));
+  HIPCHK(ghx::zinv(p->ctx, 6, 9, in, q, prune, nullptr));
+  HIPCHK(ghx::yinv(p->ctx, 9, q, pp, prune));
+  HIPCHK(ghx::xinv(p->ctx, 9, pp, grad9, prune));
   return GHX_OK;
 }
 
@@ -1772,7 +1774,7 @@ extern "C" int ghx_sgs_targets(ghx_grid *g, ghx_grid *t, const void *const *v, d
 
 // The subgrid forcing of one substep: gradients of vin, tau from the network,
 // its transform, and f + P(-div tau) into the solver's buffers.
-static int sgs_force(ghx_hd *s, const cplx *const *vin, const cplx **fs) {
+static int sgs_force(ghx_hd *s, const cplx *const *vin, const cplx **fs, int prune) {
   ghx_grid *g = s->g;
   const size_t nc = g->nc, nr = g->nr;
   double *gr = static_cast<double *>(s->sgs_mem);
@@ -1791,7 +1793,7 @@ static int sgs_force(ghx_hd *s, const cplx *const *vin, const cplx **fs) {
   for (int k = 0; k < 6; ++k) tau6[k] = tr + k * nr;
   int rc;
   const void *v[3] = {vin[0], vin[1], vin[2]};
-  if ((rc = velocity_gradients(g, v, grad9))) return rc;
+  if ((rc = velocity_gradients(g, v, grad9, prune))) return rc;
   if ((rc = ghx_mlp_set_stream(s->sgs, g->stream))) return rc;
   const double *cg[9];
   for (int k = 0; k < 9; ++k) cg[k] = grad9[k];
@@ -1878,7 +1880,7 @@ static int hd_step_once(ghx_hd *s) {
     const uint8_t *fl = s->fline;
     const cplx *fs[3];
     if (s->sgs || s->sgs_nuk) {  // the forcing of this substep is f + P(-div tau(vin)) - nu(k) kk2 vin
-      int rc = sgs_force(s, vin, fs);
+      int rc = sgs_force(s, vin, fs, prune);
       if (rc) return rc;
       fo = fs;
       fl = s->sgs_fline;
@@ -2872,7 +2874,10 @@ static int ens_schedules(int n) { return (n == 32 || n == 64 || n == 128 || n ==
 
 extern "C" int ghx_ensemble_autotune(ghx_ensemble *e, int steps, int *tz_out, int *ty_out) {
   ARGCHK(e && steps > 0, "bad ensemble_autotune arguments");
-  static const int cand[][2] = {{4, 8}, {2, 4}, {4, 4}, {8, 8}, {8, 4}, {2, 8}, {4, 16}};
+  // (z, y) tile widths; the launchers clamp each to what the line length,
+  // the lines there are, the block bound and the LDS allow
+  static const int cand[][2] = {{4, 8},  {2, 4},  {4, 4},   {8, 8},   {8, 4},  {2, 8},
+                                {4, 16}, {8, 16}, {16, 8}, {16, 16}, {32, 16}, {32, 32}};
   ens_drop_graph(e);
   hipEvent_t a = nullptr, b = nullptr;
   HIPCHK(hipEventCreate(&a));

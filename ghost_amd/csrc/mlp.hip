@@ -51,26 +51,26 @@ DEV f4 mma16(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1
 DEV short bfbits(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 
 // GELU(x) = x Phi(x) with Phi the standard normal distribution.  Phi(x) =
-// 1/2 + x P(x^2) for |x| < 4, P the degree-9 least-squares fit of
-// (Phi(x) - 1/2)/x in Chebyshev polynomials of x^2 on [0, 16], here in
-// monomial form (|error| <= 5e-6 inside); beyond 4 Phi is taken as 0 or 1
-// (error <= 3.2e-5).  Both are far below the bfloat16 rounding the
-// activations then receive, and the form needs no transcendental: the
-// activations are the inference kernel's VALU cost.
+// 1/2 + x P(min(x^2, 16)), clamped to [0, 1], with P the degree-7 minimax fit
+// (Lawson's iteration, weighted by x) of (Phi(x) - 1/2)/x in x^2 on [0, 16].
+// Inside |x| < 4 the fit's error is below 3.2e-5; beyond, the clamped line
+// reaches 0 or 1 within |x| < 4.01 and Phi differs from them by less than
+// 3.2e-5, so |Phi error| < 3.2e-5 and |GELU error| < 1.3e-4 everywhere.
+// Both are far below the bfloat16 rounding the activations then receive,
+// and the form needs no transcendental and no branch (the clamp is the
+// output modifier of the last fma): the activations are the inference
+// kernel's VALU cost, 11 operations each.
 DEV float phi_poly(float x) {
   const float t = fminf(x * x, 16.0f);
-  float p = -4.4071326e-12f;
-  p = fmaf(p, t, 4.1298920e-10f);
-  p = fmaf(p, t, -1.7545830e-08f);
-  p = fmaf(p, t, 4.5426575e-07f);
-  p = fmaf(p, t, -8.1727212e-06f);
-  p = fmaf(p, t, 1.1055289e-04f);
-  p = fmaf(p, t, -1.1762398e-03f);
-  p = fmaf(p, t, 9.9605145e-03f);
-  p = fmaf(p, t, -6.6484347e-02f);
-  p = fmaf(p, t, 3.9894181e-01f);
-  const float ph = fmaf(x, p, 0.5f);
-  return fabsf(x) < 4.0f ? ph : (x > 0.0f ? 1.0f : 0.0f);
+  float p = -1.58079339e-09f;
+  p = fmaf(p, t, 1.21711508e-07f);
+  p = fmaf(p, t, -4.10087569e-06f);
+  p = fmaf(p, t, 8.06675016e-05f);
+  p = fmaf(p, t, -1.04820507e-03f);
+  p = fmaf(p, t, 9.66487639e-03f);
+  p = fmaf(p, t, -6.61753789e-02f);
+  p = fmaf(p, t, 3.98847520e-01f);
+  return __builtin_amdgcn_fmed3f(fmaf(x, p, 0.5f), 0.0f, 1.0f);
 }
 DEV float gelu(float x) { return x * phi_poly(x); }
 DEV float gelu_d(float x) { return phi_poly(x) + x * 0.39894228040143268f * __expf(-0.5f * x * x); }
@@ -244,6 +244,12 @@ __global__ __launch_bounds__(IBLOCK) void k_mlp_fields(Ptr9 g, double delta, con
   const long long groups = (n + 63) / 64;
   const long long first = (long long)blockIdx.x * IW + w, stride = (long long)gridDim.x * IW;
   const float dn = (float)(delta - (double)nm.mx[9]) / nm.sx[9];
+  // the features' scales as reciprocals: one multiply per feature and trip
+  // instead of a division (the product differs from the quotient by at most
+  // an ulp of float, far below the bfloat16 rounding that follows)
+  float isx[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) isx[f] = 1.0f / nm.sx[f];
   auto fetch = [&](long long gi, double (&raw)[9]) {
     const long long p = gi * 64 + lane;
     const long long pp = p < n ? p : n - 1;
@@ -256,8 +262,8 @@ __global__ __launch_bounds__(IBLOCK) void k_mlp_fields(Ptr9 g, double delta, con
     if (gi + stride < groups) fetch(gi + stride, nxt);
     bf8 f0, f1;
 #pragma unroll
-    for (int f = 0; f < 8; ++f) f0[f] = (__bf16)((float)(cur[f] - (double)nm.mx[f]) / nm.sx[f]);
-    f1[0] = (__bf16)((float)(cur[8] - (double)nm.mx[8]) / nm.sx[8]);
+    for (int f = 0; f < 8; ++f) f0[f] = (__bf16)((float)(cur[f] - (double)nm.mx[f]) * isx[f]);
+    f1[0] = (__bf16)((float)(cur[8] - (double)nm.mx[8]) * isx[8]);
     f1[1] = (__bf16)dn;
 #pragma unroll
     for (int f = 2; f < 8; ++f) f1[f] = (__bf16)0.0f;

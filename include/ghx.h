@@ -353,7 +353,7 @@ int ghx_ensemble_set_graph(ghx_ensemble *e, int enabled);
 /* choose the radix schedules and the z and y tile widths of each size group
  * by timing `steps` steps per candidate (member states are restored): the z
  * and y passes each on the default schedule or on four points per lane where
- * that differs (sizes 32..256), times seven tile-width pairs; tz/ty receive
+ * that differs (sizes 32..256), times twelve tile-width pairs; tz/ty receive
  * one value per group, ghx_ensemble_schedules all three */
 int ghx_ensemble_autotune(ghx_ensemble *e, int steps, int *tz, int *ty);
 /* per group: schedule bits (1: z passes, 2: y passes on four points per
@@ -504,7 +504,9 @@ int ghx_prof_read(int slot, const char **name, int *count, double *total_ms, dou
  * (a multiple of 16) on `blocks` blocks of 256 lanes, asynchronous on the
  * stream.  kind 0: copy src -> dst; 1: read src (dst: an 8-byte device sink that
  * the kernel writes only for a sum no input produces); 2: write dst;
- * 3: copy with streaming stores; 4: write with streaming stores. */
+ * 3: copy with streaming stores; 4: write with streaming stores.  Kinds 5-9
+ * are kinds 0, 3, 1, 2, 4 with each block moving one contiguous chunk of
+ * bytes / blocks instead of a grid-stride loop. */
 int ghx_stream_probe(int kind, void *dst, const void *src, size_t bytes, int blocks, void *stream);
 
 /* Launch shape of an FFT pass (no counterpart in GHOST; host-only, no GPU

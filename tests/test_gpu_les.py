@@ -106,6 +106,40 @@ def test_closure_forcing_in_the_step_matches_oracle(gpu):
     assert rel(got[0], plain[0]) > 1e-8
 
 
+def test_closure_steps_pruned_equal_full(gpu):
+    """After the first step the LES state is dealiased, and the velocity
+    gradients the network reads skip the zero modes (pruned passes); the
+    steps must equal the unpruned ones (GHX_NO_PRUNE) to round-off."""
+    import os
+    from ghost_amd import les
+    from ghost_amd.closure import MLP
+    from ghost_amd.pseudospec import Grid
+    from ghost_amd.solvers import HDSolver
+    from oracle import oracle as O
+    n, dt, nu, delta = 48, 2e-3, 5e-3, 0.3
+    g, og = Grid(n), O.Grid(n)
+    v, _ = og.initialv_k4exp(3.0, 1.0, 73)
+    f, _ = og.initialfv_random(2.0, 3.0, 0.5, 74)
+    m = MLP(6)
+    grad = les.velocity_gradients(g, [to_gpu(x) for x in v])
+    m.fit_norm(grad, delta, [x * 0.01 + 1e-3 * (c + 1) for c, x in enumerate(grad[:6])])
+    outs = []
+    for env in (None, "1"):
+        if env:
+            os.environ["GHX_NO_PRUNE"] = env
+        try:
+            s = HDSolver(g, dt=dt, nu=nu, ord=2)
+            s.set_velocity([to_gpu(x) for x in v])
+            s.set_forcing([to_gpu(x) for x in f])
+            s.set_sgs(m, delta)
+            s.step(3)
+            outs.append([to_np(x) for x in s.velocity()])
+        finally:
+            os.environ.pop("GHX_NO_PRUNE", None)
+    for k in range(3):
+        assert rel(outs[0][k], outs[1][k]) < 1e-14, k
+
+
 def test_spectral_part_in_the_step_matches_oracle(gpu):
     """The closure's spectral part: -nu(shell) kk2 v_in joins the forcing of
     each substep (ord 2, so v_in differs between the substeps); with a
