@@ -123,6 +123,18 @@ static hipError_t launch_check(const void *kern, const char *name, const PassSha
     if (ce_ != hipSuccess) return ce_;                                                         \
   } while (0)
 
+#define GHX_X2_SIZES(X) X(24) X(32) X(48) X(64) X(96) X(128) X(256) X(512)
+static bool x2_size(int nx) {
+  switch (nx) {
+#define GHX_CASE(n_) case n_:
+    GHX_X2_SIZES(GHX_CASE)
+#undef GHX_CASE
+    return true;
+  default:
+    return false;
+  }
+}
+
 #if GHX_PART == 1
 // The launch shape a pass of line length n takes (kind 0: z and y tile
 // passes; 1: the Boussinesq z forward, whose LDS rows hold n more points; 2:
@@ -138,8 +150,7 @@ extern "C" int ghx_pass_shape(int kind, int n, int nlines, int want, long long l
   case n_:                                                                             \
     if (kind == 0 || kind == 1) T = FFTCfg<n_>::T, rows = lds_line_stride(n_) + (kind == 1 ? n_ : 0); \
     if (kind == 2 && FFTCfgWide<n_>::E != FFTCfg<n_>::E) T = FFTCfgWide<n_>::T, rows = lds_line_stride(n_); \
-    if (kind == 3 && (n_ == 32 || n_ == 64 || n_ == 128 || n_ == 256 || n_ == 512))    \
-      T = FFTCfg<n_>::T, rows = lds_line_stride(n_);                                   \
+    if (kind == 3 && x2_size(n_)) T = FFTCfg<n_>::T, rows = lds_line_stride(n_);      \
     if (kind == 4) T = FFTCfgLow<n_>::T, rows = lds_line_stride(n_);                 \
     break;
     GHX_SIZES(GHX_CASE)
@@ -179,13 +190,12 @@ static int tile_pick(const char *env, int ctx, int dflt) {
   return e > 0 ? e : (ctx > 0 ? ctx : dflt);
 }
 
-// Row lengths with a two-for-one x kernel (a full-length schedule of E = 8
-// points per lane and at most 64 lanes; ghx_pass_shape lists the same sizes);
+// Row lengths with a two-for-one x kernel (a full-length schedule of 6 or 8
+// points per lane and at most 64 lanes; ghx_pass_shape takes the same list);
 // GHX_XC2=2 selects the half-length kernels instead, for comparisons.
-#define GHX_X2_SIZES(X) X(32) X(64) X(128) X(256) X(512)
 static bool use_x2(int nx) {
   if (tile_want("GHX_XC2", 1) != 1) return false;
-  return nx == 32 || nx == 64 || nx == 128 || nx == 256 || nx == 512;
+  return x2_size(nx);
 }
 
 template <typename K> static void allow_lds(K kern, size_t bytes) {

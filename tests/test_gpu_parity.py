@@ -18,6 +18,9 @@ SHAPES = [(8, 8, 8), (6, 4, 10), (12, 12, 12), (16, 8, 32), (32, 32, 32), (48, 4
 
 # the two configurations' own grids (one transform each way, oracle on all threads)
 FULL_SHAPES = [(512, 512, 512), (512, 512, 256)]
+# one GPU's share of the 512^3 transform in an 8-way slab decomposition: the
+# y and x passes on 64 z-planes, and the z pass on the 64-wide x slab
+SLAB_SHAPES = [(512, 512, 64), (64, 512, 512)]
 
 
 def rel(a, b):
@@ -68,9 +71,10 @@ def test_r2c_c2r_match_oracle(gpu, shape):
     assert bool((cin == keep).all()), "c2r must preserve its input"
 
 
-@pytest.mark.parametrize("shape", FULL_SHAPES)
+@pytest.mark.parametrize("shape", FULL_SHAPES + SLAB_SHAPES)
 def test_r2c_c2r_full_size_match_oracle(gpu, shape):
-    """The plain transforms at the workload sizes (1 GB per field at 512^3)."""
+    """The plain transforms at the workload sizes (1 GB per field at 512^3)
+    and at the per-GPU slab shapes of the 8-way decomposition."""
     import torch
     from oracle import oracle as O
     O.set_threads(16)

@@ -12,7 +12,7 @@ import pytest
 import ghost_amd
 
 SIZES = [2, 3, 4, 5, 6, 8, 10, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1024]
-X2 = [32, 64, 128, 256, 512]
+X2 = [24, 32, 48, 64, 96, 128, 256, 512]
 WANTS = [-7, 0, 1, 2, 3, 4, 5, 8, 9, 16, 31, 32, 33, 64, 512, 1 << 20]
 LIMITS = [65536, 163840]
 TILE_BOUND, ROW_BOUND = 512, 256
