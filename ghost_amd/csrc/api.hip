@@ -316,7 +316,9 @@ extern "C" int ghx_plan_set_stream(ghx_plan *plan, void *stream) {
   return GHX_OK;
 }
 
-static int do_r2c(ghx_plan *p, int nf, const double *const *in, void *const *out) {
+// prune: the caller reads only the kept modes of `out` (the dealias set), so
+// the y and z passes skip the rest (those modes of `out` keep what they held)
+static int do_r2c(ghx_plan *p, int nf, const double *const *in, void *const *out, int prune = 0) {
   ARGCHK(p && in && out && nf >= 1 && nf <= ghx::kMaxFields, "bad r2c arguments");
   int rc = plan_scratch(p, 2 * nf);
   if (rc) return rc;
@@ -331,8 +333,8 @@ static int do_r2c(ghx_plan *p, int nf, const double *const *in, void *const *out
   const double sc = (double)nc * sizeof(cplx), sr = (double)p->nx * p->ny * p->nz * sizeof(double);
   timer_begin(p);
   PROF(P_XFWD, nf * (sr + sc), p->ctx.stream, ghx::xfwd(p->ctx, nf, in, pp));
-  PROF(P_YFWD, nf * 2 * sc, p->ctx.stream, ghx::yfwd(p->ctx, nf, pp, q));
-  PROF(P_ZFWD, nf * 2 * sc, p->ctx.stream, ghx::zfwd(p->ctx, nf, q, o));
+  PROF(P_YFWD, nf * 2 * sc, p->ctx.stream, ghx::yfwd(p->ctx, nf, pp, q, prune));
+  PROF(P_ZFWD, nf * 2 * sc, p->ctx.stream, ghx::zfwd(p->ctx, nf, q, o, prune));
   timer_end(p);
   return GHX_OK;
 }
@@ -1801,7 +1803,8 @@ static int sgs_force(ghx_hd *s, const cplx *const *vin, const cplx **fs, int pru
   const double *ct[6];
   void *tcv[6];
   for (int k = 0; k < 6; ++k) { ct[k] = tau6[k]; tcv[k] = tc + k * nc; }
-  if ((rc = do_r2c(&g->plan, 6, ct, tcv))) return rc;
+  // k_sgs_force reads the stress spectra on the kept modes only
+  if ((rc = do_r2c(&g->plan, 6, ct, tcv, 1))) return rc;
   const cplx *tau[6];
   for (int k = 0; k < 6; ++k) tau[k] = tc + k * nc;
   HIPCHK(ghx::op_sgs_force(g->gk, g->stream, tau, s->forced ? f0 : nullptr, out, s->sgs_nuk ? vin : nullptr,

@@ -101,7 +101,7 @@ hipError_t xfwd(const FFTCtx &c, int nf, const double *const *in, const FieldSet
 // -> out.p[0..2] = FFT[(A x B)/N^6] in GHOST's operand order.
 hipError_t xcross(const FFTCtx &c, int mode, const FieldSet &in, const FieldSet &out, int prune = 0);
 // Z forward, Q -> S (mode 0), or fused with projection + RK update (HD/MHD).
-hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out);
+hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune = 0);
 hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, int prune = 0);
 hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &rk, int prune = 0);
 // Fused X^-1 . dot product . X for advect3: in.p[0..2] = A, in.p[3..5] = B

@@ -495,7 +495,12 @@ GHX_DEV void zfwd_load_diff(cplx (&v)[E], cplx *lds, cplx *row, int stride, int 
 }
 
 template <int N>
-__global__ __launch_bounds__(kTileBound) void k_zfwd(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twt) {
+// prune: only the kept modes of the output are wanted (the caller reads no
+// other), so only the dealias disc's lines are loaded (the y forward wrote no
+// others) and only their kept modes are written; the rest of `out` keeps
+// whatever it held.
+__global__ __launch_bounds__(kTileBound) void k_zfwd(FieldSet in, FieldSet out, int nf, GridK g, const cplx *__restrict__ twt,
+                                             int prune) {
   using F = LineFFT<N, -1>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
@@ -506,16 +511,21 @@ __global__ __launch_bounds__(kTileBound) void k_zfwd(FieldSet in, FieldSet out, 
   int bx, by;
   xcd_tile(bx, by);
   const int i = by, j0 = bx * L, j = j0 + lt;
-  const bool ok = j < g.ny;
+  const unsigned lmask = prune ? tile_lines_active(g, i, j0, L) : ~0u;
+  if (lmask == 0u) return;  // uniform over the block
+  const bool ok = j < g.ny && ((lmask >> lt) & 1u);
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
   const size_t sline = ((size_t)i * g.ny + (ok ? j : 0)) * N;
   for (int f = 0; f < nf; ++f) {
     cplx v[E];
-    zfwd_load<N, E>(v, lds, row, stride, L, t, tw, in.p[f], g, i, j0);
+    zfwd_load<N, E>(v, lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
     if (ok) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) out.p[f][sline + t + T * m] = v[m];
+      for (int m = 0; m < E; ++m) {
+        const int k = t + T * m;
+        if (!prune || kn2_of(g, i, j, k) <= g.kmax) out.p[f][sline + k] = v[m];
+      }
     }
     __syncthreads();
   }
@@ -1652,7 +1662,7 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
 #endif
 
 #if GHX_PART == 3
-hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out) {
+hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out, int prune) {
   const GridK &g = c.gk;
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
@@ -1662,7 +1672,7 @@ hipError_t zfwd(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
     const int L = s.lines; const size_t lds = s.lds;                            \
     dim3 grid((g.ny + L - 1) / L, g.nxh);                                                   \
     allow_lds(k_zfwd<n_>, lds);                                                             \
-    GHX_LAUNCH_CHECK((k_zfwd<n_>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zfwd<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z); \
+    GHX_LAUNCH_CHECK((k_zfwd<n_>), s, kTileBound, T, true); hipLaunchKernelGGL((k_zfwd<n_>), grid, dim3(L * T), lds, c.stream, in, out, nf, g, c.tw_z, prune); \
     break;                                                                                  \
   }
     GHX_SIZES(GHX_CASE)
