@@ -183,6 +183,12 @@ static int tile_want(const char *env, int dflt) {
   return v > 0 ? v : dflt;
 }
 
+// An on/off switch for measurements: its value when set (0 included), else dflt.
+static int env_switch(const char *env, int dflt) {
+  const char *e = getenv(env);
+  return e && *e ? atoi(e) : dflt;
+}
+
 // tile width: the environment override (tuning sweeps), else the context's
 // choice, else the built-in default
 static int tile_pick(const char *env, int ctx, int dflt) {
@@ -194,7 +200,7 @@ static int tile_pick(const char *env, int ctx, int dflt) {
 // points per lane and at most 64 lanes; ghx_pass_shape takes the same list);
 // GHX_XC2=2 selects the half-length kernels instead, for comparisons.
 static bool use_x2(int nx) {
-  if (tile_want("GHX_XC2", 1) != 1) return false;
+  if (env_switch("GHX_XC2", 1) != 1) return false;
   return x2_size(nx);
 }
 
@@ -1617,7 +1623,7 @@ hipError_t zinv(const FFTCtx &c, int mode, int nf, const FieldSet &in, const Fie
   }
   // the spline prefilter pass at 512 points runs the wide schedule (GHX_ZWIDE=0
   // selects the default one); the plain and curl passes are slower on it
-  if (mode == 3 && g.nz == 512 && tile_want("GHX_ZWIDE", 1) != 0 &&
+  if (mode == 3 && g.nz == 512 && env_switch("GHX_ZWIDE", 1) != 0 &&
       wide_shape(FFTCfgWide<512>::T, lds_line_stride(512), g.ny).lds <= max_lds())
     return launch_zinv_wide<3>(c, nf, in, out, prune, aux);
   switch (g.nz) {
@@ -1693,7 +1699,7 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, i
   // against 224, two blocks per CU, so the update's loads of v and C overlap
   // more (0.71 -> 0.70 ms at 512^3; GHX_ZLOW=2 selects the default schedule).
   // Other sizes take it on request (FFTCtx::sched bit 0).
-  const bool low = (c.sched & 1) || (g.nz == 512 && tile_want("GHX_ZLOW", 1) == 1);
+  const bool low = (c.sched & 1) || (g.nz == 512 && env_switch("GHX_ZLOW", 1) == 1);
   if (low) {
     switch (g.nz) {
 #define GHX_LOW(n_)                                                                                     \
@@ -1807,7 +1813,7 @@ hipError_t yinv(const FFTCtx &c, int nf, const FieldSet &in, const FieldSet &out
   case n_: {                                                                                \
     constexpr int T = FFTCfg<n_>::T;                                                        \
     if constexpr (FFTCfgWide<n_>::E != FFTCfg<n_>::E)                                        \
-    if (tile_want("GHX_YWIDE", 1) == 1 &&                                                    \
+    if (env_switch("GHX_YWIDE", 1) == 1 &&                                                   \
         wide_shape(FFTCfgWide<n_>::T, lds_line_stride(n_), g.nxh).lds <= max_lds()) {         \
       /* 16 points per lane: 8-line tiles from 256-lane blocks, two blocks per CU */       \
       constexpr int TW = FFTCfgWide<n_>::T;                                                  \
