@@ -51,9 +51,9 @@ GHX_FFTCFG(80, 40, 3, 5, 2, 8)
 GHX_FFTCFG(96, 6, 6, 3, 2, 2, 2, 2, 2)
 GHX_FFTCFG(128, 8, 3, 2, 8, 8)
 GHX_FFTCFG(160, 40, 3, 5, 4, 8)
-GHX_FFTCFG(192, 24, 3, 3, 8, 8)
+GHX_FFTCFG(192, 6, 7, 3, 2, 2, 2, 2, 2, 2)
 GHX_FFTCFG(256, 8, 3, 4, 8, 8)
-GHX_FFTCFG(384, 24, 4, 3, 2, 8, 8)
+GHX_FFTCFG(384, 6, 8, 3, 2, 2, 2, 2, 2, 2, 2)
 GHX_FFTCFG(512, 8, 3, 8, 8, 8)
 GHX_FFTCFG(768, 24, 4, 3, 4, 8, 8)
 GHX_FFTCFG(1024, 8, 4, 2, 8, 8, 8)

@@ -123,7 +123,7 @@ static hipError_t launch_check(const void *kern, const char *name, const PassSha
     if (ce_ != hipSuccess) return ce_;                                                         \
   } while (0)
 
-#define GHX_X2_SIZES(X) X(24) X(32) X(48) X(64) X(96) X(128) X(256) X(512)
+#define GHX_X2_SIZES(X) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 static bool x2_size(int nx) {
   switch (nx) {
 #define GHX_CASE(n_) case n_:

@@ -12,7 +12,7 @@ import pytest
 import ghost_amd
 
 SIZES = [2, 3, 4, 5, 6, 8, 10, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1024]
-X2 = [24, 32, 48, 64, 96, 128, 256, 512]
+X2 = [24, 32, 48, 64, 96, 128, 192, 256, 384, 512]
 WANTS = [-7, 0, 1, 2, 3, 4, 5, 8, 9, 16, 31, 32, 33, 64, 512, 1 << 20]
 LIMITS = [65536, 163840]
 TILE_BOUND, ROW_BOUND = 512, 256
@@ -74,10 +74,11 @@ def test_unknown_sizes_and_kinds():
     assert shape(0, 64, 0, 8, 65536)[0] == -1
 
 
-def test_24_point_schedules_fit_64k():
-    """The 24-points-per-lane schedules (192, 384 points) ask 111 KB for a
-    full 256-lane x block; under a 64 KiB limit the block holds fewer rows."""
-    rc, R, block, lds, T = shape(4, 192, 1, 0, 65536)
-    assert rc == 0 and T == 8 and R == 16 and lds <= 65536
+def test_row_block_shrinks_under_lds_limit():
+    """A full 256-lane x block of 192-point rows (6 points per lane, 32 lanes,
+    8 rows) asks 8 padded rows of LDS, 27.8 KB; under a 16 KiB limit the
+    block holds half the rows."""
     rc, R, block, lds, T = shape(4, 192, 1, 0, 163840)
-    assert rc == 0 and R == 32 and lds == 32 * 217 * 16
+    assert rc == 0 and T == 32 and R == 8 and lds == 8 * 217 * 16
+    rc, R, block, lds, T = shape(4, 192, 1, 0, 16384)
+    assert rc == 0 and T == 32 and R == 4 and block == 128 and lds == 4 * 217 * 16 <= 16384
