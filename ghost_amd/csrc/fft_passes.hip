@@ -297,12 +297,12 @@ GHX_DEV unsigned tile_lines_active(const GridK &g, int i, int j0, int L) {
 // ---------------------------------------------------------------------------
 // Z^-1: S -> Q.  Block = (kx index i, ky chunk j0..j0+L-1).
 // ---------------------------------------------------------------------------
+// The store half of zinv_emit: the transformed line v (natural order, lane t
+// holding points t + T m) into the block's LDS rows, then the tile to Q.
 template <int N, int E, typename Cfg = FFTCfg<N>>
-GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N, Cfg> &tw,
-                       cplx *dst, const GridK &g, int i, int j0, unsigned lmask) {
-  using F = LineFFT<N, +1, Cfg>;
-  constexpr int T = F::T;
-  F::run(v, row, t, tw);
+GHX_DEV void zinv_store(const cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, cplx *dst,
+                        const GridK &g, int i, int j0, unsigned lmask) {
+  constexpr int T = LineFFT<N, +1, Cfg>::T;
 #pragma unroll
   for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
   __syncthreads();
@@ -310,6 +310,57 @@ GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
   tile_store_rows<E, false>(lds, stride, L, dst + (size_t)i * ny + j0, (size_t)g.nxh * ny,
                      [=](int l) { return j0 + l < ny && ((lmask >> l) & 1u); });
   __syncthreads();
+}
+
+template <int N, int E, typename Cfg = FFTCfg<N>>
+GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N, Cfg> &tw,
+                       cplx *dst, const GridK &g, int i, int j0, unsigned lmask) {
+  LineFFT<N, +1, Cfg>::run(v, row, t, tw);
+  zinv_store<N, E, Cfg>(v, lds, row, stride, L, t, dst, g, i, j0, lmask);
+}
+
+// A vector field's z inverse and its curl's with five transforms instead of
+// six.  kx and ky are constant along a z line, so Z^-1(i ky a) = i ky Z^-1(a);
+// only the kz terms of rotor3 (pseudospec3D_hd.f90:124-200) need transforms of
+// their own:
+//   curl_x = i ky Z^-1(a_z) - Z^-1(i kz a_y)
+//   curl_y = Z^-1(i kz a_x) - i kx Z^-1(a_z)
+//   curl_z = i kx Z^-1(a_y) - i ky Z^-1(a_x)
+// ax, ay, az (the spectral lines, lane t holding points t + T m) are
+// transformed in place; the six results go to o[0..2] (a) and o[3..5] (curl a).
+template <int N, int E, typename Cfg>
+GHX_DEV void zinv_curl5(cplx (&ax)[E], cplx (&ay)[E], cplx (&az)[E], double kx, double ky, cplx *lds, cplx *row,
+                        int stride, int L, int t, const TwCache<N, Cfg> &tw, cplx *const *o, const GridK &g, int i,
+                        int j0, unsigned lmask) {
+  using F = LineFFT<N, +1, Cfg>;
+  constexpr int T = F::T;
+  zinv_emit<N, E, Cfg>(az, lds, row, stride, L, t, tw, o[2], g, i, j0, lmask);  // az := Z^-1(a_z)
+  {
+    cplx w[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) w[m] = c_ik(wavenum(t + T * m, g.nz, g.dkz), ay[m]);
+    F::run(w, row, t, tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) w[m] = c_sub(c_ik(ky, az[m]), w[m]);
+    zinv_store<N, E, Cfg>(w, lds, row, stride, L, t, o[3], g, i, j0, lmask);
+  }
+  {
+    cplx w[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) w[m] = c_ik(wavenum(t + T * m, g.nz, g.dkz), ax[m]);
+    F::run(w, row, t, tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) w[m] = c_sub(w[m], c_ik(kx, az[m]));
+    zinv_store<N, E, Cfg>(w, lds, row, stride, L, t, o[4], g, i, j0, lmask);
+  }
+  zinv_emit<N, E, Cfg>(ax, lds, row, stride, L, t, tw, o[0], g, i, j0, lmask);  // ax := Z^-1(a_x)
+  zinv_emit<N, E, Cfg>(ay, lds, row, stride, L, t, tw, o[1], g, i, j0, lmask);  // ay := Z^-1(a_y)
+  {
+    cplx w[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) w[m] = c_sub(c_ik(kx, ay[m]), c_ik(ky, ax[m]));
+    zinv_store<N, E, Cfg>(w, lds, row, stride, L, t, o[5], g, i, j0, lmask);
+  }
 }
 
 template <int N, int MODE, typename Cfg = FFTCfg<N>>
@@ -358,6 +409,44 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
       }
       zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0, lmask);
     }
+  } else if constexpr (MODE == 1) {
+    // v -> (v, curl v)
+    const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
+    cplx vx[E], vy[E], vz[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      vx[m] = load(in.p[0], t + T * m);
+      vy[m] = load(in.p[1], t + T * m);
+      vz[m] = load(in.p[2], t + T * m);
+    }
+    cplx *const o[6] = {out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]};
+    zinv_curl5<N, E, Cfg>(vx, vy, vz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
+  } else if constexpr (MODE == 2) {
+    // (v, A) -> (v, curl v, b = curl A, curl b)
+    const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
+    {
+      cplx vx[E], vy[E], vz[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        vx[m] = load(in.p[0], t + T * m);
+        vy[m] = load(in.p[1], t + T * m);
+        vz[m] = load(in.p[2], t + T * m);
+      }
+      cplx *const o[6] = {out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]};
+      zinv_curl5<N, E, Cfg>(vx, vy, vz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
+    }
+    // b = curl A per mode (rotor3), then b and curl b
+    cplx bx[E], by[E], bz[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double kz = wavenum(t + T * m, g.nz, g.dkz);
+      const cplx ax = load(in.p[3], t + T * m), ay = load(in.p[4], t + T * m), az = load(in.p[5], t + T * m);
+      bx[m] = c_sub(c_ik(ky, az), c_ik(kz, ay));
+      by[m] = c_sub(c_ik(kz, ax), c_ik(kx, az));
+      bz[m] = c_sub(c_ik(kx, ay), c_ik(ky, ax));
+    }
+    cplx *const o[6] = {out.p[6], out.p[7], out.p[8], out.p[9], out.p[10], out.p[11]};
+    zinv_curl5<N, E, Cfg>(bx, by, bz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
   } else if constexpr (MODE == 6) {
     // velocity gradients: v -> d v_i / d x_j for i, j = 1..3 (out.p[3 i + j],
     // i major), derivk3 (pseudospec3D_hd.f90:22-88) scaled by 1/(nx ny nz)
@@ -381,85 +470,44 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
         zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[3 * c + dir], g, i, j0, lmask);
       }
   } else {
-    // MODE 1: v -> (v, curl v).  MODE 2: (v, A) -> (v, curl v, b = curl A, curl b).
     // MODE 4: (a, b, c, d) -> (a, b, c, grad d).  MODE 5: (v, th) -> (v, curl v, grad th).
+    using F = LineFFT<N, +1, Cfg>;
     const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
-    cplx u[3][E];
+    cplx ux[E], uy[E], uz[E];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int m = 0; m < E; ++m) u[c][m] = load(in.p[c], t + T * m);
-    // rotor3 (pseudospec3D_hd.f90:124-200) on (a0,a1,a2) -> component c
-    auto curl = [&](const cplx (&a)[3][E], int c, cplx (&o)[E]) {
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const double kz = wavenum(t + T * m, g.nz, g.dkz);
-        if (c == 0) o[m] = c_sub(c_ik(ky, a[2][m]), c_ik(kz, a[1][m]));
-        else if (c == 1) o[m] = c_sub(c_ik(kz, a[0][m]), c_ik(kx, a[2][m]));
-        else o[m] = c_sub(c_ik(kx, a[1][m]), c_ik(ky, a[0][m]));
-      }
-    };
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      cplx v[E];
-#pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = u[c][m];
-      zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[c], g, i, j0, lmask);
+    for (int m = 0; m < E; ++m) {
+      ux[m] = load(in.p[0], t + T * m);
+      uy[m] = load(in.p[1], t + T * m);
+      uz[m] = load(in.p[2], t + T * m);
     }
-    if constexpr (MODE != 4) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        cplx v[E];
-        curl(u, c, v);
-        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[3 + c], g, i, j0, lmask);
-      }
+    if constexpr (MODE == 5) {
+      cplx *const o[6] = {out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]};
+      zinv_curl5<N, E, Cfg>(ux, uy, uz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
+    } else {
+      zinv_emit<N, E, Cfg>(ux, lds, row, stride, L, t, tw, out.p[0], g, i, j0, lmask);
+      zinv_emit<N, E, Cfg>(uy, lds, row, stride, L, t, tw, out.p[1], g, i, j0, lmask);
+      zinv_emit<N, E, Cfg>(uz, lds, row, stride, L, t, tw, out.p[2], g, i, j0, lmask);
     }
-    if constexpr (MODE == 4 || MODE == 5) {
-      // derivk3 (pseudospec3D_hd.f90:22-88) of the scalar along x, y, z
-      constexpr int base = MODE == 4 ? 3 : 6;
-      cplx d[E];
+    // derivk3 (pseudospec3D_hd.f90:22-88) of the scalar along x, y, z:
+    // Z^-1(i kx d) = i kx Z^-1(d) and likewise for ky, so two transforms
+    constexpr int base = MODE == 4 ? 3 : 6;
+    cplx d[E];
 #pragma unroll
-      for (int m = 0; m < E; ++m) d[m] = load(in.p[3], t + T * m);
+    for (int m = 0; m < E; ++m) d[m] = load(in.p[3], t + T * m);
+    {
+      cplx w[E];
 #pragma unroll
-      for (int dir = 0; dir < 3; ++dir) {
-        cplx v[E];
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-          const double kd = dir == 0 ? kx : (dir == 1 ? ky : wavenum(t + T * m, g.nz, g.dkz));
-          v[m] = c_ik(kd, d[m]);
-        }
-        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[base + dir], g, i, j0, lmask);
-      }
+      for (int m = 0; m < E; ++m) w[m] = c_ik(wavenum(t + T * m, g.nz, g.dkz), d[m]);
+      zinv_emit<N, E, Cfg>(w, lds, row, stride, L, t, tw, out.p[base + 2], g, i, j0, lmask);
     }
-    if constexpr (MODE == 2) {
-      cplx a[3][E];
+    F::run(d, row, t, tw);
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
+    for (int dir = 0; dir < 2; ++dir) {
+      cplx w[E];
+      const double kd = dir == 0 ? kx : ky;
 #pragma unroll
-        for (int m = 0; m < E; ++m) a[c][m] = load(in.p[3 + c], t + T * m);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        cplx v[E];
-        curl(a, c, v);
-        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[6 + c], g, i, j0, lmask);
-      }
-      // j = curl b with the two components of b = curl A it needs recomputed
-      // per mode (the same operations as rotor3 of the stored b)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        cplx v[E];
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-          const double kz = wavenum(t + T * m, g.nz, g.dkz);
-          const cplx b0 = c_sub(c_ik(ky, a[2][m]), c_ik(kz, a[1][m]));
-          const cplx b1 = c_sub(c_ik(kz, a[0][m]), c_ik(kx, a[2][m]));
-          const cplx b2 = c_sub(c_ik(kx, a[1][m]), c_ik(ky, a[0][m]));
-          if (c == 0) v[m] = c_sub(c_ik(ky, b2), c_ik(kz, b1));
-          else if (c == 1) v[m] = c_sub(c_ik(kz, b0), c_ik(kx, b2));
-          else v[m] = c_sub(c_ik(kx, b1), c_ik(ky, b0));
-        }
-        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[9 + c], g, i, j0, lmask);
-      }
+      for (int m = 0; m < E; ++m) w[m] = c_ik(kd, d[m]);
+      zinv_store<N, E, Cfg>(w, lds, row, stride, L, t, out.p[base + dir], g, i, j0, lmask);
     }
   }
 }
