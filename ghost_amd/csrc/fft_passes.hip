@@ -198,7 +198,7 @@ static int tile_pick(const char *env, int ctx, int dflt) {
 
 // Row lengths with a two-for-one x kernel (a full-length schedule of 6 or 8
 // points per lane and at most 64 lanes; ghx_pass_shape takes the same list);
-// GHX_XC2=2 selects the half-length kernels instead, for comparisons.
+// GHX_XC2=0 selects the half-length kernels instead, for comparisons.
 static bool use_x2(int nx) {
   if (env_switch("GHX_XC2", 1) != 1) return false;
   return x2_size(nx);
