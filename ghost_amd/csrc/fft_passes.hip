@@ -285,6 +285,30 @@ GHX_DEV void tile_load_q(cplx *lds, int stride, int L, const cplx *src, size_t p
   }
 }
 
+// tile_load_q in two halves, so that a tile's loads can be in flight while
+// the previous tile is transformed: the E loads of a thread into registers,
+// then the registers into the LDS rows.
+template <int E>
+GHX_DEV void tile_fetch_q(cplx (&w)[E], int L, const cplx *src, size_t pstride, size_t base, int j0, int ny,
+                          unsigned lmask) {
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * blockDim.x;
+    const int l = idx % L, p = idx / L;
+    const bool inside = j0 + l < ny && ((lmask >> l) & 1u);
+    const cplx x = src[base + (size_t)p * pstride + j0 + (inside ? l : 0)];
+    w[q] = inside ? x : c_make(0.0, 0.0);
+  }
+}
+template <int E>
+GHX_DEV void tile_put_q(cplx *lds, int stride, int L, const cplx (&w)[E]) {
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * blockDim.x;
+    lds[(idx % L) * stride + lds_pad(idx / L)] = w[q];
+  }
+}
+
 // A (kx, ky) line carries dealiased modes iff kn2(kx, ky, 0) <= kmax.
 GHX_DEV bool line_active(const GridK &g, int i, int j) { return kn2_of(g, i, j, 0) <= g.kmax; }
 GHX_DEV unsigned tile_lines_active(const GridK &g, int i, int j0, int L) {
@@ -610,7 +634,7 @@ GHX_DEV cplx rk_update(cplx cc, cplx vv, cplx nl, cplx ff, double kk2, double nu
   return c_make(cc.x + dt * sx * rmp, cc.y + dt * sy * rmp);
 }
 
-template <int N, typename Cfg = FFTCfg<N>>
+template <int N, typename Cfg = FFTCfg<N>, int PF = 0>
 __global__ __launch_bounds__(kTileBound) void k_zfwd_rk_hd(FieldSet in_, GridK g, const cplx *__restrict__ twt, RKParams rk,
                                                    int prune) {
   using F = LineFFT<N, -1, Cfg>;
@@ -644,10 +668,28 @@ __global__ __launch_bounds__(kTileBound) void k_zfwd_rk_hd(FieldSet in_, GridK g
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
   cplx P[3][E];
+  if constexpr (PF) {
+    // the next field's tile loads are in flight while this field is transformed
+    const size_t qs = (size_t)g.nxh * g.ny, qb = (size_t)i * g.ny;
+    cplx w[E];
+    tile_fetch_q<E>(w, L, in.p[0], qs, qb, j0, g.ny, lmask);
 #pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    zfwd_load<N, E, Cfg>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
-    __syncthreads();
+    for (int f = 0; f < 3; ++f) {
+      tile_put_q<E>(lds, stride, L, w);
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < E; ++m) P[f][m] = row[lds_pad(t + T * m)];
+      __syncthreads();
+      if (f < 2) tile_fetch_q<E>(w, L, in.p[f + 1], qs, qb, j0, g.ny, lmask);
+      F::run(P[f], row, t, tw);
+      __syncthreads();
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      zfwd_load<N, E, Cfg>(P[f], lds, row, stride, L, t, tw, in.p[f], g, i, j0, lmask);
+      __syncthreads();
+    }
   }
   if (!ok) return;
   const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
@@ -1745,9 +1787,13 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, i
   // 512-point lines take the four-points-per-lane schedule by default (128
   // lanes per line, four lines per 512-lane block): 124 registers per lane
   // against 224, two blocks per CU, so the update's loads of v and C overlap
-  // more (0.71 -> 0.70 ms at 512^3; GHX_ZLOW=2 selects the default schedule).
+  // more (0.71 -> 0.70 ms at 512^3; GHX_ZLOW=0 selects the default schedule).
   // Other sizes take it on request (FFTCtx::sched bit 0).
   const bool low = (c.sched & 1) || (g.nz == 512 && env_switch("GHX_ZLOW", 1) == 1);
+  // On the four-points-per-lane schedule the next field's Q tile is loaded
+  // while this one is transformed (138 registers against 124; 0.698 -> 0.671
+  // ms at 512^3; GHX_ZPF=0 loads one tile at a time)
+  const int pf = env_switch("GHX_ZPF", 1) != 0 ? 1 : 0;
   if (low) {
     switch (g.nz) {
 #define GHX_LOW(n_)                                                                                     \
@@ -1757,9 +1803,15 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, i
     const PassShape s = tile_shape(T, lds_line_stride(n_), g.ny, tile_pick("GHX_LZ", c.tile_z, 4), max_lds()); \
     const int L = s.lines;                                                                              \
     dim3 grid((g.ny + L - 1) / L, g.nxh, c.batch);                                                      \
-    allow_lds(k_zfwd_rk_hd<n_, Cfg>, s.lds);                                                            \
-    GHX_LAUNCH_CHECK((k_zfwd_rk_hd<n_, Cfg>), s, kTileBound, T, true);                                  \
-    hipLaunchKernelGGL((k_zfwd_rk_hd<n_, Cfg>), grid, dim3(L * T), s.lds, c.stream, in, g, c.tw_z, rk, prune); \
+    if (pf) {                                                                                           \
+      allow_lds(k_zfwd_rk_hd<n_, Cfg, 1>, s.lds);                                                       \
+      GHX_LAUNCH_CHECK((k_zfwd_rk_hd<n_, Cfg, 1>), s, kTileBound, T, true);                             \
+      hipLaunchKernelGGL((k_zfwd_rk_hd<n_, Cfg, 1>), grid, dim3(L * T), s.lds, c.stream, in, g, c.tw_z, rk, prune); \
+    } else {                                                                                            \
+      allow_lds(k_zfwd_rk_hd<n_, Cfg>, s.lds);                                                          \
+      GHX_LAUNCH_CHECK((k_zfwd_rk_hd<n_, Cfg>), s, kTileBound, T, true);                                \
+      hipLaunchKernelGGL((k_zfwd_rk_hd<n_, Cfg>), grid, dim3(L * T), s.lds, c.stream, in, g, c.tw_z, rk, prune); \
+    }                                                                                                   \
     return hipGetLastError();                                                                           \
   }
       GHX_LOW(32) GHX_LOW(64) GHX_LOW(128) GHX_LOW(256) GHX_LOW(512)
