@@ -112,7 +112,7 @@ def kernel_name(slot, n):
     if slot in ("ypass_inv", "ypass_inv_tracer") and n == 512:
         return "ghx::k_yinv<512, FFTCfgWide<512> >"
     if slot == "zfwd_rk_hd" and n == 512:
-        return "ghx::k_zfwd_rk_hd<512, FFTCfgLow<512> >"
+        return "ghx::k_zfwd_rk_hd<512, FFTCfgLow<512>, 1>"
     if slot == "zinv_spline" and n == 512:
         return "ghx::k_zinv<512, 3, FFTCfgWide<512> >"
     return KERNEL_NAME.get(slot, slot).replace("<N", f"<{n}")
