@@ -815,15 +815,15 @@ __global__ __launch_bounds__(kTileBound) void k_zfwd_rk_bouss(FieldSet in, GridK
 // (w x u, j x b) (PART 0) and gauge3 (:230-336) for the potential from u x b
 // (PART 1).  The two parts are separate launches so that each keeps its
 // fields in registers.
-template <int N, int PART>
+template <int N, int PART, typename Cfg = FFTCfg<N>>
 __global__ __launch_bounds__(kTileBound) void k_zfwd_rk_mhd(FieldSet in, GridK g, const cplx *__restrict__ twt, MHDRKParams rk,
                                                     int prune) {
-  using F = LineFFT<N, -1>;
+  using F = LineFFT<N, -1, Cfg>;
   constexpr int T = F::T, E = F::E;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
-  TwCache<N> tw;  // stage twiddle roots of this lane, loaded once
+  TwCache<N, Cfg> tw;  // stage twiddle roots of this lane, loaded once
   tw.load(twt, t);
   int bx, by;
   xcd_tile(bx, by);
@@ -833,22 +833,33 @@ __global__ __launch_bounds__(kTileBound) void k_zfwd_rk_mhd(FieldSet in, GridK g
   const bool ok = j < g.ny && ((lmask >> lt) & 1u);
   const int stride = lds_line_stride(N);
   cplx *row = lds + lt * stride;
+  // PART 0, velocity: fields 0..2 = w x u, 3..5 = j x b; nonlin3 needs only
+  // the difference of their transforms, D = FFT(w x u - j x b), and the
+  // difference is taken on the tiles before one transform.  PART 1, vector
+  // potential: fields 6..8 = u x b.  The next field's tiles are in flight
+  // while this one is transformed.
+  const size_t qs = (size_t)g.nxh * g.ny, qb = (size_t)i * g.ny;
+  cplx wa[E], wb[E];
+  auto fetch = [&](int f) {
+    tile_fetch_q<E>(wa, L, in.p[PART == 0 ? f : 6 + f], qs, qb, j0, g.ny, lmask);
+    if constexpr (PART == 0) tile_fetch_q<E>(wb, L, in.p[3 + f], qs, qb, j0, g.ny, lmask);
+  };
   cplx D[3][E];
-  if constexpr (PART == 0) {
-    // velocity: fields 0..2 = w x u, 3..5 = j x b; nonlin3 needs only the
-    // difference of their transforms, D = FFT(w x u - j x b)
+  fetch(0);
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      zfwd_load_diff<N, E>(D[f], lds, row, stride, L, t, tw, in.p[f], in.p[3 + f], g, i, j0, lmask);
-      __syncthreads();
-    }
-  } else {
-    // vector potential: fields 6..8 = u x b
+  for (int f = 0; f < 3; ++f) {
+    if constexpr (PART == 0) {
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      zfwd_load<N, E>(D[f], lds, row, stride, L, t, tw, in.p[6 + f], g, i, j0, lmask);
-      __syncthreads();
+      for (int q = 0; q < E; ++q) wa[q] = c_sub(wa[q], wb[q]);
     }
+    tile_put_q<E>(lds, stride, L, wa);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) D[f][m] = row[lds_pad(t + T * m)];
+    __syncthreads();
+    if (f < 2) fetch(f + 1);
+    F::run(D[f], row, t, tw);
+    __syncthreads();
   }
   if (!ok) return;
   constexpr int c0 = PART == 0 ? 0 : 3;
@@ -1844,9 +1855,26 @@ hipError_t zfwd_rk_hd(const FFTCtx &c, const FieldSet &in, const RKParams &rk, i
 #if GHX_PART == 5
 hipError_t zfwd_rk_mhd(const FFTCtx &c, const FieldSet &in, const MHDRKParams &rk, int prune) {
   const GridK &g = c.gk;
+  // the four-points-per-lane schedule where a size has one (GHX_ZLOW=0: the
+  // default schedule)
+  const bool low = env_switch("GHX_ZLOW", 1) == 1;
   switch (g.nz) {
 #define GHX_CASE(n_)                                                                        \
   case n_: {                                                                                \
+    if constexpr (FFTCfgLow<n_>::E != FFTCfg<n_>::E) if (low) {                              \
+      using Cl = FFTCfgLow<n_>;                                                              \
+      constexpr int T = Cl::T;                                                               \
+      const PassShape s = tile_shape(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4), max_lds()); \
+      const int L = s.lines; const size_t lds = s.lds;                                       \
+      dim3 grid((g.ny + L - 1) / L, g.nxh);                                                  \
+      allow_lds(k_zfwd_rk_mhd<n_, 0, Cl>, lds);                                              \
+      allow_lds(k_zfwd_rk_mhd<n_, 1, Cl>, lds);                                              \
+      GHX_LAUNCH_CHECK((k_zfwd_rk_mhd<n_, 0, Cl>), s, kTileBound, T, true);                  \
+      hipLaunchKernelGGL((k_zfwd_rk_mhd<n_, 0, Cl>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
+      GHX_LAUNCH_CHECK((k_zfwd_rk_mhd<n_, 1, Cl>), s, kTileBound, T, true);                  \
+      hipLaunchKernelGGL((k_zfwd_rk_mhd<n_, 1, Cl>), grid, dim3(L * T), lds, c.stream, in, g, c.tw_z, rk, prune); \
+      break;                                                                                 \
+    }                                                                                        \
     constexpr int T = FFTCfg<n_>::T;                                                        \
     const PassShape s = tile_shape(T, lds_line_stride(n_), g.ny, tile_want("GHX_LZ", 4), max_lds());                                        \
     const int L = s.lines; const size_t lds = s.lds;                            \
