@@ -247,20 +247,46 @@ GHX_DEV void xcd_tile(int &bx, int &by) {
 // dealias pruning and the tile's edge; the loader zero-fills the rest without
 // touching memory).  The block is exactly L x T lanes with L a power of two
 // and E * T points per row, so every lane moves E points: line
-// l = idx & (L - 1), point idx >> log2 L.  The loop stays rolled (two points
+// l = idx & (L - 1), point idx >> log2 L.  The loop stays rolled (G points
 // per trip): a fully unrolled one holds all E points and costs the y and z
 // kernels a wave per SIMD or spills, for no gain.  NT selects streaming
 // stores: the y inverse (128-byte chunks) gains 10-14% with them, the z
 // inverse (64-byte chunks at L = 4) loses 5-11%.
-template <int E, bool NT, typename Keep>
+//
+// The block is L x T lanes with L a power of two, so l is the same on every
+// trip: the keep test is made once per lane, and the LDS reads go in groups
+// of G (up to 4; GW > 0 sets it) ahead of their stores, so that a group's
+// reads are in flight together instead of each store waiting on its own read.
+template <int E, bool NT, int GW = 0, typename Keep>
 GHX_DEV void tile_store_rows(const cplx *lds, int stride, int L, cplx *dst, size_t pstride, Keep keep) {
+  constexpr int G = GW > 0 ? GW : E % 4 == 0 ? 4 : E % 3 == 0 ? 3 : E % 2 == 0 ? 2 : 1;
+  static_assert(E % G == 0, "the group width divides the points per lane");
   const int lsh = 31 - __builtin_clz((unsigned)L);
+  if constexpr (G == 1) {
+    // one point per trip, the keep test in the loop: the fewest live
+    // registers, for kernels at the register limit
 #pragma unroll 2
-  for (int q = 0; q < E; ++q) {
-    const int idx = threadIdx.x + q * blockDim.x;
-    const int l = idx & (L - 1);
-    const cplx x = lds[l * stride + lds_pad(idx >> lsh)];
-    if (keep(l)) c_put<NT>(dst + (size_t)(idx >> lsh) * pstride + l, x);
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * blockDim.x;
+      const int l = idx & (L - 1);
+      const cplx x = lds[l * stride + lds_pad(idx >> lsh)];
+      if (keep(l)) c_put<NT>(dst + (size_t)(idx >> lsh) * pstride + l, x);
+    }
+    return;
+  }
+  const int l = threadIdx.x & (L - 1);
+  if (!keep(l)) return;
+  const int p0 = threadIdx.x >> lsh, dp = blockDim.x >> lsh;
+  const cplx *src = lds + l * stride;
+  cplx *d = dst + (size_t)p0 * pstride + l;
+  const size_t dstep = (size_t)dp * pstride;
+#pragma unroll 1
+  for (int q0 = 0; q0 < E; q0 += G) {
+    cplx x[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) x[u] = src[lds_pad(p0 + (q0 + u) * dp)];
+#pragma unroll
+    for (int u = 0; u < G; ++u) c_put<NT>(d + (size_t)(q0 + u) * dstep, x[u]);
   }
 }
 // Block = L lines x T lanes, E = N / T points per thread.  The E loads of a
@@ -323,7 +349,7 @@ GHX_DEV unsigned tile_lines_active(const GridK &g, int i, int j0, int L) {
 // ---------------------------------------------------------------------------
 // The store half of zinv_emit: the transformed line v (natural order, lane t
 // holding points t + T m) into the block's LDS rows, then the tile to Q.
-template <int N, int E, typename Cfg = FFTCfg<N>>
+template <int N, int E, typename Cfg = FFTCfg<N>, int GW = 0>
 GHX_DEV void zinv_store(const cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, cplx *dst,
                         const GridK &g, int i, int j0, unsigned lmask) {
   constexpr int T = LineFFT<N, +1, Cfg>::T;
@@ -331,16 +357,16 @@ GHX_DEV void zinv_store(const cplx (&v)[E], cplx *lds, cplx *row, int stride, in
   for (int m = 0; m < E; ++m) row[lds_pad(t + T * m)] = v[m];
   __syncthreads();
   const int ny = g.ny;
-  tile_store_rows<E, false>(lds, stride, L, dst + (size_t)i * ny + j0, (size_t)g.nxh * ny,
+  tile_store_rows<E, false, GW>(lds, stride, L, dst + (size_t)i * ny + j0, (size_t)g.nxh * ny,
                      [=](int l) { return j0 + l < ny && ((lmask >> l) & 1u); });
   __syncthreads();
 }
 
-template <int N, int E, typename Cfg = FFTCfg<N>>
+template <int N, int E, typename Cfg = FFTCfg<N>, int GW = 0>
 GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, int t, const TwCache<N, Cfg> &tw,
                        cplx *dst, const GridK &g, int i, int j0, unsigned lmask) {
   LineFFT<N, +1, Cfg>::run(v, row, t, tw);
-  zinv_store<N, E, Cfg>(v, lds, row, stride, L, t, dst, g, i, j0, lmask);
+  zinv_store<N, E, Cfg, GW>(v, lds, row, stride, L, t, dst, g, i, j0, lmask);
 }
 
 // A vector field's z inverse and its curl's with five transforms instead of
@@ -352,13 +378,13 @@ GHX_DEV void zinv_emit(cplx (&v)[E], cplx *lds, cplx *row, int stride, int L, in
 //   curl_z = i kx Z^-1(a_y) - i ky Z^-1(a_x)
 // ax, ay, az (the spectral lines, lane t holding points t + T m) are
 // transformed in place; the six results go to o[0..2] (a) and o[3..5] (curl a).
-template <int N, int E, typename Cfg>
+template <int N, int E, typename Cfg, int GW = 0>
 GHX_DEV void zinv_curl5(cplx (&ax)[E], cplx (&ay)[E], cplx (&az)[E], double kx, double ky, cplx *lds, cplx *row,
                         int stride, int L, int t, const TwCache<N, Cfg> &tw, cplx *const *o, const GridK &g, int i,
                         int j0, unsigned lmask) {
   using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T;
-  zinv_emit<N, E, Cfg>(az, lds, row, stride, L, t, tw, o[2], g, i, j0, lmask);  // az := Z^-1(a_z)
+  zinv_emit<N, E, Cfg, GW>(az, lds, row, stride, L, t, tw, o[2], g, i, j0, lmask);  // az := Z^-1(a_z)
   {
     cplx w[E];
 #pragma unroll
@@ -366,7 +392,7 @@ GHX_DEV void zinv_curl5(cplx (&ax)[E], cplx (&ay)[E], cplx (&az)[E], double kx, 
     F::run(w, row, t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) w[m] = c_sub(c_ik(ky, az[m]), w[m]);
-    zinv_store<N, E, Cfg>(w, lds, row, stride, L, t, o[3], g, i, j0, lmask);
+    zinv_store<N, E, Cfg, GW>(w, lds, row, stride, L, t, o[3], g, i, j0, lmask);
   }
   {
     cplx w[E];
@@ -375,15 +401,15 @@ GHX_DEV void zinv_curl5(cplx (&ax)[E], cplx (&ay)[E], cplx (&az)[E], double kx, 
     F::run(w, row, t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) w[m] = c_sub(w[m], c_ik(kx, az[m]));
-    zinv_store<N, E, Cfg>(w, lds, row, stride, L, t, o[4], g, i, j0, lmask);
+    zinv_store<N, E, Cfg, GW>(w, lds, row, stride, L, t, o[4], g, i, j0, lmask);
   }
-  zinv_emit<N, E, Cfg>(ax, lds, row, stride, L, t, tw, o[0], g, i, j0, lmask);  // ax := Z^-1(a_x)
-  zinv_emit<N, E, Cfg>(ay, lds, row, stride, L, t, tw, o[1], g, i, j0, lmask);  // ay := Z^-1(a_y)
+  zinv_emit<N, E, Cfg, GW>(ax, lds, row, stride, L, t, tw, o[0], g, i, j0, lmask);  // ax := Z^-1(a_x)
+  zinv_emit<N, E, Cfg, GW>(ay, lds, row, stride, L, t, tw, o[1], g, i, j0, lmask);  // ay := Z^-1(a_y)
   {
     cplx w[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) w[m] = c_sub(c_ik(kx, ay[m]), c_ik(ky, ax[m]));
-    zinv_store<N, E, Cfg>(w, lds, row, stride, L, t, o[5], g, i, j0, lmask);
+    zinv_store<N, E, Cfg, GW>(w, lds, row, stride, L, t, o[5], g, i, j0, lmask);
   }
 }
 
@@ -392,6 +418,10 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
                                              int prune, const double *__restrict__ aux) {
   using F = LineFFT<N, +1, Cfg>;
   constexpr int T = F::T, E = F::E;
+  // store groups: the MHD and Boussinesq passes (MODE 2, 5) are at the
+  // register limit and spill more with grouped reads; they store one point
+  // per trip
+  constexpr int SG = MODE == 2 || MODE == 5 ? 1 : 0;
   extern __shared__ cplx lds[];
   const int L = blockDim.x / T;
   const int lt = threadIdx.x / T, t = threadIdx.x % T;
@@ -431,7 +461,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
         v[m] = load(in.p[f], t + T * m);
         if constexpr (MODE == 3) v[m] = c_scale(v[m], sxy * aux[g.nxh + g.ny + t + T * m]);
       }
-      zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0, lmask);
+      zinv_emit<N, E, Cfg, SG>(v, lds, row, stride, L, t, tw, out.p[f], g, i, j0, lmask);
     }
   } else if constexpr (MODE == 1) {
     // v -> (v, curl v)
@@ -444,7 +474,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
       vz[m] = load(in.p[2], t + T * m);
     }
     cplx *const o[6] = {out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]};
-    zinv_curl5<N, E, Cfg>(vx, vy, vz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
+    zinv_curl5<N, E, Cfg, SG>(vx, vy, vz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
   } else if constexpr (MODE == 2) {
     // (v, A) -> (v, curl v, b = curl A, curl b)
     const double kx = wavenum(i, g.nx, g.dkx), ky = wavenum(j, g.ny, g.dky);
@@ -457,7 +487,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
         vz[m] = load(in.p[2], t + T * m);
       }
       cplx *const o[6] = {out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]};
-      zinv_curl5<N, E, Cfg>(vx, vy, vz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
+      zinv_curl5<N, E, Cfg, SG>(vx, vy, vz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
     }
     // b = curl A per mode (rotor3), then b and curl b
     cplx bx[E], by[E], bz[E];
@@ -470,7 +500,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
       bz[m] = c_sub(c_ik(kx, ay), c_ik(ky, ax));
     }
     cplx *const o[6] = {out.p[6], out.p[7], out.p[8], out.p[9], out.p[10], out.p[11]};
-    zinv_curl5<N, E, Cfg>(bx, by, bz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
+    zinv_curl5<N, E, Cfg, SG>(bx, by, bz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
   } else if constexpr (MODE == 6) {
     // velocity gradients: v -> d v_i / d x_j for i, j = 1..3 (out.p[3 i + j],
     // i major), derivk3 (pseudospec3D_hd.f90:22-88) scaled by 1/(nx ny nz)
@@ -491,7 +521,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
           const double kd = dir == 0 ? kx : (dir == 1 ? ky : wavenum(t + T * m, g.nz, g.dkz));
           v[m] = c_scale(c_ik(kd, u[c][m]), sc);
         }
-        zinv_emit<N, E, Cfg>(v, lds, row, stride, L, t, tw, out.p[3 * c + dir], g, i, j0, lmask);
+        zinv_emit<N, E, Cfg, SG>(v, lds, row, stride, L, t, tw, out.p[3 * c + dir], g, i, j0, lmask);
       }
   } else {
     // MODE 4: (a, b, c, d) -> (a, b, c, grad d).  MODE 5: (v, th) -> (v, curl v, grad th).
@@ -506,11 +536,11 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
     }
     if constexpr (MODE == 5) {
       cplx *const o[6] = {out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]};
-      zinv_curl5<N, E, Cfg>(ux, uy, uz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
+      zinv_curl5<N, E, Cfg, SG>(ux, uy, uz, kx, ky, lds, row, stride, L, t, tw, o, g, i, j0, lmask);
     } else {
-      zinv_emit<N, E, Cfg>(ux, lds, row, stride, L, t, tw, out.p[0], g, i, j0, lmask);
-      zinv_emit<N, E, Cfg>(uy, lds, row, stride, L, t, tw, out.p[1], g, i, j0, lmask);
-      zinv_emit<N, E, Cfg>(uz, lds, row, stride, L, t, tw, out.p[2], g, i, j0, lmask);
+      zinv_emit<N, E, Cfg, SG>(ux, lds, row, stride, L, t, tw, out.p[0], g, i, j0, lmask);
+      zinv_emit<N, E, Cfg, SG>(uy, lds, row, stride, L, t, tw, out.p[1], g, i, j0, lmask);
+      zinv_emit<N, E, Cfg, SG>(uz, lds, row, stride, L, t, tw, out.p[2], g, i, j0, lmask);
     }
     // derivk3 (pseudospec3D_hd.f90:22-88) of the scalar along x, y, z:
     // Z^-1(i kx d) = i kx Z^-1(d) and likewise for ky, so two transforms
@@ -522,7 +552,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
       cplx w[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) w[m] = c_ik(wavenum(t + T * m, g.nz, g.dkz), d[m]);
-      zinv_emit<N, E, Cfg>(w, lds, row, stride, L, t, tw, out.p[base + 2], g, i, j0, lmask);
+      zinv_emit<N, E, Cfg, SG>(w, lds, row, stride, L, t, tw, out.p[base + 2], g, i, j0, lmask);
     }
     F::run(d, row, t, tw);
 #pragma unroll
@@ -531,7 +561,7 @@ __global__ __launch_bounds__(kTileBound) void k_zinv(FieldSet in_, FieldSet out_
       const double kd = dir == 0 ? kx : ky;
 #pragma unroll
       for (int m = 0; m < E; ++m) w[m] = c_ik(kd, d[m]);
-      zinv_store<N, E, Cfg>(w, lds, row, stride, L, t, out.p[base + dir], g, i, j0, lmask);
+      zinv_store<N, E, Cfg, SG>(w, lds, row, stride, L, t, out.p[base + dir], g, i, j0, lmask);
     }
   }
 }
