@@ -248,8 +248,9 @@ GHX_DEV void xcd_tile(int &bx, int &by) {
 // touching memory).  The block is exactly L x T lanes with L a power of two
 // and E * T points per row, so every lane moves E points: line
 // l = idx & (L - 1), point idx >> log2 L.  The loop stays rolled (G points
-// per trip): a fully unrolled one holds all E points and costs the y and z
-// kernels a wave per SIMD or spills, for no gain.  NT selects streaming
+// per trip): holding all E points at once costs most y and z kernels a wave
+// per SIMD or spills; a caller with registers to spare sets G = E through GW
+// (the 512-point curl z inverse, 8 points).  NT selects streaming
 // stores: the y inverse (128-byte chunks) gains 10-14% with them, the z
 // inverse (64-byte chunks at L = 4) loses 5-11%.
 //
